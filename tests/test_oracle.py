@@ -1,0 +1,328 @@
+"""Pins the CPU oracle against the reference's own known-answer tests,
+closed-form physics and invariants (no GPU).
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+quadrant, a swapped half-width, a wrong sign in the elimination, a missing
+D^1/2 in the residual) fails it.
+"""
+
+import warnings
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+import fs_oracle as O
+from fieldspice_amd.grid import RectilinearGrid, graded_1d
+from fieldspice_amd.units import eps0
+
+
+def _graded():
+    return RectilinearGrid(graded_1d(0, 1e-3, 7, 1.3), graded_1d(0, 2e-3, 5, 0.8),
+                           graded_1d(0, 3e-4, 4, 1.5))
+
+
+def test_eps0_matches_reference_constant():
+    # fieldspice/units.py:23-26 -> the value quoted in BASELINE (not CODATA)
+    assert eps0 == 8.854187817131639e-12
+    assert O.EPS0 == eps0
+
+
+def test_grid_volumes_and_dual_areas_tile_domain():
+    g = _graded()
+    total = np.prod([b[1] - b[0] for b in g.bounds])
+    assert g.node_volumes().sum() == pytest.approx(total, rel=1e-14)
+    ax, ay, az = g.edge_dual_areas()
+    Lx, Ly, Lz = g.bounds
+    assert ax[0].sum() == pytest.approx((Ly[1] - Ly[0]) * (Lz[1] - Lz[0]), rel=1e-13)
+    assert ay[:, 0, :].sum() == pytest.approx((Lx[1] - Lx[0]) * (Lz[1] - Lz[0]), rel=1e-13)
+    assert az[:, :, 0].sum() == pytest.approx((Lx[1] - Lx[0]) * (Ly[1] - Ly[0]), rel=1e-13)
+
+
+def test_collapsed_axis_has_two_node_planes():
+    g = RectilinearGrid(np.linspace(0, 1, 137))
+    assert g.shape_nodes == (137, 2, 2) and g.n_nodes == 548
+
+
+def test_edge_weight_single_cell_kat():
+    # tests/test_core.py:185-191: x-edge conductance 7*(1.5*2.5)/2
+    g = RectilinearGrid.uniform([(0, 2.0), (0, 3.0), (0, 5.0)], [1, 1, 1])
+    w = O.edge_weights(g, O.cell_to_edge(g, np.full(g.shape_cells, 7.0)))
+    assert w[0] == pytest.approx(7.0 * (1.5 * 2.5) / 2.0)
+
+
+def test_cell_to_edge_area_weighting_kat():
+    # tests/test_core.py:194-205 and 208-214
+    g = RectilinearGrid.uniform([(0, 1.0), (0, 1.0), (0, 1.0)], [1, 2, 1])
+    prop = np.zeros(g.shape_cells)
+    prop[0, 0, 0] = 10.0
+    prop[0, 1, 0] = 20.0
+    e = O.cell_to_edge(g, prop)
+    ex = e[:g.n_edges_each[0]].reshape(g.shape_edges[0])
+    assert ex[0, 1, 0] == pytest.approx(15.0)
+    assert ex[0, 0, 0] == pytest.approx(10.0)
+    assert ex[0, 2, 0] == pytest.approx(20.0)
+    eh = O.cell_to_edge(g, prop, "harmonic")[:g.n_edges_each[0]].reshape(g.shape_edges[0])
+    assert eh[0, 1, 0] == pytest.approx(2.0 / (1 / 10.0 + 1 / 20.0))
+
+
+def test_cell_to_edge_uniform_and_quadrant_brute_force():
+    g = _graded()
+    assert np.allclose(O.cell_to_edge(g, np.full(g.shape_cells, 3.7)), 3.7, rtol=1e-15)
+    # brute force over the four cells around each y-edge
+    rng = np.random.default_rng(3)
+    prop = rng.uniform(1, 9, g.shape_cells)
+    e = O.cell_to_edge(g, prop)
+    ney = g.n_edges_each[1]
+    ey = e[g.n_edges_each[0]:g.n_edges_each[0] + ney].reshape(g.shape_edges[1])
+    Nx, Ny, Nz = g.ncell
+    for (i, j, k) in [(0, 0, 0), (3, 2, 1), (Nx, Ny - 1, Nz), (Nx, 1, 2), (2, 4, 0)]:
+        num = den = 0.0
+        for ci, wi in ((i - 1, g.hx[i - 1] / 2 if i > 0 else 0), (i, g.hx[i] / 2 if i < Nx else 0)):
+            for ck, wk in ((k - 1, g.hz[k - 1] / 2 if k > 0 else 0), (k, g.hz[k] / 2 if k < Nz else 0)):
+                if wi * wk > 0:
+                    num += prop[ci, j, ck] * wi * wk
+                    den += wi * wk
+        assert ey[i, j, k] == pytest.approx(num / den, rel=1e-14)
+
+
+def test_cell_to_node_uniform():
+    g = _graded()
+    assert np.allclose(O.cell_to_node(g, np.full(g.shape_cells, 3.7)), 3.7)
+
+
+def test_laplacian_null_space_symmetry_and_incidence():
+    g = _graded()
+    G = O.grad_node_edge(g)
+    assert set(np.unique(G.data)) <= {-1.0, 1.0}
+    assert np.abs(G @ np.ones(g.n_nodes)).max() == 0.0
+    L = O.nodal_laplacian(g, O.cell_to_edge(g, np.full(g.shape_cells, eps0)))
+    assert np.abs(L @ np.ones(g.n_nodes)).max() < 1e-24
+    nid = g.node_ids()
+    A, b = O.apply_dirichlet(L, np.zeros(g.n_nodes), nid[0].ravel(), 1.0)
+    assert (abs(A - A.T) > 1e-18).nnz == 0
+
+
+def _plate(prop_cell, n=(24, 10, 8), d=2e-6, W=5e-6, H=4e-6):
+    g = RectilinearGrid(np.linspace(0, d, n[0] + 1), np.linspace(0, W, n[1] + 1),
+                        np.linspace(0, H, n[2] + 1))
+    L = O.nodal_laplacian(g, O.cell_to_edge(g, prop_cell(g)))
+    nid = g.node_ids()
+    lo, hi = nid[0].ravel(), nid[-1].ravel()
+    A, b = O.apply_dirichlet(L, np.zeros(g.n_nodes), np.concatenate([lo, hi]),
+                             np.concatenate([np.ones(lo.size), np.zeros(hi.size)]))
+    return g, L, spl.spsolve(A.tocsc(), b), lo
+
+
+def test_plate_capacitance_slab_resistance_linear_profile():
+    er, d, W, H = 3.9, 2e-6, 5e-6, 4e-6
+    g, L, phi, lo = _plate(lambda g: np.full(g.shape_cells, er * eps0))
+    assert (L @ phi)[lo].sum() == pytest.approx(er * eps0 * W * H / d, rel=1e-12)
+    g, L, phi, lo = _plate(lambda g: np.full(g.shape_cells, 1e4))
+    assert 1.0 / (L @ phi)[lo].sum() == pytest.approx(d / (1e4 * W * H), rel=1e-12)
+    prof = phi.reshape(g.shape_nodes)[:, 4, 4]
+    assert np.abs(prof - np.linspace(1, 0, prof.size)).max() < 1e-12
+
+
+def test_series_stack():
+    er1, er2, d, W, H = 4.0, 2.0, 4e-6, 3e-6, 2e-6
+
+    def prop(g):
+        e = np.empty(g.shape_cells)
+        left = g.xc < d / 2
+        e[left] = er1 * eps0
+        e[~left] = er2 * eps0
+        return e
+    g, L, phi, lo = _plate(prop, n=(40, 4, 3), d=d, W=W, H=H)
+    C1 = er1 * eps0 * W * H / (d / 2)
+    C2 = er2 * eps0 * W * H / (d / 2)
+    assert (L @ phi)[lo].sum() == pytest.approx(C1 * C2 / (C1 + C2), rel=1e-10)
+
+
+@pytest.mark.parametrize("method", ["direct", "cg_jacobi"])
+def test_oracle_capacitance_plate_config(method):
+    # config: 24x10x8 cells on 2x5x4 um, eps_r 3.9, x=0 plane vs x=last plane
+    g = RectilinearGrid.uniform([(0, 2e-6), (0, 5e-6), (0, 4e-6)], [24, 10, 8])
+    nid = g.node_ids()
+    C, rep = O.capacitance_matrix(g, np.full(g.shape_cells, 3.9 * eps0),
+                                  [nid[0].ravel(), nid[-1].ravel()], method=method)
+    ref = 3.4531332486813394e-16
+    assert C[0, 0] == pytest.approx(ref, rel=1e-12 if method == "direct" else 1e-9)
+    assert C[0, 1] == pytest.approx(-ref, rel=1e-9)
+    assert rep["row_sum_rel"] < 1e-9 and rep["asymmetry_rel"] < 1e-9
+
+
+def test_oracle_cg_matches_direct_on_multimaterial():
+    rng = np.random.default_rng(0)
+    g = RectilinearGrid(graded_1d(0, 3e-6, 12, 1.1), np.linspace(0, 2e-6, 9),
+                        graded_1d(0, 2e-6, 10, 0.9))
+    eps = rng.choice([2.7, 3.9, 7.5], size=g.shape_cells) * eps0
+    nid = g.node_ids()
+    terms = [nid[2:4, 2:4, 2:4].ravel(), nid[8:10, 5:7, 6:9].ravel()]
+    Cd, _ = O.capacitance_matrix(g, eps, terms, method="direct")
+    Cc, rep = O.capacitance_matrix(g, eps, terms, method="cg_jacobi")
+    assert np.abs(Cc - Cd).max() / np.abs(Cd).max() < 1e-9
+    assert rep["linear"]["residual"] < 1e-9
+    assert Cd[0, 1] < 0 and Cd[0, 0] > 0
+
+
+def test_oracle_pins_empty_rows_and_gauge():
+    A = sp.diags([-1.0, 2.0, -1.0], [-1, 0, 1], shape=(5, 5)).tocsr()
+    ls = O.LinearSystem(A)
+    assert np.allclose(A @ ls.solve(np.ones(5)), np.ones(5))
+    # floating Laplacian: singular, gauge pinned at the largest diagonal
+    g = RectilinearGrid.uniform([(0, 1.0), (0, 1.0), (0, 1.0)], [3, 2, 2])
+    L = O.nodal_laplacian(g, O.cell_to_edge(g, np.ones(g.shape_cells)))
+    with pytest.warns(RuntimeWarning, match="gauge"):
+        ls = O.LinearSystem(L)
+    assert ls.gauge == int(np.argmax(L.diagonal()))
+    # empty row
+    M = sp.csr_matrix(np.diag([1.0, 0.0, 2.0]) + np.diag([0.0, 0.0], 0)[:0].sum())
+    M = sp.csr_matrix(np.array([[2.0, 0, -1.0], [0, 0, 0], [-1.0, 0, 2.0]]))
+    with pytest.warns(RuntimeWarning, match="empty rows"):
+        ls = O.LinearSystem(M)
+    x = ls.solve(np.array([1.0, 5.0, 1.0]))
+    assert x[1] == 0.0 and np.allclose(x[[0, 2]], [1.0, 1.0])
+
+
+def test_oracle_cg_forced_nonconvergence():
+    # A system with no identity (Dirichlet) rows, so ||b|| is not dominated by
+    # prescribed values: two CG iterations cannot reach 1e-10 -> hard failure
+    # above max(1e-6, 100 tol) (poisson.py:576-582).
+    n = 200
+    A = sp.diags([-1.0, 2.0, -1.0], [-1, 0, 1], shape=(n, n)).tocsr()
+    ls = O.LinearSystem(A, method="cg_jacobi", maxiter=2)
+    with pytest.raises(O.OracleConvergenceError):
+        ls.solve(np.ones(n))
+
+
+def _rc_stack(n=60):
+    d, W, H = 4e-6, 3e-6, 2e-6
+    g = RectilinearGrid(np.linspace(0, d, n + 1), np.linspace(0, W, 4), np.linspace(0, H, 3))
+    left = g.xc < d / 2
+    sig = np.zeros(g.shape_cells)
+    sig[left] = 1e3
+    eps = np.empty(g.shape_cells)
+    eps[left] = 4.0 * eps0
+    eps[~left] = 2.0 * eps0
+    A = W * H
+    R1 = (d / 2) / (1e3 * A)
+    C1, C2 = 4.0 * eps0 * A / (d / 2), 2.0 * eps0 * A / (d / 2)
+    return g, eps, sig, R1, C1, C2
+
+
+def _eqs(steps_per_tau):
+    g, eps, sig, R1, C1, C2 = _rc_stack()
+    tau = R1 * (C1 + C2)
+    v0 = C1 / (C1 + C2)
+    nid = g.node_ids()
+    hi, lo = nid[0].ravel(), nid[-1].ravel()
+    fixed = np.concatenate([hi, lo])
+    vals = np.concatenate([np.ones(hi.size), np.zeros(lo.size)])
+    fx = O.unique_last(fixed, vals)
+    out = O.eqs_transient(g, eps, sig, lambda t: fx, 8 * tau, tau / steps_per_tau,
+                          terminal_nodes=[hi, lo])
+    vm = np.array([p[nid[30, 1, 1]] for p in out["phis"]])
+    ana = 1.0 - (1.0 - v0) * np.exp(-out["t"] / tau)
+    return out, vm, ana, v0
+
+
+def test_oracle_eqs_rc_divider_order_and_currents():
+    out, vm, ana, v0 = _eqs(50)
+    assert v0 == pytest.approx(2.0 / 3.0, rel=1e-12)
+    assert vm[0] == pytest.approx(v0, rel=1e-9)
+    errs = [np.abs(v - a).max() for _, v, a, _ in (_eqs(s) for s in (50, 100, 200, 400))]
+    assert errs[0] == pytest.approx(1.216e-3, rel=0.05)   # docs/CONTRACTS.md:474-476
+    assert errs[-1] < 2e-4
+    for a, b in zip(errs[:-1], errs[1:]):
+        assert 1.8 < a / b < 2.2
+    out, *_ = _eqs(100)
+    tot = out["currents"][0] + out["currents"][1]
+    assert np.abs(tot).max() / np.abs(out["currents"][0]).max() < 1e-8
+    assert out["n_factorisations"] == 1
+
+
+def _bar(n=40, L=1e-3):
+    return RectilinearGrid(np.linspace(0, L, n + 1), np.linspace(0, 1e-3, 2), np.linspace(0, 1e-3, 2))
+
+
+def test_oracle_thermal_parabolic_and_order():
+    L, k, T0, qv = 1e-3, 150.0, 300.0, 1e9
+    g = _bar(40, L)
+    fixed = np.concatenate([g.node_ids()[0].ravel(), g.node_ids()[-1].ravel()])
+    T = O.thermal_steady(g, np.full(g.shape_cells, k), qv * g.node_volumes().ravel(),
+                         fixed, np.full(fixed.size, T0))
+    prof = T.reshape(g.shape_nodes)[:, 0, 0]
+    assert np.abs(prof - (T0 + qv * g.xn * (L - g.xn) / (2 * k))).max() < 1e-9
+
+
+def test_oracle_thermal_convective_resistance():
+    from fieldspice_amd.boundaries import wall_nodes, wall_dual_areas, WALLS
+    Lc, Wc, Hc, h = 100e-6, 20e-6, 20e-6, 2e4
+    g = RectilinearGrid(np.linspace(0, Lc, 21), np.linspace(0, Wc, 5), np.linspace(0, Hc, 5))
+    diag = np.zeros(g.n_nodes)
+    rhs = np.zeros(g.n_nodes)
+    for w in WALLS:
+        diag[wall_nodes(g, w)] += h * wall_dual_areas(g, w)
+        rhs[wall_nodes(g, w)] += h * wall_dual_areas(g, w) * 300.0
+    P = 1e-3
+    T = O.thermal_steady(g, np.full(g.shape_cells, 5e6), np.full(g.n_nodes, P / g.n_nodes),
+                         conv_diag=diag, conv_rhs=rhs)
+    A = 2 * (Lc * Wc + Lc * Hc + Wc * Hc)
+    assert (T.mean() - 300.0) / P == pytest.approx(1.0 / (h * A), rel=1e-5)
+
+
+def test_oracle_thermal_transient_matches_independent_rc_ladder():
+    # The 1D bar is an RC ladder: R = dx/(kA) between nodes, C = rho cp A dx
+    # (half at the ends).  Build it directly and integrate with BE.
+    Lb, k, rcp, N, T0, P = 2e-4, 150.0, 2329.0 * 700.0, 20, 300.0, 0.05
+    W = H = 1e-5
+    A = W * H
+    g = RectilinearGrid(np.linspace(0, Lb, N + 1), np.linspace(0, W, 2), np.linspace(0, H, 2))
+    nid = g.node_ids()
+    src = nid[0].ravel()
+    sink = nid[-1].ravel()
+    tau = (Lb / (k * A)) * (rcp * A * Lb)
+    dt = tau / 200
+    q0 = np.zeros(g.n_nodes)
+    q0[src] = P / src.size
+    times, Ts = O.thermal_transient(g, np.full(g.shape_cells, k), np.full(g.shape_cells, rcp),
+                                    lambda t: q0, 0.5 * tau, dt, T0,
+                                    fixed_fn=lambda t: (sink, np.full(sink.size, T0)))
+    dx = Lb / N
+    Gm = np.zeros((N + 1, N + 1))
+    for i in range(N):
+        gc = k * A / dx
+        Gm[i, i] += gc; Gm[i + 1, i + 1] += gc; Gm[i, i + 1] -= gc; Gm[i + 1, i] -= gc
+    Cv = rcp * A * dx * np.array([0.5] + [1.0] * (N - 1) + [0.5])
+    v = np.zeros(N + 1)
+    Af = np.diag(Cv / dt) + Gm
+    Af[N, :] = 0
+    Af[N, N] = 1
+    out = []
+    for _ in times[1:]:
+        b = Cv / dt * v
+        b[0] += P
+        b[N] = 0.0
+        v = np.linalg.solve(Af, b)
+        out.append(v[0])
+    Tf = np.array([T.reshape(g.shape_nodes)[0, 0, 0] for T in Ts[1:]]) - T0
+    assert np.abs(Tf - np.array(out)).max() < 1e-9 * max(1.0, np.abs(out).max())
+
+
+def test_reference_ilu_branch_is_not_a_usable_oracle():
+    """The reference's ILU-preconditioned CG (poisson.py:472-476 + cg at :552)
+    uses a non-symmetric incomplete LU inside CG.  On the c1 plate it runs to
+    maxiter and leaves C[0,0] off by ~0.6%, while the residual it reports
+    (normalised by ||b|| that includes the 1 V Dirichlet rows) is ~1e-20.
+    The parity oracle is therefore the direct solve / Jacobi branch."""
+    g = RectilinearGrid.uniform([(0, 2e-6), (0, 5e-6), (0, 4e-6)], [24, 10, 8])
+    nid = g.node_ids()
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        C, rep = O.capacitance_matrix(g, np.full(g.shape_cells, 3.9 * eps0),
+                                      [nid[0].ravel(), nid[-1].ravel()], method="cg")
+    assert rep["linear"]["iterations"] == 500
+    assert abs(C[0, 0] / 3.4531332486813394e-16 - 1) > 1e-4
+    assert rep["linear"]["residual"] < 1e-15
