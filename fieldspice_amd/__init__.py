@@ -1,0 +1,20 @@
+"""fieldspice_amd -- fieldspice's elliptic field solve on AMD MI355X (gfx950).
+
+The public names match fieldspice's (``RectilinearGrid``, ``MaterialMap``,
+``Terminal``, ``SolverConfig``, ``extraction.capacitance_matrix``, the Poisson,
+EQS and thermal solvers); every solve runs on the GPU through the C ABI of
+``libfieldspice_hip.so`` (include/fieldspice_hip.h).  There is no CPU solver.
+"""
+
+from . import boundaries, extraction, units
+from .boundaries import BoundarySpec, Dirichlet, Neumann
+from .grid import RectilinearGrid, graded_1d
+from .materials import MaterialMap
+from .solvers.base import ConvergenceError, Result, SolverConfig, Terminal
+from .solvers.poisson import PoissonSolver
+
+__version__ = "0.1.0"
+
+__all__ = ["RectilinearGrid", "graded_1d", "MaterialMap", "BoundarySpec", "Dirichlet",
+           "Neumann", "Terminal", "SolverConfig", "Result", "ConvergenceError",
+           "PoissonSolver", "extraction", "units", "boundaries"]

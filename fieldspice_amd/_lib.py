@@ -1,0 +1,348 @@
+"""ctypes binding of ``libfieldspice_hip.so`` (C ABI in include/fieldspice_hip.h).
+
+The library is built in-tree by ``fieldspice_amd.build``.  There is no CPU
+fallback: if the library or a GPU is missing, every solve raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfieldspice_hip.so")
+
+FS_OK = 0
+FS_NOT_CONVERGED = 1
+FS_EINVAL = -1
+FS_EHIP = -2
+FS_ENOMEM = -3
+
+FS_AVG = {"parallel": 0, "arithmetic": 0, "harmonic": 1, "series": 1, "min": 2, "max": 3}
+FS_PREC_FP64 = 0
+FS_PREC_MIXED = 1
+
+EXPORTS = (
+    "fs_ctx_create", "fs_ctx_destroy", "fs_last_error", "fs_ctx_device_info",
+    "fs_op_create_edges", "fs_op_create_cells", "fs_op_combine", "fs_op_destroy",
+    "fs_op_apply", "fs_op_get_edges", "fs_sys_create", "fs_sys_destroy",
+    "fs_sys_solve", "fs_capacitance", "fs_cap_prepare", "fs_cap_solve", "fs_cap_destroy",
+    "fs_sys_set_profiling",
+)
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("tol", C.c_double), ("maxiter", C.c_int), ("precision", C.c_int),
+                ("inner_tol", C.c_double), ("check_every", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("refreshes", C.c_int), ("converged", C.c_int),
+                ("residual", C.c_double), ("residual_eq", C.c_double),
+                ("solve_time", C.c_double), ("stencil_time", C.c_double),
+                ("stencil_launches", C.c_int64), ("stencil_bytes", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SysInfo(C.Structure):
+    _fields_ = [("n_nodes", C.c_int64), ("n_free", C.c_int64), ("n_fixed", C.c_int64),
+                ("n_empty", C.c_int64), ("gauge", C.c_int64), ("singular", C.c_int)]
+
+
+class LibraryError(RuntimeError):
+    """A failure reported by the HIP library (or the library is unavailable)."""
+
+
+_lock = threading.Lock()
+_lib = None
+_ctx = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+def _declare(lib):
+    lib.fs_last_error.restype = C.c_char_p
+    lib.fs_last_error.argtypes = []
+    lib.fs_ctx_create.argtypes = [C.c_int, C.POINTER(_vp)]
+    lib.fs_ctx_destroy.argtypes = [_vp]
+    lib.fs_ctx_device_info.argtypes = [_vp, C.c_char_p, C.c_int, C.POINTER(C.c_int)]
+    lib.fs_op_create_edges.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
+                                       _vp, C.POINTER(_vp)]
+    lib.fs_op_create_cells.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
+                                       C.c_int, C.POINTER(_vp)]
+    lib.fs_op_combine.argtypes = [C.c_double, _vp, C.c_double, _vp, C.c_double, _vp,
+                                  C.POINTER(_vp)]
+    lib.fs_op_destroy.argtypes = [_vp]
+    lib.fs_op_apply.argtypes = [_vp, _dp, _dp]
+    lib.fs_op_get_edges.argtypes = [_vp, _dp, _dp, _dp]
+    lib.fs_sys_create.argtypes = [_vp, _vp, C.c_int, C.POINTER(_vp), C.POINTER(SysInfo)]
+    lib.fs_sys_destroy.argtypes = [_vp]
+    lib.fs_sys_solve.argtypes = [_vp, C.c_int, _dp, _vp, _dp, C.POINTER(SolveParams),
+                                 C.POINTER(SolveStats)]
+    lib.fs_capacitance.argtypes = [_vp, _vp, C.c_int, _i32p, _dp, _vp, _vp,
+                                   C.POINTER(SolveParams), C.POINTER(SolveStats)]
+    lib.fs_cap_prepare.argtypes = [_vp, C.c_int, _i32p, C.POINTER(_vp)]
+    lib.fs_cap_solve.argtypes = [_vp, _vp, _dp, _vp, _vp, C.POINTER(SolveParams),
+                                 C.POINTER(SolveStats)]
+    lib.fs_cap_destroy.argtypes = [_vp]
+    lib.fs_sys_set_profiling.argtypes = [_vp, C.c_int]
+    for name in EXPORTS:
+        if name != "fs_last_error":
+            getattr(lib, name).restype = C.c_int
+
+
+def load():
+    """Load the library (no GPU needed to load it)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise LibraryError(
+                    f"{LIB_PATH} is missing: build it with `python -m fieldspice_amd.build` "
+                    "(there is no CPU fallback)")
+            lib = C.CDLL(LIB_PATH)
+            _declare(lib)
+            _lib = lib
+        return _lib
+
+
+def last_error() -> str:
+    msg = load().fs_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, what: str) -> int:
+    if rc < 0:
+        raise LibraryError(f"{what}: {last_error()} (status {rc})")
+    return rc
+
+
+def context():
+    """Process-wide context on device 0 (created on first use)."""
+    global _ctx
+    lib = load()
+    with _lock:
+        if _ctx is None:
+            h = _vp()
+            rc = lib.fs_ctx_create(int(os.environ.get("FIELDSPICE_DEVICE", "0")), C.byref(h))
+            if rc != FS_OK:
+                raise LibraryError(f"fs_ctx_create: {last_error()} (status {rc})")
+            _ctx = h
+        return _ctx
+
+
+def device_info() -> tuple[str, int]:
+    lib = load()
+    buf = C.create_string_buffer(256)
+    ncu = C.c_int(0)
+    check(lib.fs_ctx_device_info(context(), buf, 256, C.byref(ncu)), "fs_ctx_device_info")
+    return buf.value.decode(), ncu.value
+
+
+def as_f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class DeviceOperator:
+    """``A = G^T diag(w) G + diag(extra)`` resident on the GPU (an ``fs_op``)."""
+
+    def __init__(self, handle, shape_cells):
+        self._h = handle
+        self.shape_cells = tuple(shape_cells)
+
+    @classmethod
+    def from_cells(cls, grid, prop_cell, mode: str = "parallel") -> "DeviceOperator":
+        lib = load()
+        if mode not in FS_AVG:
+            raise ValueError(f"unknown averaging mode {mode!r}")
+        Nx, Ny, Nz = grid.shape_cells
+        h = _vp()
+        check(lib.fs_op_create_cells(context(), Nx, Ny, Nz, as_f64(grid.hx), as_f64(grid.hy),
+                                     as_f64(grid.hz), as_f64(prop_cell), FS_AVG[mode],
+                                     C.byref(h)), "fs_op_create_cells")
+        return cls(h, grid.shape_cells)
+
+    @classmethod
+    def from_edges(cls, grid, w_edges, extra=None) -> "DeviceOperator":
+        """``w_edges``: concatenated [x, y, z] edge weights (fieldspice order)."""
+        lib = load()
+        w = as_f64(w_edges).ravel()
+        nex, ney, nez = grid.n_edges_each
+        if w.size != nex + ney + nez:
+            raise ValueError(f"expected {nex + ney + nez} edge weights, got {w.size}")
+        wx = np.ascontiguousarray(w[:nex])
+        wy = np.ascontiguousarray(w[nex:nex + ney])
+        wz = np.ascontiguousarray(w[nex + ney:])
+        ex = None if extra is None else as_f64(extra).ravel()
+        if ex is not None and ex.size != grid.n_nodes:
+            raise ValueError("extra must have one value per node")
+        Nx, Ny, Nz = grid.shape_cells
+        h = _vp()
+        check(lib.fs_op_create_edges(context(), Nx, Ny, Nz, wx, wy, wz,
+                                     None if ex is None else ex.ctypes.data_as(_vp),
+                                     C.byref(h)), "fs_op_create_edges")
+        return cls(h, grid.shape_cells)
+
+    def combine(self, a: float, b: float = 0.0, other: "DeviceOperator | None" = None,
+                c: float = 0.0, extra=None) -> "DeviceOperator":
+        """``a*self + b*other + diag(c*extra)``."""
+        lib = load()
+        ex = None if extra is None else as_f64(extra).ravel()
+        h = _vp()
+        check(lib.fs_op_combine(float(a), self._h, float(b), None if other is None else other._h,
+                                float(c), None if ex is None else ex.ctypes.data_as(_vp),
+                                C.byref(h)), "fs_op_combine")
+        return DeviceOperator(h, self.shape_cells)
+
+    @property
+    def n_nodes(self) -> int:
+        Nx, Ny, Nz = self.shape_cells
+        return (Nx + 1) * (Ny + 1) * (Nz + 1)
+
+    def apply(self, x) -> np.ndarray:
+        x = as_f64(x).ravel()
+        if x.size != self.n_nodes:
+            raise ValueError(f"vector must have {self.n_nodes} entries")
+        y = np.empty_like(x)
+        check(load().fs_op_apply(self._h, x, y), "fs_op_apply")
+        return y
+
+    def edges(self) -> np.ndarray:
+        Nx, Ny, Nz = self.shape_cells
+        wx = np.empty(Nx * (Ny + 1) * (Nz + 1))
+        wy = np.empty((Nx + 1) * Ny * (Nz + 1))
+        wz = np.empty((Nx + 1) * (Ny + 1) * Nz)
+        check(load().fs_op_get_edges(self._h, wx, wy, wz), "fs_op_get_edges")
+        return np.concatenate([wx, wy, wz])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().fs_op_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceSystem:
+    """A prepared solve of ``op`` with a Dirichlet mask (an ``fs_sys``)."""
+
+    def __init__(self, op: DeviceOperator, mask=None, pin: bool = True):
+        lib = load()
+        self.op = op  # keeps the operator alive
+        m = None
+        if mask is not None:
+            m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+            if m.size != op.n_nodes:
+                raise ValueError("mask must have one entry per node")
+        h = _vp()
+        info = SysInfo()
+        rc = lib.fs_sys_create(op._h, None if m is None else m.ctypes.data_as(_vp),
+                               1 if pin else 0, C.byref(h), C.byref(info))
+        if rc == FS_EINVAL and "singular" in last_error():
+            raise ValueError(last_error())
+        check(rc, "fs_sys_create")
+        self._h = h
+        self.info = {k: getattr(info, k) for k, _ in info._fields_}
+
+    @staticmethod
+    def params(tol, maxiter=None, precision=FS_PREC_FP64, inner_tol=1e-4,
+               check_every=0) -> SolveParams:
+        return SolveParams(float(tol), int(maxiter or 0), int(precision), float(inner_tol),
+                           int(check_every))
+
+    def solve(self, b, params: SolveParams):
+        """Solve for one (n,) or several (k, n) right-hand sides."""
+        b = as_f64(b)
+        single = b.ndim == 1
+        b2 = b.reshape(1, -1) if single else b
+        n = self.op.n_nodes
+        if b2.shape[1] != n:
+            raise ValueError(f"right-hand side must have shape ({n},), got {b.shape}")
+        x = np.empty_like(b2)
+        st = SolveStats()
+        rc = load().fs_sys_solve(self._h, b2.shape[0], np.ascontiguousarray(b2), None, x,
+                                 C.byref(params), C.byref(st))
+        check(rc, "fs_sys_solve")
+        return (x[0] if single else x), st.as_dict(), rc
+
+    def capacitance(self, L: DeviceOperator, term_id, nt: int, params: SolveParams,
+                    want_phi: bool = False):
+        tid = np.ascontiguousarray(term_id, dtype=np.int32).ravel()
+        Cm = np.zeros((nt, nt))
+        qsh = np.zeros(nt)
+        phi = np.empty((nt, self.op.n_nodes)) if want_phi else None
+        st = SolveStats()
+        rc = load().fs_capacitance(self._h, L._h, nt, tid, Cm, qsh.ctypes.data_as(_vp),
+                                   None if phi is None else phi.ctypes.data_as(_vp),
+                                   C.byref(params), C.byref(st))
+        check(rc, "fs_capacitance")
+        return Cm, qsh, phi, st.as_dict(), rc
+
+    def prepare_capacitance(self, term_id, nt: int) -> "DeviceCapacitance":
+        return DeviceCapacitance(self, term_id, nt)
+
+    def set_profiling(self, on: bool) -> None:
+        check(load().fs_sys_set_profiling(self._h, 1 if on else 0), "fs_sys_set_profiling")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().fs_sys_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceCapacitance:
+    """Terminal map and node lists of one extraction resident on the GPU
+    (an ``fs_cap``), solved repeatedly with ``solve``."""
+
+    def __init__(self, system: DeviceSystem, term_id, nt: int):
+        self.system = system
+        self.nt = int(nt)
+        tid = np.ascontiguousarray(term_id, dtype=np.int32).ravel()
+        if tid.size != system.op.n_nodes:
+            raise ValueError("term_id must have one entry per node")
+        h = _vp()
+        check(load().fs_cap_prepare(system._h, self.nt, tid, C.byref(h)), "fs_cap_prepare")
+        self._h = h
+
+    def solve(self, L: DeviceOperator, params: SolveParams, want_phi: bool = False):
+        nt = self.nt
+        Cm = np.zeros((nt, nt))
+        qsh = np.zeros(nt)
+        phi = np.empty((nt, self.system.op.n_nodes)) if want_phi else None
+        st = SolveStats()
+        rc = load().fs_cap_solve(self._h, L._h, Cm, qsh.ctypes.data_as(_vp),
+                                 None if phi is None else phi.ctypes.data_as(_vp),
+                                 C.byref(params), C.byref(st))
+        check(rc, "fs_cap_solve")
+        if want_phi:
+            return Cm, qsh, phi, st.as_dict(), rc
+        return Cm, qsh, st.as_dict(), rc
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().fs_cap_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

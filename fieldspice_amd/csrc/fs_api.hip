@@ -1,0 +1,891 @@
+// C ABI of the fieldspice MI355X field solve (declared in include/fieldspice_hip.h).
+//
+// Every entry point validates its arguments, converts between fieldspice's
+// z-fastest host layout and the x-fastest device layout, and reports failures
+// through a status code plus a thread-local message.  No exception escapes.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "fieldspice_hip.h"
+#include "fs_internal.hpp"
+#include "fs_launch.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define FS_HIP(expr)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(FS_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),    \
+                  __FILE__, __LINE__);                                               \
+  } while (0)
+
+#define FS_GUARD_BEGIN try {
+#define FS_GUARD_END                                                      \
+  }                                                                       \
+  catch (const std::bad_alloc&) {                                         \
+    return fail(FS_ENOMEM, "host allocation failed");                     \
+  }                                                                       \
+  catch (...) {                                                           \
+    return fail(FS_EINVAL, "unexpected internal exception");              \
+  }
+
+template <typename T>
+int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(FS_ENOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
+                hipGetErrorString(e));
+  }
+  return FS_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+fs::Dims make_dims(int Nx, int Ny, int Nz) {
+  fs::Dims d;
+  d.nx = Nx + 1;
+  d.ny = Ny + 1;
+  d.nz = Nz + 1;
+  d.sxy = (int64_t)d.nx * d.ny;
+  d.n = d.sxy * d.nz;
+  return d;
+}
+
+int check_dims(int Nx, int Ny, int Nz) {
+  if (Nx < 1 || Ny < 1 || Nz < 1)
+    return fail(FS_EINVAL, "cell counts must be >= 1, got (%d, %d, %d)", Nx, Ny, Nz);
+  const int64_t n = (int64_t)(Nx + 1) * (Ny + 1) * (Nz + 1);
+  if (n > ((int64_t)1 << 31) - 1)
+    return fail(FS_EINVAL, "grid of %lld nodes exceeds the 2^31 node limit", (long long)n);
+  return FS_OK;
+}
+
+// Upload a host array laid out C-order (A, B, C) into an x-fast device array
+// with leading dims (OA, OB) (entries outside (A, B, C) are zero).
+template <typename T>
+int upload_permuted(fs_ctx* ctx, const T* host, int A, int B, int C, int OA, int OB, int OC,
+                    T* dev) {
+  T* tmp = nullptr;
+  const size_t cnt = (size_t)A * B * C;
+  if (int rc = dalloc(&tmp, cnt)) return rc;
+  hipError_t e = hipMemcpyAsync(tmp, host, cnt * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemsetAsync(dev, 0, (size_t)OA * OB * OC * sizeof(T), ctx->stream);
+  if (e == hipSuccess) {
+    fs::permute_in<T>(ctx->stream, tmp, dev, A, B, C, OA, OB);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  dfree(tmp);
+  if (e != hipSuccess) return fail(FS_EHIP, "upload failed: %s", hipGetErrorString(e));
+  return FS_OK;
+}
+
+// Download an x-fast device node-shaped array to a host C-order array.
+int download_nodes(fs_ctx* ctx, const fs::Dims& d, const double* dev, double* host,
+                   double* tmp /* d.n doubles */) {
+  fs::permute_out<double>(ctx->stream, dev, tmp, d.nx, d.ny, d.nz, d.nx, d.ny);
+  FS_HIP(hipGetLastError());
+  FS_HIP(hipMemcpyAsync(host, tmp, (size_t)d.n * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  FS_HIP(hipStreamSynchronize(ctx->stream));
+  return FS_OK;
+}
+
+fs_op* new_op(fs_ctx* ctx, int Nx, int Ny, int Nz) {
+  fs_op* op = new fs_op();
+  op->ctx = ctx;
+  op->Nx = Nx;
+  op->Ny = Ny;
+  op->Nz = Nz;
+  op->d = make_dims(Nx, Ny, Nz);
+  return op;
+}
+
+void free_op(fs_op* op) {
+  if (!op) return;
+  for (auto& w : op->w) dfree(w);
+  dfree(op->extra);
+  delete op;
+}
+
+void free_sys_ws(fs_sys* s) {
+  dfree(s->bs);
+  dfree(s->x);
+  dfree(s->r);
+  dfree(s->p);
+  dfree(s->q);
+  dfree(s->x32);
+  dfree(s->r32);
+  dfree(s->p32);
+  dfree(s->q32);
+  dfree(s->node_tmp);
+  s->ws_nr = 0;
+}
+
+void free_sys(fs_sys* s) {
+  if (!s) return;
+  free_sys_ws(s);
+  dfree(s->mask);
+  dfree(s->dhalf);
+  dfree(s->dinv);
+  for (auto& c : s->c64) dfree(c);
+  for (auto& c : s->c32) dfree(c);
+  dfree(s->partials);
+  dfree(s->counters);
+  dfree(s->sc);
+  dfree(s->prof_dev);
+  if (s->sc_host) (void)hipHostFree(s->sc_host);
+  for (int k = 0; k < s->ev_cap; ++k) {
+    (void)hipEventDestroy(s->ev_a[k]);
+    (void)hipEventDestroy(s->ev_b[k]);
+  }
+  delete[] s->ev_a;
+  delete[] s->ev_b;
+  delete s;
+}
+
+constexpr int kMaxChunk = 512;
+
+void prof_begin(fs_sys* s) {
+  s->prof_time = 0.0;
+  s->prof_launches = 0;
+  if (s->prof_dev) (void)hipMemsetAsync(s->prof_dev, 0, 2 * sizeof(unsigned long long), s->ctx->stream);
+}
+
+void prof_end(fs_sys* s, fs_solve_stats* st, bool mixed) {
+  if (!st || !s->profiling || !s->prof_dev) return;
+  unsigned long long h[2] = {0, 0};
+  if (hipMemcpy(h, s->prof_dev, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) return;
+  const double elt = mixed ? 4.0 : 8.0;
+  st->stencil_time += s->prof_time;
+  st->stencil_launches += s->prof_launches;
+  st->stencil_bytes += (double)s->d.n * (3.0 * elt * (double)h[0] + 2.0 * elt * (double)h[1]);
+}
+
+int ensure_ws(fs_sys* s, int nr) {
+  if (s->ws_nr >= nr) return FS_OK;
+  free_sys_ws(s);
+  const size_t cnt = (size_t)nr * s->vstride;
+  int rc;
+  if ((rc = dalloc(&s->bs, cnt)) || (rc = dalloc(&s->x, cnt)) || (rc = dalloc(&s->r, cnt)) ||
+      (rc = dalloc(&s->p, cnt)) || (rc = dalloc(&s->q, cnt)) || (rc = dalloc(&s->x32, cnt)) ||
+      (rc = dalloc(&s->r32, cnt)) || (rc = dalloc(&s->p32, cnt)) ||
+      (rc = dalloc(&s->q32, cnt)) || (rc = dalloc(&s->node_tmp, cnt))) {
+    free_sys_ws(s);
+    return rc;
+  }
+  s->ws_nr = nr;
+  return FS_OK;
+}
+
+int default_maxiter(const fs_sys* s, const fs_solve_params* P) {
+  if (P && P->maxiter > 0) return P->maxiter;
+  const double nf = (double)std::max<int64_t>(s->info.n_free, 1);
+  return std::max(500, (int)std::floor(10.0 * std::sqrt(nf)));
+}
+
+// Krylov loop after init: FP64 CG (mode 0) or FP32 cycles with FP64
+// refreshes.  Host checks the device flags between chunks of iterations.
+int run_krylov(fs_sys* s, int nr, const fs_solve_params& P) {
+  hipStream_t st = s->ctx->stream;
+  const bool mixed = P.precision == FS_PREC_MIXED;
+  const double inner_tol = P.inner_tol > 0 ? P.inner_tol : 1e-4;
+  const size_t scb = sizeof(fs::CgScalars) * (size_t)nr;
+  auto fetch = [&]() -> int {
+    FS_HIP(hipMemcpyAsync(s->sc_host, s->sc, scb, hipMemcpyDeviceToHost, st));
+    FS_HIP(hipStreamSynchronize(st));
+    return FS_OK;
+  };
+  if (int rc = fetch()) return rc;
+  auto all = [&](auto pred) {
+    for (int r = 0; r < nr; ++r)
+      if (!pred(s->sc_host[r])) return false;
+    return true;
+  };
+  if (all([](const fs::CgScalars& c) { return c.done != 0; })) return FS_OK;
+  if (mixed) fs::launch_cycle_start(s, nr, inner_tol * inner_tol);
+  int chunk = P.check_every > 0 ? P.check_every : 16;
+  const int maxiter = s->sc_host[0].maxiter;
+  std::vector<double> last_rho(nr, 0.0);
+  std::vector<int> last_it(nr, 0);
+  for (int guard = 0; guard < 4 * maxiter + 64; ++guard) {
+    const bool prof = s->profiling && chunk <= s->ev_cap;
+    for (int c = 0; c < chunk; ++c)
+      fs::launch_iteration(s, nr, mixed ? 1 : 0, prof ? s->ev_a[c] : nullptr,
+                           prof ? s->ev_b[c] : nullptr);
+    FS_HIP(hipGetLastError());
+    if (int rc = fetch()) return rc;
+    if (prof) {
+      for (int c = 0; c < chunk; ++c) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, s->ev_a[c], s->ev_b[c]) == hipSuccess)
+          s->prof_time += 1e-3 * ms;
+      }
+      s->prof_launches += chunk;
+    }
+    if (!mixed) {
+      if (all([](const fs::CgScalars& c) { return c.done != 0; })) return FS_OK;
+    } else {
+      if (all([](const fs::CgScalars& c) { return c.done != 0 || c.cycle_done != 0; })) {
+        fs::launch_refresh(s, nr);
+        FS_HIP(hipGetLastError());
+        if (int rc = fetch()) return rc;
+        if (all([](const fs::CgScalars& c) { return c.done != 0; })) return FS_OK;
+        fs::launch_cycle_start(s, nr, inner_tol * inner_tol);
+        FS_HIP(hipGetLastError());
+        chunk = P.check_every > 0 ? P.check_every : 16;
+        continue;
+      }
+    }
+    if (P.check_every > 0) continue;
+    // Adaptive chunk: extrapolate the residual decay of the slowest live RHS
+    // to its threshold, aiming slightly past it (idle launches exit at once).
+    int want = 8;
+    for (int r = 0; r < nr; ++r) {
+      const fs::CgScalars& c = s->sc_host[r];
+      if (c.done != 0 || (mixed && c.cycle_done)) continue;
+      const double rho = mixed ? c.aux0 : c.rho;
+      const double target = mixed ? c.aux1 : c.thresh2;
+      const int it = mixed ? c.cycle_iter : c.iter;
+      if (last_it[r] > 0 && it > last_it[r] && rho > 0 && last_rho[r] > rho) {
+        const double rate = std::log(rho / last_rho[r]) / (double)(it - last_it[r]);
+        const double remain = std::log(target / rho) / rate;
+        if (std::isfinite(remain)) want = std::max(want, (int)std::ceil(remain) + 2);
+      } else {
+        want = std::max(want, chunk * 2);
+      }
+      last_rho[r] = rho;
+      last_it[r] = it;
+    }
+    chunk = std::min(std::max(want, 4), kMaxChunk);
+  }
+  return fail(FS_EINVAL, "Krylov driver exceeded its loop guard");
+}
+
+void collect_stats(fs_sys* s, int nr, fs_solve_stats* st, int& status) {
+  for (int r = 0; r < nr; ++r) {
+    const fs::CgScalars& c = s->sc_host[r];
+    if (st) {
+      st->iterations = std::max(st->iterations, c.iter);
+      st->refreshes = std::max(st->refreshes, c.refreshes);
+      double res = c.aux2 > 0 ? std::sqrt(c.aux1 / c.aux2) : 0.0;
+      double req = c.bnorm2 > 0 ? std::sqrt(c.aux3 / c.bnorm2) : 0.0;
+      // a non-finite residual (NaN input, breakdown) must not read as converged
+      if (!(res >= 0.0) || !std::isfinite(c.aux2)) res = INFINITY;
+      if (!(req >= 0.0) || !std::isfinite(c.bnorm2)) req = INFINITY;
+      st->residual = std::max(st->residual, res);
+      st->residual_eq = std::max(st->residual_eq, req);
+    }
+    if (c.done != 1) status = FS_NOT_CONVERGED;
+  }
+}
+
+}  // namespace
+
+// ==========================================================================
+extern "C" {
+
+const char* fs_last_error(void) { return g_err.c_str(); }
+
+int fs_ctx_create(int device, fs_ctx** out) {
+  FS_GUARD_BEGIN
+  if (!out) return fail(FS_EINVAL, "out is NULL");
+  *out = nullptr;
+  int n = 0;
+  FS_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n)
+    return fail(FS_EINVAL, "device %d out of range (found %d)", device, n);
+  FS_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  FS_HIP(hipGetDeviceProperties(&prop, device));
+  fs_ctx* ctx = new fs_ctx();
+  ctx->device = device;
+  ctx->n_cu = prop.multiProcessorCount;
+  snprintf(ctx->name, sizeof(ctx->name), "%s", prop.gcnArchName);
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    delete ctx;
+    return fail(FS_EHIP, "stream/event creation failed");
+  }
+  *out = ctx;
+  return FS_OK;
+  FS_GUARD_END
+}
+
+int fs_ctx_destroy(fs_ctx* ctx) {
+  if (!ctx) return FS_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipEventDestroy(ctx->ev0);
+  (void)hipEventDestroy(ctx->ev1);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return FS_OK;
+}
+
+int fs_ctx_device_info(fs_ctx* ctx, char* name, int name_len, int* n_cu) {
+  if (!ctx) return fail(FS_EINVAL, "ctx is NULL");
+  if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", ctx->name);
+  if (n_cu) *n_cu = ctx->n_cu;
+  return FS_OK;
+}
+
+int fs_op_create_edges(fs_ctx* ctx, int Nx, int Ny, int Nz, const double* w_x,
+                       const double* w_y, const double* w_z, const double* extra,
+                       fs_op** out) {
+  FS_GUARD_BEGIN
+  if (!ctx || !out || !w_x || !w_y || !w_z) return fail(FS_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (int rc = check_dims(Nx, Ny, Nz)) return rc;
+  FS_HIP(hipSetDevice(ctx->device));
+  fs_op* op = new_op(ctx, Nx, Ny, Nz);
+  const fs::Dims& d = op->d;
+  int rc = FS_OK;
+  for (int a = 0; a < 3 && rc == FS_OK; ++a) rc = dalloc(&op->w[a], (size_t)d.n);
+  if (rc == FS_OK) rc = upload_permuted(ctx, w_x, Nx, d.ny, d.nz, d.nx, d.ny, d.nz, op->w[0]);
+  if (rc == FS_OK) rc = upload_permuted(ctx, w_y, d.nx, Ny, d.nz, d.nx, d.ny, d.nz, op->w[1]);
+  if (rc == FS_OK) rc = upload_permuted(ctx, w_z, d.nx, d.ny, Nz, d.nx, d.ny, d.nz, op->w[2]);
+  if (rc == FS_OK && extra) {
+    rc = dalloc(&op->extra, (size_t)d.n);
+    if (rc == FS_OK)
+      rc = upload_permuted(ctx, extra, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz, op->extra);
+  }
+  if (rc != FS_OK) {
+    free_op(op);
+    return rc;
+  }
+  *out = op;
+  return FS_OK;
+  FS_GUARD_END
+}
+
+int fs_op_create_cells(fs_ctx* ctx, int Nx, int Ny, int Nz, const double* hx,
+                       const double* hy, const double* hz, const double* prop_cell,
+                       int avg_mode, fs_op** out) {
+  FS_GUARD_BEGIN
+  if (!ctx || !out || !hx || !hy || !hz || !prop_cell) return fail(FS_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (int rc = check_dims(Nx, Ny, Nz)) return rc;
+  if (avg_mode < FS_AVG_PARALLEL || avg_mode > FS_AVG_MAX)
+    return fail(FS_EINVAL, "unknown averaging mode %d", avg_mode);
+  const int N[3] = {Nx, Ny, Nz};
+  const double* hh[3] = {hx, hy, hz};
+  std::vector<double> host_h[3], host_hd[3];
+  for (int a = 0; a < 3; ++a) {
+    host_h[a].assign(hh[a], hh[a] + N[a]);
+    host_hd[a].resize((size_t)N[a] + 1);
+    for (int i = 0; i < N[a]; ++i)
+      if (!(host_h[a][i] > 0.0) || !std::isfinite(host_h[a][i]))
+        return fail(FS_EINVAL, "cell widths along axis %d must be finite and > 0", a);
+    // dual widths: half cells at the two ends (fieldspice/grid.py:234-238)
+    host_hd[a][0] = 0.5 * host_h[a][0];
+    host_hd[a][N[a]] = 0.5 * host_h[a][N[a] - 1];
+    for (int i = 1; i < N[a]; ++i) host_hd[a][i] = 0.5 * (host_h[a][i - 1] + host_h[a][i]);
+  }
+  FS_HIP(hipSetDevice(ctx->device));
+  fs_op* op = new_op(ctx, Nx, Ny, Nz);
+  const fs::Dims& d = op->d;
+  double *dh[3] = {nullptr, nullptr, nullptr}, *dhd[3] = {nullptr, nullptr, nullptr};
+  double* cells = nullptr;
+  int rc = FS_OK;
+  for (int a = 0; a < 3 && rc == FS_OK; ++a) {
+    rc = dalloc(&op->w[a], (size_t)d.n);
+    if (rc == FS_OK) rc = dalloc(&dh[a], (size_t)N[a]);
+    if (rc == FS_OK) rc = dalloc(&dhd[a], (size_t)N[a] + 1);
+    if (rc == FS_OK &&
+        (hipMemcpy(dh[a], host_h[a].data(), N[a] * sizeof(double), hipMemcpyHostToDevice) !=
+             hipSuccess ||
+         hipMemcpy(dhd[a], host_hd[a].data(), (N[a] + 1) * sizeof(double),
+                   hipMemcpyHostToDevice) != hipSuccess))
+      rc = fail(FS_EHIP, "metric upload failed");
+  }
+  if (rc == FS_OK) rc = dalloc(&cells, (size_t)Nx * Ny * Nz);
+  if (rc == FS_OK) rc = upload_permuted(ctx, prop_cell, Nx, Ny, Nz, Nx, Ny, Nz, cells);
+  if (rc == FS_OK) {
+    const double* ch[3] = {dh[0], dh[1], dh[2]};
+    const double* chd[3] = {dhd[0], dhd[1], dhd[2]};
+    for (int a = 0; a < 3; ++a) fs::cell_to_edge(ctx->stream, cells, ch, chd, N, d, a, avg_mode, op->w[a]);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
+      rc = fail(FS_EHIP, "cell_to_edge kernel failed");
+  }
+  for (int a = 0; a < 3; ++a) {
+    dfree(dh[a]);
+    dfree(dhd[a]);
+  }
+  dfree(cells);
+  if (rc != FS_OK) {
+    free_op(op);
+    return rc;
+  }
+  *out = op;
+  return FS_OK;
+  FS_GUARD_END
+}
+
+int fs_op_combine(double a, const fs_op* A, double b, const fs_op* B, double c,
+                  const double* extra, fs_op** out) {
+  FS_GUARD_BEGIN
+  if (!A || !out) return fail(FS_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (B && (B->Nx != A->Nx || B->Ny != A->Ny || B->Nz != A->Nz || B->ctx != A->ctx))
+    return fail(FS_EINVAL, "operators differ in grid or context");
+  fs_ctx* ctx = A->ctx;
+  FS_HIP(hipSetDevice(ctx->device));
+  fs_op* op = new_op(ctx, A->Nx, A->Ny, A->Nz);
+  const int64_t n = op->d.n;
+  int rc = FS_OK;
+  for (int k = 0; k < 3 && rc == FS_OK; ++k) {
+    rc = dalloc(&op->w[k], (size_t)n);
+    if (rc == FS_OK) fs::axpby(ctx->stream, n, a, A->w[k], b, B ? B->w[k] : nullptr, op->w[k]);
+  }
+  const bool any_extra = A->extra || (B && B->extra) || extra;
+  if (rc == FS_OK && any_extra) {
+    rc = dalloc(&op->extra, (size_t)n);
+    if (rc == FS_OK) {
+      FS_HIP(hipMemsetAsync(op->extra, 0, n * sizeof(double), ctx->stream));
+      if (A->extra) fs::axpby(ctx->stream, n, a, A->extra, 1.0, op->extra, op->extra);
+      if (B && B->extra) fs::axpby(ctx->stream, n, b, B->extra, 1.0, op->extra, op->extra);
+      if (extra) {
+        double* tmp = nullptr;
+        rc = dalloc(&tmp, (size_t)n);
+        if (rc == FS_OK)
+          rc = upload_permuted(ctx, extra, op->d.nx, op->d.ny, op->d.nz, op->d.nx, op->d.ny,
+                               op->d.nz, tmp);
+        if (rc == FS_OK) fs::axpby(ctx->stream, n, c, tmp, 1.0, op->extra, op->extra);
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(FS_EHIP, "combine failed");
+        dfree(tmp);
+      }
+    }
+  }
+  if (rc == FS_OK && (hipGetLastError() != hipSuccess ||
+                      hipStreamSynchronize(ctx->stream) != hipSuccess))
+    rc = fail(FS_EHIP, "combine kernels failed");
+  if (rc != FS_OK) {
+    free_op(op);
+    return rc;
+  }
+  *out = op;
+  return FS_OK;
+  FS_GUARD_END
+}
+
+int fs_op_destroy(fs_op* op) {
+  if (op) (void)hipSetDevice(op->ctx->device);
+  free_op(op);
+  return FS_OK;
+}
+
+int fs_op_apply(fs_op* op, const double* x, double* y) {
+  FS_GUARD_BEGIN
+  if (!op || !x || !y) return fail(FS_EINVAL, "NULL argument");
+  fs_ctx* ctx = op->ctx;
+  FS_HIP(hipSetDevice(ctx->device));
+  const fs::Dims& d = op->d;
+  double *dx = nullptr, *dy = nullptr;
+  int rc = dalloc(&dx, (size_t)d.n);
+  if (rc == FS_OK) rc = dalloc(&dy, (size_t)d.n);
+  if (rc == FS_OK) rc = upload_permuted(ctx, x, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz, dx);
+  if (rc == FS_OK) {
+    fs::apply_raw(ctx->stream, op, dx, dy);
+    rc = download_nodes(ctx, d, dy, y, dx);
+  }
+  dfree(dx);
+  dfree(dy);
+  return rc;
+  FS_GUARD_END
+}
+
+int fs_op_get_edges(fs_op* op, double* w_x, double* w_y, double* w_z) {
+  FS_GUARD_BEGIN
+  if (!op || !w_x || !w_y || !w_z) return fail(FS_EINVAL, "NULL argument");
+  fs_ctx* ctx = op->ctx;
+  FS_HIP(hipSetDevice(ctx->device));
+  const fs::Dims& d = op->d;
+  double* tmp = nullptr;
+  if (int rc = dalloc(&tmp, (size_t)d.n)) return rc;
+  double* outs[3] = {w_x, w_y, w_z};
+  const int shp[3][3] = {{op->Nx, d.ny, d.nz}, {d.nx, op->Ny, d.nz}, {d.nx, d.ny, op->Nz}};
+  int rc = FS_OK;
+  for (int a = 0; a < 3 && rc == FS_OK; ++a) {
+    fs::permute_out<double>(ctx->stream, op->w[a], tmp, shp[a][0], shp[a][1], shp[a][2], d.nx,
+                            d.ny);
+    const size_t cnt = (size_t)shp[a][0] * shp[a][1] * shp[a][2];
+    if (hipMemcpyAsync(outs[a], tmp, cnt * sizeof(double), hipMemcpyDeviceToHost,
+                       ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+      rc = fail(FS_EHIP, "edge download failed");
+  }
+  dfree(tmp);
+  return rc;
+  FS_GUARD_END
+}
+
+int fs_sys_create(fs_op* op, const uint8_t* mask, int pin, fs_sys** out, fs_sys_info* info) {
+  FS_GUARD_BEGIN
+  if (!op || !out) return fail(FS_EINVAL, "NULL argument");
+  *out = nullptr;
+  fs_ctx* ctx = op->ctx;
+  FS_HIP(hipSetDevice(ctx->device));
+  fs_sys* s = new fs_sys();
+  s->ctx = ctx;
+  s->op = op;
+  s->d = op->d;
+  const fs::Dims& d = s->d;
+  s->vstride = (d.n + 63) / 64 * 64;
+  // stencil tiling: ~4K tiles, at least 4 planes per tile
+  fs::Tiling& tl = s->tl;
+  tl.tiles_x = (d.nx + fs::kTileX - 1) / fs::kTileX;
+  tl.tiles_y = (d.ny + fs::kTileY - 1) / fs::kTileY;
+  const int64_t txy = (int64_t)tl.tiles_x * tl.tiles_y;
+  tl.kz = (int)std::max<int64_t>(4, (d.nz * txy + 4095) / 4096);
+  tl.kz = std::min(tl.kz, d.nz);
+  tl.tiles_z = (d.nz + tl.kz - 1) / tl.kz;
+  tl.ntiles = (int)(txy * tl.tiles_z);
+  tl.nblocks = (tl.ntiles + 7) / 8 * 8;
+  s->vec_blocks = (int)std::min<int64_t>((d.n + fs::kBlock - 1) / fs::kBlock,
+                                         (int64_t)std::max(ctx->n_cu, 1) * 8);
+  s->vec_blocks = std::max(s->vec_blocks, 1);
+  const int64_t maxb = std::max<int64_t>(tl.nblocks, s->vec_blocks);
+  s->partials_cap = maxb * fs::kMaxRhs * 2 + 64 * (fs::kMaxRhs + 1) * fs::kMaxRhs;
+  int rc = FS_OK;
+  double* dsum = nullptr;
+  void* cnt_dev = nullptr;
+  void* cnt_host = nullptr;
+  if ((rc = dalloc(&s->mask, (size_t)d.n)) || (rc = dalloc(&s->dhalf, (size_t)d.n)) ||
+      (rc = dalloc(&s->dinv, (size_t)d.n)) || (rc = dalloc(&dsum, (size_t)d.n)) ||
+      (rc = dalloc(&s->partials, (size_t)s->partials_cap)) ||
+      (rc = dalloc(&s->counters, (size_t)fs::kNumCounters)) ||
+      (rc = dalloc(&s->sc, (size_t)fs::kMaxRhs)) ||
+      (rc = dalloc((char**)&cnt_dev, fs::kSetupCountsBytes))) {
+    dfree(dsum);
+    free_sys(s);
+    return rc;
+  }
+  for (int a = 0; a < 3 && rc == FS_OK; ++a) {
+    rc = dalloc(&s->c64[a], (size_t)d.n);
+    if (rc == FS_OK) rc = dalloc(&s->c32[a], (size_t)d.n);
+  }
+  if (rc == FS_OK && hipHostMalloc((void**)&s->sc_host, sizeof(fs::CgScalars) * fs::kMaxRhs) != hipSuccess)
+    rc = fail(FS_ENOMEM, "pinned allocation failed");
+  if (rc == FS_OK && hipHostMalloc(&cnt_host, fs::kSetupCountsBytes) != hipSuccess)
+    rc = fail(FS_ENOMEM, "pinned allocation failed");
+  if (rc == FS_OK && hipMemset(s->counters, 0, sizeof(unsigned) * fs::kNumCounters) != hipSuccess)
+    rc = fail(FS_EHIP, "counter init failed");
+  if (rc == FS_OK && hipMemset(s->sc, 0, sizeof(fs::CgScalars) * fs::kMaxRhs) != hipSuccess)
+    rc = fail(FS_EHIP, "scalar init failed");
+  if (rc == FS_OK) {
+    if (mask) {
+      rc = upload_permuted(ctx, mask, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz, s->mask);
+    } else if (hipMemset(s->mask, 0, (size_t)d.n) != hipSuccess) {
+      rc = fail(FS_EHIP, "mask init failed");
+    }
+  }
+  if (rc == FS_OK) {
+    const int st = fs::setup_system(s, dsum, cnt_dev, cnt_host, pin);
+    if (st == 1)
+      rc = fail(FS_EINVAL,
+                "system is singular: every row sums to zero, so there is no Dirichlet node "
+                "anywhere and the potential is only defined up to a constant.  Drive a "
+                "terminal, ground a wall, or pass pin=True.");
+    else if (st != 0)
+      rc = fail(FS_EHIP, "system setup failed: %s", hipGetErrorString(hipGetLastError()));
+  }
+  if (rc == FS_OK && (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess))
+    rc = fail(FS_EHIP, "system setup kernels failed");
+  dfree(dsum);
+  dfree(cnt_dev);
+  if (cnt_host) (void)hipHostFree(cnt_host);
+  if (rc != FS_OK) {
+    free_sys(s);
+    return rc;
+  }
+  if (info) *info = s->info;
+  *out = s;
+  return FS_OK;
+  FS_GUARD_END
+}
+
+int fs_sys_destroy(fs_sys* sys) {
+  if (sys) (void)hipSetDevice(sys->ctx->device);
+  free_sys(sys);
+  return FS_OK;
+}
+
+int fs_sys_solve(fs_sys* s, int nrhs, const double* b, const double* x0, double* x,
+                 const fs_solve_params* params, fs_solve_stats* stats) {
+  FS_GUARD_BEGIN
+  if (!s || !b || !x || !params) return fail(FS_EINVAL, "NULL argument");
+  if (nrhs < 1) return fail(FS_EINVAL, "nrhs must be >= 1");
+  if (x0) return fail(FS_EINVAL, "an initial guess is not supported by this build");
+  if (!(params->tol > 0.0)) return fail(FS_EINVAL, "tol must be > 0");
+  if (params->precision != FS_PREC_FP64 && params->precision != FS_PREC_MIXED)
+    return fail(FS_EINVAL, "unknown precision %d", params->precision);
+  fs_ctx* ctx = s->ctx;
+  FS_HIP(hipSetDevice(ctx->device));
+  const fs::Dims& d = s->d;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  const int maxiter = default_maxiter(s, params);
+  int status = FS_OK;
+  float ms_total = 0.0f;
+  if (stats) stats->converged = 1;
+  for (int r0 = 0; r0 < nrhs; r0 += fs::kMaxRhs) {
+    const int nr = std::min(fs::kMaxRhs, nrhs - r0);
+    if (int rc = ensure_ws(s, nr)) return rc;
+    // full right-hand sides (x-fast) into q
+    for (int r = 0; r < nr; ++r) {
+      const double* hb = b + (size_t)(r0 + r) * d.n;
+      if (int rc = upload_permuted(ctx, hb, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz,
+                                   s->q + (size_t)r * s->vstride))
+        return rc;
+    }
+    prof_begin(s);
+    FS_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    fs::launch_lift_general(s, nr, s->q);
+    // keep the full b for the Dirichlet values: move it to node_tmp
+    FS_HIP(hipMemcpyAsync(s->node_tmp, s->q, (size_t)nr * s->vstride * sizeof(double),
+                          hipMemcpyDeviceToDevice, ctx->stream));
+    fs::launch_init(s, nr, params->tol, maxiter);
+    FS_HIP(hipGetLastError());
+    if (int rc = run_krylov(s, nr, *params)) return rc;
+    fs::launch_residual(s, nr, s->r);
+    fs::launch_expand(s, nr, s->node_tmp, nullptr, 0, s->q);
+    FS_HIP(hipGetLastError());
+    FS_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    FS_HIP(hipMemcpyAsync(s->sc_host, s->sc, sizeof(fs::CgScalars) * nr, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    FS_HIP(hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    FS_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ms_total += ms;
+    int st = FS_OK;
+    collect_stats(s, nr, stats, st);
+    prof_end(s, stats, params->precision == FS_PREC_MIXED);
+    if (st != FS_OK) {
+      status = st;
+      if (stats) stats->converged = 0;
+    }
+    for (int r = 0; r < nr; ++r) {
+      if (int rc = download_nodes(ctx, d, s->q + (size_t)r * s->vstride,
+                                  x + (size_t)(r0 + r) * d.n, s->r))
+        return rc;
+    }
+  }
+  if (stats) stats->solve_time = ms_total * 1e-3;
+  if (status == FS_NOT_CONVERGED)
+    fail(FS_NOT_CONVERGED, "Krylov iteration did not reach tol %.3g within %d iterations",
+         params->tol, maxiter);
+  return status;
+  FS_GUARD_END
+}
+
+int fs_cap_prepare(fs_sys* s, int nt, const int32_t* term_id, fs_cap** out) {
+  FS_GUARD_BEGIN
+  if (!s || !term_id || !out) return fail(FS_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (nt < 1) return fail(FS_EINVAL, "capacitance extraction needs at least one terminal");
+  fs_ctx* ctx = s->ctx;
+  FS_HIP(hipSetDevice(ctx->device));
+  const fs::Dims& d = s->d;
+  // node lists per bin (terminal 0..nt-1, shield nt) in device order
+  const int nbins = nt + 1;
+  std::vector<int64_t> counts(nbins, 0);
+  for (int64_t ref = 0; ref < d.n; ++ref) {
+    const int t = term_id[ref];
+    if (t < -1 || t > nt)
+      return fail(FS_EINVAL, "term_id[%lld] = %d out of range", (long long)ref, t);
+    if (t >= 0) counts[t] += 1;
+  }
+  for (int t = 0; t < nt; ++t)
+    if (counts[t] == 0) return fail(FS_EINVAL, "terminal %d has no nodes", t);
+  std::vector<int64_t> offs(nbins + 1, 0);
+  for (int t = 0; t < nbins; ++t) offs[t + 1] = offs[t] + counts[t];
+  std::vector<int64_t> nodes((size_t)std::max<int64_t>(offs[nbins], 1));
+  std::vector<int64_t> fill(offs.begin(), offs.end() - 1);
+  for (int64_t i = 0; i < d.nx; ++i)
+    for (int64_t j = 0; j < d.ny; ++j)
+      for (int64_t k = 0; k < d.nz; ++k) {
+        const int t = term_id[(i * d.ny + j) * d.nz + k];
+        if (t >= 0) nodes[fill[t]++] = (k * d.ny + j) * d.nx + i;
+      }
+  fs_cap* cap = new fs_cap();
+  cap->sys = s;
+  cap->nt = nt;
+  cap->nbins = nbins;
+  cap->has_shield = counts[nt] > 0;
+  int64_t maxcnt = 1;
+  for (int t = 0; t < nbins; ++t) maxcnt = std::max(maxcnt, counts[t]);
+  cap->nbx = (int)std::min<int64_t>((maxcnt + fs::kBlock - 1) / fs::kBlock, 64);
+  int rc = FS_OK;
+  if ((rc = dalloc(&cap->tid, (size_t)d.n)) || (rc = dalloc(&cap->nodes, nodes.size())) ||
+      (rc = dalloc(&cap->offs, offs.size())) ||
+      (rc = dalloc(&cap->out, (size_t)nbins * fs::kMaxRhs))) {
+    fs_cap_destroy(cap);
+    return rc;
+  }
+  rc = upload_permuted(ctx, term_id, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz, cap->tid);
+  if (rc == FS_OK &&
+      (hipMemcpy(cap->nodes, nodes.data(), nodes.size() * sizeof(int64_t),
+                 hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemcpy(cap->offs, offs.data(), offs.size() * sizeof(int64_t),
+                 hipMemcpyHostToDevice) != hipSuccess))
+    rc = fail(FS_EHIP, "node list upload failed");
+  const int64_t need = (int64_t)nbins * fs::kMaxRhs * cap->nbx;
+  if (rc == FS_OK && need > s->partials_cap) {
+    dfree(s->partials);
+    rc = dalloc(&s->partials, (size_t)need);
+    s->partials_cap = rc == FS_OK ? need : 0;
+  }
+  if (rc != FS_OK) {
+    fs_cap_destroy(cap);
+    return rc;
+  }
+  *out = cap;
+  return FS_OK;
+  FS_GUARD_END
+}
+
+int fs_cap_destroy(fs_cap* cap) {
+  if (!cap) return FS_OK;
+  (void)hipSetDevice(cap->sys->ctx->device);
+  dfree(cap->tid);
+  dfree(cap->nodes);
+  dfree(cap->offs);
+  dfree(cap->out);
+  delete cap;
+  return FS_OK;
+}
+
+int fs_cap_solve(fs_cap* cap, const fs_op* L, double* C, double* q_shield, double* phi_out,
+                 const fs_solve_params* params, fs_solve_stats* stats) {
+  FS_GUARD_BEGIN
+  if (!cap || !L || !C || !params) return fail(FS_EINVAL, "NULL argument");
+  fs_sys* s = cap->sys;
+  if (L->Nx != s->op->Nx || L->Ny != s->op->Ny || L->Nz != s->op->Nz)
+    return fail(FS_EINVAL, "L and the system differ in grid");
+  if (!(params->tol > 0.0)) return fail(FS_EINVAL, "tol must be > 0");
+  if (params->precision != FS_PREC_FP64 && params->precision != FS_PREC_MIXED)
+    return fail(FS_EINVAL, "unknown precision %d", params->precision);
+  fs_ctx* ctx = s->ctx;
+  FS_HIP(hipSetDevice(ctx->device));
+  const fs::Dims& d = s->d;
+  const int nt = cap->nt, nbins = cap->nbins;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->converged = 1;
+  }
+  const int maxiter = default_maxiter(s, params);
+  const bool mixed = params->precision == FS_PREC_MIXED;
+  int status = FS_OK;
+  float ms_total = 0.0f;
+  std::vector<double> hout((size_t)nbins * fs::kMaxRhs);
+  for (int j0 = 0; j0 < nt; j0 += fs::kMaxRhs) {
+    const int nr = std::min(fs::kMaxRhs, nt - j0);
+    if (int rc = ensure_ws(s, nr)) return rc;
+    prof_begin(s);
+    FS_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    fs::launch_lift_cap(s, nr, L, cap->tid, j0);
+    fs::launch_init(s, nr, params->tol, maxiter);
+    FS_HIP(hipGetLastError());
+    if (int rc = run_krylov(s, nr, *params)) return rc;
+    fs::launch_residual(s, nr, s->r);
+    fs::launch_expand(s, nr, nullptr, cap->tid, j0, s->q);
+    fs::launch_charge(s, nr, L, s->q, cap->nodes, cap->offs, nbins, cap->nbx, cap->out);
+    FS_HIP(hipGetLastError());
+    FS_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    FS_HIP(hipMemcpyAsync(hout.data(), cap->out, sizeof(double) * nbins * nr,
+                          hipMemcpyDeviceToHost, ctx->stream));
+    FS_HIP(hipMemcpyAsync(s->sc_host, s->sc, sizeof(fs::CgScalars) * nr, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    FS_HIP(hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    ms_total += ms;
+    int st = FS_OK;
+    collect_stats(s, nr, stats, st);
+    prof_end(s, stats, mixed);
+    if (st != FS_OK) {
+      status = st;
+      if (stats) stats->converged = 0;
+    }
+    for (int r = 0; r < nr; ++r) {
+      for (int i = 0; i < nt; ++i) C[(size_t)i * nt + j0 + r] = hout[(size_t)i * nr + r];
+      if (q_shield) q_shield[j0 + r] = cap->has_shield ? hout[(size_t)nt * nr + r] : 0.0;
+    }
+    if (phi_out) {
+      for (int r = 0; r < nr; ++r)
+        if (int rc = download_nodes(ctx, d, s->q + (size_t)r * s->vstride,
+                                    phi_out + (size_t)(j0 + r) * d.n, s->r))
+          return rc;
+    }
+  }
+  if (stats) stats->solve_time = ms_total * 1e-3;
+  if (status == FS_NOT_CONVERGED)
+    fail(FS_NOT_CONVERGED, "Krylov iteration did not reach tol %.3g within %d iterations",
+         params->tol, maxiter);
+  return status;
+  FS_GUARD_END
+}
+
+int fs_capacitance(fs_sys* s, const fs_op* L, int nt, const int32_t* term_id, double* C,
+                   double* q_shield, double* phi_out, const fs_solve_params* params,
+                   fs_solve_stats* stats) {
+  fs_cap* cap = nullptr;
+  if (int rc = fs_cap_prepare(s, nt, term_id, &cap)) return rc;
+  const int rc = fs_cap_solve(cap, L, C, q_shield, phi_out, params, stats);
+  const std::string keep = g_err;
+  fs_cap_destroy(cap);
+  g_err = keep;
+  return rc;
+}
+
+int fs_sys_set_profiling(fs_sys* s, int on) {
+  FS_GUARD_BEGIN
+  if (!s) return fail(FS_EINVAL, "NULL argument");
+  FS_HIP(hipSetDevice(s->ctx->device));
+  if (on && s->ev_cap == 0) {
+    s->ev_a = new hipEvent_t[kMaxChunk];
+    s->ev_b = new hipEvent_t[kMaxChunk];
+    for (int k = 0; k < kMaxChunk; ++k) {
+      FS_HIP(hipEventCreate(&s->ev_a[k]));
+      FS_HIP(hipEventCreate(&s->ev_b[k]));
+      s->ev_cap = k + 1;
+    }
+    if (int rc = dalloc(&s->prof_dev, 2)) return rc;
+    FS_HIP(hipMemset(s->prof_dev, 0, 2 * sizeof(unsigned long long)));
+  }
+  s->profiling = on ? 1 : 0;
+  return FS_OK;
+  FS_GUARD_END
+}
+
+}  // extern "C"

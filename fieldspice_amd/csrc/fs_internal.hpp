@@ -1,0 +1,127 @@
+// Internal definitions shared by the kernels and the C ABI of the
+// fieldspice MI355X library.  Nothing here crosses the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "fieldspice_hip.h"
+
+namespace fs {
+
+constexpr int kWave = 64;         // CDNA wavefront
+constexpr int kBlock = 256;       // threads per block for every kernel here
+constexpr int kMaxRhs = 8;        // right-hand sides one launch carries
+constexpr int kTileX = 64;        // stencil tile: one wavefront along x ...
+constexpr int kTileY = 4;         // ... times four rows
+constexpr int kNumCounters = 16;  // last-block counters per system
+
+// Device layout: node (i, j, k) of an (nx, ny, nz) node grid lives at
+// n = (k*ny + j)*nx + i (x fastest).  Every node vector of a system is padded
+// to `vstride` elements so that right-hand sides start on 256-byte bounds.
+struct Dims {
+  int nx, ny, nz;
+  int64_t n;       // nx*ny*nz
+  int64_t sxy;     // nx*ny
+};
+
+// Tile decomposition of the stencil kernels.  Block b of the launch is
+// remapped so that the 8 XCDs (which receive blocks round robin) each sweep
+// one contiguous range of tiles: z-neighbour planes and the halo rows of a
+// tile are then produced and consumed inside one XCD's L2.
+struct Tiling {
+  int tiles_x, tiles_y, tiles_z;
+  int kz;          // z planes marched per tile
+  int ntiles;      // tiles_x*tiles_y*tiles_z
+  int nblocks;     // ntiles rounded up to a multiple of 8
+};
+
+// Per right-hand-side Krylov scalars, written only by the last block of a
+// reduction and read by every block of the following kernels.
+struct CgScalars {
+  double rho;       // r.r of the current residual
+  double pq;        // p.q
+  double alpha;
+  double beta;
+  double thresh2;   // stop when rho < thresh2  (== (tol*||b_s||)^2)
+  double bnorm2;    // ||b_s||^2
+  double rho_cycle; // rho at the start of an FP32 cycle (mixed precision)
+  double scale;     // FP32 cycle scaling: r32 = r64 * scale
+  double aux0, aux1, aux2, aux3;
+  int iter;         // iterations completed (over all cycles)
+  int cycle_iter;   // iterations in the current FP32 cycle
+  int maxiter;
+  int done;         // 0 running, 1 converged, 2 iteration cap, 3 breakdown
+  int cycle_done;   // mixed precision: FP32 cycle reached its target
+  int refreshes;
+  int pad0, pad1;
+};
+
+}  // namespace fs
+
+// --------------------------------------------------------------------------
+// ABI objects
+// --------------------------------------------------------------------------
+struct fs_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n_cu = 0;
+  char name[256] = {0};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct fs_op {
+  fs_ctx* ctx = nullptr;
+  int Nx = 0, Ny = 0, Nz = 0;  // cells
+  fs::Dims d{};
+  double* w[3] = {nullptr, nullptr, nullptr};  // node-padded edge weights (x-fast)
+  double* extra = nullptr;                     // node diagonal, or null
+};
+
+struct fs_sys {
+  fs_ctx* ctx = nullptr;
+  const fs_op* op = nullptr;  // not owned; must outlive the system
+  fs::Dims d{};
+  fs::Tiling tl{};
+  int64_t vstride = 0;
+  uint8_t* mask = nullptr;     // 0 free, 1 Dirichlet, 2 pinned (x-fast)
+  double* dhalf = nullptr;     // sqrt(D) on free nodes, 0 elsewhere
+  double* dinv = nullptr;      // 1/sqrt(D) on free nodes, 0 elsewhere
+  double* c64[3] = {nullptr, nullptr, nullptr};  // equilibrated masked couplings
+  float* c32[3] = {nullptr, nullptr, nullptr};
+  fs_sys_info info{};
+  // solve workspace, grown on demand (capacity in right-hand sides)
+  int ws_nr = 0;
+  double *bs = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
+  float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *q32 = nullptr;
+  double* node_tmp = nullptr;        // host-layout staging (nr node vectors)
+  double* partials = nullptr;        // reduction partials
+  int64_t partials_cap = 0;
+  unsigned* counters = nullptr;      // last-block counters
+  fs::CgScalars* sc = nullptr;       // device scalars (kMaxRhs)
+  fs::CgScalars* sc_host = nullptr;  // pinned host mirror
+  int vec_blocks = 0;                // grid of the node-wise vector kernels
+  // optional timing of the stencil launches of the Krylov loop
+  int profiling = 0;
+  int ev_cap = 0;
+  hipEvent_t* ev_a = nullptr;
+  hipEvent_t* ev_b = nullptr;
+  double prof_time = 0.0;
+  int64_t prof_launches = 0;
+  unsigned long long* prof_dev = nullptr;  // [0] live launches, [1] live RHS summed
+};
+
+struct fs_cap {
+  fs_sys* sys = nullptr;
+  int nt = 0;
+  int nbins = 0;
+  int nbx = 0;
+  int has_shield = 0;
+  int32_t* tid = nullptr;     // x-fast terminal map
+  int64_t* nodes = nullptr;   // node lists per bin (x-fast indices)
+  int64_t* offs = nullptr;
+  double* out = nullptr;      // nbins x kMaxRhs charges
+};

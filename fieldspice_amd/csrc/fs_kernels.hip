@@ -1,0 +1,1163 @@
+// HIP kernels of the fieldspice MI355X field solve (gfx950, wave64).
+//
+// Operator: A = G^T diag(w) G + diag(extra) on a tensor-product node grid,
+// stored matrix-free as three node-padded edge-weight arrays in an
+// x-fastest layout (see fs_internal.hpp).  A prepared system keeps the
+// Jacobi-equilibrated, Dirichlet-masked couplings
+//     c_e = w_e / sqrt(D_n D_m)   (0 when either end is fixed or pinned)
+// so the Krylov operator is  y = x - sum_e c_e x_other  with an implicit unit
+// diagonal: the equilibrated free block S A_ff S of
+// fieldspice/solvers/poisson.py:364-381, with no mask or diagonal stream.
+//
+// Reductions are deterministic: each block writes one partial per quantity,
+// and the last block to arrive (atomic ticket) sums the partials in index
+// order and updates the per-RHS scalars on the device, so the host never has
+// to synchronise inside the Krylov loop.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fs_internal.hpp"
+#include "fs_launch.hpp"
+
+namespace fs {
+
+// --------------------------------------------------------------------------
+// reductions
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+  return v;
+}
+
+// Sum NV values over the block; result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  __shared__ double s[NV][kBlock / kWave];
+  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double t = wave_sum(v[k]);
+    if (lane == 0) s[k][wv] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < kBlock / kWave; ++w) t += s[k][w];
+      v[k] = t;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void store_partial(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_partial(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by every thread after thread 0 stored this block's partials.
+// Returns true in every thread of the last block to arrive.
+__device__ __forceinline__ bool arrive_last(unsigned* counter) {
+  __shared__ int s_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __atomic_thread_fence(__ATOMIC_RELEASE);
+    unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == gridDim.x * gridDim.y - 1);
+  }
+  __syncthreads();
+  const bool last = s_last != 0;
+  if (last) __atomic_thread_fence(__ATOMIC_ACQUIRE);
+  return last;
+}
+
+__device__ __forceinline__ void reset_counter(unsigned* counter) {
+  if (threadIdx.x == 0)
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// In the last block: sum partials[0..nb) in a fixed order (result in thread 0).
+__device__ __forceinline__ double final_sum(const double* partials, int nb) {
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < nb; b += kBlock) v[0] += load_partial(partials + b);
+  block_sum<1>(v);
+  return v[0];
+}
+
+// --------------------------------------------------------------------------
+// layout permutation between fieldspice's z-fastest arrays and x-fast
+// --------------------------------------------------------------------------
+// in: C order (A, B, C), c fastest.  out[(c*OB + b)*OA + a] (a fastest).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_permute_in(const T* __restrict__ in,
+                                                       T* __restrict__ out, int A, int B,
+                                                       int C, int OA, int OB) {
+  __shared__ T tile[64][65];
+  const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+  const int c0 = blockIdx.x * 64, a0 = blockIdx.y * 64, b = blockIdx.z;
+  for (int ra = ty; ra < 64; ra += 4) {
+    const int a = a0 + ra, c = c0 + tx;
+    if (a < A && c < C) tile[ra][tx] = in[((int64_t)a * B + b) * C + c];
+  }
+  __syncthreads();
+  for (int rc = ty; rc < 64; rc += 4) {
+    const int c = c0 + rc, a = a0 + tx;
+    if (a < A && c < C) out[((int64_t)c * OB + b) * OA + a] = tile[tx][rc];
+  }
+}
+
+// in: x-fast [(c*IB + b)*IA + a];  out: C order (A, B, C).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_permute_out(const T* __restrict__ in,
+                                                        T* __restrict__ out, int A, int B,
+                                                        int C, int IA, int IB) {
+  __shared__ T tile[64][65];
+  const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+  const int a0 = blockIdx.x * 64, c0 = blockIdx.y * 64, b = blockIdx.z;
+  for (int rc = ty; rc < 64; rc += 4) {
+    const int c = c0 + rc, a = a0 + tx;
+    if (a < A && c < C) tile[rc][tx] = in[((int64_t)c * IB + b) * IA + a];
+  }
+  __syncthreads();
+  for (int ra = ty; ra < 64; ra += 4) {
+    const int a = a0 + ra, c = c0 + tx;
+    if (a < A && c < C) out[((int64_t)a * B + b) * C + c] = tile[tx][ra];
+  }
+}
+
+template <typename T>
+void permute_in(hipStream_t s, const T* in, T* out, int A, int B, int C, int OA, int OB) {
+  dim3 grid((C + 63) / 64, (A + 63) / 64, B);
+  hipLaunchKernelGGL(k_permute_in<T>, grid, dim3(kBlock), 0, s, in, out, A, B, C, OA, OB);
+}
+template <typename T>
+void permute_out(hipStream_t s, const T* in, T* out, int A, int B, int C, int IA, int IB) {
+  dim3 grid((A + 63) / 64, (C + 63) / 64, B);
+  hipLaunchKernelGGL(k_permute_out<T>, grid, dim3(kBlock), 0, s, in, out, A, B, C, IA, IB);
+}
+template void permute_in<double>(hipStream_t, const double*, double*, int, int, int, int, int);
+template void permute_in<uint8_t>(hipStream_t, const uint8_t*, uint8_t*, int, int, int, int, int);
+template void permute_in<int32_t>(hipStream_t, const int32_t*, int32_t*, int, int, int, int, int);
+template void permute_out<double>(hipStream_t, const double*, double*, int, int, int, int, int);
+
+// --------------------------------------------------------------------------
+// edge weights from cell properties: cell_to_edge + edge_mass
+// (fieldspice/operators.py:144-237, 302-315) fused in one pass
+// --------------------------------------------------------------------------
+struct GridMetric {
+  const double *h[3];   // primal widths, per axis (N_d entries)
+  const double *hd[3];  // dual widths, per axis (N_d + 1 entries)
+  int N[3];             // cells per axis
+};
+
+__global__ __launch_bounds__(kBlock) void k_cell_to_edge(const double* __restrict__ prop,
+                                                         GridMetric g, Dims d, int dir,
+                                                         int mode, double* __restrict__ w) {
+#pragma clang fp contract(off)
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= d.n) return;
+  int idx[3];
+  idx[0] = (int)(n % d.nx);
+  idx[1] = (int)((n / d.nx) % d.ny);
+  idx[2] = (int)(n / d.sxy);
+  if (idx[dir] >= g.N[dir]) {  // past the last cell along dir: no edge
+    w[n] = 0.0;
+    return;
+  }
+  const int t0 = dir == 0 ? 1 : 0;
+  const int t1 = dir == 2 ? 1 : 2;
+  double q[4], wt[4];
+  int m = 0;
+  for (int s0 = 0; s0 < 2; ++s0) {
+    for (int s1 = 0; s1 < 2; ++s1, ++m) {
+      const int i0 = idx[t0], i1 = idx[t1];
+      // half width of the cell below (s=0) / above (s=1) the node plane
+      const double w0 = s0 == 0 ? (i0 > 0 ? 0.5 * g.h[t0][i0 - 1] : 0.0)
+                                : (i0 < g.N[t0] ? 0.5 * g.h[t0][i0] : 0.0);
+      const double w1 = s1 == 0 ? (i1 > 0 ? 0.5 * g.h[t1][i1 - 1] : 0.0)
+                                : (i1 < g.N[t1] ? 0.5 * g.h[t1][i1] : 0.0);
+      wt[m] = w0 * w1;
+      // edge-replicated padding: missing cells repeat the boundary cell
+      int c[3];
+      c[dir] = idx[dir];
+      c[t0] = min(max(i0 - 1 + s0, 0), g.N[t0] - 1);
+      c[t1] = min(max(i1 - 1 + s1, 0), g.N[t1] - 1);
+      q[m] = prop[((int64_t)c[2] * g.N[1] + c[1]) * g.N[0] + c[0]];
+    }
+  }
+  const double wsum = ((wt[0] + wt[1]) + wt[2]) + wt[3];
+  const double safe = wsum > 0.0 ? wsum : 1.0;
+  double e;
+  if (mode == FS_AVG_PARALLEL) {
+    e = (((q[0] * wt[0] + q[1] * wt[1]) + q[2] * wt[2]) + q[3] * wt[3]) / safe;
+  } else if (mode == FS_AVG_HARMONIC) {
+    double acc = 0.0;
+    for (int k = 0; k < 4; ++k) acc = acc + wt[k] / (q[k] > 0.0 ? q[k] : 2.2250738585072014e-308);
+    e = safe / acc;
+  } else if (mode == FS_AVG_MIN) {
+    e = fmin(fmin(q[0], q[1]), fmin(q[2], q[3]));
+  } else {
+    e = fmax(fmax(q[0], q[1]), fmax(q[2], q[3]));
+  }
+  const double area = g.hd[t0][idx[t0]] * g.hd[t1][idx[t1]];
+  w[n] = (e * area) / g.h[dir][idx[dir]];
+}
+
+void cell_to_edge(hipStream_t s, const double* prop, const double* const h[3],
+                  const double* const hd[3], const int N[3], const Dims& d, int dir,
+                  int mode, double* w) {
+  GridMetric g;
+  for (int a = 0; a < 3; ++a) {
+    g.h[a] = h[a];
+    g.hd[a] = hd[a];
+    g.N[a] = N[a];
+  }
+  const int nb = (int)((d.n + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_cell_to_edge, dim3(nb), dim3(kBlock), 0, s, prop, g, d, dir, mode, w);
+}
+
+// out = a*x + b*y elementwise (y may be null)
+__global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, double a, const double* x,
+                                                  double b, const double* y, double* out) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    double v = x ? a * x[i] : 0.0;
+    if (y) v += b * y[i];
+    out[i] = v;
+  }
+}
+
+void axpby(hipStream_t s, int64_t n, double a, const double* x, double b, const double* y,
+           double* out) {
+  const int nb = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, 4096);
+  hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(kBlock), 0, s, n, a, x, b, y, out);
+}
+
+// --------------------------------------------------------------------------
+// raw operator apply: y = sum_e w_e (x_n - x_m) + extra_n x_n   (FP64)
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double raw_row(const double* __restrict__ x,
+                                          const double* __restrict__ wx,
+                                          const double* __restrict__ wy,
+                                          const double* __restrict__ wz,
+                                          const double* __restrict__ extra, const Dims& d,
+                                          int i, int j, int k, int64_t n) {
+  const double xc = x[n];
+  double y = 0.0;
+  if (i + 1 < d.nx) y += wx[n] * (xc - x[n + 1]);
+  if (i > 0) y += wx[n - 1] * (xc - x[n - 1]);
+  if (j + 1 < d.ny) y += wy[n] * (xc - x[n + d.nx]);
+  if (j > 0) y += wy[n - d.nx] * (xc - x[n - d.nx]);
+  if (k + 1 < d.nz) y += wz[n] * (xc - x[n + d.sxy]);
+  if (k > 0) y += wz[n - d.sxy] * (xc - x[n - d.sxy]);
+  if (extra) y += extra[n] * xc;
+  return y;
+}
+
+__global__ __launch_bounds__(kBlock) void k_apply_raw(const double* __restrict__ x,
+                                                      double* __restrict__ y,
+                                                      const double* __restrict__ wx,
+                                                      const double* __restrict__ wy,
+                                                      const double* __restrict__ wz,
+                                                      const double* __restrict__ extra,
+                                                      Dims d) {
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= d.n) return;
+  const int i = (int)(n % d.nx), j = (int)((n / d.nx) % d.ny), k = (int)(n / d.sxy);
+  y[n] = raw_row(x, wx, wy, wz, extra, d, i, j, k, n);
+}
+
+void apply_raw(hipStream_t s, const fs_op* op, const double* x, double* y) {
+  const int nb = (int)((op->d.n + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_apply_raw, dim3(nb), dim3(kBlock), 0, s, x, y, op->w[0], op->w[1],
+                     op->w[2], op->extra, op->d);
+}
+
+// --------------------------------------------------------------------------
+// system setup: diagonal, masks, gauge, equilibration, couplings
+// --------------------------------------------------------------------------
+struct SetupCounts {
+  unsigned long long n_fixed;
+  unsigned long long n_empty;
+  unsigned long long n_nonzero_rowsum;  // free rows whose sum is not ~0
+  unsigned long long dmax_bits;         // max D over non-empty rows (bit pattern)
+  long long gauge_ref;                  // min reference index among argmax rows
+};
+
+// D_n = sum of the <= 6 incident weights + extra_n; classify rows.  Counts
+// and the maximum are combined per wavefront before one atomic per wave.
+__global__ __launch_bounds__(kBlock) void k_setup_diag(const fs_op op, uint8_t* __restrict__ mask,
+                                                       double* __restrict__ dsum,
+                                                       SetupCounts* cnt) {
+  const Dims d = op.d;
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  bool is_fixed = false, is_empty = false, nonzero_sum = false;
+  unsigned long long dbits = 0ull;
+  if (n < d.n) {
+    const int i = (int)(n % d.nx), j = (int)((n / d.nx) % d.ny), k = (int)(n / d.sxy);
+    double ws = op.w[0][n] + op.w[1][n] + op.w[2][n];
+    if (i > 0) ws += op.w[0][n - 1];
+    if (j > 0) ws += op.w[1][n - d.nx];
+    if (k > 0) ws += op.w[2][n - d.sxy];
+    const double ex = op.extra ? op.extra[n] : 0.0;
+    const double D = ws + ex;
+    dsum[n] = D;
+    if (mask[n] == FS_NODE_DIRICHLET) {
+      is_fixed = true;
+    } else {
+      const double row_abs = 2.0 * ws + fabs(ex);
+      if (row_abs == 0.0) {  // coupled to nothing: pinned to 0
+        mask[n] = 2;
+        is_empty = true;
+      } else {
+        nonzero_sum = fabs(ex) > 1e-9 * row_abs;
+        dbits = (unsigned long long)__double_as_longlong(fabs(D));
+      }
+    }
+  }
+  const unsigned long long bf = __ballot(is_fixed), be = __ballot(is_empty),
+                           bn = __ballot(nonzero_sum);
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_down(dbits, off, kWave);
+    dbits = o > dbits ? o : dbits;
+  }
+  if (threadIdx.x % kWave == 0) {
+    if (bf) atomicAdd(&cnt->n_fixed, (unsigned long long)__popcll(bf));
+    if (be) atomicAdd(&cnt->n_empty, (unsigned long long)__popcll(be));
+    if (bn) atomicAdd(&cnt->n_nonzero_rowsum, (unsigned long long)__popcll(bn));
+    if (dbits) atomicMax(&cnt->dmax_bits, dbits);
+  }
+}
+
+// Among free non-empty rows with |D| == dmax, the smallest reference index
+// (fieldspice picks np.argmax(|diag|), i.e. the first in its C order).
+__global__ __launch_bounds__(kBlock) void k_setup_gauge(const uint8_t* __restrict__ mask,
+                                                        const double* __restrict__ dsum,
+                                                        Dims d, SetupCounts* cnt) {
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= d.n) return;
+  if (mask[n] != 0) return;
+  if ((unsigned long long)__double_as_longlong(fabs(dsum[n])) != cnt->dmax_bits) return;
+  const int64_t i = n % d.nx, j = (n / d.nx) % d.ny, k = n / d.sxy;
+  const long long ref = (long long)((i * d.ny + j) * d.nz + k);
+  atomicMin(&cnt->gauge_ref, ref);
+}
+
+__global__ __launch_bounds__(kBlock) void k_setup_pin(uint8_t* mask, Dims d, long long ref) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int64_t i = ref / ((int64_t)d.ny * d.nz);
+  const int64_t j = (ref / d.nz) % d.ny;
+  const int64_t k = ref % d.nz;
+  mask[(k * d.ny + j) * d.nx + i] = 2;
+}
+
+__global__ __launch_bounds__(kBlock) void k_setup_equil(const uint8_t* __restrict__ mask,
+                                                        const double* __restrict__ dsum,
+                                                        double* __restrict__ dhalf,
+                                                        double* __restrict__ dinv, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  if (mask[i] == 0) {
+    const double h = sqrt(dsum[i]);
+    dhalf[i] = h;
+    dinv[i] = 1.0 / h;
+  } else {
+    dhalf[i] = 0.0;
+    dinv[i] = 0.0;
+  }
+}
+
+// c_e = dinv_n * (w_e * dinv_m); zero whenever either end is not free.
+__global__ __launch_bounds__(kBlock) void k_setup_couplings(const fs_op op,
+                                                            const double* __restrict__ dinv,
+                                                            double* cx, double* cy, double* cz,
+                                                            float* fx, float* fy, float* fz) {
+  const Dims d = op.d;
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= d.n) return;
+  const int i = (int)(n % d.nx), j = (int)((n / d.nx) % d.ny), k = (int)(n / d.sxy);
+  const double s = dinv[n];
+  const double vx = (i + 1 < d.nx) ? s * (op.w[0][n] * dinv[n + 1]) : 0.0;
+  const double vy = (j + 1 < d.ny) ? s * (op.w[1][n] * dinv[n + d.nx]) : 0.0;
+  const double vz = (k + 1 < d.nz) ? s * (op.w[2][n] * dinv[n + d.sxy]) : 0.0;
+  cx[n] = vx;
+  cy[n] = vy;
+  cz[n] = vz;
+  fx[n] = (float)vx;
+  fy[n] = (float)vy;
+  fz[n] = (float)vz;
+}
+
+int setup_system(fs_sys* sys, double* dsum, void* counts_dev, void* counts_host, int pin) {
+  hipStream_t s = sys->ctx->stream;
+  const Dims d = sys->d;
+  const int nb = (int)((d.n + kBlock - 1) / kBlock);
+  SetupCounts* cd = (SetupCounts*)counts_dev;
+  SetupCounts* ch = (SetupCounts*)counts_host;  // pinned
+  *ch = SetupCounts{0, 0, 0, 0, (long long)INT64_MAX};
+  if (hipMemcpyAsync(cd, ch, sizeof(SetupCounts), hipMemcpyHostToDevice, s) != hipSuccess)
+    return -1;
+  hipLaunchKernelGGL(k_setup_diag, dim3(nb), dim3(kBlock), 0, s, *sys->op, sys->mask, dsum, cd);
+  if (hipMemcpyAsync(ch, cd, sizeof(SetupCounts), hipMemcpyDeviceToHost, s) != hipSuccess)
+    return -1;
+  if (hipStreamSynchronize(s) != hipSuccess) return -1;
+  sys->info.n_nodes = d.n;
+  sys->info.n_fixed = (int64_t)ch->n_fixed;
+  sys->info.n_empty = (int64_t)ch->n_empty;
+  const int64_t n_keep = d.n - sys->info.n_fixed - sys->info.n_empty;
+  sys->info.singular = (sys->info.n_fixed == 0 && n_keep > 0 && ch->n_nonzero_rowsum == 0) ? 1 : 0;
+  sys->info.gauge = -1;
+  if (sys->info.singular) {
+    if (!pin) return 1;
+    hipLaunchKernelGGL(k_setup_gauge, dim3(nb), dim3(kBlock), 0, s, sys->mask, dsum, d, cd);
+    if (hipMemcpyAsync(ch, cd, sizeof(SetupCounts), hipMemcpyDeviceToHost, s) != hipSuccess)
+      return -1;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    if (ch->gauge_ref < 0 || ch->gauge_ref >= d.n) return -1;  // no candidate row
+    sys->info.gauge = ch->gauge_ref;
+    hipLaunchKernelGGL(k_setup_pin, dim3(1), dim3(kBlock), 0, s, sys->mask, d, ch->gauge_ref);
+  }
+  sys->info.n_free = d.n - sys->info.n_fixed - sys->info.n_empty - (sys->info.gauge >= 0 ? 1 : 0);
+  hipLaunchKernelGGL(k_setup_equil, dim3(nb), dim3(kBlock), 0, s, sys->mask, dsum, sys->dhalf,
+                     sys->dinv, d.n);
+  hipLaunchKernelGGL(k_setup_couplings, dim3(nb), dim3(kBlock), 0, s, *sys->op, sys->dinv,
+                     sys->c64[0], sys->c64[1], sys->c64[2], sys->c32[0], sys->c32[1],
+                     sys->c32[2]);
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// Krylov kernels.  NR = compile-time right-hand-side capacity, nr <= NR live.
+// Vectors of right-hand side r start at r*vs.
+// --------------------------------------------------------------------------
+struct VecArgs {
+  int64_t n;
+  int64_t vs;
+  int nr;
+};
+
+__device__ __forceinline__ bool rhs_live(const CgScalars* sc, int r, int mode) {
+  return mode == 0 ? sc[r].done == 0 : (sc[r].done == 0 && sc[r].cycle_done == 0);
+}
+
+__device__ __forceinline__ bool all_idle(const CgScalars* sc, int nr, int mode) {
+  for (int r = 0; r < nr; ++r)
+    if (rhs_live(sc, r, mode)) return false;
+  return true;
+}
+
+// Bind a tile of the XCD-remapped launch; returns false for padding blocks.
+__device__ __forceinline__ bool tile_of_block(const Tiling& tl, int& tx, int& ty, int& tz) {
+  const int bid = blockIdx.x;
+  const int per = tl.nblocks >> 3;
+  const int t = (bid & 7) * per + (bid >> 3);
+  if (t >= tl.ntiles) return false;
+  tx = t % tl.tiles_x;
+  ty = (t / tl.tiles_x) % tl.tiles_y;
+  tz = t / (tl.tiles_x * tl.tiles_y);
+  return true;
+}
+
+// q = A_s p for each live RHS (z-marching, couplings read once for all RHS),
+// fused with p.q; the last block sets alpha = rho / (p.q).
+template <typename T, int NR>
+__global__ __launch_bounds__(kBlock) void k_stencil_dot(const T* __restrict__ p, T* __restrict__ q,
+                                                        const T* __restrict__ cx,
+                                                        const T* __restrict__ cy,
+                                                        const T* __restrict__ cz, Dims d,
+                                                        Tiling tl, VecArgs va, int mode,
+                                                        CgScalars* sc, double* partials,
+                                                        unsigned* counter,
+                                                        unsigned long long* prof) {
+  if (all_idle(sc, va.nr, mode)) return;
+  bool live[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) live[r] = r < va.nr && rhs_live(sc, r, mode);
+  double acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+
+  int tx, ty, tz;
+  if (tile_of_block(tl, tx, ty, tz)) {
+    const int i = tx * kTileX + (int)(threadIdx.x % kTileX);
+    const int j = ty * kTileY + (int)(threadIdx.x / kTileX);
+    const int k0 = tz * tl.kz;
+    const int k1 = min(d.nz, k0 + tl.kz);
+    if (i < d.nx && j < d.ny) {
+      const int64_t base = (int64_t)j * d.nx + i;
+      const bool has_e = i + 1 < d.nx, has_w = i > 0, has_n = j + 1 < d.ny, has_s = j > 0;
+      int64_t nn = (int64_t)k0 * d.sxy + base;
+      T czm = k0 > 0 ? cz[nn - d.sxy] : T(0);
+      T pm[NR], pc[NR];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        pm[r] = T(0);
+        pc[r] = T(0);
+        if (live[r]) {
+          const T* pr = p + r * va.vs;
+          pm[r] = k0 > 0 ? pr[nn - d.sxy] : T(0);
+          pc[r] = pr[nn];
+        }
+      }
+      for (int k = k0; k < k1; ++k, nn += d.sxy) {
+        const T cxc = cx[nn], czc = cz[nn], cyc = cy[nn];
+        const T cxm = has_w ? cx[nn - 1] : T(0);
+        const T cym = has_s ? cy[nn - d.nx] : T(0);
+        const bool has_u = k + 1 < d.nz;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          if (!live[r]) continue;
+          const T* pr = p + r * va.vs;
+          const T pe = has_e ? pr[nn + 1] : T(0);
+          const T pw = has_w ? pr[nn - 1] : T(0);
+          const T pn = has_n ? pr[nn + d.nx] : T(0);
+          const T ps = has_s ? pr[nn - d.nx] : T(0);
+          const T pu = has_u ? pr[nn + d.sxy] : T(0);
+          const T y = pc[r] - (cxc * pe + cxm * pw + cyc * pn + cym * ps + czc * pu + czm * pm[r]);
+          q[r * va.vs + nn] = y;
+          acc[r] += (double)pc[r] * (double)y;
+          pm[r] = pc[r];
+          pc[r] = pu;
+        }
+        czm = czc;
+      }
+    }
+  }
+  block_sum<NR>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (r < va.nr) store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
+  }
+  if (arrive_last(counter)) {
+    if (prof && threadIdx.x == 0) {
+      unsigned long long nlive = 0;
+      for (int r = 0; r < va.nr; ++r) nlive += live[r] ? 1ull : 0ull;
+      prof[0] += 1ull;
+      prof[1] += nlive;
+    }
+    for (int r = 0; r < va.nr; ++r) {
+      const double pq = final_sum(partials + (int64_t)r * gridDim.x, gridDim.x);
+      if (threadIdx.x == 0 && live[r]) {
+        const double rho = mode == 0 ? sc[r].rho : sc[r].aux0;
+        sc[r].pq = pq;
+        if (!(pq > 0.0) || !isfinite(pq)) {
+          sc[r].done = 3;  // breakdown: A_s not positive definite on p
+          sc[r].alpha = 0.0;
+        } else {
+          sc[r].alpha = rho / pq;
+        }
+      }
+    }
+    reset_counter(counter);
+  }
+}
+
+// x += alpha p; r -= alpha q; rho_new = r.r; beta = rho_new/rho; stop tests.
+// mode 0: FP64 main iteration (threshold thresh2, sets done).
+// mode 1: FP32 cycle of mixed precision (threshold aux1 on the cycle's own
+//         rho kept in aux0, sets cycle_done; the iteration cap still ends it).
+template <typename T, int NR>
+__global__ __launch_bounds__(kBlock) void k_update(T* __restrict__ x, T* __restrict__ r_,
+                                                   const T* __restrict__ p,
+                                                   const T* __restrict__ q, VecArgs va,
+                                                   int mode, CgScalars* sc, double* partials,
+                                                   unsigned* counter) {
+  if (all_idle(sc, va.nr, mode)) return;
+  double acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+  bool live[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) live[r] = r < va.nr && rhs_live(sc, r, mode);
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (!live[r]) continue;
+    const T a = (T)sc[r].alpha;
+    T* xr = x + r * va.vs;
+    T* rr = r_ + r * va.vs;
+    const T* pr = p + r * va.vs;
+    const T* qr = q + r * va.vs;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride) {
+      xr[i] += a * pr[i];
+      const T rn = rr[i] - a * qr[i];
+      rr[i] = rn;
+      s += (double)rn * (double)rn;
+    }
+    acc[r] = s;
+  }
+  block_sum<NR>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (r < va.nr) store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
+  }
+  if (arrive_last(counter)) {
+    for (int r = 0; r < va.nr; ++r) {
+      const double rho_new = final_sum(partials + (int64_t)r * gridDim.x, gridDim.x);
+      if (threadIdx.x == 0 && live[r]) {
+        CgScalars& c = sc[r];
+        c.iter += 1;
+        c.cycle_iter += 1;
+        if (mode == 0) {
+          c.beta = rho_new / c.rho;
+          c.rho = rho_new;
+          if (rho_new < c.thresh2) c.done = 1;
+          else if (c.iter >= c.maxiter) c.done = 2;
+          else if (!isfinite(rho_new)) c.done = 3;
+        } else {
+          c.beta = rho_new / c.aux0;
+          c.aux0 = rho_new;
+          if (rho_new < c.aux1) c.cycle_done = 1;
+          else if (c.iter >= c.maxiter) c.cycle_done = 1;
+          else if (!isfinite(rho_new)) c.done = 3;
+        }
+      }
+    }
+    reset_counter(counter);
+  }
+}
+
+// p = r + beta p
+template <typename T, int NR>
+__global__ __launch_bounds__(kBlock) void k_pupdate(T* __restrict__ p, const T* __restrict__ r_,
+                                                    VecArgs va, int mode, const CgScalars* sc) {
+  if (all_idle(sc, va.nr, mode)) return;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (!(r < va.nr && rhs_live(sc, r, mode))) continue;
+    const T b = (T)sc[r].beta;
+    T* pr = p + r * va.vs;
+    const T* rr = r_ + r * va.vs;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride)
+      pr[i] = rr[i] + b * pr[i];
+  }
+}
+
+// --------------------------------------------------------------------------
+// right-hand sides, initial state, expansion
+// --------------------------------------------------------------------------
+// General LinearSystem.solve RHS: b (x-fast, full) -> b_s = b*dinv on free
+// rows; ||b||^2 with pinned rows zeroed goes to sc[r].aux2.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_lift_general(const double* __restrict__ b,
+                                                         const uint8_t* __restrict__ mask,
+                                                         const double* __restrict__ dinv,
+                                                         double* __restrict__ bs, VecArgs va,
+                                                         CgScalars* sc, double* partials,
+                                                         unsigned* counter) {
+  double acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride) {
+    const uint8_t m = mask[i];
+    const double di = dinv[i];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      if (r >= va.nr) continue;
+      const double v = m == 2 ? 0.0 : b[r * va.vs + i];
+      acc[r] += v * v;
+      bs[r * va.vs + i] = m == 0 ? v * di : 0.0;
+    }
+  }
+  block_sum<NR>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (r < va.nr) store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
+  }
+  if (arrive_last(counter)) {
+    for (int r = 0; r < va.nr; ++r) {
+      const double s = final_sum(partials + (int64_t)r * gridDim.x, gridDim.x);
+      if (threadIdx.x == 0) sc[r].aux2 = s;
+    }
+    reset_counter(counter);
+  }
+}
+
+// Capacitance RHS for excitation r (terminal id r at 1 V, other fixed at 0):
+// b_n = sum over neighbours m with tid[m]==r of w(n,m) on free rows, so
+// b_s = dinv * b; ||b_full||^2 = sum b^2 + |terminal r|  -> aux2.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_lift_cap(const fs_op op, const int32_t* __restrict__ tid,
+                                                     const uint8_t* __restrict__ mask,
+                                                     const double* __restrict__ dinv,
+                                                     double* __restrict__ bs, VecArgs va,
+                                                     int j0, CgScalars* sc, double* partials,
+                                                     unsigned* counter) {
+  const Dims d = op.d;
+  double acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < va.n; n += stride) {
+    const int i = (int)(n % d.nx), j = (int)((n / d.nx) % d.ny), k = (int)(n / d.sxy);
+    const int t_self = tid[n];
+    if (mask[n] != 0) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        if (r >= va.nr) continue;
+        bs[r * va.vs + n] = 0.0;
+        if (t_self == j0 + r) acc[r] += 1.0;
+      }
+      continue;
+    }
+    int tn[6];
+    double wn[6];
+    tn[0] = i + 1 < d.nx ? tid[n + 1] : -1;        wn[0] = op.w[0][n];
+    tn[1] = i > 0 ? tid[n - 1] : -1;               wn[1] = i > 0 ? op.w[0][n - 1] : 0.0;
+    tn[2] = j + 1 < d.ny ? tid[n + d.nx] : -1;     wn[2] = op.w[1][n];
+    tn[3] = j > 0 ? tid[n - d.nx] : -1;            wn[3] = j > 0 ? op.w[1][n - d.nx] : 0.0;
+    tn[4] = k + 1 < d.nz ? tid[n + d.sxy] : -1;    wn[4] = op.w[2][n];
+    tn[5] = k > 0 ? tid[n - d.sxy] : -1;           wn[5] = k > 0 ? op.w[2][n - d.sxy] : 0.0;
+    const double di = dinv[n];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      if (r >= va.nr) continue;
+      double v = 0.0;
+#pragma unroll
+      for (int e = 0; e < 6; ++e)
+        if (tn[e] == j0 + r) v += wn[e];
+      acc[r] += v * v;
+      bs[r * va.vs + n] = v * di;
+    }
+  }
+  block_sum<NR>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (r < va.nr) store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
+  }
+  if (arrive_last(counter)) {
+    for (int r = 0; r < va.nr; ++r) {
+      const double s = final_sum(partials + (int64_t)r * gridDim.x, gridDim.x);
+      if (threadIdx.x == 0) sc[r].aux2 = s;
+    }
+    reset_counter(counter);
+  }
+}
+
+// x = 0, p = 0, r = b_s, rho = ||b_s||^2; thresholds.  (FP64 state)
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_cg_init(const double* __restrict__ bs,
+                                                    double* __restrict__ x,
+                                                    double* __restrict__ r_,
+                                                    double* __restrict__ p, VecArgs va,
+                                                    double tol, int maxiter, CgScalars* sc,
+                                                    double* partials, unsigned* counter) {
+  double acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (r >= va.nr) continue;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride) {
+      const double b = bs[r * va.vs + i];
+      x[r * va.vs + i] = 0.0;
+      p[r * va.vs + i] = 0.0;
+      r_[r * va.vs + i] = b;
+      s += b * b;
+    }
+    acc[r] = s;
+  }
+  block_sum<NR>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (r < va.nr) store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
+  }
+  if (arrive_last(counter)) {
+    for (int r = 0; r < va.nr; ++r) {
+      const double rho = final_sum(partials + (int64_t)r * gridDim.x, gridDim.x);
+      if (threadIdx.x == 0) {
+        CgScalars& c = sc[r];
+        c.rho = rho;
+        c.bnorm2 = rho;
+        c.thresh2 = tol * tol * rho;
+        c.beta = 0.0;
+        c.alpha = 0.0;
+        c.iter = 0;
+        c.cycle_iter = 0;
+        c.maxiter = maxiter;
+        c.done = rho == 0.0 ? 1 : 0;
+        c.cycle_done = 0;
+        c.refreshes = 0;
+      }
+    }
+    reset_counter(counter);
+  }
+}
+
+// phi = x_s*dinv on free rows, the Dirichlet value on fixed rows, 0 pinned.
+// Dirichlet values come from b (general solve) or from the terminal map
+// (capacitance: 1 on terminal r, 0 elsewhere).
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_expand(const double* __restrict__ xs,
+                                                   const uint8_t* __restrict__ mask,
+                                                   const double* __restrict__ dinv,
+                                                   const double* __restrict__ b,
+                                                   const int32_t* __restrict__ tid,
+                                                   double* __restrict__ phi, VecArgs va,
+                                                   int j0) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride) {
+    const uint8_t m = mask[i];
+    const double di = dinv[i];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      if (r >= va.nr) continue;
+      double v;
+      if (m == 0) v = xs[r * va.vs + i] * di;
+      else if (m == 2) v = 0.0;
+      else v = b ? b[r * va.vs + i] : (tid[i] == j0 + r ? 1.0 : 0.0);
+      phi[r * va.vs + i] = v;
+    }
+  }
+}
+
+// True equilibrated residual r = b_s - A_s x (FP64 couplings), written to
+// `res`, with ||r||^2 -> aux3 and ||r * dhalf||^2 -> aux1 per RHS.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_residual(const double* __restrict__ bs,
+                                                     const double* __restrict__ x,
+                                                     double* __restrict__ res,
+                                                     const double* __restrict__ cx,
+                                                     const double* __restrict__ cy,
+                                                     const double* __restrict__ cz,
+                                                     const double* __restrict__ dhalf, Dims d,
+                                                     VecArgs va, CgScalars* sc,
+                                                     double* partials, unsigned* counter) {
+  double acc[2 * NR];
+#pragma unroll
+  for (int r = 0; r < 2 * NR; ++r) acc[r] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < va.n; n += stride) {
+    const int i = (int)(n % d.nx), j = (int)((n / d.nx) % d.ny), k = (int)(n / d.sxy);
+    const double cxc = cx[n], cyc = cy[n], czc = cz[n];
+    const double cxm = i > 0 ? cx[n - 1] : 0.0;
+    const double cym = j > 0 ? cy[n - d.nx] : 0.0;
+    const double czm = k > 0 ? cz[n - d.sxy] : 0.0;
+    const double dh = dhalf[n];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      if (r >= va.nr) continue;
+      const double* xr = x + r * va.vs;
+      double s = 0.0;
+      if (i + 1 < d.nx) s += cxc * xr[n + 1];
+      if (i > 0) s += cxm * xr[n - 1];
+      if (j + 1 < d.ny) s += cyc * xr[n + d.nx];
+      if (j > 0) s += cym * xr[n - d.nx];
+      if (k + 1 < d.nz) s += czc * xr[n + d.sxy];
+      if (k > 0) s += czm * xr[n - d.sxy];
+      // Dirichlet and pinned rows are identity rows with x_s = b_s = 0 there
+      const double rv = bs[r * va.vs + n] - (xr[n] - s);
+      res[r * va.vs + n] = rv;
+      acc[r] += rv * rv;
+      acc[NR + r] += (rv * dh) * (rv * dh);
+    }
+  }
+  block_sum<2 * NR>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      if (r >= va.nr) continue;
+      store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
+      store_partial(partials + (int64_t)(va.nr + r) * gridDim.x + blockIdx.x, acc[NR + r]);
+    }
+  }
+  if (arrive_last(counter)) {
+    for (int r = 0; r < va.nr; ++r) {
+      const double s0 = final_sum(partials + (int64_t)r * gridDim.x, gridDim.x);
+      const double s1 = final_sum(partials + (int64_t)(va.nr + r) * gridDim.x, gridDim.x);
+      if (threadIdx.x == 0) {
+        sc[r].aux3 = s0;
+        sc[r].aux1 = s1;
+      }
+    }
+    reset_counter(counter);
+  }
+}
+
+// --------------------------------------------------------------------------
+// mixed precision: FP32 cycles with FP64 residual refresh
+// --------------------------------------------------------------------------
+// Start (or restart) an FP32 cycle from the FP64 residual r64:
+// scale = 1/||r64||, r32 = r64*scale, d32 = p32 = 0, rho32 = ||r32||^2 (aux0),
+// cycle threshold aux1 = max(inner_tol^2, 0.25*thresh2/rho64) * rho32.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_cycle_start(const double* __restrict__ r64,
+                                                        float* __restrict__ r32,
+                                                        float* __restrict__ d32,
+                                                        float* __restrict__ p32, VecArgs va,
+                                                        double inner_tol2, CgScalars* sc,
+                                                        double* partials, unsigned* counter) {
+  double acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (!(r < va.nr && sc[r].done == 0)) continue;
+    const double sc_r = 1.0 / sqrt(sc[r].rho);
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride) {
+      const float v = (float)(r64[r * va.vs + i] * sc_r);
+      r32[r * va.vs + i] = v;
+      d32[r * va.vs + i] = 0.0f;
+      p32[r * va.vs + i] = 0.0f;
+      s += (double)v * (double)v;
+    }
+    acc[r] = s;
+  }
+  block_sum<NR>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (r < va.nr) store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
+  }
+  if (arrive_last(counter)) {
+    for (int r = 0; r < va.nr; ++r) {
+      const double s = final_sum(partials + (int64_t)r * gridDim.x, gridDim.x);
+      if (threadIdx.x == 0 && sc[r].done == 0) {
+        CgScalars& c = sc[r];
+        c.scale = 1.0 / sqrt(c.rho);
+        c.aux0 = s;
+        c.aux1 = fmax(inner_tol2, 0.25 * c.thresh2 / c.rho) * s;
+        c.beta = 0.0;
+        c.cycle_iter = 0;
+        c.cycle_done = 0;
+      }
+    }
+    reset_counter(counter);
+  }
+}
+
+// x64 += d32/scale for RHS whose cycle ended (then refreshed residual next)
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_cycle_accumulate(double* __restrict__ x64,
+                                                             const float* __restrict__ d32,
+                                                             VecArgs va, const CgScalars* sc) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (!(r < va.nr && sc[r].done == 0)) continue;
+    const double inv = 1.0 / sc[r].scale;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride)
+      x64[r * va.vs + i] += (double)d32[r * va.vs + i] * inv;
+  }
+}
+
+// After k_residual wrote r64 and aux3 = ||r64||^2: new rho, refresh count,
+// convergence test (done=1), iteration cap (done=2).
+__global__ void k_refresh_finish(int nr, CgScalars* sc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int r = 0; r < nr; ++r) {
+    CgScalars& c = sc[r];
+    if (c.done != 0) continue;
+    c.rho = c.aux3;
+    c.refreshes += 1;
+    if (c.rho < c.thresh2) c.done = 1;
+    else if (c.iter >= c.maxiter) c.done = 2;
+    else if (!isfinite(c.rho)) c.done = 3;
+  }
+}
+
+// --------------------------------------------------------------------------
+// capacitance: C[i][r] = sum_{n in terminal i} (L phi_r)_n
+// --------------------------------------------------------------------------
+// grid: (blocks_x, nbins); node lists per bin (x-fast indices) in CSR form.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_charge(const fs_op op, const double* __restrict__ phi,
+                                                   const int64_t* __restrict__ nodes,
+                                                   const int64_t* __restrict__ offs, VecArgs va,
+                                                   double* __restrict__ partials) {
+  const Dims d = op.d;
+  const int bin = blockIdx.y;
+  const int64_t lo = offs[bin], hi = offs[bin + 1];
+  double acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+  for (int64_t t = lo + (int64_t)blockIdx.x * kBlock + threadIdx.x; t < hi;
+       t += (int64_t)gridDim.x * kBlock) {
+    const int64_t n = nodes[t];
+    const int i = (int)(n % d.nx), j = (int)((n / d.nx) % d.ny), k = (int)(n / d.sxy);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      if (r >= va.nr) continue;
+      acc[r] += raw_row(phi + r * va.vs, op.w[0], op.w[1], op.w[2], op.extra, d, i, j, k, n);
+    }
+  }
+  block_sum<NR>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (r < va.nr)
+        partials[((int64_t)bin * va.nr + r) * gridDim.x + blockIdx.x] = acc[r];
+  }
+}
+
+// one block: out[bin*nr + r] = ordered sum of the k_charge partials
+__global__ __launch_bounds__(kBlock) void k_charge_final(const double* __restrict__ partials,
+                                                         int nbins, int nr, int nbx,
+                                                         double* __restrict__ out) {
+  for (int br = 0; br < nbins * nr; ++br) {
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < nbx; b += kBlock) v[0] += partials[(int64_t)br * nbx + b];
+    block_sum<1>(v);
+    if (threadIdx.x == 0) out[br] = v[0];
+  }
+}
+
+// --------------------------------------------------------------------------
+// host launchers
+// --------------------------------------------------------------------------
+template <int NR>
+static void launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {
+  hipStream_t st = s->ctx->stream;
+  VecArgs va{s->d.n, s->vstride, nr};
+  const int vb = s->vec_blocks;
+  const int sb = s->tl.nblocks;
+  unsigned long long* prof = ea ? s->prof_dev : nullptr;
+  if (mode == 0) {
+    hipLaunchKernelGGL((k_pupdate<double, NR>), dim3(vb), dim3(kBlock), 0, st, s->p, s->r, va,
+                       mode, s->sc);
+    if (ea) (void)hipEventRecord(ea, st);
+    hipLaunchKernelGGL((k_stencil_dot<double, NR>), dim3(sb), dim3(kBlock), 0, st, s->p, s->q,
+                       s->c64[0], s->c64[1], s->c64[2], s->d, s->tl, va, mode, s->sc,
+                       s->partials, s->counters + 0, prof);
+    if (eb) (void)hipEventRecord(eb, st);
+    hipLaunchKernelGGL((k_update<double, NR>), dim3(vb), dim3(kBlock), 0, st, s->x, s->r, s->p,
+                       s->q, va, mode, s->sc, s->partials, s->counters + 1);
+  } else {
+    hipLaunchKernelGGL((k_pupdate<float, NR>), dim3(vb), dim3(kBlock), 0, st, s->p32, s->r32,
+                       va, mode, s->sc);
+    if (ea) (void)hipEventRecord(ea, st);
+    hipLaunchKernelGGL((k_stencil_dot<float, NR>), dim3(sb), dim3(kBlock), 0, st, s->p32,
+                       s->q32, s->c32[0], s->c32[1], s->c32[2], s->d, s->tl, va, mode, s->sc,
+                       s->partials, s->counters + 0, prof);
+    if (eb) (void)hipEventRecord(eb, st);
+    hipLaunchKernelGGL((k_update<float, NR>), dim3(vb), dim3(kBlock), 0, st, s->x32, s->r32,
+                       s->p32, s->q32, va, mode, s->sc, s->partials, s->counters + 1);
+  }
+}
+
+#define FS_DISPATCH_NR(nr, FN, ...)        \
+  do {                                     \
+    if ((nr) <= 1) FN<1>(__VA_ARGS__);     \
+    else if ((nr) <= 2) FN<2>(__VA_ARGS__); \
+    else if ((nr) <= 4) FN<4>(__VA_ARGS__); \
+    else FN<8>(__VA_ARGS__);               \
+  } while (0)
+
+void launch_iteration(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {
+  FS_DISPATCH_NR(nr, launch_iteration_t, s, nr, mode, ea, eb);
+}
+
+template <int NR>
+static void launch_init_t(fs_sys* s, int nr, double tol, int maxiter) {
+  VecArgs va{s->d.n, s->vstride, nr};
+  hipLaunchKernelGGL((k_cg_init<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream,
+                     s->bs, s->x, s->r, s->p, va, tol, maxiter, s->sc, s->partials,
+                     s->counters + 2);
+}
+void launch_init(fs_sys* s, int nr, double tol, int maxiter) {
+  FS_DISPATCH_NR(nr, launch_init_t, s, nr, tol, maxiter);
+}
+
+template <int NR>
+static void launch_lift_general_t(fs_sys* s, int nr, const double* b) {
+  VecArgs va{s->d.n, s->vstride, nr};
+  hipLaunchKernelGGL((k_lift_general<NR>), dim3(s->vec_blocks), dim3(kBlock), 0,
+                     s->ctx->stream, b, s->mask, s->dinv, s->bs, va, s->sc, s->partials,
+                     s->counters + 3);
+}
+void launch_lift_general(fs_sys* s, int nr, const double* b) {
+  FS_DISPATCH_NR(nr, launch_lift_general_t, s, nr, b);
+}
+
+template <int NR>
+static void launch_lift_cap_t(fs_sys* s, int nr, const fs_op* L, const int32_t* tid, int j0) {
+  VecArgs va{s->d.n, s->vstride, nr};
+  hipLaunchKernelGGL((k_lift_cap<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream,
+                     *L, tid, s->mask, s->dinv, s->bs, va, j0, s->sc, s->partials,
+                     s->counters + 4);
+}
+void launch_lift_cap(fs_sys* s, int nr, const fs_op* L, const int32_t* tid, int j0) {
+  FS_DISPATCH_NR(nr, launch_lift_cap_t, s, nr, L, tid, j0);
+}
+
+template <int NR>
+static void launch_expand_t(fs_sys* s, int nr, const double* b, const int32_t* tid,
+                            int j0, double* phi) {
+  VecArgs va{s->d.n, s->vstride, nr};
+  hipLaunchKernelGGL((k_expand<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream,
+                     s->x, s->mask, s->dinv, b, tid, phi, va, j0);
+}
+void launch_expand(fs_sys* s, int nr, const double* b, const int32_t* tid, int j0,
+                   double* phi) {
+  FS_DISPATCH_NR(nr, launch_expand_t, s, nr, b, tid, j0, phi);
+}
+
+template <int NR>
+static void launch_residual_t(fs_sys* s, int nr, double* res) {
+  VecArgs va{s->d.n, s->vstride, nr};
+  hipLaunchKernelGGL((k_residual<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream,
+                     s->bs, s->x, res, s->c64[0], s->c64[1], s->c64[2], s->dhalf, s->d, va,
+                     s->sc, s->partials, s->counters + 5);
+}
+void launch_residual(fs_sys* s, int nr, double* res) {
+  FS_DISPATCH_NR(nr, launch_residual_t, s, nr, res);
+}
+
+template <int NR>
+static void launch_cycle_start_t(fs_sys* s, int nr, double inner_tol2) {
+  VecArgs va{s->d.n, s->vstride, nr};
+  hipLaunchKernelGGL((k_cycle_start<NR>), dim3(s->vec_blocks), dim3(kBlock), 0,
+                     s->ctx->stream, s->r, s->r32, s->x32, s->p32, va, inner_tol2, s->sc,
+                     s->partials, s->counters + 6);
+}
+void launch_cycle_start(fs_sys* s, int nr, double inner_tol2) {
+  FS_DISPATCH_NR(nr, launch_cycle_start_t, s, nr, inner_tol2);
+}
+
+template <int NR>
+static void launch_refresh_t(fs_sys* s, int nr) {
+  VecArgs va{s->d.n, s->vstride, nr};
+  hipLaunchKernelGGL((k_cycle_accumulate<NR>), dim3(s->vec_blocks), dim3(kBlock), 0,
+                     s->ctx->stream, s->x, s->x32, va, s->sc);
+  launch_residual_t<NR>(s, nr, s->r);
+  hipLaunchKernelGGL(k_refresh_finish, dim3(1), dim3(64), 0, s->ctx->stream, nr, s->sc);
+}
+void launch_refresh(fs_sys* s, int nr) { FS_DISPATCH_NR(nr, launch_refresh_t, s, nr); }
+
+template <int NR>
+static void launch_charge_t(fs_sys* s, int nr, const fs_op* L, const double* phi,
+                            const int64_t* nodes, const int64_t* offs, int nbins, int nbx,
+                            double* out) {
+  VecArgs va{s->d.n, s->vstride, nr};
+  hipLaunchKernelGGL((k_charge<NR>), dim3(nbx, nbins), dim3(kBlock), 0, s->ctx->stream, *L,
+                     phi, nodes, offs, va, s->partials);
+  hipLaunchKernelGGL(k_charge_final, dim3(1), dim3(kBlock), 0, s->ctx->stream, s->partials,
+                     nbins, nr, nbx, out);
+}
+void launch_charge(fs_sys* s, int nr, const fs_op* L, const double* phi, const int64_t* nodes,
+                   const int64_t* offs, int nbins, int nbx, double* out) {
+  FS_DISPATCH_NR(nr, launch_charge_t, s, nr, L, phi, nodes, offs, nbins, nbx, out);
+}
+
+}  // namespace fs

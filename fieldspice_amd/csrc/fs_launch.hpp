@@ -1,0 +1,40 @@
+// Host-side launchers of the kernels in fs_kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fs_internal.hpp"
+
+namespace fs {
+
+template <typename T>
+void permute_in(hipStream_t s, const T* in, T* out, int A, int B, int C, int OA, int OB);
+template <typename T>
+void permute_out(hipStream_t s, const T* in, T* out, int A, int B, int C, int IA, int IB);
+
+void cell_to_edge(hipStream_t s, const double* prop, const double* const h[3],
+                  const double* const hd[3], const int N[3], const Dims& d, int dir, int mode,
+                  double* w);
+void axpby(hipStream_t s, int64_t n, double a, const double* x, double b, const double* y,
+           double* out);
+void apply_raw(hipStream_t s, const fs_op* op, const double* x, double* y);
+
+// returns 0 ok, 1 singular with pin == 0, -1 HIP failure
+int setup_system(fs_sys* sys, double* dsum, void* counts_dev, void* counts_host, int pin);
+constexpr size_t kSetupCountsBytes = 64;
+
+void launch_iteration(fs_sys* s, int nr, int mode, hipEvent_t ev_a, hipEvent_t ev_b);
+void launch_init(fs_sys* s, int nr, double tol, int maxiter);
+void launch_lift_general(fs_sys* s, int nr, const double* b);
+void launch_lift_cap(fs_sys* s, int nr, const fs_op* L, const int32_t* tid, int j0);
+void launch_expand(fs_sys* s, int nr, const double* b, const int32_t* tid, int j0,
+                   double* phi);
+void launch_residual(fs_sys* s, int nr, double* res);
+void launch_cycle_start(fs_sys* s, int nr, double inner_tol2);
+void launch_refresh(fs_sys* s, int nr);
+void launch_charge(fs_sys* s, int nr, const fs_op* L, const double* phi, const int64_t* nodes,
+                   const int64_t* offs, int nbins, int nbx, double* out);
+
+}  // namespace fs

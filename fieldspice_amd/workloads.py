@@ -1,0 +1,91 @@
+"""Seeded extraction geometries: the parallel plate and layered multi-conductor
+problems the solver is measured on.
+
+``layered_boxes`` builds a graded grid, a stack of dielectric layers whose
+interfaces sit on node planes, and rectangular conductor boxes placed by
+rejection sampling with at least one free cell between boxes (so terminal node
+sets stay disjoint, as ``fieldspice/solvers/poisson.py:780-786`` requires).
+Each box becomes a terminal owning every node of its closed box; its cells keep
+the permittivity of the layer they sit in.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .grid import RectilinearGrid
+from .solvers.base import Terminal
+from .units import eps0
+
+__all__ = ["Extraction", "parallel_plate", "layered_boxes", "c2_geometry"]
+
+
+@dataclass
+class Extraction:
+    grid: RectilinearGrid
+    eps_cell: np.ndarray
+    terminals: list
+    boxes: list          # (i0, i1, j0, j1, k0, k1) cell ranges, half-open
+    layer_eps_r: list
+    seed: int
+
+
+def parallel_plate(n=(24, 10, 8), extent=(2e-6, 5e-6, 4e-6), eps_r=3.9) -> Extraction:
+    """Plates on the x=0 (1 V) and x=last (0 V) node planes of a uniform grid."""
+    g = RectilinearGrid.uniform([(0, extent[0]), (0, extent[1]), (0, extent[2])], n)
+    nid = g.node_ids()
+    terms = [Terminal("hi", nid[0].ravel(), voltage=1.0),
+             Terminal("lo", nid[-1].ravel(), voltage=0.0)]
+    return Extraction(g, np.full(g.shape_cells, eps_r * eps0), terms, [], [eps_r], 0)
+
+
+def layered_boxes(nodes=(192, 192, 96), h_xy=50e-9, z_first=20e-9, z_last=400e-9,
+                  n_layers=3, eps_choices=(2.7, 3.9, 7.5), n_boxes=8,
+                  side=(4, 40), seed=0, max_tries=100000) -> Extraction:
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = nodes
+    Nx, Ny, Nz = nx - 1, ny - 1, nz - 1
+    x = np.arange(nx) * h_xy
+    y = np.arange(ny) * h_xy
+    if Nz > 1:
+        ratio = (z_last / z_first) ** (1.0 / (Nz - 1))
+        hz = z_first * ratio ** np.arange(Nz)
+    else:
+        hz = np.array([z_first])
+    z = np.concatenate([[0.0], np.cumsum(hz)])
+    g = RectilinearGrid(x, y, z)
+    # layer interfaces on node planes (distinct, strictly inside)
+    cuts = np.sort(rng.choice(np.arange(1, Nz), size=n_layers - 1, replace=False))
+    bounds = np.concatenate([[0], cuts, [Nz]])
+    layer_eps = [float(rng.choice(eps_choices)) for _ in range(n_layers)]
+    eps_r = np.empty(g.shape_cells)
+    for L in range(n_layers):
+        eps_r[:, :, bounds[L]:bounds[L + 1]] = layer_eps[L]
+    occ = np.zeros(g.shape_cells, dtype=bool)  # cells claimed incl. a 1-cell gap
+    boxes = []
+    lo, hi = side
+    tries = 0
+    while len(boxes) < n_boxes:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("could not place the conductor boxes; enlarge the grid")
+        s = [int(rng.integers(lo, min(hi, N - 2) + 1)) for N in (Nx, Ny, Nz)]
+        o = [int(rng.integers(1, N - sd)) for N, sd in zip((Nx, Ny, Nz), s)]
+        sl = tuple(slice(max(a - 1, 0), a + b + 1) for a, b in zip(o, s))
+        if occ[sl].any():
+            continue
+        occ[sl] = True
+        boxes.append((o[0], o[0] + s[0], o[1], o[1] + s[1], o[2], o[2] + s[2]))
+    nid = g.node_ids()
+    terms = []
+    for m, (i0, i1, j0, j1, k0, k1) in enumerate(boxes):
+        terms.append(Terminal(f"c{m}", nid[i0:i1 + 1, j0:j1 + 1, k0:k1 + 1].ravel()))
+    return Extraction(g, eps_r * eps0, terms, boxes, layer_eps, seed)
+
+
+def c2_geometry(seed: int = 0, nodes=(192, 192, 96)) -> Extraction:
+    """192x192x96 nodes, x/y 50 nm, z log-graded 20 -> 400 nm, three layers with
+    eps_r from {2.7, 3.9, 7.5}, eight boxes of 4-40 cells per side."""
+    return layered_boxes(nodes=nodes, seed=seed)

@@ -1,0 +1,165 @@
+/*
+ * fieldspice_hip.h -- C ABI of the MI355X (gfx950) elliptic field solve.
+ *
+ * The library solves the node-Laplacian systems fieldspice builds as
+ *     A = G^T diag(w) G + diag(extra)          (w >= 0 per edge)
+ * with Dirichlet rows eliminated symmetrically, entirely on one GPU and
+ * matrix-free: the operator is stored as three per-edge weight arrays, never
+ * as a CSR matrix.
+ *
+ * Conventions (all entry points):
+ *   - Host arrays use fieldspice's layout: C order over (i, j, k) with k (z)
+ *     fastest.  Node arrays have (Nx+1)*(Ny+1)*(Nz+1) entries; x-edge arrays
+ *     (Nx, Ny+1, Nz+1), y-edge (Nx+1, Ny, Nz+1), z-edge (Nx+1, Ny+1, Nz).
+ *     The library permutes to its own x-fastest device layout on upload and
+ *     back on download.
+ *   - Host pointers are caller-owned and only read (or written, for outputs)
+ *     during the call.  Inputs are never modified.
+ *   - Return value: FS_OK (0), FS_NOT_CONVERGED (1), or a negative error code;
+ *     fs_last_error() describes the most recent failure of the calling thread.
+ *   - A context and the objects created from it may be used by one host
+ *     thread at a time.  Work runs on the context's own stream (default
+ *     priority).  No C++ exception crosses this boundary.
+ */
+#ifndef FIELDSPICE_HIP_H
+#define FIELDSPICE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FS_OK 0
+#define FS_NOT_CONVERGED 1
+#define FS_EINVAL (-1)
+#define FS_EHIP (-2)
+#define FS_ENOMEM (-3)
+
+/* cell -> edge averaging rules of fieldspice/operators.py:144-237 */
+#define FS_AVG_PARALLEL 0
+#define FS_AVG_HARMONIC 1
+#define FS_AVG_MIN 2
+#define FS_AVG_MAX 3
+
+/* node mask values (fs_sys_create) */
+#define FS_NODE_FREE 0
+#define FS_NODE_DIRICHLET 1
+
+/* precision of the Krylov iteration */
+#define FS_PREC_FP64 0          /* FP64 stencil and vectors throughout */
+#define FS_PREC_MIXED 1         /* FP32 stencil/vectors, FP64 residual refresh and dots */
+
+typedef struct fs_ctx fs_ctx;
+typedef struct fs_op fs_op;
+typedef struct fs_sys fs_sys;
+typedef struct fs_cap fs_cap;
+
+typedef struct {
+    double tol;          /* relative residual ||r_s|| < tol*||b_s|| in the equilibrated space */
+    int maxiter;         /* <= 0: max(500, floor(10*sqrt(n_free))) as the reference */
+    int precision;       /* FS_PREC_FP64 or FS_PREC_MIXED */
+    double inner_tol;    /* FS_PREC_MIXED: residual drop per FP32 cycle before an FP64 refresh */
+    int check_every;     /* iterations between host convergence checks (<= 0: adaptive) */
+} fs_solve_params;
+
+typedef struct {
+    int iterations;      /* Krylov iterations (maximum over right-hand sides) */
+    int refreshes;       /* FP64 residual recomputations (mixed precision) */
+    int converged;       /* 1 when every right-hand side met tol */
+    double residual;     /* fieldspice's info["residual"]: ||D^1/2 (b_s - A_s x_s)|| / ||b|| */
+    double residual_eq;  /* FP64 recomputed ||b_s - A_s x_s|| / ||b_s|| (max over RHS) */
+    double solve_time;   /* seconds, device work of this call */
+    /* with fs_sys_set_profiling(sys, 1): the stencil kernel of the Krylov loop */
+    double stencil_time;      /* seconds summed over its launches (HIP events) */
+    int64_t stencil_launches; /* launches, including ones that found no live RHS */
+    double stencil_bytes;     /* algorithmic bytes: couplings once + p read + q write per live RHS */
+} fs_solve_stats;
+
+typedef struct {
+    int64_t n_nodes;
+    int64_t n_free;      /* unknowns carried into the Krylov solve */
+    int64_t n_fixed;     /* Dirichlet rows */
+    int64_t n_empty;     /* rows coupled to nothing, pinned to 0 */
+    int64_t gauge;       /* node pinned to fix the gauge of a floating system, or -1 */
+    int singular;        /* 1 if the operator had the constant null vector */
+} fs_sys_info;
+
+/* ---- context ------------------------------------------------------------ */
+int fs_ctx_create(int device, fs_ctx** out);
+int fs_ctx_destroy(fs_ctx* ctx);
+const char* fs_last_error(void);
+/* device name and compute-unit count, for reports */
+int fs_ctx_device_info(fs_ctx* ctx, char* name, int name_len, int* n_cu);
+
+/* ---- operators: A = G^T diag(w) G + diag(extra) ------------------------- */
+/* Replaces edge_mass + nodal_laplacian (fieldspice/operators.py:302-315,
+ * 340-351) for given edge weights.  extra may be NULL. */
+int fs_op_create_edges(fs_ctx* ctx, int Nx, int Ny, int Nz,
+                       const double* w_x, const double* w_y, const double* w_z,
+                       const double* extra, fs_op** out);
+
+/* Replaces cell_to_edge + edge_mass (fieldspice/operators.py:144-237,302-315):
+ * w_e = avg(prop over the <=4 cells around e) * A_dual(e) / L(e), built on the
+ * device from per-cell prop (Nx,Ny,Nz) and the primal cell widths hx,hy,hz. */
+int fs_op_create_cells(fs_ctx* ctx, int Nx, int Ny, int Nz,
+                       const double* hx, const double* hy, const double* hz,
+                       const double* prop_cell, int avg_mode, fs_op** out);
+
+/* out = a*A + b*B + diag(c*extra): the EQS/thermal step matrices
+ * (fieldspice/solvers/eqs.py:279-280, thermal.py:259-263).  B and extra may
+ * be NULL. */
+int fs_op_combine(double a, const fs_op* A, double b, const fs_op* B,
+                  double c, const double* extra, fs_op** out);
+int fs_op_destroy(fs_op* op);
+
+/* y = A x (node vectors, host layout): L @ phi (fieldspice/solvers/poisson.py:1219-1221) */
+int fs_op_apply(fs_op* op, const double* x, double* y);
+/* edge weights back to host (edge layouts above), for inspection */
+int fs_op_get_edges(fs_op* op, double* w_x, double* w_y, double* w_z);
+
+/* ---- prepared solves: LinearSystem (fieldspice/solvers/poisson.py:260-436) */
+/* mask: per node FS_NODE_FREE / FS_NODE_DIRICHLET (host layout), may be NULL.
+ * Dirichlet rows become identity rows (apply_dirichlet, operators.py:362-407).
+ * Empty rows are pinned to 0; a floating system (no Dirichlet row, zero row
+ * sums) gets its gauge node pinned when pin != 0, else FS_EINVAL. */
+int fs_sys_create(fs_op* op, const uint8_t* mask, int pin, fs_sys** out,
+                  fs_sys_info* info);
+int fs_sys_destroy(fs_sys* sys);
+
+/* LinearSystem.solve (fieldspice/solvers/poisson.py:487-589) for nrhs
+ * right-hand sides at once: b and x are nrhs node vectors back to back.
+ * b holds the Dirichlet values on Dirichlet rows (the output of
+ * apply_dirichlet); x0 (may be NULL) is an initial guess.  Returns
+ * FS_NOT_CONVERGED when maxiter is reached. */
+int fs_sys_solve(fs_sys* sys, int nrhs, const double* b, const double* x0,
+                 double* x, const fs_solve_params* params, fs_solve_stats* stats);
+
+/* PoissonSolver.capacitance_matrix inner loop
+ * (fieldspice/solvers/poisson.py:1208-1223): sys was created from op L with
+ * every terminal and shield node marked Dirichlet.  term_id per node:
+ * -1 free, 0..nt-1 terminal, nt = grounded shield.  For each j, terminal j at
+ * 1 V and all other fixed nodes at 0 V are solved (all nt at once) and
+ * C[i*nt + j] = sum over terminal i of (L phi_j), q_shield[j] the shield
+ * charge.  C is returned unsymmetrised. phi_out (nt node vectors) may be
+ * NULL. */
+int fs_capacitance(fs_sys* sys, const fs_op* L, int nt, const int32_t* term_id,
+                   double* C, double* q_shield, double* phi_out,
+                   const fs_solve_params* params, fs_solve_stats* stats);
+
+/* The same extraction split into a setup (terminal map and node lists
+ * resident on the device) and repeated solves, so that repeated extractions
+ * of one geometry pay the setup once. */
+int fs_cap_prepare(fs_sys* sys, int nt, const int32_t* term_id, fs_cap** out);
+int fs_cap_solve(fs_cap* cap, const fs_op* L, double* C, double* q_shield, double* phi_out,
+                 const fs_solve_params* params, fs_solve_stats* stats);
+int fs_cap_destroy(fs_cap* cap);
+
+/* Record HIP events around every stencil launch of the Krylov loop of this
+ * system and report their summed duration in fs_solve_stats (off by default). */
+int fs_sys_set_profiling(fs_sys* sys, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FIELDSPICE_HIP_H */
