@@ -161,7 +161,6 @@ void free_sys(fs_sys* s) {
   for (auto& c : s->c64) dfree(c);
   for (auto& c : s->c32) dfree(c);
   dfree(s->partials);
-  dfree(s->counters);
   dfree(s->sc);
   dfree(s->prof_dev);
   if (s->sc_host) (void)hipHostFree(s->sc_host);
@@ -562,12 +561,12 @@ int fs_sys_create(fs_op* op, const uint8_t* mask, int pin, fs_sys** out, fs_sys_
   s->d = op->d;
   const fs::Dims& d = s->d;
   s->vstride = (d.n + 63) / 64 * 64;
-  // stencil tiling: ~4K tiles, at least 4 planes per tile
+  // stencil tiling: ~2K tiles, at least 8 planes per tile
   fs::Tiling& tl = s->tl;
   tl.tiles_x = (d.nx + fs::kTileX - 1) / fs::kTileX;
   tl.tiles_y = (d.ny + fs::kTileY - 1) / fs::kTileY;
   const int64_t txy = (int64_t)tl.tiles_x * tl.tiles_y;
-  tl.kz = (int)std::max<int64_t>(4, (d.nz * txy + 4095) / 4096);
+  tl.kz = (int)std::max<int64_t>(8, (d.nz * txy + 2047) / 2048);
   tl.kz = std::min(tl.kz, d.nz);
   tl.tiles_z = (d.nz + tl.kz - 1) / tl.kz;
   tl.ntiles = (int)(txy * tl.tiles_z);
@@ -584,7 +583,6 @@ int fs_sys_create(fs_op* op, const uint8_t* mask, int pin, fs_sys** out, fs_sys_
   if ((rc = dalloc(&s->mask, (size_t)d.n)) || (rc = dalloc(&s->dhalf, (size_t)d.n)) ||
       (rc = dalloc(&s->dinv, (size_t)d.n)) || (rc = dalloc(&dsum, (size_t)d.n)) ||
       (rc = dalloc(&s->partials, (size_t)s->partials_cap)) ||
-      (rc = dalloc(&s->counters, (size_t)fs::kNumCounters)) ||
       (rc = dalloc(&s->sc, (size_t)fs::kMaxRhs)) ||
       (rc = dalloc((char**)&cnt_dev, fs::kSetupCountsBytes))) {
     dfree(dsum);
@@ -599,8 +597,6 @@ int fs_sys_create(fs_op* op, const uint8_t* mask, int pin, fs_sys** out, fs_sys_
     rc = fail(FS_ENOMEM, "pinned allocation failed");
   if (rc == FS_OK && hipHostMalloc(&cnt_host, fs::kSetupCountsBytes) != hipSuccess)
     rc = fail(FS_ENOMEM, "pinned allocation failed");
-  if (rc == FS_OK && hipMemset(s->counters, 0, sizeof(unsigned) * fs::kNumCounters) != hipSuccess)
-    rc = fail(FS_EHIP, "counter init failed");
   if (rc == FS_OK && hipMemset(s->sc, 0, sizeof(fs::CgScalars) * fs::kMaxRhs) != hipSuccess)
     rc = fail(FS_EHIP, "scalar init failed");
   if (rc == FS_OK) {

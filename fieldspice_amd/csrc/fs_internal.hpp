@@ -17,7 +17,6 @@ constexpr int kBlock = 256;       // threads per block for every kernel here
 constexpr int kMaxRhs = 8;        // right-hand sides one launch carries
 constexpr int kTileX = 64;        // stencil tile: one wavefront along x ...
 constexpr int kTileY = 4;         // ... times four rows
-constexpr int kNumCounters = 16;  // last-block counters per system
 
 // Device layout: node (i, j, k) of an (nx, ny, nz) node grid lives at
 // n = (k*ny + j)*nx + i (x fastest).  Every node vector of a system is padded
@@ -39,8 +38,8 @@ struct Tiling {
   int nblocks;     // ntiles rounded up to a multiple of 8
 };
 
-// Per right-hand-side Krylov scalars, written only by the last block of a
-// reduction and read by every block of the following kernels.
+// Per right-hand-side Krylov scalars, written only by the single-block
+// finalisation kernels and read by every block of the following kernels.
 struct CgScalars {
   double rho;       // r.r of the current residual
   double pq;        // p.q
@@ -100,7 +99,6 @@ struct fs_sys {
   double* node_tmp = nullptr;        // host-layout staging (nr node vectors)
   double* partials = nullptr;        // reduction partials
   int64_t partials_cap = 0;
-  unsigned* counters = nullptr;      // last-block counters
   fs::CgScalars* sc = nullptr;       // device scalars (kMaxRhs)
   fs::CgScalars* sc_host = nullptr;  // pinned host mirror
   int vec_blocks = 0;                // grid of the node-wise vector kernels

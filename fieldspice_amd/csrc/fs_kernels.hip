@@ -9,10 +9,10 @@
 // diagonal: the equilibrated free block S A_ff S of
 // fieldspice/solvers/poisson.py:364-381, with no mask or diagonal stream.
 //
-// Reductions are deterministic: each block writes one partial per quantity,
-// and the last block to arrive (atomic ticket) sums the partials in index
-// order and updates the per-RHS scalars on the device, so the host never has
-// to synchronise inside the Krylov loop.
+// Reductions are deterministic: each block writes one partial per quantity
+// and a single-block finalisation kernel sums them in index order and updates
+// the per-RHS scalars on the device, so the host never has to synchronise
+// inside the Krylov loop.
 
 #include <hip/hip_runtime.h>
 
@@ -55,42 +55,32 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
   __syncthreads();
 }
 
-__device__ __forceinline__ void store_partial(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_partial(const double* p) {
-  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-}
+__device__ __forceinline__ void store_partial(double* p, double v) { *p = v; }
 
-// Called by every thread after thread 0 stored this block's partials.
-// Returns true in every thread of the last block to arrive.
-__device__ __forceinline__ bool arrive_last(unsigned* counter) {
-  __shared__ int s_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __atomic_thread_fence(__ATOMIC_RELEASE);
-    unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (prev == gridDim.x * gridDim.y - 1);
+// out[r] = sum over b of partials[r*nb + b] for r < nr (NR >= nr), each in a
+// fixed order, all quantities in one pass.  Results valid in thread 0.
+template <int NR>
+__device__ __forceinline__ void final_sums(const double* __restrict__ partials, int nb, int nr,
+                                           double (&out)[NR]) {
+#pragma unroll
+  for (int r = 0; r < NR; ++r) out[r] = 0.0;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (r >= nr) continue;
+    const double* pr = partials + (int64_t)r * nb;
+    // four independent chains keep four loads in flight per thread
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int b = threadIdx.x;
+    for (; b + 3 * kBlock < nb; b += 4 * kBlock) {
+      a0 += pr[b];
+      a1 += pr[b + kBlock];
+      a2 += pr[b + 2 * kBlock];
+      a3 += pr[b + 3 * kBlock];
+    }
+    for (; b < nb; b += kBlock) a0 += pr[b];
+    out[r] = (a0 + a1) + (a2 + a3);
   }
-  __syncthreads();
-  const bool last = s_last != 0;
-  if (last) __atomic_thread_fence(__ATOMIC_ACQUIRE);
-  return last;
-}
-
-__device__ __forceinline__ void reset_counter(unsigned* counter) {
-  if (threadIdx.x == 0)
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// In the last block: sum partials[0..nb) in a fixed order (result in thread 0).
-__device__ __forceinline__ double final_sum(const double* partials, int nb) {
-  double v[1] = {0.0};
-  for (int b = threadIdx.x; b < nb; b += kBlock) v[0] += load_partial(partials + b);
-  block_sum<1>(v);
-  return v[0];
+  block_sum<NR>(out);
 }
 
 // --------------------------------------------------------------------------
@@ -468,7 +458,7 @@ __device__ __forceinline__ bool tile_of_block(const Tiling& tl, int& tx, int& ty
 }
 
 // q = A_s p for each live RHS (z-marching, couplings read once for all RHS),
-// fused with p.q; the last block sets alpha = rho / (p.q).
+// fused with the block partials of p.q (alpha is set by k_finalize).
 template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_stencil_dot(const T* __restrict__ p, T* __restrict__ q,
                                                         const T* __restrict__ cx,
@@ -476,12 +466,17 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot(const T* __restrict__ p,
                                                         const T* __restrict__ cz, Dims d,
                                                         Tiling tl, VecArgs va, int mode,
                                                         CgScalars* sc, double* partials,
-                                                        unsigned* counter,
                                                         unsigned long long* prof) {
   if (all_idle(sc, va.nr, mode)) return;
   bool live[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) live[r] = r < va.nr && rhs_live(sc, r, mode);
+  if (prof && blockIdx.x == 0 && threadIdx.x == 0) {  // launches with work, live RHS summed
+    unsigned long long nl = 0;
+    for (int r = 0; r < NR; ++r) nl += live[r] ? 1ull : 0ull;
+    prof[0] += 1ull;
+    prof[1] += nl;
+  }
   double acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) acc[r] = 0.0;
@@ -495,40 +490,36 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot(const T* __restrict__ p,
     if (i < d.nx && j < d.ny) {
       const int64_t base = (int64_t)j * d.nx + i;
       const bool has_e = i + 1 < d.nx, has_w = i > 0, has_n = j + 1 < d.ny, has_s = j > 0;
-      int64_t nn = (int64_t)k0 * d.sxy + base;
-      T czm = k0 > 0 ? cz[nn - d.sxy] : T(0);
-      T pm[NR], pc[NR];
+      // One right-hand side at a time through the tile: the tile's couplings
+      // come from HBM for the first live RHS and from L1/L2 for the others,
+      // and each pass is the register-light single-RHS z-march.
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
-        pm[r] = T(0);
-        pc[r] = T(0);
-        if (live[r]) {
-          const T* pr = p + r * va.vs;
-          pm[r] = k0 > 0 ? pr[nn - d.sxy] : T(0);
-          pc[r] = pr[nn];
-        }
-      }
-      for (int k = k0; k < k1; ++k, nn += d.sxy) {
-        const T cxc = cx[nn], czc = cz[nn], cyc = cy[nn];
-        const T cxm = has_w ? cx[nn - 1] : T(0);
-        const T cym = has_s ? cy[nn - d.nx] : T(0);
-        const bool has_u = k + 1 < d.nz;
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-          if (!live[r]) continue;
-          const T* pr = p + r * va.vs;
+        if (!live[r]) continue;
+        const T* __restrict__ pr = p + r * va.vs;
+        T* __restrict__ qr = q + r * va.vs;
+        int64_t nn = (int64_t)k0 * d.sxy + base;
+        T czm = k0 > 0 ? cz[nn - d.sxy] : T(0);
+        T pm = k0 > 0 ? pr[nn - d.sxy] : T(0);
+        T pc = pr[nn];
+        double a = 0.0;
+        for (int k = k0; k < k1; ++k, nn += d.sxy) {
+          const T cxc = cx[nn], cyc = cy[nn], czc = cz[nn];
+          const T cxm = has_w ? cx[nn - 1] : T(0);
+          const T cym = has_s ? cy[nn - d.nx] : T(0);
+          const T pu = k + 1 < d.nz ? pr[nn + d.sxy] : T(0);
           const T pe = has_e ? pr[nn + 1] : T(0);
           const T pw = has_w ? pr[nn - 1] : T(0);
           const T pn = has_n ? pr[nn + d.nx] : T(0);
           const T ps = has_s ? pr[nn - d.nx] : T(0);
-          const T pu = has_u ? pr[nn + d.sxy] : T(0);
-          const T y = pc[r] - (cxc * pe + cxm * pw + cyc * pn + cym * ps + czc * pu + czm * pm[r]);
-          q[r * va.vs + nn] = y;
-          acc[r] += (double)pc[r] * (double)y;
-          pm[r] = pc[r];
-          pc[r] = pu;
+          const T y = pc - (cxc * pe + cxm * pw + cyc * pn + cym * ps + czc * pu + czm * pm);
+          qr[nn] = y;
+          a += (double)pc * (double)y;
+          pm = pc;
+          pc = pu;
+          czm = czc;
         }
-        czm = czc;
+        acc[r] = a;
       }
     }
   }
@@ -537,28 +528,6 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot(const T* __restrict__ p,
 #pragma unroll
     for (int r = 0; r < NR; ++r)
       if (r < va.nr) store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
-  }
-  if (arrive_last(counter)) {
-    if (prof && threadIdx.x == 0) {
-      unsigned long long nlive = 0;
-      for (int r = 0; r < va.nr; ++r) nlive += live[r] ? 1ull : 0ull;
-      prof[0] += 1ull;
-      prof[1] += nlive;
-    }
-    for (int r = 0; r < va.nr; ++r) {
-      const double pq = final_sum(partials + (int64_t)r * gridDim.x, gridDim.x);
-      if (threadIdx.x == 0 && live[r]) {
-        const double rho = mode == 0 ? sc[r].rho : sc[r].aux0;
-        sc[r].pq = pq;
-        if (!(pq > 0.0) || !isfinite(pq)) {
-          sc[r].done = 3;  // breakdown: A_s not positive definite on p
-          sc[r].alpha = 0.0;
-        } else {
-          sc[r].alpha = rho / pq;
-        }
-      }
-    }
-    reset_counter(counter);
   }
 }
 
@@ -570,8 +539,7 @@ template <typename T, int NR>
 __global__ __launch_bounds__(kBlock) void k_update(T* __restrict__ x, T* __restrict__ r_,
                                                    const T* __restrict__ p,
                                                    const T* __restrict__ q, VecArgs va,
-                                                   int mode, CgScalars* sc, double* partials,
-                                                   unsigned* counter) {
+                                                   int mode, CgScalars* sc, double* partials) {
   if (all_idle(sc, va.nr, mode)) return;
   double acc[NR];
 #pragma unroll
@@ -603,30 +571,6 @@ __global__ __launch_bounds__(kBlock) void k_update(T* __restrict__ x, T* __restr
     for (int r = 0; r < NR; ++r)
       if (r < va.nr) store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
   }
-  if (arrive_last(counter)) {
-    for (int r = 0; r < va.nr; ++r) {
-      const double rho_new = final_sum(partials + (int64_t)r * gridDim.x, gridDim.x);
-      if (threadIdx.x == 0 && live[r]) {
-        CgScalars& c = sc[r];
-        c.iter += 1;
-        c.cycle_iter += 1;
-        if (mode == 0) {
-          c.beta = rho_new / c.rho;
-          c.rho = rho_new;
-          if (rho_new < c.thresh2) c.done = 1;
-          else if (c.iter >= c.maxiter) c.done = 2;
-          else if (!isfinite(rho_new)) c.done = 3;
-        } else {
-          c.beta = rho_new / c.aux0;
-          c.aux0 = rho_new;
-          if (rho_new < c.aux1) c.cycle_done = 1;
-          else if (c.iter >= c.maxiter) c.cycle_done = 1;
-          else if (!isfinite(rho_new)) c.done = 3;
-        }
-      }
-    }
-    reset_counter(counter);
-  }
 }
 
 // p = r + beta p
@@ -646,6 +590,126 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(T* __restrict__ p, const T* 
   }
 }
 
+
+// --------------------------------------------------------------------------
+// Finalisation of the block partials: one single-block kernel launched right
+// after each reducing kernel.  Kernel boundaries make the partials of every
+// XCD visible, so no device-scope fence is needed inside the reducing kernels
+// (a per-block release fence writes back the whole L2 and quadruples the
+// cost of a streaming kernel on this part).
+// --------------------------------------------------------------------------
+enum FinOp : int {
+  FIN_ALPHA = 0,   // q = p.q          -> alpha = rho / (p.q)
+  FIN_BETA = 1,    // q = r_new.r_new  -> beta, rho, stop tests
+  FIN_INIT = 2,    // q = ||b_s||^2    -> fresh Krylov state
+  FIN_BNORM = 3,   // q = ||b||^2      -> aux2
+  FIN_RESID = 4,   // q0 = ||r||^2, q1 = ||r dhalf||^2 -> aux3, aux1
+  FIN_CYCLE = 5,   // q = ||r32||^2    -> FP32 cycle state
+};
+
+struct FinArgs {
+  int op;
+  int nr;
+  int nb;          // partials per quantity
+  int mode;        // 0 FP64 CG, 1 FP32 cycle
+  double tol;
+  int maxiter;
+  double inner_tol2;
+};
+
+// Sum of one quantity's partials (result in thread 0).
+__device__ __forceinline__ double sum_partials(const double* __restrict__ pr, int nb) {
+  double v[1];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int b = threadIdx.x;
+  for (; b + 3 * kBlock < nb; b += 4 * kBlock) {
+    a0 += pr[b];
+    a1 += pr[b + kBlock];
+    a2 += pr[b + 2 * kBlock];
+    a3 += pr[b + 3 * kBlock];
+  }
+  for (; b < nb; b += kBlock) a0 += pr[b];
+  v[0] = (a0 + a1) + (a2 + a3);
+  block_sum<1>(v);
+  return v[0];
+}
+
+// Launched with one block per right-hand side: block r owns sc[r].
+__global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ partials,
+                                                     FinArgs fa, CgScalars* sc,
+                                                     unsigned long long* prof) {
+  const int r = blockIdx.x;
+  if (fa.op == FIN_ALPHA || fa.op == FIN_BETA) {
+    if (!rhs_live(sc, r, fa.mode)) return;  // uniform over the block
+  }
+  const double s0 = sum_partials(partials + (int64_t)r * fa.nb, fa.nb);
+  double s1 = 0.0;
+  if (fa.op == FIN_RESID) s1 = sum_partials(partials + (int64_t)(fa.nr + r) * fa.nb, fa.nb);
+  if (threadIdx.x != 0) return;
+  CgScalars& c = sc[r];
+  if (fa.op == FIN_ALPHA) {
+    const double pq = s0;
+    const double rho = fa.mode == 0 ? c.rho : c.aux0;
+    c.pq = pq;
+    if (!(pq > 0.0) || !isfinite(pq)) {
+      c.done = 3;  // breakdown: A_s not positive definite on p
+      c.alpha = 0.0;
+    } else {
+      c.alpha = rho / pq;
+    }
+  } else if (fa.op == FIN_BETA) {
+    const double rho_new = s0;
+    c.iter += 1;
+    c.cycle_iter += 1;
+    if (fa.mode == 0) {
+      c.beta = rho_new / c.rho;
+      c.rho = rho_new;
+      if (rho_new < c.thresh2) c.done = 1;
+      else if (c.iter >= c.maxiter) c.done = 2;
+      else if (!isfinite(rho_new)) c.done = 3;
+    } else {
+      c.beta = rho_new / c.aux0;
+      c.aux0 = rho_new;
+      if (rho_new < c.aux1) c.cycle_done = 1;
+      else if (c.iter >= c.maxiter) c.cycle_done = 1;
+      else if (!isfinite(rho_new)) c.done = 3;
+    }
+  } else if (fa.op == FIN_INIT) {
+    c.rho = s0;
+    c.bnorm2 = s0;
+    c.thresh2 = fa.tol * fa.tol * s0;
+    c.beta = 0.0;
+    c.alpha = 0.0;
+    c.iter = 0;
+    c.cycle_iter = 0;
+    c.maxiter = fa.maxiter;
+    c.done = s0 == 0.0 ? 1 : 0;
+    c.cycle_done = 0;
+    c.refreshes = 0;
+  } else if (fa.op == FIN_BNORM) {
+    c.aux2 = s0;
+  } else if (fa.op == FIN_RESID) {
+    c.aux3 = s0;
+    c.aux1 = s1;
+  } else if (fa.op == FIN_CYCLE) {
+    if (c.done != 0) return;
+    c.scale = 1.0 / sqrt(c.rho);
+    c.aux0 = s0;
+    c.aux1 = fmax(fa.inner_tol2, 0.25 * c.thresh2 / c.rho) * s0;
+    c.beta = 0.0;
+    c.cycle_iter = 0;
+    c.cycle_done = 0;
+  }
+}
+
+template <int NR>
+static void finalize(hipStream_t st, const double* partials, int op, int nr, int nb, int mode,
+                     CgScalars* sc, unsigned long long* prof = nullptr, double tol = 0.0,
+                     int maxiter = 0, double inner_tol2 = 0.0) {
+  FinArgs fa{op, nr, nb, mode, tol, maxiter, inner_tol2};
+  hipLaunchKernelGGL(k_finalize, dim3(nr), dim3(kBlock), 0, st, partials, fa, sc, prof);
+}
+
 // --------------------------------------------------------------------------
 // right-hand sides, initial state, expansion
 // --------------------------------------------------------------------------
@@ -656,8 +720,7 @@ __global__ __launch_bounds__(kBlock) void k_lift_general(const double* __restric
                                                          const uint8_t* __restrict__ mask,
                                                          const double* __restrict__ dinv,
                                                          double* __restrict__ bs, VecArgs va,
-                                                         CgScalars* sc, double* partials,
-                                                         unsigned* counter) {
+                                                         double* partials) {
   double acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) acc[r] = 0.0;
@@ -679,13 +742,6 @@ __global__ __launch_bounds__(kBlock) void k_lift_general(const double* __restric
     for (int r = 0; r < NR; ++r)
       if (r < va.nr) store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
   }
-  if (arrive_last(counter)) {
-    for (int r = 0; r < va.nr; ++r) {
-      const double s = final_sum(partials + (int64_t)r * gridDim.x, gridDim.x);
-      if (threadIdx.x == 0) sc[r].aux2 = s;
-    }
-    reset_counter(counter);
-  }
 }
 
 // Capacitance RHS for excitation r (terminal id r at 1 V, other fixed at 0):
@@ -696,8 +752,7 @@ __global__ __launch_bounds__(kBlock) void k_lift_cap(const fs_op op, const int32
                                                      const uint8_t* __restrict__ mask,
                                                      const double* __restrict__ dinv,
                                                      double* __restrict__ bs, VecArgs va,
-                                                     int j0, CgScalars* sc, double* partials,
-                                                     unsigned* counter) {
+                                                     int j0, double* partials) {
   const Dims d = op.d;
   double acc[NR];
 #pragma unroll
@@ -741,13 +796,6 @@ __global__ __launch_bounds__(kBlock) void k_lift_cap(const fs_op op, const int32
     for (int r = 0; r < NR; ++r)
       if (r < va.nr) store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
   }
-  if (arrive_last(counter)) {
-    for (int r = 0; r < va.nr; ++r) {
-      const double s = final_sum(partials + (int64_t)r * gridDim.x, gridDim.x);
-      if (threadIdx.x == 0) sc[r].aux2 = s;
-    }
-    reset_counter(counter);
-  }
 }
 
 // x = 0, p = 0, r = b_s, rho = ||b_s||^2; thresholds.  (FP64 state)
@@ -756,8 +804,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(const double* __restrict__ b
                                                     double* __restrict__ x,
                                                     double* __restrict__ r_,
                                                     double* __restrict__ p, VecArgs va,
-                                                    double tol, int maxiter, CgScalars* sc,
-                                                    double* partials, unsigned* counter) {
+                                                    double* partials) {
   double acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) acc[r] = 0.0;
@@ -780,26 +827,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(const double* __restrict__ b
 #pragma unroll
     for (int r = 0; r < NR; ++r)
       if (r < va.nr) store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
-  }
-  if (arrive_last(counter)) {
-    for (int r = 0; r < va.nr; ++r) {
-      const double rho = final_sum(partials + (int64_t)r * gridDim.x, gridDim.x);
-      if (threadIdx.x == 0) {
-        CgScalars& c = sc[r];
-        c.rho = rho;
-        c.bnorm2 = rho;
-        c.thresh2 = tol * tol * rho;
-        c.beta = 0.0;
-        c.alpha = 0.0;
-        c.iter = 0;
-        c.cycle_iter = 0;
-        c.maxiter = maxiter;
-        c.done = rho == 0.0 ? 1 : 0;
-        c.cycle_done = 0;
-        c.refreshes = 0;
-      }
-    }
-    reset_counter(counter);
   }
 }
 
@@ -840,8 +867,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(const double* __restrict__ 
                                                      const double* __restrict__ cy,
                                                      const double* __restrict__ cz,
                                                      const double* __restrict__ dhalf, Dims d,
-                                                     VecArgs va, CgScalars* sc,
-                                                     double* partials, unsigned* counter) {
+                                                     VecArgs va, double* partials) {
   double acc[2 * NR];
 #pragma unroll
   for (int r = 0; r < 2 * NR; ++r) acc[r] = 0.0;
@@ -880,17 +906,6 @@ __global__ __launch_bounds__(kBlock) void k_residual(const double* __restrict__ 
       store_partial(partials + (int64_t)(va.nr + r) * gridDim.x + blockIdx.x, acc[NR + r]);
     }
   }
-  if (arrive_last(counter)) {
-    for (int r = 0; r < va.nr; ++r) {
-      const double s0 = final_sum(partials + (int64_t)r * gridDim.x, gridDim.x);
-      const double s1 = final_sum(partials + (int64_t)(va.nr + r) * gridDim.x, gridDim.x);
-      if (threadIdx.x == 0) {
-        sc[r].aux3 = s0;
-        sc[r].aux1 = s1;
-      }
-    }
-    reset_counter(counter);
-  }
 }
 
 // --------------------------------------------------------------------------
@@ -904,8 +919,7 @@ __global__ __launch_bounds__(kBlock) void k_cycle_start(const double* __restrict
                                                         float* __restrict__ r32,
                                                         float* __restrict__ d32,
                                                         float* __restrict__ p32, VecArgs va,
-                                                        double inner_tol2, CgScalars* sc,
-                                                        double* partials, unsigned* counter) {
+                                                        CgScalars* sc, double* partials) {
   double acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) acc[r] = 0.0;
@@ -929,21 +943,6 @@ __global__ __launch_bounds__(kBlock) void k_cycle_start(const double* __restrict
 #pragma unroll
     for (int r = 0; r < NR; ++r)
       if (r < va.nr) store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
-  }
-  if (arrive_last(counter)) {
-    for (int r = 0; r < va.nr; ++r) {
-      const double s = final_sum(partials + (int64_t)r * gridDim.x, gridDim.x);
-      if (threadIdx.x == 0 && sc[r].done == 0) {
-        CgScalars& c = sc[r];
-        c.scale = 1.0 / sqrt(c.rho);
-        c.aux0 = s;
-        c.aux1 = fmax(inner_tol2, 0.25 * c.thresh2 / c.rho) * s;
-        c.beta = 0.0;
-        c.cycle_iter = 0;
-        c.cycle_done = 0;
-      }
-    }
-    reset_counter(counter);
   }
 }
 
@@ -1039,20 +1038,24 @@ static void launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEv
     if (ea) (void)hipEventRecord(ea, st);
     hipLaunchKernelGGL((k_stencil_dot<double, NR>), dim3(sb), dim3(kBlock), 0, st, s->p, s->q,
                        s->c64[0], s->c64[1], s->c64[2], s->d, s->tl, va, mode, s->sc,
-                       s->partials, s->counters + 0, prof);
+                       s->partials, prof);
     if (eb) (void)hipEventRecord(eb, st);
+    finalize<NR>(st, s->partials, FIN_ALPHA, nr, sb, mode, s->sc);
     hipLaunchKernelGGL((k_update<double, NR>), dim3(vb), dim3(kBlock), 0, st, s->x, s->r, s->p,
-                       s->q, va, mode, s->sc, s->partials, s->counters + 1);
+                       s->q, va, mode, s->sc, s->partials);
+    finalize<NR>(st, s->partials, FIN_BETA, nr, vb, mode, s->sc);
   } else {
     hipLaunchKernelGGL((k_pupdate<float, NR>), dim3(vb), dim3(kBlock), 0, st, s->p32, s->r32,
                        va, mode, s->sc);
     if (ea) (void)hipEventRecord(ea, st);
     hipLaunchKernelGGL((k_stencil_dot<float, NR>), dim3(sb), dim3(kBlock), 0, st, s->p32,
                        s->q32, s->c32[0], s->c32[1], s->c32[2], s->d, s->tl, va, mode, s->sc,
-                       s->partials, s->counters + 0, prof);
+                       s->partials, prof);
     if (eb) (void)hipEventRecord(eb, st);
+    finalize<NR>(st, s->partials, FIN_ALPHA, nr, sb, mode, s->sc);
     hipLaunchKernelGGL((k_update<float, NR>), dim3(vb), dim3(kBlock), 0, st, s->x32, s->r32,
-                       s->p32, s->q32, va, mode, s->sc, s->partials, s->counters + 1);
+                       s->p32, s->q32, va, mode, s->sc, s->partials);
+    finalize<NR>(st, s->partials, FIN_BETA, nr, vb, mode, s->sc);
   }
 }
 
@@ -1072,8 +1075,9 @@ template <int NR>
 static void launch_init_t(fs_sys* s, int nr, double tol, int maxiter) {
   VecArgs va{s->d.n, s->vstride, nr};
   hipLaunchKernelGGL((k_cg_init<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream,
-                     s->bs, s->x, s->r, s->p, va, tol, maxiter, s->sc, s->partials,
-                     s->counters + 2);
+                     s->bs, s->x, s->r, s->p, va, s->partials);
+  finalize<NR>(s->ctx->stream, s->partials, FIN_INIT, nr, s->vec_blocks, 0, s->sc, nullptr, tol,
+               maxiter);
 }
 void launch_init(fs_sys* s, int nr, double tol, int maxiter) {
   FS_DISPATCH_NR(nr, launch_init_t, s, nr, tol, maxiter);
@@ -1083,8 +1087,8 @@ template <int NR>
 static void launch_lift_general_t(fs_sys* s, int nr, const double* b) {
   VecArgs va{s->d.n, s->vstride, nr};
   hipLaunchKernelGGL((k_lift_general<NR>), dim3(s->vec_blocks), dim3(kBlock), 0,
-                     s->ctx->stream, b, s->mask, s->dinv, s->bs, va, s->sc, s->partials,
-                     s->counters + 3);
+                     s->ctx->stream, b, s->mask, s->dinv, s->bs, va, s->partials);
+  finalize<NR>(s->ctx->stream, s->partials, FIN_BNORM, nr, s->vec_blocks, 0, s->sc);
 }
 void launch_lift_general(fs_sys* s, int nr, const double* b) {
   FS_DISPATCH_NR(nr, launch_lift_general_t, s, nr, b);
@@ -1094,8 +1098,8 @@ template <int NR>
 static void launch_lift_cap_t(fs_sys* s, int nr, const fs_op* L, const int32_t* tid, int j0) {
   VecArgs va{s->d.n, s->vstride, nr};
   hipLaunchKernelGGL((k_lift_cap<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream,
-                     *L, tid, s->mask, s->dinv, s->bs, va, j0, s->sc, s->partials,
-                     s->counters + 4);
+                     *L, tid, s->mask, s->dinv, s->bs, va, j0, s->partials);
+  finalize<NR>(s->ctx->stream, s->partials, FIN_BNORM, nr, s->vec_blocks, 0, s->sc);
 }
 void launch_lift_cap(fs_sys* s, int nr, const fs_op* L, const int32_t* tid, int j0) {
   FS_DISPATCH_NR(nr, launch_lift_cap_t, s, nr, L, tid, j0);
@@ -1118,7 +1122,8 @@ static void launch_residual_t(fs_sys* s, int nr, double* res) {
   VecArgs va{s->d.n, s->vstride, nr};
   hipLaunchKernelGGL((k_residual<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream,
                      s->bs, s->x, res, s->c64[0], s->c64[1], s->c64[2], s->dhalf, s->d, va,
-                     s->sc, s->partials, s->counters + 5);
+                     s->partials);
+  finalize<NR>(s->ctx->stream, s->partials, FIN_RESID, nr, s->vec_blocks, 0, s->sc);
 }
 void launch_residual(fs_sys* s, int nr, double* res) {
   FS_DISPATCH_NR(nr, launch_residual_t, s, nr, res);
@@ -1128,8 +1133,9 @@ template <int NR>
 static void launch_cycle_start_t(fs_sys* s, int nr, double inner_tol2) {
   VecArgs va{s->d.n, s->vstride, nr};
   hipLaunchKernelGGL((k_cycle_start<NR>), dim3(s->vec_blocks), dim3(kBlock), 0,
-                     s->ctx->stream, s->r, s->r32, s->x32, s->p32, va, inner_tol2, s->sc,
-                     s->partials, s->counters + 6);
+                     s->ctx->stream, s->r, s->r32, s->x32, s->p32, va, s->sc, s->partials);
+  finalize<NR>(s->ctx->stream, s->partials, FIN_CYCLE, nr, s->vec_blocks, 1, s->sc, nullptr,
+               0.0, 0, inner_tol2);
 }
 void launch_cycle_start(fs_sys* s, int nr, double inner_tol2) {
   FS_DISPATCH_NR(nr, launch_cycle_start_t, s, nr, inner_tol2);
