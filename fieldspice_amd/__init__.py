@@ -12,9 +12,12 @@ from .grid import RectilinearGrid, graded_1d
 from .materials import MaterialMap
 from .solvers.base import ConvergenceError, Result, SolverConfig, Terminal
 from .solvers.poisson import PoissonSolver
+from .solvers.eqs import EQSSolver
+from .solvers.thermal import ThermalSolver, joule_heating_nodes
 
 __version__ = "0.1.0"
 
 __all__ = ["RectilinearGrid", "graded_1d", "MaterialMap", "BoundarySpec", "Dirichlet",
            "Neumann", "Terminal", "SolverConfig", "Result", "ConvergenceError",
-           "PoissonSolver", "extraction", "units", "boundaries"]
+           "PoissonSolver", "EQSSolver", "ThermalSolver", "joule_heating_nodes", "extraction",
+           "units", "boundaries"]

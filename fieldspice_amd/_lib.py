@@ -30,7 +30,7 @@ EXPORTS = (
     "fs_op_create_edges", "fs_op_create_cells", "fs_op_combine", "fs_op_destroy",
     "fs_op_apply", "fs_op_get_edges", "fs_sys_create", "fs_sys_destroy",
     "fs_sys_solve", "fs_capacitance", "fs_cap_prepare", "fs_cap_solve", "fs_cap_destroy",
-    "fs_sys_set_profiling",
+    "fs_sys_set_profiling", "fs_transient_run", "fs_op_edge_power",
 )
 
 
@@ -52,6 +52,15 @@ class SolveStats(C.Structure):
 class SysInfo(C.Structure):
     _fields_ = [("n_nodes", C.c_int64), ("n_free", C.c_int64), ("n_fixed", C.c_int64),
                 ("n_empty", C.c_int64), ("gauge", C.c_int64), ("singular", C.c_int)]
+
+
+class TransientDesc(C.Structure):
+    _fields_ = [("nsteps", C.c_int), ("ngroups", C.c_int), ("group", C.c_void_p),
+                ("values", C.c_void_p), ("nsrc", C.c_int), ("src", C.c_void_p),
+                ("src_scale", C.c_void_p), ("warm_start", C.c_int), ("nprobe", C.c_int),
+                ("probe", C.c_void_p), ("probe_w", C.c_void_p), ("cur_a", C.c_void_p),
+                ("cur_b", C.c_void_p), ("quad_a", C.c_void_p), ("quad_b", C.c_void_p),
+                ("store_every", C.c_int)]
 
 
 class LibraryError(RuntimeError):
@@ -94,6 +103,9 @@ def _declare(lib):
                                  C.POINTER(SolveStats)]
     lib.fs_cap_destroy.argtypes = [_vp]
     lib.fs_sys_set_profiling.argtypes = [_vp, C.c_int]
+    lib.fs_transient_run.argtypes = [_vp, _vp, C.POINTER(TransientDesc), _dp, _vp, _vp, _vp,
+                                     _vp, C.POINTER(SolveParams), C.POINTER(SolveStats)]
+    lib.fs_op_edge_power.argtypes = [_vp, _dp, _dp]
     for name in EXPORTS:
         if name != "fs_last_error":
             getattr(lib, name).restype = C.c_int
@@ -215,6 +227,13 @@ class DeviceOperator:
         check(load().fs_op_apply(self._h, x, y), "fs_op_apply")
         return y
 
+    def edge_power(self, x) -> np.ndarray:
+        """``q_n = 1/2 sum_e w_e (x_n - x_m)^2`` (Joule heat per node)."""
+        x = as_f64(x).ravel()
+        q = np.empty_like(x)
+        check(load().fs_op_edge_power(self._h, x, q), "fs_op_edge_power")
+        return q
+
     def edges(self) -> np.ndarray:
         Nx, Ny, Nz = self.shape_cells
         wx = np.empty(Nx * (Ny + 1) * (Nz + 1))
@@ -289,6 +308,44 @@ class DeviceSystem:
                                    C.byref(params), C.byref(st))
         check(rc, "fs_capacitance")
         return Cm, qsh, phi, st.as_dict(), rc
+
+    def transient(self, Aexp, x0, nsteps, group=None, values=None, src=None, src_scale=None,
+                  probe=None, probe_w=None, nprobe=0, cur_a=None, cur_b=None, quad_a=None,
+                  quad_b=None, store_every=0, warm_start=True, params=None):
+        """Device time loop (``fs_transient_run``); returns a dict of outputs."""
+        n = self.op.n_nodes
+        keep = []
+
+        def ptr(a, dtype):
+            if a is None:
+                return None
+            arr = np.ascontiguousarray(a, dtype=dtype)
+            keep.append(arr)
+            return arr.ctypes.data_as(_vp)
+        ng = 0 if values is None else int(np.asarray(values).reshape(nsteps + 1, -1).shape[1])
+        nsrc = 0 if src is None else int(np.asarray(src).reshape(-1, n).shape[0])
+        d = TransientDesc(int(nsteps), ng, ptr(group, np.int32), ptr(values, np.float64), nsrc,
+                          ptr(src, np.float64), ptr(src_scale, np.float64),
+                          1 if warm_start else 0, int(nprobe), ptr(probe, np.int32),
+                          ptr(probe_w, np.float64),
+                          None if cur_a is None else cur_a._h, None if cur_b is None else cur_b._h,
+                          None if quad_a is None else quad_a._h,
+                          None if quad_b is None else quad_b._h, int(store_every))
+        x = as_f64(x0).ravel().copy()
+        nstore = (nsteps // store_every + 1) if store_every > 0 else 0
+        stored = np.empty((nstore, n)) if nstore else None
+        pi = np.empty((nsteps + 1, nprobe)) if nprobe else None
+        pv = np.empty((nsteps + 1, nprobe)) if nprobe else None
+        ss = np.empty((nsteps + 1, 4))
+        st = SolveStats()
+        rc = load().fs_transient_run(self._h, None if Aexp is None else Aexp._h, C.byref(d), x,
+                                     None if stored is None else stored.ctypes.data_as(_vp),
+                                     None if pi is None else pi.ctypes.data_as(_vp),
+                                     None if pv is None else pv.ctypes.data_as(_vp),
+                                     ss.ctypes.data_as(_vp), C.byref(params), C.byref(st))
+        check(rc, "fs_transient_run")
+        return {"x": x, "stored": stored, "probe_i": pi, "probe_v": pv, "step_stats": ss,
+                "stats": st.as_dict(), "rc": rc}
 
     def prepare_capacitance(self, term_id, nt: int) -> "DeviceCapacitance":
         return DeviceCapacitance(self, term_id, nt)

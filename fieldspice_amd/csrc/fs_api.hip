@@ -311,6 +311,43 @@ void collect_stats(fs_sys* s, int nr, fs_solve_stats* st, int& status) {
 
 }  // namespace
 
+namespace {
+// per-probe node lists in device order (host C-order ids -> x-fast indices)
+int build_lists(const fs::Dims& d, const int32_t* id, int nbins, std::vector<int64_t>& nodes,
+                std::vector<int64_t>& offs) {
+  std::vector<int64_t> counts(nbins, 0);
+  for (int64_t ref = 0; ref < d.n; ++ref) {
+    const int t = id[ref];
+    if (t < -1 || t >= nbins) return fail(FS_EINVAL, "id[%lld] = %d out of range", (long long)ref, t);
+    if (t >= 0) counts[t] += 1;
+  }
+  offs.assign(nbins + 1, 0);
+  for (int t = 0; t < nbins; ++t) offs[t + 1] = offs[t] + counts[t];
+  nodes.assign((size_t)std::max<int64_t>(offs[nbins], 1), 0);
+  std::vector<int64_t> fill(offs.begin(), offs.end() - 1);
+  for (int64_t i = 0; i < d.nx; ++i)
+    for (int64_t j = 0; j < d.ny; ++j)
+      for (int64_t k = 0; k < d.nz; ++k) {
+        const int t = id[(i * d.ny + j) * d.nz + k];
+        if (t >= 0) nodes[fill[t]++] = (k * d.ny + j) * d.nx + i;
+      }
+  return FS_OK;
+}
+
+struct DevBufs {
+  std::vector<void*> ptrs;
+  ~DevBufs() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  int get(T** p, size_t count) {
+    int rc = dalloc(p, count);
+    if (rc == FS_OK) ptrs.push_back((void*)*p);
+    return rc;
+  }
+};
+}  // namespace
+
 // ==========================================================================
 extern "C" {
 
@@ -862,6 +899,194 @@ int fs_capacitance(fs_sys* s, const fs_op* L, int nt, const int32_t* term_id, do
   fs_cap_destroy(cap);
   g_err = keep;
   return rc;
+}
+
+
+int fs_op_edge_power(fs_op* op, const double* x, double* q) {
+  FS_GUARD_BEGIN
+  if (!op || !x || !q) return fail(FS_EINVAL, "NULL argument");
+  fs_ctx* ctx = op->ctx;
+  FS_HIP(hipSetDevice(ctx->device));
+  const fs::Dims& d = op->d;
+  double *dx = nullptr, *dq = nullptr;
+  int rc = dalloc(&dx, (size_t)d.n);
+  if (rc == FS_OK) rc = dalloc(&dq, (size_t)d.n);
+  if (rc == FS_OK) rc = upload_permuted(ctx, x, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz, dx);
+  if (rc == FS_OK) {
+    fs::launch_edge_power(ctx->stream, op, dx, dq, ctx->n_cu);
+    rc = download_nodes(ctx, d, dq, q, dx);
+  }
+  dfree(dx);
+  dfree(dq);
+  return rc;
+  FS_GUARD_END
+}
+
+
+int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, double* x,
+                     double* stored, double* probe_i, double* probe_v, double* step_stats,
+                     const fs_solve_params* params, fs_solve_stats* stats) {
+  FS_GUARD_BEGIN
+  if (!s || !D || !x || !params) return fail(FS_EINVAL, "NULL argument");
+  if (D->nsteps < 1) return fail(FS_EINVAL, "nsteps must be >= 1");
+  if (D->ngroups < 0 || (D->ngroups > 0 && (!D->group || !D->values)))
+    return fail(FS_EINVAL, "groups need group ids and values");
+  if (D->nsrc < 0 || (D->nsrc > 0 && (!D->src || !D->src_scale)))
+    return fail(FS_EINVAL, "sources need vectors and scales");
+  if (D->nprobe < 0 || (D->nprobe > 0 && (!D->probe || !D->probe_w)))
+    return fail(FS_EINVAL, "probes need ids and weights");
+  if (params->precision != FS_PREC_FP64 && params->precision != FS_PREC_MIXED)
+    return fail(FS_EINVAL, "unknown precision %d", params->precision);
+  if (!(params->tol > 0.0)) return fail(FS_EINVAL, "tol must be > 0");
+  const fs_op* ops[5] = {Aexp, D->cur_a, D->cur_b, D->quad_a, D->quad_b};
+  for (const fs_op* o : ops)
+    if (o && (o->Nx != s->op->Nx || o->Ny != s->op->Ny || o->Nz != s->op->Nz))
+      return fail(FS_EINVAL, "operators differ in grid");
+  if (D->nprobe > 0 && !D->cur_a) return fail(FS_EINVAL, "probe currents need cur_a");
+  fs_ctx* ctx = s->ctx;
+  FS_HIP(hipSetDevice(ctx->device));
+  const fs::Dims& d = s->d;
+  const int nsteps = D->nsteps, ng = D->ngroups, np = D->nprobe;
+  if (int rc = ensure_ws(s, 1)) return rc;
+  DevBufs B;
+  double *xa = nullptr, *xb = nullptr, *vals = nullptr, *src = nullptr, *scale = nullptr;
+  double *pw = nullptr, *oi = nullptr, *ov = nullptr, *ost = nullptr;
+  int32_t *grp = nullptr;
+  int64_t *pnodes = nullptr, *poffs = nullptr;
+  int rc = FS_OK;
+  if ((rc = B.get(&xa, (size_t)s->vstride)) || (rc = B.get(&xb, (size_t)s->vstride)) ||
+      (rc = B.get(&grp, (size_t)d.n)) ||
+      (rc = B.get(&vals, (size_t)std::max(1, (nsteps + 1) * ng))) ||
+      (rc = B.get(&ost, (size_t)(nsteps + 1) * 4)))
+    return rc;
+  if ((rc = upload_permuted(ctx, x, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz, xa))) return rc;
+  if (ng > 0) {
+    // group ids must agree with the system's Dirichlet rows
+    for (int64_t i = 0; i < d.n; ++i)
+      if (D->group[i] < -1 || D->group[i] >= ng)
+        return fail(FS_EINVAL, "group[%lld] = %d out of range", (long long)i, D->group[i]);
+    if ((rc = upload_permuted(ctx, D->group, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz, grp))) return rc;
+    FS_HIP(hipMemcpy(vals, D->values, sizeof(double) * (nsteps + 1) * ng, hipMemcpyHostToDevice));
+  } else {
+    FS_HIP(hipMemset(grp, 0xff, sizeof(int32_t) * d.n));  // all -1
+  }
+  if (D->nsrc > 0) {
+    if ((rc = B.get(&src, (size_t)D->nsrc * s->vstride)) ||
+        (rc = B.get(&scale, (size_t)(nsteps + 1) * D->nsrc)))
+      return rc;
+    for (int k = 0; k < D->nsrc; ++k)
+      if ((rc = upload_permuted(ctx, D->src + (size_t)k * d.n, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz,
+                                src + (size_t)k * s->vstride)))
+        return rc;
+    FS_HIP(hipMemcpy(scale, D->src_scale, sizeof(double) * (nsteps + 1) * D->nsrc,
+                     hipMemcpyHostToDevice));
+  }
+  int nbx = 1;
+  if (np > 0) {
+    std::vector<int64_t> nodes, offs;
+    if ((rc = build_lists(d, D->probe, np, nodes, offs))) return rc;
+    int64_t maxc = 1;
+    for (int t = 0; t < np; ++t) maxc = std::max(maxc, offs[t + 1] - offs[t]);
+    nbx = (int)std::min<int64_t>((maxc + fs::kBlock - 1) / fs::kBlock, 64);
+    if ((rc = B.get(&pnodes, nodes.size())) || (rc = B.get(&poffs, offs.size())) ||
+        (rc = B.get(&pw, (size_t)d.n)) || (rc = B.get(&oi, (size_t)(nsteps + 1) * np)) ||
+        (rc = B.get(&ov, (size_t)(nsteps + 1) * np)))
+      return rc;
+    FS_HIP(hipMemcpy(pnodes, nodes.data(), nodes.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    FS_HIP(hipMemcpy(poffs, offs.data(), offs.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    if ((rc = upload_permuted(ctx, D->probe_w, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz, pw))) return rc;
+    const int64_t need = (int64_t)np * 2 * nbx;
+    if (need > s->partials_cap) {
+      dfree(s->partials);
+      if ((rc = dalloc(&s->partials, (size_t)need))) {
+        s->partials_cap = 0;
+        return rc;
+      }
+      s->partials_cap = need;
+    }
+  }
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->converged = 1;
+  }
+  const int maxiter = default_maxiter(s, params);
+  const int store_every = D->store_every;
+  int64_t nstored = 0;
+  auto store = [&](const double* dev) -> int {
+    if (!stored) return FS_OK;
+    return download_nodes(ctx, d, dev, stored + (size_t)nstored++ * d.n, s->r);
+  };
+  // step 0 records
+  fs::launch_step_stats(s, xa, D->quad_a, D->quad_b, ost);
+  if (np > 0)
+    fs::launch_probe(s, D->cur_a, D->cur_b, xa, xa, pw, pnodes, poffs, np, nbx, nullptr, ov);
+  FS_HIP(hipGetLastError());
+  if (store_every > 0 && (rc = store(xa))) return rc;
+  int status = FS_OK;
+  double worst_res = 0.0, worst_req = 0.0;
+  int64_t total_it = 0;
+  FS_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  for (int k = 1; k <= nsteps; ++k) {
+    const double* vk = vals + (size_t)k * ng;
+    fs::launch_explicit(s, Aexp, xa, src, scale ? scale + (size_t)k * D->nsrc : nullptr, D->nsrc,
+                        s->vstride, s->node_tmp);
+    fs::launch_lift_groups(s, s->node_tmp, grp, vk);
+    if (D->warm_start) fs::launch_init_warm(s, xa, params->tol, maxiter);
+    else fs::launch_init(s, 1, params->tol, maxiter);
+    FS_HIP(hipGetLastError());
+    if ((rc = run_krylov(s, 1, *params))) return rc;
+    fs::launch_residual(s, 1, s->r);
+    fs::launch_expand_groups(s, grp, vk, xb);
+    fs::launch_step_stats(s, xb, D->quad_a, D->quad_b, ost + (size_t)k * 4);
+    if (np > 0)
+      fs::launch_probe(s, D->cur_a, D->cur_b, xb, xa, pw, pnodes, poffs, np, nbx,
+                       oi + (size_t)k * np, ov + (size_t)k * np);
+    FS_HIP(hipGetLastError());
+    FS_HIP(hipMemcpyAsync(s->sc_host, s->sc, sizeof(fs::CgScalars), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    FS_HIP(hipStreamSynchronize(ctx->stream));
+    const fs::CgScalars& c = s->sc_host[0];
+    total_it += c.iter;
+    double res = c.aux2 > 0 ? std::sqrt(c.aux1 / c.aux2) : 0.0;
+    double req = c.bnorm2 > 0 ? std::sqrt(c.aux3 / c.bnorm2) : 0.0;
+    if (!(res >= 0.0)) res = INFINITY;
+    if (!(req >= 0.0)) req = INFINITY;
+    worst_res = std::max(worst_res, res);
+    worst_req = std::max(worst_req, req);
+    if (stats) stats->refreshes += c.refreshes;
+    if (c.done != 1) {
+      status = FS_NOT_CONVERGED;
+      if (stats) stats->converged = 0;
+    }
+    std::swap(xa, xb);
+    if (store_every > 0 && k % store_every == 0 && (rc = store(xa))) return rc;
+  }
+  FS_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  FS_HIP(hipEventSynchronize(ctx->ev1));
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  if ((rc = download_nodes(ctx, d, xa, x, s->r))) return rc;
+  if (step_stats)
+    FS_HIP(hipMemcpy(step_stats, ost, sizeof(double) * (nsteps + 1) * 4, hipMemcpyDeviceToHost));
+  if (np > 0) {
+    if (probe_i) {
+      FS_HIP(hipMemcpy(probe_i, oi, sizeof(double) * (nsteps + 1) * np, hipMemcpyDeviceToHost));
+      for (int m = 0; m < np; ++m) probe_i[m] = nsteps >= 1 ? probe_i[np + m] : 0.0;
+    }
+    if (probe_v)
+      FS_HIP(hipMemcpy(probe_v, ov, sizeof(double) * (nsteps + 1) * np, hipMemcpyDeviceToHost));
+  }
+  if (stats) {
+    stats->iterations = (int)std::min<int64_t>(total_it, INT32_MAX);
+    stats->residual = worst_res;
+    stats->residual_eq = worst_req;
+    stats->solve_time = ms * 1e-3;
+  }
+  if (status == FS_NOT_CONVERGED)
+    fail(FS_NOT_CONVERGED, "a time step did not reach tol %.3g within %d iterations", params->tol,
+         maxiter);
+  return status;
+  FS_GUARD_END
 }
 
 int fs_sys_set_profiling(fs_sys* s, int on) {

@@ -1,0 +1,131 @@
+// Device helpers shared by the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fs_internal.hpp"
+
+namespace fs {
+
+// --------------------------------------------------------------------------
+// reductions
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+  return v;
+}
+
+// Sum NV values over the block; result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  __shared__ double s[NV][kBlock / kWave];
+  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double t = wave_sum(v[k]);
+    if (lane == 0) s[k][wv] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < kBlock / kWave; ++w) t += s[k][w];
+      v[k] = t;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void store_partial(double* p, double v) { *p = v; }
+
+// out[r] = sum over b of partials[r*nb + b] for r < nr (NR >= nr), each in a
+// fixed order, all quantities in one pass.  Results valid in thread 0.
+template <int NR>
+__device__ __forceinline__ void final_sums(const double* __restrict__ partials, int nb, int nr,
+                                           double (&out)[NR]) {
+#pragma unroll
+  for (int r = 0; r < NR; ++r) out[r] = 0.0;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (r >= nr) continue;
+    const double* pr = partials + (int64_t)r * nb;
+    // four independent chains keep four loads in flight per thread
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int b = threadIdx.x;
+    for (; b + 3 * kBlock < nb; b += 4 * kBlock) {
+      a0 += pr[b];
+      a1 += pr[b + kBlock];
+      a2 += pr[b + 2 * kBlock];
+      a3 += pr[b + 3 * kBlock];
+    }
+    for (; b < nb; b += kBlock) a0 += pr[b];
+    out[r] = (a0 + a1) + (a2 + a3);
+  }
+  block_sum<NR>(out);
+}
+
+// Sum of one quantity's partials (result in thread 0).
+__device__ __forceinline__ double sum_partials(const double* __restrict__ pr, int nb) {
+  double v[1];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int b = threadIdx.x;
+  for (; b + 3 * kBlock < nb; b += 4 * kBlock) {
+    a0 += pr[b];
+    a1 += pr[b + kBlock];
+    a2 += pr[b + 2 * kBlock];
+    a3 += pr[b + 3 * kBlock];
+  }
+  for (; b < nb; b += kBlock) a0 += pr[b];
+  v[0] = (a0 + a1) + (a2 + a3);
+  block_sum<1>(v);
+  return v[0];
+}
+
+// ----------------------------------------------------------------------------
+// raw operator row: (A x)_n = sum_e w_e (x_n - x_m) + extra_n x_n   (FP64)
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ double raw_row(const double* __restrict__ x,
+                                          const double* __restrict__ wx,
+                                          const double* __restrict__ wy,
+                                          const double* __restrict__ wz,
+                                          const double* __restrict__ extra, const Dims& d,
+                                          int i, int j, int k, int64_t n) {
+  const double xc = x[n];
+  double y = 0.0;
+  if (i + 1 < d.nx) y += wx[n] * (xc - x[n + 1]);
+  if (i > 0) y += wx[n - 1] * (xc - x[n - 1]);
+  if (j + 1 < d.ny) y += wy[n] * (xc - x[n + d.nx]);
+  if (j > 0) y += wy[n - d.nx] * (xc - x[n - d.nx]);
+  if (k + 1 < d.nz) y += wz[n] * (xc - x[n + d.sxy]);
+  if (k > 0) y += wz[n - d.sxy] * (xc - x[n - d.sxy]);
+  if (extra) y += extra[n] * xc;
+  return y;
+}
+
+
+// (A (x - y))_n without forming x - y
+__device__ __forceinline__ double raw_row_diff(const double* __restrict__ x,
+                                               const double* __restrict__ y,
+                                               const double* __restrict__ wx,
+                                               const double* __restrict__ wy,
+                                               const double* __restrict__ wz,
+                                               const double* __restrict__ extra, const Dims& d,
+                                               int i, int j, int k, int64_t n) {
+  const double c = x[n] - y[n];
+  double v = 0.0;
+  if (i + 1 < d.nx) v += wx[n] * (c - (x[n + 1] - y[n + 1]));
+  if (i > 0) v += wx[n - 1] * (c - (x[n - 1] - y[n - 1]));
+  if (j + 1 < d.ny) v += wy[n] * (c - (x[n + d.nx] - y[n + d.nx]));
+  if (j > 0) v += wy[n - d.nx] * (c - (x[n - d.nx] - y[n - d.nx]));
+  if (k + 1 < d.nz) v += wz[n] * (c - (x[n + d.sxy] - y[n + d.sxy]));
+  if (k > 0) v += wz[n - d.sxy] * (c - (x[n - d.sxy] - y[n - d.sxy]));
+  if (extra) v += extra[n] * c;
+  return v;
+}
+
+}  // namespace fs

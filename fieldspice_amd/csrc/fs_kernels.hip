@@ -19,69 +19,10 @@
 #include <cstdint>
 
 #include "fs_internal.hpp"
+#include "fs_device.hpp"
 #include "fs_launch.hpp"
 
 namespace fs {
-
-// --------------------------------------------------------------------------
-// reductions
-// --------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
-  return v;
-}
-
-// Sum NV values over the block; result valid in thread 0.
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV]) {
-  __shared__ double s[NV][kBlock / kWave];
-  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double t = wave_sum(v[k]);
-    if (lane == 0) s[k][wv] = t;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double t = 0.0;
-#pragma unroll
-      for (int w = 0; w < kBlock / kWave; ++w) t += s[k][w];
-      v[k] = t;
-    }
-  }
-  __syncthreads();
-}
-
-__device__ __forceinline__ void store_partial(double* p, double v) { *p = v; }
-
-// out[r] = sum over b of partials[r*nb + b] for r < nr (NR >= nr), each in a
-// fixed order, all quantities in one pass.  Results valid in thread 0.
-template <int NR>
-__device__ __forceinline__ void final_sums(const double* __restrict__ partials, int nb, int nr,
-                                           double (&out)[NR]) {
-#pragma unroll
-  for (int r = 0; r < NR; ++r) out[r] = 0.0;
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    if (r >= nr) continue;
-    const double* pr = partials + (int64_t)r * nb;
-    // four independent chains keep four loads in flight per thread
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int b = threadIdx.x;
-    for (; b + 3 * kBlock < nb; b += 4 * kBlock) {
-      a0 += pr[b];
-      a1 += pr[b + kBlock];
-      a2 += pr[b + 2 * kBlock];
-      a3 += pr[b + 3 * kBlock];
-    }
-    for (; b < nb; b += kBlock) a0 += pr[b];
-    out[r] = (a0 + a1) + (a2 + a3);
-  }
-  block_sum<NR>(out);
-}
 
 // --------------------------------------------------------------------------
 // layout permutation between fieldspice's z-fastest arrays and x-fast
@@ -235,24 +176,6 @@ void axpby(hipStream_t s, int64_t n, double a, const double* x, double b, const 
 // --------------------------------------------------------------------------
 // raw operator apply: y = sum_e w_e (x_n - x_m) + extra_n x_n   (FP64)
 // --------------------------------------------------------------------------
-__device__ __forceinline__ double raw_row(const double* __restrict__ x,
-                                          const double* __restrict__ wx,
-                                          const double* __restrict__ wy,
-                                          const double* __restrict__ wz,
-                                          const double* __restrict__ extra, const Dims& d,
-                                          int i, int j, int k, int64_t n) {
-  const double xc = x[n];
-  double y = 0.0;
-  if (i + 1 < d.nx) y += wx[n] * (xc - x[n + 1]);
-  if (i > 0) y += wx[n - 1] * (xc - x[n - 1]);
-  if (j + 1 < d.ny) y += wy[n] * (xc - x[n + d.nx]);
-  if (j > 0) y += wy[n - d.nx] * (xc - x[n - d.nx]);
-  if (k + 1 < d.nz) y += wz[n] * (xc - x[n + d.sxy]);
-  if (k > 0) y += wz[n - d.sxy] * (xc - x[n - d.sxy]);
-  if (extra) y += extra[n] * xc;
-  return y;
-}
-
 __global__ __launch_bounds__(kBlock) void k_apply_raw(const double* __restrict__ x,
                                                       double* __restrict__ y,
                                                       const double* __restrict__ wx,
@@ -616,23 +539,6 @@ struct FinArgs {
   int maxiter;
   double inner_tol2;
 };
-
-// Sum of one quantity's partials (result in thread 0).
-__device__ __forceinline__ double sum_partials(const double* __restrict__ pr, int nb) {
-  double v[1];
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  int b = threadIdx.x;
-  for (; b + 3 * kBlock < nb; b += 4 * kBlock) {
-    a0 += pr[b];
-    a1 += pr[b + kBlock];
-    a2 += pr[b + 2 * kBlock];
-    a3 += pr[b + 3 * kBlock];
-  }
-  for (; b < nb; b += kBlock) a0 += pr[b];
-  v[0] = (a0 + a1) + (a2 + a3);
-  block_sum<1>(v);
-  return v[0];
-}
 
 // Launched with one block per right-hand side: block r owns sc[r].
 __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ partials,

@@ -37,4 +37,17 @@ void launch_refresh(fs_sys* s, int nr);
 void launch_charge(fs_sys* s, int nr, const fs_op* L, const double* phi, const int64_t* nodes,
                    const int64_t* offs, int nbins, int nbx, double* out);
 
+// time loops (fs_transient.hip)
+void launch_explicit(fs_sys* s, const fs_op* A, const double* x, const double* src,
+                     const double* scale, int nsrc, int64_t vs, double* rhs);
+void launch_lift_groups(fs_sys* s, const double* rhs, const int32_t* group, const double* vals);
+void launch_init_warm(fs_sys* s, const double* xprev, double tol, int maxiter);
+void launch_expand_groups(fs_sys* s, const int32_t* group, const double* vals, double* x);
+void launch_probe(fs_sys* s, const fs_op* A1, const fs_op* A2, const double* x,
+                  const double* xprev, const double* w, const int64_t* nodes,
+                  const int64_t* offs, int nprobe, int nbx, double* out_i, double* out_v);
+void launch_step_stats(fs_sys* s, const double* x, const fs_op* Qa, const fs_op* Qb,
+                       double* out);
+void launch_edge_power(hipStream_t st, const fs_op* A, const double* x, double* q, int ncu);
+
 }  // namespace fs

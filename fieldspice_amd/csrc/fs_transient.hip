@@ -1,0 +1,346 @@
+// Kernels of the implicit time loops (EQS and thermal steps) and of the
+// Joule-heat product.  One field per loop; all node vectors x-fast.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fs_device.hpp"
+#include "fs_internal.hpp"
+#include "fs_launch.hpp"
+
+namespace fs {
+
+namespace {
+
+__device__ __forceinline__ void node_ijk(const Dims& d, int64_t n, int& i, int& j, int& k) {
+  i = (int)(n % d.nx);
+  j = (int)((n / d.nx) % d.ny);
+  k = (int)(n / d.sxy);
+}
+
+// rhs = Aexp x + sum_s scale[s] * src[s]   (Aexp may be absent)
+__global__ __launch_bounds__(kBlock) void k_explicit(const fs_op A, int has_a,
+                                                     const double* __restrict__ x,
+                                                     const double* __restrict__ src,
+                                                     const double* __restrict__ scale, int nsrc,
+                                                     int64_t vs, double* __restrict__ rhs) {
+  const Dims d = A.d;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < d.n;
+       n += (int64_t)gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(d, n, i, j, k);
+    double y = has_a ? raw_row(x, A.w[0], A.w[1], A.w[2], A.extra, d, i, j, k, n) : 0.0;
+    for (int s = 0; s < nsrc; ++s) y += scale[s] * src[(int64_t)s * vs + n];
+    rhs[n] = y;
+  }
+}
+
+// Dirichlet lifting with per-group values (apply_dirichlet,
+// fieldspice/operators.py:395-407): free rows b = rhs + sum over fixed
+// neighbours of w*v, b_s = b*dinv; ||b||^2 (Dirichlet rows carry their value,
+// pinned rows 0) into the partials.
+__global__ __launch_bounds__(kBlock) void k_lift_groups(const fs_op K,
+                                                        const double* __restrict__ rhs,
+                                                        const int32_t* __restrict__ group,
+                                                        const double* __restrict__ vals,
+                                                        const uint8_t* __restrict__ mask,
+                                                        const double* __restrict__ dinv,
+                                                        double* __restrict__ bs,
+                                                        double* __restrict__ partials) {
+  const Dims d = K.d;
+  double acc[1] = {0.0};
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < d.n;
+       n += (int64_t)gridDim.x * kBlock) {
+    const uint8_t m = mask[n];
+    if (m == 2) {
+      bs[n] = 0.0;
+      continue;
+    }
+    if (m == 1) {
+      const double v = vals[group[n]];
+      acc[0] += v * v;
+      bs[n] = 0.0;
+      continue;
+    }
+    int i, j, k;
+    node_ijk(d, n, i, j, k);
+    double b = rhs ? rhs[n] : 0.0;
+    int g;
+    if (i + 1 < d.nx && (g = group[n + 1]) >= 0) b += K.w[0][n] * vals[g];
+    if (i > 0 && (g = group[n - 1]) >= 0) b += K.w[0][n - 1] * vals[g];
+    if (j + 1 < d.ny && (g = group[n + d.nx]) >= 0) b += K.w[1][n] * vals[g];
+    if (j > 0 && (g = group[n - d.nx]) >= 0) b += K.w[1][n - d.nx] * vals[g];
+    if (k + 1 < d.nz && (g = group[n + d.sxy]) >= 0) b += K.w[2][n] * vals[g];
+    if (k > 0 && (g = group[n - d.sxy]) >= 0) b += K.w[2][n - d.sxy] * vals[g];
+    acc[0] += b * b;
+    bs[n] = b * dinv[n];
+  }
+  block_sum<1>(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
+}
+
+// Warm start: x_s = x_prev*dhalf on free rows, r = b_s - A_s x_s, p = 0.
+// Partials: ||r||^2 (quantity 0) and ||b_s||^2 (quantity 1).
+__global__ __launch_bounds__(kBlock) void k_init_warm(const double* __restrict__ xprev,
+                                                      const double* __restrict__ dhalf,
+                                                      const double* __restrict__ cx,
+                                                      const double* __restrict__ cy,
+                                                      const double* __restrict__ cz, Dims d,
+                                                      const double* __restrict__ bs,
+                                                      double* __restrict__ xs,
+                                                      double* __restrict__ r,
+                                                      double* __restrict__ p,
+                                                      double* __restrict__ partials) {
+  double acc[2] = {0.0, 0.0};
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < d.n;
+       n += (int64_t)gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(d, n, i, j, k);
+    const double xc = xprev[n] * dhalf[n];  // dhalf = 0 off the free rows
+    double s = 0.0;
+    if (i + 1 < d.nx) s += cx[n] * (xprev[n + 1] * dhalf[n + 1]);
+    if (i > 0) s += cx[n - 1] * (xprev[n - 1] * dhalf[n - 1]);
+    if (j + 1 < d.ny) s += cy[n] * (xprev[n + d.nx] * dhalf[n + d.nx]);
+    if (j > 0) s += cy[n - d.nx] * (xprev[n - d.nx] * dhalf[n - d.nx]);
+    if (k + 1 < d.nz) s += cz[n] * (xprev[n + d.sxy] * dhalf[n + d.sxy]);
+    if (k > 0) s += cz[n - d.sxy] * (xprev[n - d.sxy] * dhalf[n - d.sxy]);
+    const double b = bs[n];
+    const double rv = b - (xc - s);
+    xs[n] = xc;
+    r[n] = rv;
+    p[n] = 0.0;
+    acc[0] += rv * rv;
+    acc[1] += b * b;
+  }
+  block_sum<2>(acc);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = acc[0];
+    partials[gridDim.x + blockIdx.x] = acc[1];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_init_warm_final(const double* __restrict__ partials,
+                                                            int nb, double tol, int maxiter,
+                                                            CgScalars* sc) {
+  const double s0 = sum_partials(partials, nb);
+  const double s1 = sum_partials(partials + nb, nb);
+  if (threadIdx.x != 0) return;
+  CgScalars& c = sc[0];
+  c.rho = s0;
+  c.bnorm2 = s1;
+  c.thresh2 = tol * tol * s1;
+  c.beta = 0.0;
+  c.alpha = 0.0;
+  c.iter = 0;
+  c.cycle_iter = 0;
+  c.maxiter = maxiter;
+  c.done = (s1 == 0.0 || s0 < c.thresh2) ? 1 : 0;
+  c.cycle_done = 0;
+  c.refreshes = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bnorm_final(const double* __restrict__ partials,
+                                                        int nb, CgScalars* sc) {
+  const double s0 = sum_partials(partials, nb);
+  if (threadIdx.x == 0) sc[0].aux2 = s0;
+}
+
+// x = x_s*dinv on free rows, the group value on Dirichlet rows, 0 pinned.
+__global__ __launch_bounds__(kBlock) void k_expand_groups(const double* __restrict__ xs,
+                                                          const uint8_t* __restrict__ mask,
+                                                          const double* __restrict__ dinv,
+                                                          const int32_t* __restrict__ group,
+                                                          const double* __restrict__ vals,
+                                                          int64_t n, double* __restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    const uint8_t m = mask[i];
+    x[i] = m == 0 ? xs[i] * dinv[i] : (m == 1 ? vals[group[i]] : 0.0);
+  }
+}
+
+// Probe currents and voltages: grid (nbx, nprobe); partials
+// [(probe*2 + q)*nbx + bx], q = 0 current, 1 voltage.
+__global__ __launch_bounds__(kBlock) void k_probe(const fs_op A1, const fs_op A2, int has_b,
+                                                  const double* __restrict__ x,
+                                                  const double* __restrict__ xprev,
+                                                  const double* __restrict__ w,
+                                                  const int64_t* __restrict__ nodes,
+                                                  const int64_t* __restrict__ offs,
+                                                  double* __restrict__ partials) {
+  const Dims d = A1.d;
+  const int pb = blockIdx.y;
+  double acc[2] = {0.0, 0.0};
+  for (int64_t t = offs[pb] + (int64_t)blockIdx.x * kBlock + threadIdx.x; t < offs[pb + 1];
+       t += (int64_t)gridDim.x * kBlock) {
+    const int64_t n = nodes[t];
+    int i, j, k;
+    node_ijk(d, n, i, j, k);
+    double cur = raw_row_diff(x, xprev, A1.w[0], A1.w[1], A1.w[2], A1.extra, d, i, j, k, n);
+    if (has_b) cur += raw_row(x, A2.w[0], A2.w[1], A2.w[2], A2.extra, d, i, j, k, n);
+    acc[0] += cur;
+    acc[1] += w[n] * x[n];
+  }
+  block_sum<2>(acc);
+  if (threadIdx.x == 0) {
+    partials[((int64_t)pb * 2 + 0) * gridDim.x + blockIdx.x] = acc[0];
+    partials[((int64_t)pb * 2 + 1) * gridDim.x + blockIdx.x] = acc[1];
+  }
+}
+
+// one block per probe: out_i[pb], out_v[pb] of this step
+__global__ __launch_bounds__(kBlock) void k_probe_final(const double* __restrict__ partials,
+                                                        int nbx, double* out_i,
+                                                        double* out_v) {
+  const int pb = blockIdx.x;
+  const double si = sum_partials(partials + ((int64_t)pb * 2 + 0) * nbx, nbx);
+  const double sv = sum_partials(partials + ((int64_t)pb * 2 + 1) * nbx, nbx);
+  if (threadIdx.x == 0) {
+    if (out_i) out_i[pb] = si;
+    out_v[pb] = sv;
+  }
+}
+
+// Per-step reductions: sum x, max x, x^T Qa x, x^T Qb x (each edge once).
+__device__ __forceinline__ double quad_row(const double* __restrict__ x, const fs_op& Q,
+                                           const Dims& d, int i, int j, int k, int64_t n) {
+  const double xc = x[n];
+  double v = 0.0;
+  if (i + 1 < d.nx) { const double t = xc - x[n + 1]; v += Q.w[0][n] * t * t; }
+  if (j + 1 < d.ny) { const double t = xc - x[n + d.nx]; v += Q.w[1][n] * t * t; }
+  if (k + 1 < d.nz) { const double t = xc - x[n + d.sxy]; v += Q.w[2][n] * t * t; }
+  if (Q.extra) v += Q.extra[n] * xc * xc;
+  return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_step_stats(const double* __restrict__ x, Dims d,
+                                                       const fs_op Qa, int has_a, const fs_op Qb,
+                                                       int has_b, double* __restrict__ partials) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  double mx = -INFINITY;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < d.n;
+       n += (int64_t)gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(d, n, i, j, k);
+    const double xc = x[n];
+    acc[0] += xc;
+    mx = fmax(mx, xc);
+    if (has_a) acc[1] += quad_row(x, Qa, d, i, j, k, n);
+    if (has_b) acc[2] += quad_row(x, Qb, d, i, j, k, n);
+  }
+  block_sum<3>(acc);
+  for (int off = kWave / 2; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, kWave));
+  __shared__ double smx[kBlock / kWave];
+  if (threadIdx.x % kWave == 0) smx[threadIdx.x / kWave] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = smx[0];
+    for (int w = 1; w < kBlock / kWave; ++w) m = fmax(m, smx[w]);
+    partials[0 * gridDim.x + blockIdx.x] = acc[0];
+    partials[1 * gridDim.x + blockIdx.x] = m;
+    partials[2 * gridDim.x + blockIdx.x] = acc[1];
+    partials[3 * gridDim.x + blockIdx.x] = acc[2];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_step_stats_final(const double* __restrict__ partials,
+                                                             int nb, double* out) {
+  const double s0 = sum_partials(partials, nb);
+  const double s2 = sum_partials(partials + 2 * (int64_t)nb, nb);
+  const double s3 = sum_partials(partials + 3 * (int64_t)nb, nb);
+  double m = -INFINITY;
+  for (int b = threadIdx.x; b < nb; b += kBlock) m = fmax(m, partials[nb + b]);
+  for (int off = kWave / 2; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, kWave));
+  __shared__ double smx[kBlock / kWave];
+  if (threadIdx.x % kWave == 0) smx[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = smx[0];
+    for (int w = 1; w < kBlock / kWave; ++w) mm = fmax(mm, smx[w]);
+    out[0] = s0;
+    out[1] = mm;
+    out[2] = s2;
+    out[3] = s3;
+  }
+}
+
+// q_n = 1/2 sum over the <= 6 edges at n of w_e (x_n - x_m)^2
+__global__ __launch_bounds__(kBlock) void k_edge_power(const fs_op A, const double* __restrict__ x,
+                                                       double* __restrict__ q) {
+  const Dims d = A.d;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < d.n;
+       n += (int64_t)gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(d, n, i, j, k);
+    const double xc = x[n];
+    double v = 0.0;
+    if (i + 1 < d.nx) { const double t = xc - x[n + 1]; v += A.w[0][n] * t * t; }
+    if (i > 0) { const double t = xc - x[n - 1]; v += A.w[0][n - 1] * t * t; }
+    if (j + 1 < d.ny) { const double t = xc - x[n + d.nx]; v += A.w[1][n] * t * t; }
+    if (j > 0) { const double t = xc - x[n - d.nx]; v += A.w[1][n - d.nx] * t * t; }
+    if (k + 1 < d.nz) { const double t = xc - x[n + d.sxy]; v += A.w[2][n] * t * t; }
+    if (k > 0) { const double t = xc - x[n - d.sxy]; v += A.w[2][n - d.sxy] * t * t; }
+    q[n] = 0.5 * v;
+  }
+}
+
+int grid_for(int64_t n, int cap) {
+  const int64_t nb = (n + kBlock - 1) / kBlock;
+  return (int)(nb < cap ? (nb > 0 ? nb : 1) : cap);
+}
+
+}  // namespace
+
+void launch_explicit(fs_sys* s, const fs_op* A, const double* x, const double* src,
+                     const double* scale, int nsrc, int64_t vs, double* rhs) {
+  fs_op dummy = A ? *A : *s->op;
+  hipLaunchKernelGGL(k_explicit, dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream, dummy,
+                     A ? 1 : 0, x, src, scale, nsrc, vs, rhs);
+}
+
+void launch_lift_groups(fs_sys* s, const double* rhs, const int32_t* group, const double* vals) {
+  hipLaunchKernelGGL(k_lift_groups, dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream,
+                     *s->op, rhs, group, vals, s->mask, s->dinv, s->bs, s->partials);
+  hipLaunchKernelGGL(k_bnorm_final, dim3(1), dim3(kBlock), 0, s->ctx->stream, s->partials,
+                     s->vec_blocks, s->sc);
+}
+
+void launch_init_warm(fs_sys* s, const double* xprev, double tol, int maxiter) {
+  hipLaunchKernelGGL(k_init_warm, dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream, xprev,
+                     s->dhalf, s->c64[0], s->c64[1], s->c64[2], s->d, s->bs, s->x, s->r, s->p,
+                     s->partials);
+  hipLaunchKernelGGL(k_init_warm_final, dim3(1), dim3(kBlock), 0, s->ctx->stream, s->partials,
+                     s->vec_blocks, tol, maxiter, s->sc);
+}
+
+void launch_expand_groups(fs_sys* s, const int32_t* group, const double* vals, double* x) {
+  hipLaunchKernelGGL(k_expand_groups, dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream,
+                     s->x, s->mask, s->dinv, group, vals, s->d.n, x);
+}
+
+void launch_probe(fs_sys* s, const fs_op* A1, const fs_op* A2, const double* x,
+                  const double* xprev, const double* w, const int64_t* nodes,
+                  const int64_t* offs, int nprobe, int nbx, double* out_i, double* out_v) {
+  fs_op b = A2 ? *A2 : *A1;
+  hipLaunchKernelGGL(k_probe, dim3(nbx, nprobe), dim3(kBlock), 0, s->ctx->stream, *A1, b,
+                     A2 ? 1 : 0, x, xprev, w, nodes, offs, s->partials);
+  hipLaunchKernelGGL(k_probe_final, dim3(nprobe), dim3(kBlock), 0, s->ctx->stream, s->partials,
+                     nbx, out_i, out_v);
+}
+
+void launch_step_stats(fs_sys* s, const double* x, const fs_op* Qa, const fs_op* Qb,
+                       double* out) {
+  fs_op a = Qa ? *Qa : *s->op, b = Qb ? *Qb : *s->op;
+  hipLaunchKernelGGL(k_step_stats, dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream, x,
+                     s->d, a, Qa ? 1 : 0, b, Qb ? 1 : 0, s->partials);
+  hipLaunchKernelGGL(k_step_stats_final, dim3(1), dim3(kBlock), 0, s->ctx->stream, s->partials,
+                     s->vec_blocks, out);
+}
+
+void launch_edge_power(hipStream_t st, const fs_op* A, const double* x, double* q, int ncu) {
+  hipLaunchKernelGGL(k_edge_power, dim3(grid_for(A->d.n, 8 * ncu)), dim3(kBlock), 0, st, *A, x,
+                     q);
+}
+
+}  // namespace fs

@@ -184,6 +184,16 @@ class RectilinearGrid:
         return tuple(int(np.argmin(np.abs(a.nodes - p)))
                      for a, p in zip(self._axes, pt))  # type: ignore
 
+    def courant_dt(self, eps_r_min: float = 1.0, mu_r_min: float = 1.0) -> float:
+        """Explicit-FDTD stability step (``fieldspice/grid.py:483-501``), used to
+        report how many explicit steps one implicit step replaces."""
+        from .units import c0
+        v = 1.0 / np.sqrt(eps_r_min * mu_r_min)
+        inv2 = sum(1.0 / float(a.h.min()) ** 2 for a in self._axes if not a.collapsed)
+        if inv2 == 0.0:
+            inv2 = 1.0 / float(self.ax.h.min()) ** 2
+        return 1.0 / (c0 * v * np.sqrt(inv2))
+
     def __repr__(self) -> str:
         nx, ny, nz = self.ncell
         return f"<RectilinearGrid {nx}x{ny}x{nz}, {self.n_nodes:,} nodes>"

@@ -1,0 +1,193 @@
+"""Electroquasistatic transients on the GPU (``fieldspice/solvers/eqs.py``).
+
+Backward Euler / theta stepping of
+
+    (L_eps/dt + theta L_sigma) phi_k = (L_eps/dt - (1-theta) L_sigma) phi_{k-1} + i_inject
+
+with voltage-driven terminals and Dirichlet walls imposed every step.  The
+whole time loop runs on the device (``fs_transient_run``): the explicit
+product, the Dirichlet lifting with that step's values, the CG solve
+(warm-started from the previous step), the terminal currents and voltages and
+the stored energy / dissipation.  The matrix is prepared once, as the
+reference's cached factorisation is.
+"""
+
+from __future__ import annotations
+
+from typing import Sequence
+
+import numpy as np
+
+from .. import _lib
+from ..boundaries import BoundarySpec
+from .base import Result, SolverConfig, Terminal, _Timer, check_terminals
+from .linear import GridLinearSystem
+from .transient import DirichletGroups, source_basis
+
+__all__ = ["EQSSolver"]
+
+
+class EQSSolver:
+    """Time-domain electroquasistatic solver (``fieldspice/solvers/eqs.py:42-386``)."""
+
+    name = "eqs"
+    assumptions = ("A1a", "A2", "A3", "A10", "A12")
+
+    def __init__(self, grid, eps_cell, sigma_cell, config: SolverConfig | None = None,
+                 operators=None, *, eps_mode: str = "parallel", sigma_mode: str = "parallel"):
+        self.grid = grid
+        self.cfg = config or SolverConfig()
+        eps_cell = np.asarray(eps_cell, dtype=float)
+        sigma_cell = np.asarray(sigma_cell, dtype=float)
+        for arr, nm in ((eps_cell, "eps_cell"), (sigma_cell, "sigma_cell")):
+            if arr.shape != grid.shape_cells:
+                raise ValueError(f"{nm} must have shape {grid.shape_cells}, got {arr.shape}")
+        if np.any(eps_cell <= 0.0):
+            raise ValueError("eps_cell must be positive and ABSOLUTE [F/m]; multiply a "
+                             "relative permittivity by fieldspice_amd.units.eps0")
+        if float(np.min(eps_cell)) > 1e-6:
+            raise ValueError(f"eps_cell looks like a RELATIVE permittivity (min "
+                             f"{float(np.min(eps_cell)):.4g}); it must be absolute [F/m].")
+        if np.any(sigma_cell < 0.0):
+            raise ValueError("sigma_cell must be non-negative [S/m]")
+        self.eps_cell = eps_cell
+        self.sigma_cell = sigma_cell
+        self.L_eps = _lib.DeviceOperator.from_cells(grid, eps_cell, eps_mode)
+        self.L_sigma = _lib.DeviceOperator.from_cells(grid, sigma_cell, sigma_mode)
+        self._node_vol = grid.node_volumes().ravel()
+
+    # -- bookkeeping ---------------------------------------------------------
+    def _dirichlet(self, terminals, bc, t: float):
+        """Fixed nodes and values; walls after terminals, later wins
+        (``fieldspice/solvers/eqs.py:108-129``)."""
+        return DirichletGroups(self.grid, terminals, bc, walls_last=True).node_values(t)
+
+    def _node_weights(self, nodes: np.ndarray) -> np.ndarray:
+        """Dual-volume weights of a terminal (``fieldspice/solvers/eqs.py:131-154``)."""
+        w = self._node_vol[nodes]
+        tot = float(w.sum())
+        return w / tot if tot > 0 else np.full(nodes.size, 1.0 / nodes.size)
+
+    def _inject(self, terminals, t: float) -> np.ndarray:
+        rhs = np.zeros(self.grid.n_nodes)
+        for term in terminals:
+            if term.driven == "current":
+                rhs[term.nodes] += float(term.value_at(t)) * self._node_weights(term.nodes)
+        return rhs
+
+    def _static_solve(self, op, terminals, bc, t, rhs=None):
+        fixed, vals = self._dirichlet(terminals, bc, t)
+        ls = GridLinearSystem(op, fixed, self.cfg)
+        x0 = np.zeros(self.grid.n_nodes)
+        x0[fixed] = vals
+        b = (np.zeros(self.grid.n_nodes) if rhs is None else rhs) - op.apply(x0)
+        b[fixed] = vals
+        return ls.solve(b), ls
+
+    def consistent_initial_condition(self, terminals, bc=None, t: float = 0.0) -> np.ndarray:
+        """Electrostatic potential at ``t`` (``fieldspice/solvers/eqs.py:176-195``)."""
+        fixed, _ = self._dirichlet(terminals, bc, t)
+        if fixed.size == 0:
+            return np.zeros(self.grid.n_nodes)
+        phi, _ = self._static_solve(self.L_eps, terminals, bc, t)
+        return phi
+
+    def steady_state(self, terminals, bc=None, t: float = 0.0) -> Result:
+        """DC resistive solve (``fieldspice/solvers/eqs.py:197-227``)."""
+        timer = _Timer()
+        terms = check_terminals(terminals, self.grid)
+        phi, ls = self._static_solve(self.L_sigma, terms, bc, t, rhs=self._inject(terms, t))
+        res = Result(grid=self.grid, t=np.zeros(1), fields={"phi": phi[None, :]})
+        q = self.L_sigma.apply(phi)
+        for term in terms:
+            w = self._node_weights(term.nodes)
+            res.terminals[term.name] = {"v": np.array([float(w @ phi[term.nodes])]),
+                                        "i": np.array([float(q[term.nodes].sum())])}
+        res.scalars["dissipation"] = np.array([float(phi @ q)])
+        return timer.finish(res, self.name, self.assumptions, self.grid, mode="steady_state",
+                            linear=dict(ls.info))
+
+    # -- transient -------------------------------------------------------------
+    def solve(self, terminals: Sequence[Terminal], t_end: float, dt: float, bc=None,
+              theta: float = 1.0, monitors=None, phi0=None, store: bool = False,
+              store_every: int = 1, t_start: float = 0.0, warm_start: bool = True) -> Result:
+        """Step from ``t_start`` to ``t_end`` (``fieldspice/solvers/eqs.py:229-323``)."""
+        if dt <= 0.0:
+            raise ValueError("dt must be positive")
+        if t_end <= t_start:
+            raise ValueError("t_end must exceed t_start")
+        if not 0.0 <= theta <= 1.0:
+            raise ValueError("theta must lie in [0, 1]")
+        if monitors is not None:
+            raise NotImplementedError("field monitors are not part of the GPU time loop")
+        timer = _Timer()
+        g = self.grid
+        terms = check_terminals(terminals, g)
+        if bc is not None and not isinstance(bc, BoundarySpec):
+            raise ValueError("bc must be a BoundarySpec")
+        n = g.n_nodes
+        phi = (self.consistent_initial_condition(terms, bc, t_start) if phi0 is None
+               else np.asarray(phi0, dtype=float).ravel().copy())
+        if phi.shape != (n,):
+            raise ValueError(f"phi0 must have shape ({n},)")
+        nsteps = int(np.ceil((t_end - t_start) / dt))
+        times = t_start + dt * np.arange(nsteps + 1)
+        K = self.L_eps.combine(1.0 / dt, theta, self.L_sigma)
+        Kexp = self.L_eps.combine(1.0 / dt, -(1.0 - theta), self.L_sigma)
+        groups = DirichletGroups(g, terms, bc, walls_last=True)
+        ls = GridLinearSystem(K, groups.owner >= 0, self.cfg)
+        values = np.stack([groups.values_at(t) for t in times]) if groups.ngroups else None
+        # current-driven terminals: dual-volume weighted injection
+        src = scales = None
+        cur_terms = [t for t in terms if t.driven == "current"]
+        if cur_terms:
+            src = np.zeros((len(cur_terms), n))
+            for m, t in enumerate(cur_terms):
+                src[m, t.nodes] = self._node_weights(t.nodes)
+            scales = np.array([[float(t.value_at(tt)) for t in cur_terms] for tt in times])
+        probe = np.full(n, -1, dtype=np.int32)
+        pw = np.zeros(n)
+        for m, t in enumerate(terms):
+            probe[t.nodes] = m
+            pw[t.nodes] = self._node_weights(t.nodes)
+        cur_a = self.L_eps.combine(1.0 / dt)
+        out = ls.sys.transient(
+            Kexp, phi, nsteps, group=groups.owner if groups.ngroups else None, values=values,
+            src=src, src_scale=scales, probe=probe if terms else None, probe_w=pw,
+            nprobe=len(terms), cur_a=cur_a, cur_b=self.L_sigma, quad_a=self.L_eps,
+            quad_b=self.L_sigma, store_every=max(1, store_every) if store else 0,
+            warm_start=warm_start, params=ls.params())
+        ls._judge(out["stats"], out["rc"])
+        res = Result(grid=g, t=times)
+        if store:
+            res.fields["phi"] = out["stored"]
+            se = max(1, store_every)
+            res.scalars["t_stored"] = times[::se][:out["stored"].shape[0]]
+        for m, t in enumerate(terms):
+            res.terminals[t.name] = {"v": out["probe_v"][:, m].copy(),
+                                     "i": out["probe_i"][:, m].copy()}
+        ss = out["step_stats"]
+        res.scalars["energy_electric"] = 0.5 * ss[:, 2]
+        res.scalars["dissipation"] = ss[:, 3]
+        res.meta["linear"] = dict(ls.info)
+        res.meta["phi_final"] = out["x"]
+        return timer.finish(res, self.name, self.assumptions, g, mode="transient", theta=theta,
+                            dt=dt, n_factorisations=1,
+                            speedup_vs_courant=dt / g.courant_dt(eps_r_min=1.0))
+
+    # -- derived quantities ------------------------------------------------------
+    def terminal_current(self, phi_new, phi_old, dt, nodes, theta: float = 1.0) -> float:
+        resid = (self.L_eps.apply(np.asarray(phi_new) - np.asarray(phi_old)) / dt
+                 + self.L_sigma.apply(theta * np.asarray(phi_new)
+                                      + (1 - theta) * np.asarray(phi_old)))
+        return float(resid[nodes].sum())
+
+    def stored_energy(self, phi) -> float:
+        return 0.5 * float(np.asarray(phi) @ self.L_eps.apply(phi))
+
+    def dissipation(self, phi) -> float:
+        return float(np.asarray(phi) @ self.L_sigma.apply(phi))
+
+    def capacitance_matrix(self, terminals, bc=None):
+        from .poisson import PoissonSolver
+        return PoissonSolver(self.grid, self.eps_cell, self.cfg).capacitance_matrix(terminals, bc)

@@ -155,6 +155,48 @@ int fs_cap_solve(fs_cap* cap, const fs_op* L, double* C, double* q_shield, doubl
                  const fs_solve_params* params, fs_solve_stats* stats);
 int fs_cap_destroy(fs_cap* cap);
 
+/* ---- implicit time loops: EQS and thermal steps ------------------------- */
+/* One field x advanced by
+ *     K x_k = Aexp x_{k-1} + sum_s src_scale[k][s] * src[s]      (free rows)
+ *     x_k[n] = values[k][group[n]]                              (Dirichlet rows)
+ * for k = 1..nsteps, where K is the matrix of sys.  This is the per-step loop
+ * of EQSSolver.solve (fieldspice/solvers/eqs.py:291-307) and
+ * ThermalSolver.transient (fieldspice/solvers/thermal.py:271-285), with the
+ * solve, the explicit product, the Dirichlet lifting and the recorded
+ * quantities on the GPU.  Arrays are host layout. */
+typedef struct {
+    int nsteps;
+    int ngroups;               /* Dirichlet groups (terminals, walls) */
+    const int32_t* group;      /* per node: -1 not Dirichlet, else group id; must equal sys's mask */
+    const double* values;      /* [(nsteps+1) * ngroups] group values per time point */
+    int nsrc;                  /* source vectors */
+    const double* src;         /* [nsrc * n_nodes] */
+    const double* src_scale;   /* [(nsteps+1) * nsrc] */
+    int warm_start;            /* start each solve from the previous field */
+    int nprobe;                /* recorded terminals */
+    const int32_t* probe;      /* per node: -1 or probe id */
+    const double* probe_w;     /* per node weight of the probe voltage (sum over probe = 1) */
+    const fs_op* cur_a;        /* I_m(k) = sum_{n in m} [cur_a (x_k - x_{k-1}) + cur_b x_k]_n */
+    const fs_op* cur_b;        /* may be NULL */
+    const fs_op* quad_a;       /* per-step x^T quad_a x (may be NULL) */
+    const fs_op* quad_b;       /* per-step x^T quad_b x (may be NULL) */
+    int store_every;           /* copy x_k to `stored` every store_every steps (0: never) */
+} fs_transient_desc;
+
+/* x: in x_0, out x_nsteps.  Outputs (NULL to skip):
+ *   stored      [nstore * n_nodes], nstore = nsteps/store_every + 1 (x_0 first)
+ *   probe_i     [(nsteps+1) * nprobe] currents (row 0 copies row 1)
+ *   probe_v     [(nsteps+1) * nprobe] weighted probe voltages
+ *   step_stats  [(nsteps+1) * 4]: sum(x), max(x), x^T quad_a x, x^T quad_b x */
+int fs_transient_run(fs_sys* sys, const fs_op* Aexp, const fs_transient_desc* desc,
+                     double* x, double* stored, double* probe_i, double* probe_v,
+                     double* step_stats, const fs_solve_params* params,
+                     fs_solve_stats* stats);
+
+/* Joule heat per node: q_n = 1/2 sum_{e at n} w_e (x_n - x_m)^2
+ * (joule_heating_nodes, fieldspice/solvers/thermal.py:66-87). */
+int fs_op_edge_power(fs_op* op, const double* x, double* q);
+
 /* Record HIP events around every stencil launch of the Krylov loop of this
  * system and report their summed duration in fs_solve_stats (off by default). */
 int fs_sys_set_profiling(fs_sys* sys, int on);

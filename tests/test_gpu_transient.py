@@ -1,0 +1,235 @@
+"""GPU EQS and thermal implicit steps against closed forms and the oracle."""
+
+import warnings
+
+import numpy as np
+import pytest
+
+import fs_oracle as O
+from fieldspice_amd.boundaries import BoundarySpec, Dirichlet
+from fieldspice_amd.grid import RectilinearGrid
+from fieldspice_amd.solvers.base import SolverConfig, Terminal
+from fieldspice_amd.solvers.eqs import EQSSolver
+from fieldspice_amd.solvers.thermal import ThermalSolver, joule_heating_nodes
+from fieldspice_amd.units import eps0
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg):
+    print(f"[gpu case] {msg}", flush=True)
+
+
+def _rc_stack(n=60):
+    d, W, H = 4e-6, 3e-6, 2e-6
+    g = RectilinearGrid(np.linspace(0, d, n + 1), np.linspace(0, W, 4), np.linspace(0, H, 3))
+    left = g.xc < d / 2
+    sig = np.zeros(g.shape_cells)
+    sig[left] = 1e3
+    eps = np.empty(g.shape_cells)
+    eps[left] = 4.0 * eps0
+    eps[~left] = 2.0 * eps0
+    A = W * H
+    R1 = (d / 2) / (1e3 * A)
+    C1, C2 = 4.0 * eps0 * A / (d / 2), 2.0 * eps0 * A / (d / 2)
+    return g, eps, sig, R1, C1, C2, n
+
+
+def _eqs_run(steps_per_tau):
+    g, eps, sig, R1, C1, C2, n = _rc_stack()
+    tau = R1 * (C1 + C2)
+    v0 = C1 / (C1 + C2)
+    nid = g.node_ids()
+    s = EQSSolver(g, eps, sig, SolverConfig(verbose=0))
+    hi = Terminal("hi", nid[0].ravel(), voltage=1.0)
+    lo = Terminal("lo", nid[-1].ravel(), voltage=0.0)
+    r = s.solve([hi, lo], t_end=8 * tau, dt=tau / steps_per_tau, store=True)
+    vm = r.fields["phi"][:, nid[n // 2, 1, 1]]
+    ts = r.scalars["t_stored"]
+    ana = 1.0 - (1.0 - v0) * np.exp(-ts / tau)
+    return r, vm, ana, tau, v0
+
+
+def test_eqs_initial_condition_and_analytic_rc():
+    _case("EQS RC stack: divider and analytic step (400 steps/tau)")
+    _, vm, _, _, v0 = _eqs_run(50)
+    assert v0 == pytest.approx(2.0 / 3.0, rel=1e-12)
+    assert vm[0] == pytest.approx(v0, rel=1e-9)
+    _, vm, ana, _, _ = _eqs_run(400)
+    assert np.abs(vm - ana).max() < 2e-4
+
+
+def test_eqs_backward_euler_first_order_and_published_error():
+    _case("EQS BE order sequence 50/100/200/400")
+    errs = [np.abs(v - a).max() for _, v, a, _, _ in (_eqs_run(s) for s in (50, 100, 200, 400))]
+    assert errs[0] == pytest.approx(1.216e-3, rel=0.05)   # docs/CONTRACTS.md:474-476
+    for lo, hi in zip(errs[:-1], errs[1:]):
+        assert 1.8 < lo / hi < 2.2
+
+
+def test_eqs_currents_conserve_and_one_factorisation():
+    _case("EQS current conservation")
+    r, *_ = _eqs_run(100)
+    total = r.i("hi") + r.i("lo")
+    assert np.abs(total).max() / max(np.abs(r.i("hi")).max(), 1e-30) < 1e-8
+    assert r.meta["n_factorisations"] == 1
+    assert "A1a" in r.meta["assumptions"]
+
+
+def test_eqs_steady_state_slab_resistance():
+    _case("EQS steady slab R")
+    d, W, H, sigma = 2e-6, 5e-6, 4e-6, 1e4
+    g = RectilinearGrid(np.linspace(0, d, 16), np.linspace(0, W, 6), np.linspace(0, H, 5))
+    s = EQSSolver(g, np.full(g.shape_cells, 3.9 * eps0), np.full(g.shape_cells, sigma))
+    nid = g.node_ids()
+    r = s.steady_state([Terminal("hi", nid[0].ravel(), voltage=1.0),
+                        Terminal("lo", nid[-1].ravel(), voltage=0.0)])
+    R = 1.0 / abs(float(r.i("hi")[0]))
+    assert R == pytest.approx(d / (sigma * W * H), rel=1e-9)
+
+
+def _eqs_parity_case(floating: bool, tol: float, rtol: float):
+    g = RectilinearGrid(np.linspace(0, 1.2e-7, 13), np.linspace(0, 1.0e-7, 11),
+                        np.linspace(0, 0.8e-7, 9))
+    cu = np.zeros(g.shape_cells, dtype=bool)
+    cu[1:4, 1:4, 1:4] = True   # driven box
+    cu[8:11, 6:9, 4:7] = True  # grounded box
+    if floating:
+        cu[4:7, 2:5, 5:7] = True   # floating copper
+    sig = np.where(cu, 5.8e7, 0.0)
+    eps = np.full(g.shape_cells, 3.9 * eps0)
+    nid = g.node_ids()
+    drv = Terminal("drv", nid[1:5, 1:5, 1:5].ravel(),
+                   voltage=lambda t: np.sin(2 * np.pi * 1e9 * t))
+    gnd = Terminal("gnd", nid[8:12, 6:10, 4:8].ravel(), voltage=0.0)
+    dt, nsteps = 1e-12, 40
+    cfg = SolverConfig(tol=tol, maxiter=20000)
+    r = EQSSolver(g, eps, sig, cfg).solve([drv, gnd], t_end=nsteps * dt, dt=dt, store=True,
+                                          store_every=10)
+
+    def fixed_fn(t):
+        idx = np.concatenate([drv.nodes, gnd.nodes])
+        vals = np.concatenate([np.full(drv.nodes.size, drv.value_at(t)),
+                               np.zeros(gnd.nodes.size)])
+        return O.unique_last(idx, vals)
+    ref = O.eqs_transient(g, eps, sig, fixed_fn, nsteps * dt, dt,
+                          terminal_nodes=[drv.nodes, gnd.nodes])
+    for k_store, k in enumerate(range(0, nsteps + 1, 10)):
+        a, b = r.fields["phi"][k_store], ref["phis"][k]
+        assert np.abs(a - b).max() <= rtol * max(np.abs(b).max(), 1e-12), k
+    for m, nm in enumerate(("drv", "gnd")):
+        ri, oi = r.i(nm), ref["currents"][m]
+        assert np.abs(ri - oi).max() <= 1e3 * rtol * np.abs(oi).max(), nm
+
+
+def test_eqs_multimaterial_transient_matches_oracle():
+    _case("EQS 3D two-phase transient, driven + grounded copper, sine drive, vs oracle")
+    _eqs_parity_case(floating=False, tol=1e-12, rtol=1e-9)
+
+
+def test_eqs_floating_island_transient_within_conditioning():
+    _case("EQS 3D two-phase transient with a floating copper island vs oracle")
+    # A floating island couples to the rest only through oxide
+    # (sigma*h / (eps*h/dt) ~ 1.7e6): the equilibrated matrix has
+    # cond ~ 1.4e7, so FP64 fixes the potential only to ~cond*u ~ 1.5e-9 per
+    # step (the oracle's own LU moves by ~1e-10 under refinement).  Both
+    # solutions are checked to a few cond*u over the 40 steps.
+    _eqs_parity_case(floating=True, tol=1e-14, rtol=1e-8)
+
+
+def _bar(n=40, L=1e-3, k=150.0):
+    g = RectilinearGrid(np.linspace(0, L, n + 1), np.linspace(0, 1e-3, 2), np.linspace(0, 1e-3, 2))
+    return g, ThermalSolver(g, np.full(g.shape_cells, k))
+
+
+def test_thermal_parabolic_profile_and_second_order():
+    _case("thermal parabolic profile, O(h^2)")
+    L, k, T0, qv = 1e-3, 150.0, 300.0, 1e9
+    g, s = _bar(40, L, k)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        r = s.steady(qv * g.node_volumes().ravel(),
+                     bc=BoundarySpec(xlo=Dirichlet(T0), xhi=Dirichlet(T0)))
+    T = r.fields["T"][0].reshape(g.shape_nodes)[:, 0, 0]
+    assert np.abs(T - (T0 + qv * g.xn * (L - g.xn) / (2 * k))).max() < 1e-9
+    errs = []
+    for n in (10, 20, 40, 80):
+        g, s = _bar(n, L, k)
+        X, _, _ = g.node_coords()
+        q = 1e9 * np.sin(np.pi * X.ravel() / L) * g.node_volumes().ravel()
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            r = s.steady(q, bc=BoundarySpec(xlo=Dirichlet(T0), xhi=Dirichlet(T0)))
+        T = r.fields["T"][0].reshape(g.shape_nodes)[:, 0, 0]
+        errs.append(np.abs(T - (T0 + 1e9 * L ** 2 / (np.pi ** 2 * k)
+                                * np.sin(np.pi * g.xn / L))).max())
+    for a, b in zip(errs[:-1], errs[1:]):
+        assert 3.6 < a / b < 4.4
+
+
+def test_thermal_slab_and_convective_resistance():
+    _case("thermal slab R_th and convective R_th")
+    Lz, k, A = 2e-4, 150.0, 1e-4 * 1e-4
+    g = RectilinearGrid(np.linspace(0, 1e-4, 9), np.linspace(0, 1e-4, 9), np.linspace(0, Lz, 25))
+    s = ThermalSolver(g, np.full(g.shape_cells, k))
+    r = s.steady(np.zeros(g.n_nodes), bc=BoundarySpec(zlo=Dirichlet(300.0), zhi=Dirichlet(301.0)))
+    Q = float(s.K.apply(r.fields["T"][0])[g.node_ids()[:, :, -1].ravel()].sum())
+    assert 1.0 / Q == pytest.approx(Lz / (k * A), rel=1e-9)
+    Lc, Wc, Hc, h = 100e-6, 20e-6, 20e-6, 2e4
+    g = RectilinearGrid(np.linspace(0, Lc, 21), np.linspace(0, Wc, 5), np.linspace(0, Hc, 5))
+    s = ThermalSolver(g, np.full(g.shape_cells, 5e6))
+    conv = {w: (h, 300.0) for w in ("xlo", "xhi", "ylo", "yhi", "zlo", "zhi")}
+    P = 1e-3
+    r = s.steady(np.full(g.n_nodes, P / g.n_nodes), convection=conv)
+    Asurf = 2 * (Lc * Wc + Lc * Hc + Wc * Hc)
+    assert (float(r.fields["T"][0].mean()) - 300.0) / P == pytest.approx(1.0 / (h * Asurf), rel=1e-5)
+
+
+def test_thermal_adiabatic_warns():
+    _case("thermal adiabatic warning")
+    g, s = _bar(10)
+    with pytest.warns(RuntimeWarning, match="adiabatic"):
+        s.steady(np.full(g.n_nodes, 1e-6))
+
+
+def test_thermal_transient_matches_oracle_rc_ladder():
+    _case("thermal transient vs oracle (theta 1 and 0.5)")
+    Lb, k, rcp, N, T0, P = 2e-4, 150.0, 2329.0 * 700.0, 20, 300.0, 0.05
+    W = H = 1e-5
+    g = RectilinearGrid(np.linspace(0, Lb, N + 1), np.linspace(0, W, 2), np.linspace(0, H, 2))
+    nid = g.node_ids()
+    src, sink = nid[0].ravel(), nid[-1].ravel()
+    tau = (Lb / (k * W * H)) * (rcp * W * H * Lb)
+    dt = tau / 200
+    q0 = np.zeros(g.n_nodes)
+    q0[src] = P / src.size
+    t_on = 0.1 * tau
+    qfn = lambda t: q0 if t >= t_on else 0.0 * q0   # noqa: E731
+    s = ThermalSolver(g, np.full(g.shape_cells, k), np.full(g.shape_cells, rcp),
+                      SolverConfig(tol=1e-12))
+    for theta in (1.0, 0.5):
+        r = s.transient(qfn, 0.5 * tau, dt, T0=T0, bc=BoundarySpec(xhi=Dirichlet(T0)),
+                        theta=theta, store=True)
+        times, Ts = O.thermal_transient(g, np.full(g.shape_cells, k), np.full(g.shape_cells, rcp),
+                                        qfn, 0.5 * tau, dt, T0,
+                                        fixed_fn=lambda t: (sink, np.full(sink.size, T0)),
+                                        theta=theta)
+        Tref = np.array(Ts)
+        assert np.abs(r.fields["T"] - Tref).max() <= 1e-9 * np.abs(Tref - T0).max()
+        assert np.abs(r.scalars["T_max"] - Tref.max(axis=1)).max() <= 1e-9 * Tref.max()
+
+
+def test_joule_heating_is_exactly_conservative():
+    _case("Joule heat conservation")
+    g = RectilinearGrid(np.linspace(0, 2e-6, 13), np.linspace(0, 5e-6, 6), np.linspace(0, 4e-6, 5))
+    se = O.cell_to_edge(g, np.full(g.shape_cells, 1e4))
+    L = O.nodal_laplacian(g, se)
+    nid = g.node_ids()
+    lo, hi = nid[0].ravel(), nid[-1].ravel()
+    A, b = O.apply_dirichlet(L, np.zeros(g.n_nodes), np.concatenate([lo, hi]),
+                             np.concatenate([np.ones(lo.size), np.zeros(hi.size)]))
+    import scipy.sparse.linalg as spl
+    phi = spl.spsolve(A.tocsc(), b)
+    q = joule_heating_nodes(g, se, phi)
+    assert q.sum() == pytest.approx(float(phi @ (L @ phi)), rel=1e-12)
+    assert (q >= -1e-30).all()
