@@ -24,6 +24,7 @@ FS_ENOMEM = -3
 FS_AVG = {"parallel": 0, "arithmetic": 0, "harmonic": 1, "series": 1, "min": 2, "max": 3}
 FS_PREC_FP64 = 0
 FS_PREC_MIXED = 1
+FS_PREC_FCG = 2
 
 EXPORTS = (
     "fs_ctx_create", "fs_ctx_destroy", "fs_last_error", "fs_ctx_device_info",
@@ -31,6 +32,10 @@ EXPORTS = (
     "fs_op_apply", "fs_op_get_edges", "fs_sys_create", "fs_sys_destroy",
     "fs_sys_solve", "fs_capacitance", "fs_cap_prepare", "fs_cap_solve", "fs_cap_destroy",
     "fs_sys_set_profiling", "fs_transient_run", "fs_op_edge_power",
+    "fs_net_create", "fs_net_destroy", "fs_net_num_params", "fs_net_get_params",
+    "fs_net_set_params", "fs_net_eval", "fs_pc_create", "fs_pc_destroy", "fs_pc_level_dims",
+    "fs_sys_set_precond", "fs_pc_apply", "fs_pc_add_pairs", "fs_pc_num_pairs",
+    "fs_pc_clear_pairs", "fs_pc_loss_grad", "fs_pc_train",
 )
 
 
@@ -43,7 +48,8 @@ class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("refreshes", C.c_int), ("converged", C.c_int),
                 ("residual", C.c_double), ("residual_eq", C.c_double),
                 ("solve_time", C.c_double), ("stencil_time", C.c_double),
-                ("stencil_launches", C.c_int64), ("stencil_bytes", C.c_double)]
+                ("stencil_launches", C.c_int64), ("stencil_bytes", C.c_double),
+                ("fallbacks", C.c_int), ("pc_applies", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -74,6 +80,7 @@ _ctx = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _vp = C.c_void_p
 
 
@@ -106,6 +113,23 @@ def _declare(lib):
     lib.fs_transient_run.argtypes = [_vp, _vp, C.POINTER(TransientDesc), _dp, _vp, _vp, _vp,
                                      _vp, C.POINTER(SolveParams), C.POINTER(SolveStats)]
     lib.fs_op_edge_power.argtypes = [_vp, _dp, _dp]
+    lib.fs_net_create.argtypes = [_vp, C.c_uint64, C.POINTER(_vp)]
+    lib.fs_net_destroy.argtypes = [_vp]
+    lib.fs_net_num_params.argtypes = [_vp, C.POINTER(C.c_int64)]
+    lib.fs_net_get_params.argtypes = [_vp, _fp]
+    lib.fs_net_set_params.argtypes = [_vp, _fp]
+    lib.fs_net_eval.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _fp, _fp, _vp, _vp]
+    lib.fs_pc_create.argtypes = [_vp, _vp, C.c_int, _i32p, C.c_double, C.POINTER(_vp)]
+    lib.fs_pc_destroy.argtypes = [_vp]
+    lib.fs_pc_level_dims.argtypes = [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]
+    lib.fs_sys_set_precond.argtypes = [_vp, _vp]
+    lib.fs_pc_apply.argtypes = [_vp, C.c_int, _dp, _dp]
+    lib.fs_pc_add_pairs.argtypes = [_vp, C.c_int, _dp]
+    lib.fs_pc_num_pairs.argtypes = [_vp, C.POINTER(C.c_int)]
+    lib.fs_pc_clear_pairs.argtypes = [_vp]
+    lib.fs_pc_loss_grad.argtypes = [_vp, C.c_int, _dp, _dp, _vp]
+    lib.fs_pc_train.argtypes = [_vp, C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]
     for name in EXPORTS:
         if name != "fs_last_error":
             getattr(lib, name).restype = C.c_int
@@ -353,6 +377,14 @@ class DeviceSystem:
     def set_profiling(self, on: bool) -> None:
         check(load().fs_sys_set_profiling(self._h, 1 if on else 0), "fs_sys_set_profiling")
 
+    def set_precond(self, pc: "DevicePreconditioner | None") -> None:
+        """Preconditioner of FS_PREC_FCG solves (the preconditioner keeps this
+        system alive; the system keeps only the handle)."""
+        if pc is not None and pc.system is not self:
+            raise ValueError("the preconditioner was built for another system")
+        check(load().fs_sys_set_precond(self._h, None if pc is None else pc._h),
+              "fs_sys_set_precond")
+
     def close(self):
         if getattr(self, "_h", None):
             load().fs_sys_destroy(self._h)
@@ -396,6 +428,136 @@ class DeviceCapacitance:
     def close(self):
         if getattr(self, "_h", None):
             load().fs_cap_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceNet:
+    """The convolutional network of the learned preconditioner (an ``fs_net``):
+    11 3x3x3 convolutions on three levels (16/32/64 channels), BF16 weights
+    and activations with FP32 accumulation, trained with Adam on the GPU."""
+
+    def __init__(self, seed: int = 0):
+        lib = load()
+        h = _vp()
+        check(lib.fs_net_create(context(), int(seed) & (2 ** 64 - 1), C.byref(h)),
+              "fs_net_create")
+        self._h = h
+        cnt = C.c_int64(0)
+        check(lib.fs_net_num_params(h, C.byref(cnt)), "fs_net_num_params")
+        self.num_params = int(cnt.value)
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.num_params, np.float32)
+        check(load().fs_net_get_params(self._h, out), "fs_net_get_params")
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32).ravel()
+        if p.size != self.num_params:
+            raise ValueError(f"expected {self.num_params} parameters, got {p.size}")
+        check(load().fs_net_set_params(self._h, p), "fs_net_set_params")
+
+    def eval(self, X, dY=None):
+        """``X``: (nb, nz, ny, nx, 16) channels-last.  Returns Y (same shape)
+        and, with ``dY``, the parameter gradient of ``sum(dY * Y)``."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        if X.ndim != 5 or X.shape[-1] != 16:
+            raise ValueError("X must have shape (nb, nz, ny, nx, 16)")
+        nb, nz, ny, nx, _ = X.shape
+        Y = np.empty_like(X)
+        grad = None
+        dYa = None
+        if dY is not None:
+            dYa = np.ascontiguousarray(dY, dtype=np.float32)
+            if dYa.shape != X.shape:
+                raise ValueError("dY must have the shape of X")
+            grad = np.empty(self.num_params, np.float32)
+        check(load().fs_net_eval(self._h, nx, ny, nz, nb, X, Y,
+                                 None if dYa is None else dYa.ctypes.data_as(_vp),
+                                 None if grad is None else grad.ctypes.data_as(_vp)),
+              "fs_net_eval")
+        return (Y, grad) if dY is not None else Y
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().fs_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DevicePreconditioner:
+    """Aggregation hierarchy of one prepared system with the network (or one
+    Jacobi step) at its bottom (an ``fs_pc``)."""
+
+    def __init__(self, system: DeviceSystem, net: DeviceNet | None, groups, omega: float = 0.8):
+        self.system = system
+        self.net = net
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(g, np.int32).ravel()
+                                                    for lv in groups for g in lv]),
+                                    dtype=np.int32)
+        self.nlevels = len(groups)
+        h = _vp()
+        check(load().fs_pc_create(system._h, None if net is None else net._h, self.nlevels,
+                                  flat, float(omega), C.byref(h)), "fs_pc_create")
+        self._h = h
+
+    def level_dims(self, level: int) -> tuple[int, int, int]:
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        check(load().fs_pc_level_dims(self._h, int(level), C.byref(a), C.byref(b), C.byref(c)),
+              "fs_pc_level_dims")
+        return a.value, b.value, c.value
+
+    def apply(self, r) -> np.ndarray:
+        """z = M(r) for equilibrated vectors, (n,) or (k, n) in C order."""
+        r = as_f64(r)
+        single = r.ndim == 1
+        r2 = np.ascontiguousarray(r.reshape(1, -1) if single else r)
+        z = np.empty_like(r2)
+        check(load().fs_pc_apply(self._h, r2.shape[0], r2, z), "fs_pc_apply")
+        return z[0] if single else z
+
+    def add_pairs(self, e) -> None:
+        """Exact errors (node space, (k, n)) as training pairs."""
+        e2 = np.ascontiguousarray(as_f64(e).reshape(-1, self.system.op.n_nodes))
+        check(load().fs_pc_add_pairs(self._h, e2.shape[0], e2), "fs_pc_add_pairs")
+
+    @property
+    def num_pairs(self) -> int:
+        k = C.c_int()
+        check(load().fs_pc_num_pairs(self._h, C.byref(k)), "fs_pc_num_pairs")
+        return k.value
+
+    def clear_pairs(self) -> None:
+        check(load().fs_pc_clear_pairs(self._h), "fs_pc_clear_pairs")
+
+    def loss_grad(self, e):
+        e2 = np.ascontiguousarray(as_f64(e).reshape(-1, self.system.op.n_nodes))
+        loss = np.empty(e2.shape[0])
+        grad = np.empty(self.net.num_params, np.float32)
+        check(load().fs_pc_loss_grad(self._h, e2.shape[0], e2, loss, grad.ctypes.data_as(_vp)),
+              "fs_pc_loss_grad")
+        return loss, grad
+
+    def train(self, steps: int, batch: int = 4, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+        losses = np.empty(int(steps))
+        check(load().fs_pc_train(self._h, int(steps), int(batch), float(lr),
+                                 int(seed) & (2 ** 64 - 1), losses), "fs_pc_train")
+        return losses
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().fs_pc_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -16,10 +16,11 @@
 #include <vector>
 
 #include "fieldspice_hip.h"
+#include "fs_common.hpp"
 #include "fs_internal.hpp"
 #include "fs_launch.hpp"
 
-namespace {
+namespace fs {
 
 thread_local std::string g_err;
 
@@ -33,42 +34,15 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define FS_HIP(expr)                                                                 \
-  do {                                                                               \
-    hipError_t e_ = (expr);                                                          \
-    if (e_ != hipSuccess)                                                            \
-      return fail(FS_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),    \
-                  __FILE__, __LINE__);                                               \
-  } while (0)
+const char* last_error() { return g_err.c_str(); }
 
-#define FS_GUARD_BEGIN try {
-#define FS_GUARD_END                                                      \
-  }                                                                       \
-  catch (const std::bad_alloc&) {                                         \
-    return fail(FS_ENOMEM, "host allocation failed");                     \
-  }                                                                       \
-  catch (...) {                                                           \
-    return fail(FS_EINVAL, "unexpected internal exception");              \
-  }
+}  // namespace fs
 
-template <typename T>
-int dalloc(T** p, size_t count) {
-  *p = nullptr;
-  if (count == 0) count = 1;
-  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return fail(FS_ENOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
-                hipGetErrorString(e));
-  }
-  return FS_OK;
-}
+namespace {
 
-template <typename T>
-void dfree(T*& p) {
-  if (p) (void)hipFree((void*)p);
-  p = nullptr;
-}
+using fs::dalloc;
+using fs::dfree;
+using fs::fail;
 
 fs::Dims make_dims(int Nx, int Ny, int Nz) {
   fs::Dims d;
@@ -148,12 +122,15 @@ void free_sys_ws(fs_sys* s) {
   dfree(s->r32);
   dfree(s->p32);
   dfree(s->q32);
+  dfree(s->z32);
   dfree(s->node_tmp);
   s->ws_nr = 0;
 }
 
 void free_sys(fs_sys* s) {
   if (!s) return;
+  for (fs_pc* pc : s->pcs) fs::pc_orphan(pc);
+  s->pcs.clear();
   free_sys_ws(s);
   dfree(s->mask);
   dfree(s->dhalf);
@@ -199,9 +176,16 @@ int ensure_ws(fs_sys* s, int nr) {
   if ((rc = dalloc(&s->bs, cnt)) || (rc = dalloc(&s->x, cnt)) || (rc = dalloc(&s->r, cnt)) ||
       (rc = dalloc(&s->p, cnt)) || (rc = dalloc(&s->q, cnt)) || (rc = dalloc(&s->x32, cnt)) ||
       (rc = dalloc(&s->r32, cnt)) || (rc = dalloc(&s->p32, cnt)) ||
-      (rc = dalloc(&s->q32, cnt)) || (rc = dalloc(&s->node_tmp, cnt))) {
+      (rc = dalloc(&s->q32, cnt)) || (rc = dalloc(&s->z32, cnt)) ||
+      (rc = dalloc(&s->node_tmp, cnt))) {
     free_sys_ws(s);
     return rc;
+  }
+  // the flexible-CG start reads q32 into an unused partial: keep it finite
+  if (hipMemset(s->q32, 0, cnt * sizeof(float)) != hipSuccess ||
+      hipMemset(s->z32, 0, cnt * sizeof(float)) != hipSuccess) {
+    free_sys_ws(s);
+    return fail(FS_EHIP, "workspace initialisation failed");
   }
   s->ws_nr = nr;
   return FS_OK;
@@ -213,11 +197,12 @@ int default_maxiter(const fs_sys* s, const fs_solve_params* P) {
   return std::max(500, (int)std::floor(10.0 * std::sqrt(nf)));
 }
 
-// Krylov loop after init: FP64 CG (mode 0) or FP32 cycles with FP64
-// refreshes.  Host checks the device flags between chunks of iterations.
-int run_krylov(fs_sys* s, int nr, const fs_solve_params& P) {
+// Krylov loop after init: FP64 CG (mode 0), or FP32 cycles with FP64
+// refreshes: CG (mode 1) or flexible CG preconditioned by the attached fs_pc
+// (mode 2).  Host checks the device flags between chunks of iterations.
+int run_krylov_mode(fs_sys* s, int nr, const fs_solve_params& P, int mode) {
   hipStream_t st = s->ctx->stream;
-  const bool mixed = P.precision == FS_PREC_MIXED;
+  const bool cyc = mode != 0;
   const double inner_tol = P.inner_tol > 0 ? P.inner_tol : 1e-4;
   const size_t scb = sizeof(fs::CgScalars) * (size_t)nr;
   auto fetch = [&]() -> int {
@@ -232,16 +217,32 @@ int run_krylov(fs_sys* s, int nr, const fs_solve_params& P) {
     return true;
   };
   if (all([](const fs::CgScalars& c) { return c.done != 0; })) return FS_OK;
-  if (mixed) fs::launch_cycle_start(s, nr, inner_tol * inner_tol);
-  int chunk = P.check_every > 0 ? P.check_every : 16;
+  auto start_cycle = [&]() -> int {
+    fs::launch_cycle_start(s, nr, inner_tol * inner_tol);
+    if (mode == 2) {
+      if (int rc = fs::launch_fcg_start(s, nr)) return rc;
+      s->acc_pc_applies += 1;
+    }
+    FS_HIP(hipGetLastError());
+    return FS_OK;
+  };
+  if (cyc)
+    if (int rc = start_cycle()) return rc;
+  // a flexible-CG iteration costs several stencil applications: check often
+  const int max_chunk = mode == 2 ? 32 : kMaxChunk;
+  const int first_chunk = mode == 2 ? 4 : 16;
+  int chunk = P.check_every > 0 ? P.check_every : first_chunk;
   const int maxiter = s->sc_host[0].maxiter;
   std::vector<double> last_rho(nr, 0.0);
   std::vector<int> last_it(nr, 0);
   for (int guard = 0; guard < 4 * maxiter + 64; ++guard) {
     const bool prof = s->profiling && chunk <= s->ev_cap;
-    for (int c = 0; c < chunk; ++c)
-      fs::launch_iteration(s, nr, mixed ? 1 : 0, prof ? s->ev_a[c] : nullptr,
-                           prof ? s->ev_b[c] : nullptr);
+    for (int c = 0; c < chunk; ++c) {
+      if (int rc = fs::launch_iteration(s, nr, mode, prof ? s->ev_a[c] : nullptr,
+                                        prof ? s->ev_b[c] : nullptr))
+        return rc;
+      if (mode == 2) s->acc_pc_applies += 1;
+    }
     FS_HIP(hipGetLastError());
     if (int rc = fetch()) return rc;
     if (prof) {
@@ -252,7 +253,7 @@ int run_krylov(fs_sys* s, int nr, const fs_solve_params& P) {
       }
       s->prof_launches += chunk;
     }
-    if (!mixed) {
+    if (!cyc) {
       if (all([](const fs::CgScalars& c) { return c.done != 0; })) return FS_OK;
     } else {
       if (all([](const fs::CgScalars& c) { return c.done != 0 || c.cycle_done != 0; })) {
@@ -260,38 +261,66 @@ int run_krylov(fs_sys* s, int nr, const fs_solve_params& P) {
         FS_HIP(hipGetLastError());
         if (int rc = fetch()) return rc;
         if (all([](const fs::CgScalars& c) { return c.done != 0; })) return FS_OK;
-        fs::launch_cycle_start(s, nr, inner_tol * inner_tol);
-        FS_HIP(hipGetLastError());
-        chunk = P.check_every > 0 ? P.check_every : 16;
+        if (int rc = start_cycle()) return rc;
+        chunk = P.check_every > 0 ? P.check_every : first_chunk;
+        std::fill(last_it.begin(), last_it.end(), 0);
         continue;
       }
     }
     if (P.check_every > 0) continue;
     // Adaptive chunk: extrapolate the residual decay of the slowest live RHS
-    // to its threshold, aiming slightly past it (idle launches exit at once).
-    int want = 8;
+    // to its threshold (CG aims slightly past it: its idle launches exit at
+    // once; a flexible-CG iteration past the target is not free).
+    int want = mode == 2 ? 1 : 8;
     for (int r = 0; r < nr; ++r) {
       const fs::CgScalars& c = s->sc_host[r];
-      if (c.done != 0 || (mixed && c.cycle_done)) continue;
-      const double rho = mixed ? c.aux0 : c.rho;
-      const double target = mixed ? c.aux1 : c.thresh2;
-      const int it = mixed ? c.cycle_iter : c.iter;
+      if (c.done != 0 || (cyc && c.cycle_done)) continue;
+      const double rho = cyc ? c.aux0 : c.rho;
+      const double target = cyc ? c.aux1 : c.thresh2;
+      const int it = cyc ? c.cycle_iter : c.iter;
       if (last_it[r] > 0 && it > last_it[r] && rho > 0 && last_rho[r] > rho) {
         const double rate = std::log(rho / last_rho[r]) / (double)(it - last_it[r]);
         const double remain = std::log(target / rho) / rate;
-        if (std::isfinite(remain)) want = std::max(want, (int)std::ceil(remain) + 2);
+        if (std::isfinite(remain))
+          want = std::max(want, (int)std::ceil(remain) + (mode == 2 ? 0 : 2));
       } else {
         want = std::max(want, chunk * 2);
       }
       last_rho[r] = rho;
       last_it[r] = it;
     }
-    chunk = std::min(std::max(want, 4), kMaxChunk);
+    chunk = std::min(std::max(want, mode == 2 ? 1 : 4), max_chunk);
   }
   return fail(FS_EINVAL, "Krylov driver exceeded its loop guard");
 }
 
+int run_krylov(fs_sys* s, int nr, const fs_solve_params& P) {
+  if (P.precision != FS_PREC_FCG)
+    return run_krylov_mode(s, nr, P, P.precision == FS_PREC_MIXED ? 1 : 0);
+  if (!s->pc) return fail(FS_EINVAL, "FS_PREC_FCG needs a preconditioner (fs_sys_set_precond)");
+  if (int rc = run_krylov_mode(s, nr, P, 2)) return rc;
+  // plain FP64 CG redoes the batch when flexible CG broke down or stalled
+  bool ok = true;
+  for (int r = 0; r < nr; ++r) ok = ok && s->sc_host[r].done == 1;
+  if (ok) return FS_OK;
+  s->acc_fallbacks += 1;
+  // the fallback keeps the reference's own cap, whatever maxiter bounded the
+  // flexible-CG run
+  fs_solve_params Q = P;
+  Q.precision = FS_PREC_FP64;
+  Q.maxiter = 0;
+  fs::launch_init(s, nr, P.tol, default_maxiter(s, &Q));
+  FS_HIP(hipGetLastError());
+  return run_krylov_mode(s, nr, Q, 0);
+}
+
 void collect_stats(fs_sys* s, int nr, fs_solve_stats* st, int& status) {
+  if (st) {
+    st->fallbacks += s->acc_fallbacks;
+    st->pc_applies += s->acc_pc_applies;
+  }
+  s->acc_fallbacks = 0;
+  s->acc_pc_applies = 0;
   for (int r = 0; r < nr; ++r) {
     const fs::CgScalars& c = s->sc_host[r];
     if (st) {
@@ -310,6 +339,10 @@ void collect_stats(fs_sys* s, int nr, fs_solve_stats* st, int& status) {
 }
 
 }  // namespace
+
+namespace fs {
+int ensure_solver_ws(fs_sys* s, int nr) { return ensure_ws(s, nr); }
+}  // namespace fs
 
 namespace {
 // per-probe node lists in device order (host C-order ids -> x-fast indices)
@@ -351,7 +384,7 @@ struct DevBufs {
 // ==========================================================================
 extern "C" {
 
-const char* fs_last_error(void) { return g_err.c_str(); }
+const char* fs_last_error(void) { return fs::last_error(); }
 
 int fs_ctx_create(int device, fs_ctx** out) {
   FS_GUARD_BEGIN
@@ -681,7 +714,8 @@ int fs_sys_solve(fs_sys* s, int nrhs, const double* b, const double* x0, double*
   if (nrhs < 1) return fail(FS_EINVAL, "nrhs must be >= 1");
   if (x0) return fail(FS_EINVAL, "an initial guess is not supported by this build");
   if (!(params->tol > 0.0)) return fail(FS_EINVAL, "tol must be > 0");
-  if (params->precision != FS_PREC_FP64 && params->precision != FS_PREC_MIXED)
+  if (params->precision != FS_PREC_FP64 && params->precision != FS_PREC_MIXED &&
+      params->precision != FS_PREC_FCG)
     return fail(FS_EINVAL, "unknown precision %d", params->precision);
   fs_ctx* ctx = s->ctx;
   FS_HIP(hipSetDevice(ctx->device));
@@ -722,7 +756,7 @@ int fs_sys_solve(fs_sys* s, int nrhs, const double* b, const double* x0, double*
     ms_total += ms;
     int st = FS_OK;
     collect_stats(s, nr, stats, st);
-    prof_end(s, stats, params->precision == FS_PREC_MIXED);
+    prof_end(s, stats, params->precision != FS_PREC_FP64);
     if (st != FS_OK) {
       status = st;
       if (stats) stats->converged = 0;
@@ -826,7 +860,8 @@ int fs_cap_solve(fs_cap* cap, const fs_op* L, double* C, double* q_shield, doubl
   if (L->Nx != s->op->Nx || L->Ny != s->op->Ny || L->Nz != s->op->Nz)
     return fail(FS_EINVAL, "L and the system differ in grid");
   if (!(params->tol > 0.0)) return fail(FS_EINVAL, "tol must be > 0");
-  if (params->precision != FS_PREC_FP64 && params->precision != FS_PREC_MIXED)
+  if (params->precision != FS_PREC_FP64 && params->precision != FS_PREC_MIXED &&
+      params->precision != FS_PREC_FCG)
     return fail(FS_EINVAL, "unknown precision %d", params->precision);
   fs_ctx* ctx = s->ctx;
   FS_HIP(hipSetDevice(ctx->device));
@@ -837,7 +872,7 @@ int fs_cap_solve(fs_cap* cap, const fs_op* L, double* C, double* q_shield, doubl
     stats->converged = 1;
   }
   const int maxiter = default_maxiter(s, params);
-  const bool mixed = params->precision == FS_PREC_MIXED;
+  const bool mixed = params->precision != FS_PREC_FP64;
   int status = FS_OK;
   float ms_total = 0.0f;
   std::vector<double> hout((size_t)nbins * fs::kMaxRhs);
@@ -895,9 +930,9 @@ int fs_capacitance(fs_sys* s, const fs_op* L, int nt, const int32_t* term_id, do
   fs_cap* cap = nullptr;
   if (int rc = fs_cap_prepare(s, nt, term_id, &cap)) return rc;
   const int rc = fs_cap_solve(cap, L, C, q_shield, phi_out, params, stats);
-  const std::string keep = g_err;
+  const std::string keep = fs::g_err;
   fs_cap_destroy(cap);
-  g_err = keep;
+  fs::g_err = keep;
   return rc;
 }
 
@@ -935,7 +970,8 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
     return fail(FS_EINVAL, "sources need vectors and scales");
   if (D->nprobe < 0 || (D->nprobe > 0 && (!D->probe || !D->probe_w)))
     return fail(FS_EINVAL, "probes need ids and weights");
-  if (params->precision != FS_PREC_FP64 && params->precision != FS_PREC_MIXED)
+  if (params->precision != FS_PREC_FP64 && params->precision != FS_PREC_MIXED &&
+      params->precision != FS_PREC_FCG)
     return fail(FS_EINVAL, "unknown precision %d", params->precision);
   if (!(params->tol > 0.0)) return fail(FS_EINVAL, "tol must be > 0");
   const fs_op* ops[5] = {Aexp, D->cur_a, D->cur_b, D->quad_a, D->quad_b};
@@ -1011,6 +1047,8 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
   }
   const int maxiter = default_maxiter(s, params);
   const int store_every = D->store_every;
+  s->acc_fallbacks = 0;
+  s->acc_pc_applies = 0;
   int64_t nstored = 0;
   auto store = [&](const double* dev) -> int {
     if (!stored) return FS_OK;
@@ -1081,7 +1119,11 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
     stats->residual = worst_res;
     stats->residual_eq = worst_req;
     stats->solve_time = ms * 1e-3;
+    stats->fallbacks = s->acc_fallbacks;
+    stats->pc_applies = s->acc_pc_applies;
   }
+  s->acc_fallbacks = 0;
+  s->acc_pc_applies = 0;
   if (status == FS_NOT_CONVERGED)
     fail(FS_NOT_CONVERGED, "a time step did not reach tol %.3g within %d iterations", params->tol,
          maxiter);

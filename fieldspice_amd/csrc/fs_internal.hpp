@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "fieldspice_hip.h"
 
@@ -47,7 +48,7 @@ struct CgScalars {
   double beta;
   double thresh2;   // stop when rho < thresh2  (== (tol*||b_s||)^2)
   double bnorm2;    // ||b_s||^2
-  double rho_cycle; // rho at the start of an FP32 cycle (mixed precision)
+  double zr;        // flexible CG: <z, r> of the current preconditioned residual
   double scale;     // FP32 cycle scaling: r32 = r64 * scale
   double aux0, aux1, aux2, aux3;
   int iter;         // iterations completed (over all cycles)
@@ -80,6 +81,8 @@ struct fs_op {
   double* extra = nullptr;                     // node diagonal, or null
 };
 
+struct fs_pc;
+
 struct fs_sys {
   fs_ctx* ctx = nullptr;
   const fs_op* op = nullptr;  // not owned; must outlive the system
@@ -96,6 +99,7 @@ struct fs_sys {
   int ws_nr = 0;
   double *bs = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
   float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *q32 = nullptr;
+  float* z32 = nullptr;              // flexible CG: preconditioned residual
   double* node_tmp = nullptr;        // host-layout staging (nr node vectors)
   double* partials = nullptr;        // reduction partials
   int64_t partials_cap = 0;
@@ -110,6 +114,11 @@ struct fs_sys {
   double prof_time = 0.0;
   int64_t prof_launches = 0;
   unsigned long long* prof_dev = nullptr;  // [0] live launches, [1] live RHS summed
+  // flexible CG
+  fs_pc* pc = nullptr;        // attached preconditioner (not owned)
+  std::vector<fs_pc*> pcs;    // every preconditioner built on this system
+  int acc_fallbacks = 0;      // counters of the current solve call
+  int acc_pc_applies = 0;
 };
 
 struct fs_cap {

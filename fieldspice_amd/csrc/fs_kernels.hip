@@ -528,6 +528,8 @@ enum FinOp : int {
   FIN_BNORM = 3,   // q = ||b||^2      -> aux2
   FIN_RESID = 4,   // q0 = ||r||^2, q1 = ||r dhalf||^2 -> aux3, aux1
   FIN_CYCLE = 5,   // q = ||r32||^2    -> FP32 cycle state
+  FIN_FCG = 6,     // q0 = <z,r>, q1 = <z,q> -> flexible-CG beta, zr
+  FIN_FCG0 = 7,    // q0 = <z,r> at the start of a cycle -> zr
 };
 
 struct FinArgs {
@@ -545,17 +547,19 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
                                                      FinArgs fa, CgScalars* sc,
                                                      unsigned long long* prof) {
   const int r = blockIdx.x;
-  if (fa.op == FIN_ALPHA || fa.op == FIN_BETA) {
+  if (fa.op == FIN_ALPHA || fa.op == FIN_BETA || fa.op == FIN_FCG) {
     if (!rhs_live(sc, r, fa.mode)) return;  // uniform over the block
   }
+  if (fa.op == FIN_FCG0 && sc[r].done != 0) return;
   const double s0 = sum_partials(partials + (int64_t)r * fa.nb, fa.nb);
   double s1 = 0.0;
-  if (fa.op == FIN_RESID) s1 = sum_partials(partials + (int64_t)(fa.nr + r) * fa.nb, fa.nb);
+  if (fa.op == FIN_RESID || fa.op == FIN_FCG)
+    s1 = sum_partials(partials + (int64_t)(fa.nr + r) * fa.nb, fa.nb);
   if (threadIdx.x != 0) return;
   CgScalars& c = sc[r];
   if (fa.op == FIN_ALPHA) {
     const double pq = s0;
-    const double rho = fa.mode == 0 ? c.rho : c.aux0;
+    const double rho = fa.mode == 0 ? c.rho : (fa.mode == 1 ? c.aux0 : c.zr);
     c.pq = pq;
     if (!(pq > 0.0) || !isfinite(pq)) {
       c.done = 3;  // breakdown: A_s not positive definite on p
@@ -574,7 +578,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
       else if (c.iter >= c.maxiter) c.done = 2;
       else if (!isfinite(rho_new)) c.done = 3;
     } else {
-      c.beta = rho_new / c.aux0;
+      // FP32 cycle; flexible CG (mode 2) takes beta from FIN_FCG instead
+      if (fa.mode == 1) c.beta = rho_new / c.aux0;
       c.aux0 = rho_new;
       if (rho_new < c.aux1) c.cycle_done = 1;
       else if (c.iter >= c.maxiter) c.cycle_done = 1;
@@ -597,6 +602,12 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
   } else if (fa.op == FIN_RESID) {
     c.aux3 = s0;
     c.aux1 = s1;
+  } else if (fa.op == FIN_FCG || fa.op == FIN_FCG0) {
+    // beta = -<z_new, A p>/<p, A p> keeps the new direction A-orthogonal to
+    // the previous one whatever the preconditioner did (flexible CG)
+    c.zr = s0;
+    c.beta = fa.op == FIN_FCG ? -s1 / c.pq : 0.0;
+    if (!(s0 > 0.0) || !isfinite(s0) || !isfinite(c.beta)) c.done = 3;
   } else if (fa.op == FIN_CYCLE) {
     if (c.done != 0) return;
     c.scale = 1.0 / sqrt(c.rho);
@@ -932,7 +943,7 @@ __global__ __launch_bounds__(kBlock) void k_charge_final(const double* __restric
 // host launchers
 // --------------------------------------------------------------------------
 template <int NR>
-static void launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {
+static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {
   hipStream_t st = s->ctx->stream;
   VecArgs va{s->d.n, s->vstride, nr};
   const int vb = s->vec_blocks;
@@ -950,19 +961,26 @@ static void launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEv
     hipLaunchKernelGGL((k_update<double, NR>), dim3(vb), dim3(kBlock), 0, st, s->x, s->r, s->p,
                        s->q, va, mode, s->sc, s->partials);
     finalize<NR>(st, s->partials, FIN_BETA, nr, vb, mode, s->sc);
-  } else {
-    hipLaunchKernelGGL((k_pupdate<float, NR>), dim3(vb), dim3(kBlock), 0, st, s->p32, s->r32,
-                       va, mode, s->sc);
-    if (ea) (void)hipEventRecord(ea, st);
-    hipLaunchKernelGGL((k_stencil_dot<float, NR>), dim3(sb), dim3(kBlock), 0, st, s->p32,
-                       s->q32, s->c32[0], s->c32[1], s->c32[2], s->d, s->tl, va, mode, s->sc,
-                       s->partials, prof);
-    if (eb) (void)hipEventRecord(eb, st);
-    finalize<NR>(st, s->partials, FIN_ALPHA, nr, sb, mode, s->sc);
-    hipLaunchKernelGGL((k_update<float, NR>), dim3(vb), dim3(kBlock), 0, st, s->x32, s->r32,
-                       s->p32, s->q32, va, mode, s->sc, s->partials);
-    finalize<NR>(st, s->partials, FIN_BETA, nr, vb, mode, s->sc);
+    return 0;
   }
+  // FP32 cycle: CG searches along r32, flexible CG along z32 = M(r32)
+  const float* dir = mode == 2 ? s->z32 : s->r32;
+  hipLaunchKernelGGL((k_pupdate<float, NR>), dim3(vb), dim3(kBlock), 0, st, s->p32, dir, va, mode,
+                     s->sc);
+  if (ea) (void)hipEventRecord(ea, st);
+  hipLaunchKernelGGL((k_stencil_dot<float, NR>), dim3(sb), dim3(kBlock), 0, st, s->p32, s->q32,
+                     s->c32[0], s->c32[1], s->c32[2], s->d, s->tl, va, mode, s->sc, s->partials,
+                     prof);
+  if (eb) (void)hipEventRecord(eb, st);
+  finalize<NR>(st, s->partials, FIN_ALPHA, nr, sb, mode, s->sc);
+  hipLaunchKernelGGL((k_update<float, NR>), dim3(vb), dim3(kBlock), 0, st, s->x32, s->r32, s->p32,
+                     s->q32, va, mode, s->sc, s->partials);
+  finalize<NR>(st, s->partials, FIN_BETA, nr, vb, mode, s->sc);
+  if (mode == 2) {
+    if (int rc = pc_apply_solver(s, nr)) return rc;
+    finalize<NR>(st, s->partials, FIN_FCG, nr, vb, mode, s->sc);
+  }
+  return 0;
 }
 
 #define FS_DISPATCH_NR(nr, FN, ...)        \
@@ -973,8 +991,11 @@ static void launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEv
     else FN<8>(__VA_ARGS__);               \
   } while (0)
 
-void launch_iteration(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {
-  FS_DISPATCH_NR(nr, launch_iteration_t, s, nr, mode, ea, eb);
+int launch_iteration(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {
+  if (nr <= 1) return launch_iteration_t<1>(s, nr, mode, ea, eb);
+  if (nr <= 2) return launch_iteration_t<2>(s, nr, mode, ea, eb);
+  if (nr <= 4) return launch_iteration_t<4>(s, nr, mode, ea, eb);
+  return launch_iteration_t<8>(s, nr, mode, ea, eb);
 }
 
 template <int NR>
@@ -1042,6 +1063,18 @@ static void launch_cycle_start_t(fs_sys* s, int nr, double inner_tol2) {
                      s->ctx->stream, s->r, s->r32, s->x32, s->p32, va, s->sc, s->partials);
   finalize<NR>(s->ctx->stream, s->partials, FIN_CYCLE, nr, s->vec_blocks, 1, s->sc, nullptr,
                0.0, 0, inner_tol2);
+}
+template <int NR>
+static int launch_fcg_start_t(fs_sys* s, int nr) {
+  if (int rc = pc_apply_solver(s, nr)) return rc;
+  finalize<NR>(s->ctx->stream, s->partials, FIN_FCG0, nr, s->vec_blocks, 2, s->sc);
+  return 0;
+}
+int launch_fcg_start(fs_sys* s, int nr) {
+  if (nr <= 1) return launch_fcg_start_t<1>(s, nr);
+  if (nr <= 2) return launch_fcg_start_t<2>(s, nr);
+  if (nr <= 4) return launch_fcg_start_t<4>(s, nr);
+  return launch_fcg_start_t<8>(s, nr);
 }
 void launch_cycle_start(fs_sys* s, int nr, double inner_tol2) {
   FS_DISPATCH_NR(nr, launch_cycle_start_t, s, nr, inner_tol2);

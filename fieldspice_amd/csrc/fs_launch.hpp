@@ -25,7 +25,8 @@ void apply_raw(hipStream_t s, const fs_op* op, const double* x, double* y);
 int setup_system(fs_sys* sys, double* dsum, void* counts_dev, void* counts_host, int pin);
 constexpr size_t kSetupCountsBytes = 64;
 
-void launch_iteration(fs_sys* s, int nr, int mode, hipEvent_t ev_a, hipEvent_t ev_b);
+// mode 0 FP64 CG, 1 FP32 CG cycle, 2 FP32 flexible CG cycle (returns a status)
+int launch_iteration(fs_sys* s, int nr, int mode, hipEvent_t ev_a, hipEvent_t ev_b);
 void launch_init(fs_sys* s, int nr, double tol, int maxiter);
 void launch_lift_general(fs_sys* s, int nr, const double* b);
 void launch_lift_cap(fs_sys* s, int nr, const fs_op* L, const int32_t* tid, int j0);
@@ -33,6 +34,8 @@ void launch_expand(fs_sys* s, int nr, const double* b, const int32_t* tid, int j
                    double* phi);
 void launch_residual(fs_sys* s, int nr, double* res);
 void launch_cycle_start(fs_sys* s, int nr, double inner_tol2);
+// flexible CG: z32 = M(r32) and <z,r> at the start of an FP32 cycle
+int launch_fcg_start(fs_sys* s, int nr);
 void launch_refresh(fs_sys* s, int nr);
 void launch_charge(fs_sys* s, int nr, const fs_op* L, const double* phi, const int64_t* nodes,
                    const int64_t* offs, int nbins, int nbx, double* out);
@@ -49,5 +52,14 @@ void launch_probe(fs_sys* s, const fs_op* A1, const fs_op* A2, const double* x,
 void launch_step_stats(fs_sys* s, const double* x, const fs_op* Qa, const fs_op* Qb,
                        double* out);
 void launch_edge_power(hipStream_t st, const fs_op* A, const double* x, double* q, int ncu);
+
+// learned preconditioner (fs_pc.hip)
+int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, double* partials,
+             int nblocks);
+int pc_apply_solver(fs_sys* s, int nr);  // z32 = M(r32), FCG partials
+// the system of pc is being destroyed: pc keeps only its own memory
+void pc_orphan(fs_pc* pc);
+// solver workspace of a system for nr right-hand sides (fs_api.hip)
+int ensure_solver_ws(fs_sys* s, int nr);
 
 }  // namespace fs

@@ -1,0 +1,371 @@
+// Aggregation hierarchy of the learned preconditioner (FP32).
+//
+// Level 0 is the prepared system's equilibrated operator A = I - C (unit
+// diagonal; couplings that touch Dirichlet or pinned nodes are zero).  Level
+// l+1 aggregates boxes of level-l nodes: the tensor product of one grouping
+// of consecutive indices per axis.  With the piecewise-constant prolongation
+// P of such boxes the Galerkin product P^T A P is again a 7-point operator:
+//   d_I  = sum_{n in I} d_n - 2 sum_{edges inside I} c_e
+//   c_IJ = sum of the couplings of the level-l edges between boxes I and J.
+// One cycle (applied to a residual r on level l):
+//   z1 = omega D^-1 r
+//   r_{l+1} = P^T (r - A z1)                      (restrict_)
+//   e_{l+1} = cycle on level l+1 (the network, or one Jacobi step, at the bottom)
+//   z2 = z1 + P e_{l+1};  e_l = z2 + omega D^-1 (r - A z2)   (prolong)
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fs_device.hpp"
+#include "fs_internal.hpp"
+#include "fs_nn.hpp"
+
+namespace fs {
+namespace mg {
+
+__device__ __forceinline__ void node_ijk(const Level& L, int64_t n, int& i, int& j, int& k) {
+  i = (int)(n % L.nx);
+  j = (int)((n / L.nx) % L.ny);
+  k = (int)(n / ((int64_t)L.nx * L.ny));
+}
+
+// diagonal, inverse diagonal and activity of node n (level 0: unit, mask)
+template <bool UNIT>
+__device__ __forceinline__ float dinv_of(const Level& L, int64_t n) {
+  if constexpr (UNIT) return L.mask[n] == 0 ? 1.f : 0.f;
+  else return L.dinv[n];
+}
+template <bool UNIT>
+__device__ __forceinline__ float diag_of(const Level& L, int64_t n) {
+  if constexpr (UNIT) return 1.f;
+  else return L.dg[n];
+}
+
+// -------------------------------------------------------------------------
+// setup: Galerkin coarse operator of one aggregation step (double sums)
+// -------------------------------------------------------------------------
+template <bool UNIT>
+__global__ __launch_bounds__(kBlock) void k_coarsen(Level f, Map m, float* __restrict__ dg,
+                                                    float* __restrict__ dinv,
+                                                    float* __restrict__ c0,
+                                                    float* __restrict__ c1,
+                                                    float* __restrict__ c2) {
+  const int64_t nc = (int64_t)m.cn[0] * m.cn[1] * m.cn[2];
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < nc;
+       t += (int64_t)gridDim.x * kBlock) {
+    const int I = (int)(t % m.cn[0]), J = (int)((t / m.cn[0]) % m.cn[1]),
+              K = (int)(t / ((int64_t)m.cn[0] * m.cn[1]));
+    const int i0 = m.start[0][I], i1 = m.start[0][I + 1];
+    const int j0 = m.start[1][J], j1 = m.start[1][J + 1];
+    const int k0 = m.start[2][K], k1 = m.start[2][K + 1];
+    double d = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    int act = 0;
+    for (int k = k0; k < k1; ++k)
+      for (int j = j0; j < j1; ++j)
+        for (int i = i0; i < i1; ++i) {
+          const int64_t n = ((int64_t)k * f.ny + j) * f.nx + i;
+          if (dinv_of<UNIT>(f, n) > 0.f) {
+            d += (double)diag_of<UNIT>(f, n);
+            ++act;
+          }
+          // edges to the + neighbour: internal ones count twice against the
+          // diagonal, the ones leaving the box are the coarse couplings
+          if (i + 1 < i1) d -= 2.0 * (double)f.c[0][n];
+          else if (I + 1 < m.cn[0]) cx += (double)f.c[0][n];
+          if (j + 1 < j1) d -= 2.0 * (double)f.c[1][n];
+          else if (J + 1 < m.cn[1]) cy += (double)f.c[1][n];
+          if (k + 1 < k1) d -= 2.0 * (double)f.c[2][n];
+          else if (K + 1 < m.cn[2]) cz += (double)f.c[2][n];
+        }
+    const bool active = act > 0 && d > 0.0;
+    dg[t] = active ? (float)d : 1.f;
+    dinv[t] = active ? (float)(1.0 / d) : 0.f;
+    c0[t] = active ? (float)cx : 0.f;
+    c1[t] = active ? (float)cy : 0.f;
+    c2[t] = active ? (float)cz : 0.f;
+  }
+}
+
+// network inputs that depend on the operator only: couplings normalised by
+// sqrt(d_I d_J) along +x, +y, +z, and activity; dh = 1/sqrt(d) on active nodes
+__global__ __launch_bounds__(kBlock) void k_bottom_features(Level L, float* __restrict__ feat4,
+                                                            float* __restrict__ dh) {
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < L.n;
+       n += (int64_t)gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(L, n, i, j, k);
+    const float di = L.dinv[n];
+    const float hi = di > 0.f ? sqrtf(di) : 0.f;
+    const int64_t nb[3] = {i + 1 < L.nx ? n + 1 : -1, j + 1 < L.ny ? n + L.nx : -1,
+                           k + 1 < L.nz ? n + (int64_t)L.nx * L.ny : -1};
+    for (int a = 0; a < 3; ++a) {
+      float v = 0.f;
+      if (nb[a] >= 0) {
+        const float dj = L.dinv[nb[a]];
+        if (hi > 0.f && dj > 0.f) v = L.c[a][n] * hi * sqrtf(dj);
+      }
+      feat4[n * 4 + a] = v;
+    }
+    feat4[n * 4 + 3] = di > 0.f ? 1.f : 0.f;
+    dh[n] = hi;
+  }
+}
+
+// -------------------------------------------------------------------------
+// cycle kernels
+// -------------------------------------------------------------------------
+// t_n = (r - A omega D^-1 r)_n = (1 - omega) r_n + omega sum_m c_nm dinv_m r_m
+// (on inactive nodes r = 0 and every coupling is 0, so t = 0 there)
+template <bool UNIT>
+__device__ __forceinline__ float smooth_res(const Level& L, const float* __restrict__ r,
+                                            int i, int j, int k, int64_t n, float omega) {
+  const int64_t sxy = (int64_t)L.nx * L.ny;
+  float s = 0.f;
+  if (i + 1 < L.nx) s += L.c[0][n] * dinv_of<UNIT>(L, n + 1) * r[n + 1];
+  if (i > 0) s += L.c[0][n - 1] * dinv_of<UNIT>(L, n - 1) * r[n - 1];
+  if (j + 1 < L.ny) s += L.c[1][n] * dinv_of<UNIT>(L, n + L.nx) * r[n + L.nx];
+  if (j > 0) s += L.c[1][n - L.nx] * dinv_of<UNIT>(L, n - L.nx) * r[n - L.nx];
+  if (k + 1 < L.nz) s += L.c[2][n] * dinv_of<UNIT>(L, n + sxy) * r[n + sxy];
+  if (k > 0) s += L.c[2][n - sxy] * dinv_of<UNIT>(L, n - sxy) * r[n - sxy];
+  return (1.f - omega) * r[n] + omega * s;
+}
+
+template <bool UNIT>
+__global__ __launch_bounds__(kBlock) void k_restrict(Level f, Map m, const float* __restrict__ r,
+                                                     float* __restrict__ rc, int64_t vsc, int nr,
+                                                     float omega) {
+  const int64_t nc = (int64_t)m.cn[0] * m.cn[1] * m.cn[2];
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < nc;
+       t += (int64_t)gridDim.x * kBlock) {
+    const int I = (int)(t % m.cn[0]), J = (int)((t / m.cn[0]) % m.cn[1]),
+              K = (int)(t / ((int64_t)m.cn[0] * m.cn[1]));
+    const int i0 = m.start[0][I], i1 = m.start[0][I + 1];
+    const int j0 = m.start[1][J], j1 = m.start[1][J + 1];
+    const int k0 = m.start[2][K], k1 = m.start[2][K + 1];
+    for (int b = 0; b < nr; ++b) {
+      const float* rb = r + b * f.vs;
+      float s = 0.f;
+      for (int k = k0; k < k1; ++k)
+        for (int j = j0; j < j1; ++j)
+          for (int i = i0; i < i1; ++i) {
+            const int64_t n = ((int64_t)k * f.ny + j) * f.nx + i;
+            s += smooth_res<UNIT>(f, rb, i, j, k, n, omega);
+          }
+      rc[b * vsc + t] = s;
+    }
+  }
+}
+
+// z2_n = omega dinv_n r_n + (active ? ec[P(n)] : 0)
+template <bool UNIT>
+__device__ __forceinline__ float z2_of(const Level& L, const Map& m, const float* __restrict__ r,
+                                       const float* __restrict__ ec, int i, int j, int k,
+                                       int64_t n, float omega) {
+  const float di = dinv_of<UNIT>(L, n);
+  if (!(di > 0.f)) return 0.f;
+  const int64_t c = ((int64_t)m.f2c[2][k] * m.cn[1] + m.f2c[1][j]) * m.cn[0] + m.f2c[0][i];
+  return omega * di * r[n] + ec[c];
+}
+
+template <bool UNIT>
+__global__ __launch_bounds__(kBlock) void k_prolong(Level f, Map m, const float* __restrict__ r,
+                                                    const float* __restrict__ ec, int64_t vsc,
+                                                    float* __restrict__ e, int nr, float omega,
+                                                    const float* __restrict__ q,
+                                                    double* __restrict__ partials) {
+  double acc[2 * kMaxRhs];
+#pragma unroll
+  for (int a = 0; a < 2 * kMaxRhs; ++a) acc[a] = 0.0;
+  const int64_t sxy = (int64_t)f.nx * f.ny;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < f.n;
+       n += (int64_t)gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(f, n, i, j, k);
+    const float di = dinv_of<UNIT>(f, n);
+    const float dd = diag_of<UNIT>(f, n);
+#pragma unroll
+    for (int b = 0; b < kMaxRhs; ++b) {
+      if (b >= nr) break;
+      const float* rb = r + b * f.vs;
+      const float* eb = ec + b * vsc;
+      float out = 0.f;
+      if (di > 0.f) {
+        const float zc = z2_of<UNIT>(f, m, rb, eb, i, j, k, n, omega);
+        float s = 0.f;
+        if (i + 1 < f.nx) s += f.c[0][n] * z2_of<UNIT>(f, m, rb, eb, i + 1, j, k, n + 1, omega);
+        if (i > 0) s += f.c[0][n - 1] * z2_of<UNIT>(f, m, rb, eb, i - 1, j, k, n - 1, omega);
+        if (j + 1 < f.ny) s += f.c[1][n] * z2_of<UNIT>(f, m, rb, eb, i, j + 1, k, n + f.nx, omega);
+        if (j > 0) s += f.c[1][n - f.nx] * z2_of<UNIT>(f, m, rb, eb, i, j - 1, k, n - f.nx, omega);
+        if (k + 1 < f.nz) s += f.c[2][n] * z2_of<UNIT>(f, m, rb, eb, i, j, k + 1, n + sxy, omega);
+        if (k > 0) s += f.c[2][n - sxy] * z2_of<UNIT>(f, m, rb, eb, i, j, k - 1, n - sxy, omega);
+        out = zc + omega * di * (rb[n] - dd * zc + s);
+      }
+      e[b * f.vs + n] = out;
+      if (partials) {
+        acc[b] += (double)out * (double)rb[n];
+        acc[kMaxRhs + b] += (double)out * (double)q[b * f.vs + n];
+      }
+    }
+  }
+  if (!partials) return;  // uniform over the launch
+  block_sum<2 * kMaxRhs>(acc);
+  if (threadIdx.x == 0) {
+    for (int b = 0; b < nr; ++b) {
+      store_partial(partials + (int64_t)b * gridDim.x + blockIdx.x, acc[b]);
+      store_partial(partials + (int64_t)(nr + b) * gridDim.x + blockIdx.x, acc[kMaxRhs + b]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bottom_jacobi(Level L, const float* __restrict__ r,
+                                                          float* __restrict__ e, int nr,
+                                                          float omega) {
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < L.n;
+       n += (int64_t)gridDim.x * kBlock)
+    for (int b = 0; b < nr; ++b) e[b * L.vs + n] = omega * L.dinv[n] * r[b * L.vs + n];
+}
+
+// level-0 training loss: w = A z, g = 2 (w - r) gscale_b, partials z.w, z.r
+__global__ __launch_bounds__(kBlock) void k_loss_grad(Level L, const float* __restrict__ z,
+                                                      const float* __restrict__ r,
+                                                      const float* __restrict__ gscale,
+                                                      float* __restrict__ g, int nr,
+                                                      double* __restrict__ partials) {
+  double acc[2 * kMaxRhs];
+#pragma unroll
+  for (int a = 0; a < 2 * kMaxRhs; ++a) acc[a] = 0.0;
+  const int64_t sxy = (int64_t)L.nx * L.ny;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < L.n;
+       n += (int64_t)gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(L, n, i, j, k);
+#pragma unroll
+    for (int b = 0; b < kMaxRhs; ++b) {
+      if (b >= nr) break;
+      const float* zb = z + b * L.vs;
+      float s = 0.f;
+      if (i + 1 < L.nx) s += L.c[0][n] * zb[n + 1];
+      if (i > 0) s += L.c[0][n - 1] * zb[n - 1];
+      if (j + 1 < L.ny) s += L.c[1][n] * zb[n + L.nx];
+      if (j > 0) s += L.c[1][n - L.nx] * zb[n - L.nx];
+      if (k + 1 < L.nz) s += L.c[2][n] * zb[n + sxy];
+      if (k > 0) s += L.c[2][n - sxy] * zb[n - sxy];
+      const float w = zb[n] - s;
+      const float rv = r[b * L.vs + n];
+      g[b * L.vs + n] = 2.f * (w - rv) * gscale[b];
+      acc[b] += (double)zb[n] * (double)w;
+      acc[kMaxRhs + b] += (double)zb[n] * (double)rv;
+    }
+  }
+  block_sum<2 * kMaxRhs>(acc);
+  if (threadIdx.x == 0) {
+    for (int b = 0; b < nr; ++b) {
+      store_partial(partials + (int64_t)b * gridDim.x + blockIdx.x, acc[b]);
+      store_partial(partials + (int64_t)(nr + b) * gridDim.x + blockIdx.x, acc[kMaxRhs + b]);
+    }
+  }
+}
+
+// one training pair from an exact FP64 error e (equilibrated, level 0):
+// r = A e with the FP64 couplings; e32, r32 stored; partial of e.r (the
+// energy ||e||_A^2 that normalises the loss)
+__global__ __launch_bounds__(kBlock) void k_pair_make(Level L, const double* __restrict__ cx,
+                                                      const double* __restrict__ cy,
+                                                      const double* __restrict__ cz,
+                                                      const double* __restrict__ e,
+                                                      float* __restrict__ e32,
+                                                      float* __restrict__ r32,
+                                                      double* __restrict__ partials) {
+  double acc[1] = {0.0};
+  const int64_t sxy = (int64_t)L.nx * L.ny;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < L.n;
+       n += (int64_t)gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(L, n, i, j, k);
+    double rv = 0.0, ev = 0.0;
+    if (L.mask[n] == 0) {
+      double s = 0.0;
+      if (i + 1 < L.nx) s += cx[n] * e[n + 1];
+      if (i > 0) s += cx[n - 1] * e[n - 1];
+      if (j + 1 < L.ny) s += cy[n] * e[n + L.nx];
+      if (j > 0) s += cy[n - L.nx] * e[n - L.nx];
+      if (k + 1 < L.nz) s += cz[n] * e[n + sxy];
+      if (k > 0) s += cz[n - sxy] * e[n - sxy];
+      ev = e[n];
+      rv = ev - s;
+    }
+    e32[n] = (float)ev;
+    r32[n] = (float)rv;
+    acc[0] += ev * rv;
+  }
+  block_sum<1>(acc);
+  if (threadIdx.x == 0) store_partial(partials + blockIdx.x, acc[0]);
+}
+
+// -------------------------------------------------------------------------
+// launchers
+// -------------------------------------------------------------------------
+int vec_blocks(int64_t n) {
+  const int64_t b = (n + kBlock - 1) / kBlock;
+  return (int)(b < 2048 ? (b > 0 ? b : 1) : 2048);
+}
+
+void coarsen(hipStream_t s, const Level& f, const Map& m, float* dg, float* dinv, float* c0,
+             float* c1, float* c2) {
+  const int64_t nc = (int64_t)m.cn[0] * m.cn[1] * m.cn[2];
+  if (f.dg == nullptr)
+    hipLaunchKernelGGL(k_coarsen<true>, dim3(vec_blocks(nc)), dim3(kBlock), 0, s, f, m, dg, dinv,
+                       c0, c1, c2);
+  else
+    hipLaunchKernelGGL(k_coarsen<false>, dim3(vec_blocks(nc)), dim3(kBlock), 0, s, f, m, dg, dinv,
+                       c0, c1, c2);
+}
+
+void bottom_features(hipStream_t s, const Level& L, float* feat4, float* dh) {
+  hipLaunchKernelGGL(k_bottom_features, dim3(vec_blocks(L.n)), dim3(kBlock), 0, s, L, feat4, dh);
+}
+
+void restrict_(hipStream_t s, const Level& f, const Map& m, const float* r, float* rc,
+               int64_t vsc, int nr, float omega) {
+  const int64_t nc = (int64_t)m.cn[0] * m.cn[1] * m.cn[2];
+  if (f.dg == nullptr)
+    hipLaunchKernelGGL(k_restrict<true>, dim3(vec_blocks(nc)), dim3(kBlock), 0, s, f, m, r, rc,
+                       vsc, nr, omega);
+  else
+    hipLaunchKernelGGL(k_restrict<false>, dim3(vec_blocks(nc)), dim3(kBlock), 0, s, f, m, r, rc,
+                       vsc, nr, omega);
+}
+
+void prolong(hipStream_t s, const Level& f, const Map& m, const float* r, const float* ec,
+             int64_t vsc, float* e, int nr, float omega, const float* q, double* partials,
+             int nblocks) {
+  const int nb = nblocks > 0 ? nblocks : vec_blocks(f.n);
+  if (f.dg == nullptr)
+    hipLaunchKernelGGL(k_prolong<true>, dim3(nb), dim3(kBlock), 0, s, f, m, r, ec, vsc, e, nr,
+                       omega, q, partials);
+  else
+    hipLaunchKernelGGL(k_prolong<false>, dim3(nb), dim3(kBlock), 0, s, f, m, r, ec, vsc, e, nr,
+                       omega, q, partials);
+}
+
+void bottom_jacobi(hipStream_t s, const Level& L, const float* r, float* e, int nr,
+                   float omega) {
+  hipLaunchKernelGGL(k_bottom_jacobi, dim3(vec_blocks(L.n)), dim3(kBlock), 0, s, L, r, e, nr,
+                     omega);
+}
+
+void loss_grad(hipStream_t s, const Level& L, const float* z, const float* r,
+               const float* gscale, float* g, int nr, double* partials, int nblocks) {
+  hipLaunchKernelGGL(k_loss_grad, dim3(nblocks), dim3(kBlock), 0, s, L, z, r, gscale, g, nr,
+                     partials);
+}
+
+void pair_make(hipStream_t s, const Level& L, const double* const c64[3], const double* e,
+               float* e32, float* r32, double* partials, int nblocks) {
+  hipLaunchKernelGGL(k_pair_make, dim3(nblocks), dim3(kBlock), 0, s, L, c64[0], c64[1], c64[2],
+                     e, e32, r32, partials);
+}
+
+}  // namespace mg
+}  // namespace fs

@@ -49,17 +49,22 @@ extern "C" {
 /* precision of the Krylov iteration */
 #define FS_PREC_FP64 0          /* FP64 stencil and vectors throughout */
 #define FS_PREC_MIXED 1         /* FP32 stencil/vectors, FP64 residual refresh and dots */
+#define FS_PREC_FCG 2           /* FS_PREC_MIXED cycles of flexible CG preconditioned by the
+                                   system's attached fs_pc (fs_sys_set_precond); a run that
+                                   breaks down or hits maxiter is redone by FP64 CG */
 
 typedef struct fs_ctx fs_ctx;
 typedef struct fs_op fs_op;
 typedef struct fs_sys fs_sys;
 typedef struct fs_cap fs_cap;
+typedef struct fs_net fs_net;
+typedef struct fs_pc fs_pc;
 
 typedef struct {
     double tol;          /* relative residual ||r_s|| < tol*||b_s|| in the equilibrated space */
     int maxiter;         /* <= 0: max(500, floor(10*sqrt(n_free))) as the reference */
-    int precision;       /* FS_PREC_FP64 or FS_PREC_MIXED */
-    double inner_tol;    /* FS_PREC_MIXED: residual drop per FP32 cycle before an FP64 refresh */
+    int precision;       /* FS_PREC_FP64, FS_PREC_MIXED or FS_PREC_FCG */
+    double inner_tol;    /* MIXED/FCG: residual drop per FP32 cycle before an FP64 refresh */
     int check_every;     /* iterations between host convergence checks (<= 0: adaptive) */
 } fs_solve_params;
 
@@ -74,6 +79,8 @@ typedef struct {
     double stencil_time;      /* seconds summed over its launches (HIP events) */
     int64_t stencil_launches; /* launches, including ones that found no live RHS */
     double stencil_bytes;     /* algorithmic bytes: couplings once + p read + q write per live RHS */
+    int fallbacks;       /* FS_PREC_FCG runs redone by FP64 CG */
+    int pc_applies;      /* preconditioner applications (FS_PREC_FCG) */
 } fs_solve_stats;
 
 typedef struct {
@@ -200,6 +207,55 @@ int fs_op_edge_power(fs_op* op, const double* x, double* q);
 /* Record HIP events around every stencil launch of the Krylov loop of this
  * system and report their summed duration in fs_solve_stats (off by default). */
 int fs_sys_set_profiling(fs_sys* sys, int on);
+
+/* ---- learned multilevel preconditioner ----------------------------------
+ * No counterpart in the reference, whose CG takes ILU or Jacobi
+ * (fieldspice/solvers/poisson.py:466-484); it stands beside them for
+ * FS_PREC_FCG.  fs_pc holds an aggregation hierarchy of one prepared system
+ * (per-axis groupings of consecutive nodes, Galerkin 7-point coarse
+ * operators, damped-Jacobi smoothing) whose bottom level is solved by an
+ * fs_net: a 3-level convolutional network (16/32/64 channels, 3x3x3, BF16
+ * weights and activations, FP32 accumulation on MFMA) that maps the bottom
+ * residual to an approximate error.  Without a network the bottom takes one
+ * Jacobi step.  Vectors of fs_pc_apply live in the system's equilibrated
+ * space (r_s = D^-1/2 r, x_s = D^1/2 x), C order. */
+int fs_net_create(fs_ctx* ctx, uint64_t seed, fs_net** out);
+int fs_net_destroy(fs_net* net);
+int fs_net_num_params(const fs_net* net, int64_t* count);
+/* flat FP32 parameters: per layer W[27][cin][cout] (tap = (dz+1)*9 + (dy+1)*3
+ * + (dx+1), x fastest), then b[cout] */
+int fs_net_get_params(fs_net* net, float* params);
+int fs_net_set_params(fs_net* net, const float* params);
+/* The network alone on an nx*ny*nz voxel grid (x fastest), batch nb:
+ * X [nb][n][16] -> Y [nb][n][16] (BF16 values returned as float).  With dY
+ * non-NULL also grad[num_params] = d(sum dY*Y)/d(params). */
+int fs_net_eval(fs_net* net, int nx, int ny, int nz, int nb, const float* X, float* Y,
+                const float* dY, float* grad);
+
+/* groups: for each level l < nlevels, three arrays (x, y, z) mapping the
+ * level-l index along that axis to its level-(l+1) index (0, then steps of 0
+ * or 1), concatenated.  net may be NULL.  The fs_pc must not outlive sys. */
+int fs_pc_create(fs_sys* sys, fs_net* net, int nlevels, const int32_t* groups, double omega,
+                 fs_pc** out);
+int fs_pc_destroy(fs_pc* pc);
+/* level dims (l = 0 .. nlevels) */
+int fs_pc_level_dims(fs_pc* pc, int level, int* nx, int* ny, int* nz);
+/* attach (or detach with NULL) the preconditioner used by FS_PREC_FCG */
+int fs_sys_set_precond(fs_sys* sys, fs_pc* pc);
+/* z = M(r) for nrhs equilibrated vectors */
+int fs_pc_apply(fs_pc* pc, int nrhs, const double* r, double* z);
+/* Training pairs from exact errors e (node space, C order, as differences of
+ * solutions): the library forms e_s = D^1/2 e, r_s = A_s e_s in FP64 and keeps
+ * both on the device in FP32. */
+int fs_pc_add_pairs(fs_pc* pc, int npairs, const double* e);
+int fs_pc_num_pairs(fs_pc* pc, int* npairs);
+int fs_pc_clear_pairs(fs_pc* pc);
+/* Loss of one batch: L_b = ||e_s - M(r_s)||_A^2 / ||e_s||_A^2, and the
+ * gradient of mean_b L_b with respect to the network parameters. */
+int fs_pc_loss_grad(fs_pc* pc, int nb, const double* e, double* loss, float* grad);
+/* Adam steps on the resident pairs (batches of `batch` <= 8 drawn with the
+ * seed, random signs); losses[step] = mean batch loss before the update. */
+int fs_pc_train(fs_pc* pc, int steps, int batch, double lr, uint64_t seed, double* losses);
 
 #ifdef __cplusplus
 }
