@@ -25,6 +25,8 @@ FS_AVG = {"parallel": 0, "arithmetic": 0, "harmonic": 1, "series": 1, "min": 2, 
 FS_PREC_FP64 = 0
 FS_PREC_MIXED = 1
 FS_PREC_FCG = 2
+FS_LOSS_CYCLE = 0
+FS_LOSS_BOTTOM = 1
 
 EXPORTS = (
     "fs_ctx_create", "fs_ctx_destroy", "fs_last_error", "fs_ctx_device_info",
@@ -128,8 +130,8 @@ def _declare(lib):
     lib.fs_pc_add_pairs.argtypes = [_vp, C.c_int, _dp]
     lib.fs_pc_num_pairs.argtypes = [_vp, C.POINTER(C.c_int)]
     lib.fs_pc_clear_pairs.argtypes = [_vp]
-    lib.fs_pc_loss_grad.argtypes = [_vp, C.c_int, _dp, _dp, _vp]
-    lib.fs_pc_train.argtypes = [_vp, C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]
+    lib.fs_pc_loss_grad.argtypes = [_vp, C.c_int, C.c_int, _dp, _dp, _vp]
+    lib.fs_pc_train.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]
     for name in EXPORTS:
         if name != "fs_last_error":
             getattr(lib, name).restype = C.c_int
@@ -300,8 +302,10 @@ class DeviceSystem:
         self.info = {k: getattr(info, k) for k, _ in info._fields_}
 
     @staticmethod
-    def params(tol, maxiter=None, precision=FS_PREC_FP64, inner_tol=1e-4,
+    def params(tol, maxiter=None, precision=FS_PREC_FP64, inner_tol=0.0,
                check_every=0) -> SolveParams:
+        """``inner_tol`` 0: the library's default per method (1e-4 mixed CG,
+        1e-5 flexible CG)."""
         return SolveParams(float(tol), int(maxiter or 0), int(precision), float(inner_tol),
                            int(check_every))
 
@@ -541,17 +545,18 @@ class DevicePreconditioner:
     def clear_pairs(self) -> None:
         check(load().fs_pc_clear_pairs(self._h), "fs_pc_clear_pairs")
 
-    def loss_grad(self, e):
+    def loss_grad(self, e, kind: int = FS_LOSS_BOTTOM):
         e2 = np.ascontiguousarray(as_f64(e).reshape(-1, self.system.op.n_nodes))
         loss = np.empty(e2.shape[0])
         grad = np.empty(self.net.num_params, np.float32)
-        check(load().fs_pc_loss_grad(self._h, e2.shape[0], e2, loss, grad.ctypes.data_as(_vp)),
-              "fs_pc_loss_grad")
+        check(load().fs_pc_loss_grad(self._h, int(kind), e2.shape[0], e2, loss,
+                                     grad.ctypes.data_as(_vp)), "fs_pc_loss_grad")
         return loss, grad
 
-    def train(self, steps: int, batch: int = 4, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+    def train(self, steps: int, batch: int = 4, lr: float = 1e-3, seed: int = 0,
+              kind: int = FS_LOSS_BOTTOM) -> np.ndarray:
         losses = np.empty(int(steps))
-        check(load().fs_pc_train(self._h, int(steps), int(batch), float(lr),
+        check(load().fs_pc_train(self._h, int(kind), int(steps), int(batch), float(lr),
                                  int(seed) & (2 ** 64 - 1), losses), "fs_pc_train")
         return losses
 

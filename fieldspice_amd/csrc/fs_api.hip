@@ -203,7 +203,7 @@ int default_maxiter(const fs_sys* s, const fs_solve_params* P) {
 int run_krylov_mode(fs_sys* s, int nr, const fs_solve_params& P, int mode) {
   hipStream_t st = s->ctx->stream;
   const bool cyc = mode != 0;
-  const double inner_tol = P.inner_tol > 0 ? P.inner_tol : 1e-4;
+  const double inner_tol = P.inner_tol > 0 ? P.inner_tol : (mode == 2 ? 1e-5 : 1e-4);
   const size_t scb = sizeof(fs::CgScalars) * (size_t)nr;
   auto fetch = [&]() -> int {
     FS_HIP(hipMemcpyAsync(s->sc_host, s->sc, scb, hipMemcpyDeviceToHost, st));

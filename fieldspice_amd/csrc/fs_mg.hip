@@ -303,9 +303,86 @@ __global__ __launch_bounds__(kBlock) void k_pair_make(Level L, const double* __r
   if (threadIdx.x == 0) store_partial(partials + blockIdx.x, acc[0]);
 }
 
+// bottom training loss: per vector b, c_b = r^T D^-1 r (normalisation),
+// partials [nr][nblocks]
+__global__ __launch_bounds__(kBlock) void k_jacobi_norm(Level L, const float* __restrict__ r,
+                                                        int nr, double* __restrict__ partials) {
+  double acc[kMaxRhs];
+#pragma unroll
+  for (int a = 0; a < kMaxRhs; ++a) acc[a] = 0.0;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < L.n;
+       n += (int64_t)gridDim.x * kBlock) {
+    const double di = L.dinv[n];
+#pragma unroll
+    for (int b = 0; b < kMaxRhs; ++b) {
+      if (b >= nr) break;
+      const double v = r[b * L.vs + n];
+      acc[b] += v * v * di;
+    }
+  }
+  block_sum<kMaxRhs>(acc);
+  if (threadIdx.x == 0)
+    for (int b = 0; b < nr; ++b) store_partial(partials + (int64_t)b * gridDim.x + blockIdx.x, acc[b]);
+}
+
+// bottom energy loss L_b = (y^T A y - 2 y^T r) / c_b: gradient
+// g = 2 (A y - r) / (c_b nr) and partials [2*nr][nblocks] of y.Ay, y.r
+__global__ __launch_bounds__(kBlock) void k_energy_grad(Level L, const float* __restrict__ y,
+                                                        const float* __restrict__ r,
+                                                        const double* __restrict__ c,
+                                                        float* __restrict__ g, int nr,
+                                                        double* __restrict__ partials) {
+  double acc[2 * kMaxRhs];
+#pragma unroll
+  for (int a = 0; a < 2 * kMaxRhs; ++a) acc[a] = 0.0;
+  const int64_t sxy = (int64_t)L.nx * L.ny;
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < L.n;
+       n += (int64_t)gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(L, n, i, j, k);
+#pragma unroll
+    for (int b = 0; b < kMaxRhs; ++b) {
+      if (b >= nr) break;
+      const float* yb = y + b * L.vs;
+      float s = 0.f;
+      if (i + 1 < L.nx) s += L.c[0][n] * yb[n + 1];
+      if (i > 0) s += L.c[0][n - 1] * yb[n - 1];
+      if (j + 1 < L.ny) s += L.c[1][n] * yb[n + L.nx];
+      if (j > 0) s += L.c[1][n - L.nx] * yb[n - L.nx];
+      if (k + 1 < L.nz) s += L.c[2][n] * yb[n + sxy];
+      if (k > 0) s += L.c[2][n - sxy] * yb[n - sxy];
+      // inactive nodes: y = r = 0 and every coupling is 0
+      const float w = L.dinv[n] > 0.f ? L.dg[n] * yb[n] - s : 0.f;
+      const float rv = r[b * L.vs + n];
+      const double cb = c[b] > 0.0 ? c[b] : 1.0;
+      g[b * L.vs + n] = (float)(2.0 * ((double)w - (double)rv) / (cb * nr));
+      acc[b] += (double)yb[n] * (double)w;
+      acc[kMaxRhs + b] += (double)yb[n] * (double)rv;
+    }
+  }
+  block_sum<2 * kMaxRhs>(acc);
+  if (threadIdx.x == 0) {
+    for (int b = 0; b < nr; ++b) {
+      store_partial(partials + (int64_t)b * gridDim.x + blockIdx.x, acc[b]);
+      store_partial(partials + (int64_t)(nr + b) * gridDim.x + blockIdx.x, acc[kMaxRhs + b]);
+    }
+  }
+}
+
 // -------------------------------------------------------------------------
 // launchers
 // -------------------------------------------------------------------------
+void jacobi_norm(hipStream_t s, const Level& L, const float* r, int nr, double* partials,
+                 int nblocks) {
+  hipLaunchKernelGGL(k_jacobi_norm, dim3(nblocks), dim3(kBlock), 0, s, L, r, nr, partials);
+}
+
+void energy_grad(hipStream_t s, const Level& L, const float* y, const float* r, const double* c,
+                 float* g, int nr, double* partials, int nblocks) {
+  hipLaunchKernelGGL(k_energy_grad, dim3(nblocks), dim3(kBlock), 0, s, L, y, r, c, g, nr,
+                     partials);
+}
+
 int vec_blocks(int64_t n) {
   const int64_t b = (n + kBlock - 1) / kBlock;
   return (int)(b < 2048 ? (b > 0 ? b : 1) : 2048);

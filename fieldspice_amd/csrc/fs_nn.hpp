@@ -110,6 +110,11 @@ void loss_grad(hipStream_t s, const Level& L, const float* z, const float* r,
 // r = A e for FP64 e (pairs are made from exact errors), FP32 outputs of e and r
 void pair_make(hipStream_t s, const Level& L, const double* const c64[3], const double* e,
                float* e32, float* r32, double* partials, int nblocks);
+// bottom-level training loss: c_b = r^T D^-1 r, then g = 2 (A y - r) / (c_b nr)
+void jacobi_norm(hipStream_t s, const Level& L, const float* r, int nr, double* partials,
+                 int nblocks);
+void energy_grad(hipStream_t s, const Level& L, const float* y, const float* r, const double* c,
+                 float* g, int nr, double* partials, int nblocks);
 int vec_blocks(int64_t n);
 
 }  // namespace mg

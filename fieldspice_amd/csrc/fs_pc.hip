@@ -282,6 +282,8 @@ struct fs_pc {
   std::vector<int32_t*> owned_i;
   std::vector<float*> rv, ev;  // per level >= 1: [kMaxRhs][vs]
   float *feat4 = nullptr, *dh = nullptr, *sc = nullptr;
+  float* gK = nullptr;     // bottom-loss gradient [kMaxRhs][vs_K]
+  double* cK = nullptr;    // bottom-loss normalisation per batch element
   NetWs ws;
   // training
   int npairs = 0, cap_pairs = 0;
@@ -307,6 +309,8 @@ void free_pc(fs_pc* pc) {
   dfree(pc->feat4);
   dfree(pc->dh);
   dfree(pc->sc);
+  dfree(pc->gK);
+  dfree(pc->cK);
   free_ws(pc->ws);
   dfree(pc->pe);
   dfree(pc->pr);
@@ -393,6 +397,37 @@ int batch_loss_grad(fs_pc* pc, int nb, const float* gscale, int64_t sum_off) {
                   pc->lv[l + 1].vs, nb, pc->omega);
   NetWs& w = pc->ws;
   nn::put0(s, pc->rv[pc->K], pc->lv[pc->K].vs, pc->dh, pc->sc, w.g[1], w.t[T_DY]);
+  if (int rc = net_backward(pc->net, w)) return rc;
+  FS_HIP(hipGetLastError());
+  return FS_OK;
+}
+
+// Bottom loss of one batch held in pc->br: the down-sweep gives r_K, the
+// network y = e_K, and L_b = (y^T A_K y - 2 y^T r_K) / (r_K^T D_K^-1 r_K)
+// (the energy functional of A_K e = r_K, minimal at the exact bottom solve).
+// Leaves y.Ay, y.r and c per element in pc->sums[sum_off..] ([3*nb]).
+int batch_bottom_loss_grad(fs_pc* pc, int nb, int64_t sum_off) {
+  hipStream_t s = pc->sys->ctx->stream;
+  const int K = pc->K;
+  for (int l = 0; l < K; ++l)
+    mg::restrict_(s, pc->lv[l], pc->maps[l], l == 0 ? pc->br : pc->rv[l], pc->rv[l + 1],
+                  pc->lv[l + 1].vs, nb, pc->omega);
+  const mg::Level& B = pc->lv[K];
+  NetWs& w = pc->ws;
+  set_batch(w, nb);
+  nn::rms(s, pc->rv[K], B.vs, pc->dh, w.g[1], pc->sc);
+  nn::features(s, pc->rv[K], B.vs, pc->dh, pc->feat4, pc->sc, w.g[1], w.t[T_X0]);
+  if (int rc = net_forward(pc->net, w)) return rc;
+  nn::take0(s, w.t[T_Y], pc->dh, pc->sc, w.g[1], pc->ev[K], B.vs);
+  const int nbk = mg::vec_blocks(B.n);
+  mg::jacobi_norm(s, B, pc->rv[K], nb, pc->part, nbk);
+  hipLaunchKernelGGL(k_sum_rows, dim3(nb), dim3(256), 0, s, pc->part, nbk, pc->cK);
+  mg::energy_grad(s, B, pc->ev[K], pc->rv[K], pc->cK, pc->gK, nb, pc->part, nbk);
+  hipLaunchKernelGGL(k_sum_rows, dim3(2 * nb), dim3(256), 0, s, pc->part, nbk,
+                     pc->sums + sum_off);
+  FS_HIP(hipMemcpyAsync(pc->sums + sum_off + 2 * nb, pc->cK, nb * sizeof(double),
+                        hipMemcpyDeviceToDevice, s));
+  nn::put0(s, pc->gK, B.vs, pc->dh, pc->sc, w.g[1], w.t[T_DY]);
   if (int rc = net_backward(pc->net, w)) return rc;
   FS_HIP(hipGetLastError());
   return FS_OK;
@@ -692,6 +727,8 @@ int fs_pc_create(fs_sys* sys, fs_net* net, int nlevels, const int32_t* groups, d
                         B.ny, B.nz));
   if ((rc = dalloc(&pc->feat4, (size_t)B.n * 4)) || (rc = dalloc(&pc->dh, (size_t)B.n)) ||
       (rc = dalloc(&pc->sc, (size_t)2 * fs::kMaxRhs)) ||
+      (rc = dalloc(&pc->gK, (size_t)fs::kMaxRhs * B.vs)) ||
+      (rc = dalloc(&pc->cK, (size_t)fs::kMaxRhs)) ||
       (rc = dalloc(&pc->part, (size_t)2 * fs::kMaxRhs * std::max(pc->nblk0, 2048) + 64)) ||
       (rc = ensure_sums(pc, 64)))
     return fail_pc(rc);
@@ -853,11 +890,12 @@ int fs_pc_clear_pairs(fs_pc* pc) {
   return FS_OK;
 }
 
-int fs_pc_loss_grad(fs_pc* pc, int nb, const double* e, double* loss, float* grad) {
+int fs_pc_loss_grad(fs_pc* pc, int kind, int nb, const double* e, double* loss, float* grad) {
   FS_GUARD_BEGIN
   if (!pc || !e || !loss) return fail(FS_EINVAL, "NULL argument");
   if (!pc->sys) return fail(FS_EINVAL, "the preconditioner's system was destroyed");
   if (!pc->net) return fail(FS_EINVAL, "preconditioner has no network");
+  if (kind != FS_LOSS_CYCLE && kind != FS_LOSS_BOTTOM) return fail(FS_EINVAL, "unknown loss %d", kind);
   if (nb < 1 || nb > fs::kMaxRhs) return fail(FS_EINVAL, "nb must be in 1..%d", fs::kMaxRhs);
   fs_sys* sys = pc->sys;
   fs_ctx* ctx = sys->ctx;
@@ -870,21 +908,28 @@ int fs_pc_loss_grad(fs_pc* pc, int nb, const double* e, double* loss, float* gra
       return rc;
     if (!(E[b] > 0.0)) return fail(FS_EINVAL, "error %d has no energy on the free nodes", b);
   }
-  std::vector<float> gs(fs::kMaxRhs, 0.0f);
-  for (int b = 0; b < nb; ++b) gs[b] = (float)(1.0 / (E[b] * nb));
-  if (pc->gscale_cap < fs::kMaxRhs) {
-    dfree(pc->gscale);
-    if (int rc = dalloc(&pc->gscale, fs::kMaxRhs)) return rc;
-    pc->gscale_cap = fs::kMaxRhs;
+  if (int rc = ensure_sums(pc, 3 * nb)) return rc;
+  std::vector<double> sums(3 * nb);
+  if (kind == FS_LOSS_CYCLE) {
+    std::vector<float> gs(fs::kMaxRhs, 0.0f);
+    for (int b = 0; b < nb; ++b) gs[b] = (float)(1.0 / (E[b] * nb));
+    if (pc->gscale_cap < fs::kMaxRhs) {
+      dfree(pc->gscale);
+      if (int rc = dalloc(&pc->gscale, fs::kMaxRhs)) return rc;
+      pc->gscale_cap = fs::kMaxRhs;
+    }
+    FS_HIP(hipMemcpy(pc->gscale, gs.data(), fs::kMaxRhs * sizeof(float), hipMemcpyHostToDevice));
+    if (int rc = batch_loss_grad(pc, nb, pc->gscale, 0)) return rc;
+  } else {
+    if (int rc = batch_bottom_loss_grad(pc, nb, 0)) return rc;
   }
-  FS_HIP(hipMemcpy(pc->gscale, gs.data(), fs::kMaxRhs * sizeof(float), hipMemcpyHostToDevice));
-  if (int rc = ensure_sums(pc, 2 * nb)) return rc;
-  if (int rc = batch_loss_grad(pc, nb, pc->gscale, 0)) return rc;
-  std::vector<double> sums(2 * nb);
-  FS_HIP(hipMemcpyAsync(sums.data(), pc->sums, 2 * nb * sizeof(double), hipMemcpyDeviceToHost,
-                        ctx->stream));
+  FS_HIP(hipMemcpyAsync(sums.data(), pc->sums, (kind == FS_LOSS_CYCLE ? 2 : 3) * nb * sizeof(double),
+                        hipMemcpyDeviceToHost, ctx->stream));
   FS_HIP(hipStreamSynchronize(ctx->stream));
-  for (int b = 0; b < nb; ++b) loss[b] = (sums[b] - 2.0 * sums[nb + b]) / E[b] + 1.0;
+  for (int b = 0; b < nb; ++b) {
+    const double norm = kind == FS_LOSS_CYCLE ? E[b] : (sums[2 * nb + b] > 0 ? sums[2 * nb + b] : 1.0);
+    loss[b] = (sums[b] - 2.0 * sums[nb + b]) / norm + (kind == FS_LOSS_CYCLE ? 1.0 : 0.0);
+  }
   if (grad)
     FS_HIP(hipMemcpy(grad, pc->net->grad, pc->net->nparams * sizeof(float),
                      hipMemcpyDeviceToHost));
@@ -892,7 +937,8 @@ int fs_pc_loss_grad(fs_pc* pc, int nb, const double* e, double* loss, float* gra
   FS_GUARD_END
 }
 
-int fs_pc_train(fs_pc* pc, int steps, int batch, double lr, uint64_t seed, double* losses) {
+int fs_pc_train(fs_pc* pc, int kind, int steps, int batch, double lr, uint64_t seed,
+                double* losses) {
   FS_GUARD_BEGIN
   if (!pc) return fail(FS_EINVAL, "NULL argument");
   if (!pc->sys) return fail(FS_EINVAL, "the preconditioner's system was destroyed");
@@ -901,6 +947,7 @@ int fs_pc_train(fs_pc* pc, int steps, int batch, double lr, uint64_t seed, doubl
   if (batch < 1 || batch > fs::kMaxRhs) return fail(FS_EINVAL, "batch must be in 1..%d", fs::kMaxRhs);
   if (pc->npairs < 1) return fail(FS_EINVAL, "no training pairs (fs_pc_add_pairs)");
   if (!(lr > 0.0)) return fail(FS_EINVAL, "lr must be > 0");
+  if (kind != FS_LOSS_CYCLE && kind != FS_LOSS_BOTTOM) return fail(FS_EINVAL, "unknown loss %d", kind);
   fs_sys* sys = pc->sys;
   fs_ctx* ctx = sys->ctx;
   FS_HIP(hipSetDevice(ctx->device));
@@ -924,7 +971,8 @@ int fs_pc_train(fs_pc* pc, int steps, int batch, double lr, uint64_t seed, doubl
     pc->gscale_cap = (int64_t)steps * fs::kMaxRhs;
   }
   FS_HIP(hipMemcpy(pc->gscale, gs.data(), gs.size() * sizeof(float), hipMemcpyHostToDevice));
-  if (int rc = ensure_sums(pc, (int64_t)2 * batch * steps)) return rc;
+  const int per = kind == FS_LOSS_CYCLE ? 2 : 3;  // sums per element and step
+  if (int rc = ensure_sums(pc, (int64_t)per * batch * steps)) return rc;
   const int64_t n = sys->d.n;
   for (int st = 0; st < steps; ++st) {
     for (int b = 0; b < batch; ++b) {
@@ -933,13 +981,17 @@ int fs_pc_train(fs_pc* pc, int steps, int batch, double lr, uint64_t seed, doubl
                          pc->pr + (size_t)p * vs, n, sgn[(size_t)st * batch + b],
                          pc->br + (size_t)b * vs);
     }
-    if (int rc = batch_loss_grad(pc, batch, pc->gscale + (size_t)st * fs::kMaxRhs,
-                                 (int64_t)2 * batch * st))
+    const int64_t off = (int64_t)per * batch * st;
+    if (kind == FS_LOSS_CYCLE) {
+      if (int rc = batch_loss_grad(pc, batch, pc->gscale + (size_t)st * fs::kMaxRhs, off))
+        return rc;
+    } else if (int rc = batch_bottom_loss_grad(pc, batch, off)) {
       return rc;
+    }
     if (int rc = adam_update(pc->net, (float)lr)) return rc;
   }
   FS_HIP(hipGetLastError());
-  std::vector<double> sums((size_t)2 * batch * steps);
+  std::vector<double> sums((size_t)per * batch * steps);
   FS_HIP(hipMemcpyAsync(sums.data(), pc->sums, sums.size() * sizeof(double), hipMemcpyDeviceToHost,
                         ctx->stream));
   FS_HIP(hipStreamSynchronize(ctx->stream));
@@ -947,10 +999,15 @@ int fs_pc_train(fs_pc* pc, int steps, int batch, double lr, uint64_t seed, doubl
     for (int st = 0; st < steps; ++st) {
       double acc = 0.0;
       for (int b = 0; b < batch; ++b) {
-        const double E = pc->pE[idx[(size_t)st * batch + b]];
-        const double zaz = sums[(size_t)2 * batch * st + b];
-        const double zr = sums[(size_t)2 * batch * st + batch + b];
-        acc += (zaz - 2.0 * zr) / E + 1.0;
+        const size_t o = (size_t)per * batch * st;
+        const double zaz = sums[o + b];
+        const double zr = sums[o + batch + b];
+        if (kind == FS_LOSS_CYCLE) {
+          acc += (zaz - 2.0 * zr) / pc->pE[idx[(size_t)st * batch + b]] + 1.0;
+        } else {
+          const double c = sums[o + 2 * batch + b];
+          acc += (zaz - 2.0 * zr) / (c > 0 ? c : 1.0);
+        }
       }
       losses[st] = acc / batch;
     }

@@ -34,10 +34,10 @@ import numpy as np
 from . import _lib
 
 __all__ = ["axis_groups", "hierarchy_groups", "LearnedPreconditioner", "default_weights_path",
-           "load_net", "save_net", "DEFAULT_OMEGA", "DEFAULT_MAX_BOTTOM"]
+           "load_net", "save_net", "uniform_widths", "DEFAULT_OMEGA", "DEFAULT_MAX_BOTTOM"]
 
 DEFAULT_OMEGA = 0.8
-DEFAULT_MAX_BOTTOM = 40_000
+DEFAULT_MAX_BOTTOM = 8_000
 WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
 
 
@@ -141,16 +141,32 @@ class LearnedPreconditioner:
     def groups_for(self, grid):
         return hierarchy_groups((grid.ax.hd, grid.ay.hd, grid.az.hd), self.max_bottom)
 
-    def build(self, system: "_lib.DeviceSystem", grid) -> "_lib.DevicePreconditioner":
-        groups = self.groups_for(grid)
+    def build_widths(self, system: "_lib.DeviceSystem", widths) -> "_lib.DevicePreconditioner":
+        groups = hierarchy_groups(widths, self.max_bottom)
         if not groups:
             raise ValueError("grid too small for a multilevel preconditioner")
         return _lib.DevicePreconditioner(system, self.net, groups, self.omega)
 
-    def attach(self, system: "_lib.DeviceSystem", grid) -> "_lib.DevicePreconditioner":
-        pc = self.build(system, grid)
+    def build(self, system: "_lib.DeviceSystem", grid) -> "_lib.DevicePreconditioner":
+        return self.build_widths(system, (grid.ax.hd, grid.ay.hd, grid.az.hd))
+
+    def attach_widths(self, system, widths) -> "_lib.DevicePreconditioner":
+        pc = self.build_widths(system, widths)
         system.set_precond(pc)
         return pc
+
+    def attach(self, system: "_lib.DeviceSystem", grid) -> "_lib.DevicePreconditioner":
+        return self.attach_widths(system, (grid.ax.hd, grid.ay.hd, grid.az.hd))
+
+
+def uniform_widths(shape_cells):
+    """Dual widths of a unit-spaced grid with ``shape_cells`` cells."""
+    out = []
+    for nc in shape_cells:
+        w = np.ones(int(nc) + 1)
+        w[0] = w[-1] = 0.5
+        out.append(w)
+    return tuple(out)
 
 
 def warn_untrained():

@@ -42,7 +42,11 @@ class SolverConfig:
         algorithm), ``"hip_mixed"`` (FP32 CG cycles with FP64 residual
         refresh), ``"hip_fcg"`` (flexible CG with the learned preconditioner).
     inner_tol
-        Residual reduction of one FP32 cycle before the FP64 refresh.
+        Residual reduction of one FP32 cycle before the FP64 refresh
+        (``None``: 1e-4 for ``hip_mixed``, 1e-5 for ``hip_fcg``).
+    preconditioner
+        ``hip_fcg`` only: a ``precond.LearnedPreconditioner``; ``None`` uses
+        the trained network shipped with the package.
     maxiter
         Krylov cap; ``None`` is fieldspice's ``max(500, 10*sqrt(n))``.
     """
@@ -54,9 +58,10 @@ class SolverConfig:
     verbose: int = 0
     store_every: int = 1
     dtype: type = np.float64
-    inner_tol: float = 1e-4
+    inner_tol: float | None = None
     maxiter: int | None = None
     check_every: int = 0
+    preconditioner: object = None
 
     def __post_init__(self):
         if self.linear_solver not in LINEAR_SOLVERS:

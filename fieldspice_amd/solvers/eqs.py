@@ -77,7 +77,7 @@ class EQSSolver:
 
     def _static_solve(self, op, terminals, bc, t, rhs=None):
         fixed, vals = self._dirichlet(terminals, bc, t)
-        ls = GridLinearSystem(op, fixed, self.cfg)
+        ls = GridLinearSystem(op, fixed, self.cfg, grid=self.grid)
         x0 = np.zeros(self.grid.n_nodes)
         x0[fixed] = vals
         b = (np.zeros(self.grid.n_nodes) if rhs is None else rhs) - op.apply(x0)
@@ -135,7 +135,7 @@ class EQSSolver:
         K = self.L_eps.combine(1.0 / dt, theta, self.L_sigma)
         Kexp = self.L_eps.combine(1.0 / dt, -(1.0 - theta), self.L_sigma)
         groups = DirichletGroups(g, terms, bc, walls_last=True)
-        ls = GridLinearSystem(K, groups.owner >= 0, self.cfg)
+        ls = GridLinearSystem(K, groups.owner >= 0, self.cfg, grid=self.grid)
         values = np.stack([groups.values_at(t) for t in times]) if groups.ngroups else None
         # current-driven terminals: dual-volume weighted injection
         src = scales = None

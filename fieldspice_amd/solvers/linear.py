@@ -30,7 +30,7 @@ def method_for(cfg: SolverConfig) -> tuple[str, int]:
     if want == "hip_mixed":
         return "hip_mixed", _lib.FS_PREC_MIXED
     if want == "hip_fcg":
-        raise NotImplementedError("linear_solver='hip_fcg' needs a trained preconditioner")
+        return "hip_fcg", _lib.FS_PREC_FCG
     return "hip_pcg", _lib.FS_PREC_FP64
 
 
@@ -46,9 +46,13 @@ class GridLinearSystem:
     config : SolverConfig
     pin : bool
         Pin the gauge of a floating system (warning) instead of raising.
+    grid : RectilinearGrid, optional
+        ``hip_fcg``: the node spacings that shape the preconditioner's
+        hierarchy (uniform spacing is assumed without it).
     """
 
-    def __init__(self, op, fixed=None, config: SolverConfig | None = None, pin: bool = True):
+    def __init__(self, op, fixed=None, config: SolverConfig | None = None, pin: bool = True,
+                 grid=None):
         t0 = time.perf_counter()
         self.cfg = config or SolverConfig()
         self.method, self.precision = method_for(self.cfg)
@@ -90,17 +94,30 @@ class GridLinearSystem:
             "setup_time": time.perf_counter() - t0,
             "notes": (),
         }
+        self.pc = None
+        if self.precision == _lib.FS_PREC_FCG:
+            from ..precond import LearnedPreconditioner, uniform_widths
+            lp = self.cfg.preconditioner or LearnedPreconditioner()
+            widths = ((grid.ax.hd, grid.ay.hd, grid.az.hd) if grid is not None
+                      else uniform_widths(op.shape_cells))
+            self.pc = lp.attach_widths(self.sys, widths)
+            self.info["precond_levels"] = self.pc.nlevels
+            self.info["setup_time"] = time.perf_counter() - t0
 
     def params(self):
         return _lib.DeviceSystem.params(self.cfg.tol, self.cfg.maxiter, self.precision,
-                                        self.cfg.inner_tol, self.cfg.check_every)
+                                        self.cfg.inner_tol or 0.0, self.cfg.check_every)
 
     def _judge(self, st: dict, rc: int, what: str = "cg"):
         """Convergence verdict of ``fieldspice/solvers/poisson.py:554-587``,
         applied to both residual norms."""
         self.info.update(iterations=st["iterations"], residual=st["residual"],
                          residual_eq=st["residual_eq"], refreshes=st["refreshes"],
-                         solve_time=st["solve_time"])
+                         solve_time=st["solve_time"], fallbacks=st["fallbacks"],
+                         precond_applies=st["pc_applies"])
+        if st["fallbacks"]:
+            warnings.warn("flexible CG did not converge; the solve was redone with FP64 CG",
+                          RuntimeWarning, stacklevel=3)
         if rc == _lib.FS_NOT_CONVERGED:
             warnings.warn(f"{self.method} returned flag {st['iterations']} after "
                           f"{st['iterations']} iterations", RuntimeWarning, stacklevel=3)

@@ -19,7 +19,7 @@ from .grid import RectilinearGrid
 from .solvers.base import Terminal
 from .units import eps0
 
-__all__ = ["Extraction", "parallel_plate", "layered_boxes", "c2_geometry"]
+__all__ = ["Extraction", "parallel_plate", "layered_boxes", "c2_geometry", "multi_conductor"]
 
 
 @dataclass
@@ -89,3 +89,44 @@ def c2_geometry(seed: int = 0, nodes=(192, 192, 96)) -> Extraction:
     """192x192x96 nodes, x/y 50 nm, z log-graded 20 -> 400 nm, three layers with
     eps_r from {2.7, 3.9, 7.5}, eight boxes of 4-40 cells per side."""
     return layered_boxes(nodes=nodes, seed=seed)
+
+
+def multi_conductor(nodes=(128, 128, 128), seed: int = 0, h: float = 20e-9,
+                    n_terms=(2, 8), eps_range=(2.5, 8.0), n_layers=(1, 4),
+                    side=None, max_tries=100000) -> Extraction:
+    """Seeded multi-conductor extraction on a uniform grid: ``n_layers`` z
+    layers with eps_r drawn uniformly from ``eps_range``, and a number of
+    rectangular conductor boxes drawn from ``n_terms``.  Boxes have sides of
+    ``side`` cells (default 4 .. N/4) and at least one free cell between
+    them.  Each box is a terminal owning every node of its closed box."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = nodes
+    Nx, Ny, Nz = nx - 1, ny - 1, nz - 1
+    g = RectilinearGrid.uniform([(0, Nx * h), (0, Ny * h), (0, Nz * h)], (Nx, Ny, Nz))
+    nl = int(rng.integers(n_layers[0], n_layers[1] + 1))
+    cuts = np.sort(rng.choice(np.arange(1, Nz), size=nl - 1, replace=False)) if nl > 1 else []
+    bounds = np.concatenate([[0], cuts, [Nz]]).astype(int)
+    layer_eps = [float(rng.uniform(*eps_range)) for _ in range(nl)]
+    eps_r = np.empty(g.shape_cells)
+    for L in range(nl):
+        eps_r[:, :, bounds[L]:bounds[L + 1]] = layer_eps[L]
+    nt = int(rng.integers(n_terms[0], n_terms[1] + 1))
+    lo, hi = side if side is not None else (4, max(5, min(Nx, Ny, Nz) // 4))
+    occ = np.zeros(g.shape_cells, dtype=bool)
+    boxes = []
+    tries = 0
+    while len(boxes) < nt:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("could not place the conductor boxes; enlarge the grid")
+        s = [int(rng.integers(lo, min(hi, N - 2) + 1)) for N in (Nx, Ny, Nz)]
+        o = [int(rng.integers(1, N - sd)) for N, sd in zip((Nx, Ny, Nz), s)]
+        sl = tuple(slice(max(a - 1, 0), a + b + 1) for a, b in zip(o, s))
+        if occ[sl].any():
+            continue
+        occ[sl] = True
+        boxes.append((o[0], o[0] + s[0], o[1], o[1] + s[1], o[2], o[2] + s[2]))
+    nid = g.node_ids()
+    terms = [Terminal(f"c{m}", nid[i0:i1 + 1, j0:j1 + 1, k0:k1 + 1].ravel())
+             for m, (i0, i1, j0, j1, k0, k1) in enumerate(boxes)]
+    return Extraction(g, eps_r * eps0, terms, boxes, layer_eps, seed)

@@ -64,7 +64,8 @@ typedef struct {
     double tol;          /* relative residual ||r_s|| < tol*||b_s|| in the equilibrated space */
     int maxiter;         /* <= 0: max(500, floor(10*sqrt(n_free))) as the reference */
     int precision;       /* FS_PREC_FP64, FS_PREC_MIXED or FS_PREC_FCG */
-    double inner_tol;    /* MIXED/FCG: residual drop per FP32 cycle before an FP64 refresh */
+    double inner_tol;    /* MIXED/FCG: residual drop per FP32 cycle before an FP64 refresh
+                            (<= 0: 1e-4 MIXED, 1e-5 FCG) */
     int check_every;     /* iterations between host convergence checks (<= 0: adaptive) */
 } fs_solve_params;
 
@@ -250,12 +251,20 @@ int fs_pc_apply(fs_pc* pc, int nrhs, const double* r, double* z);
 int fs_pc_add_pairs(fs_pc* pc, int npairs, const double* e);
 int fs_pc_num_pairs(fs_pc* pc, int* npairs);
 int fs_pc_clear_pairs(fs_pc* pc);
-/* Loss of one batch: L_b = ||e_s - M(r_s)||_A^2 / ||e_s||_A^2, and the
- * gradient of mean_b L_b with respect to the network parameters. */
-int fs_pc_loss_grad(fs_pc* pc, int nb, const double* e, double* loss, float* grad);
+/* Training losses of one pair (e_s, r_s = A_s e_s):
+ *   FS_LOSS_CYCLE   L = ||e_s - M(r_s)||_A^2 / ||e_s||_A^2 (the whole cycle)
+ *   FS_LOSS_BOTTOM  L = (y^T A_K y - 2 y^T r_K) / (r_K^T D_K^-1 r_K), with r_K
+ *                   the residual the down-sweep hands to the bottom and y the
+ *                   network's bottom error (minimal at y = A_K^-1 r_K) */
+#define FS_LOSS_CYCLE 0
+#define FS_LOSS_BOTTOM 1
+/* loss[b] of nb errors (node space) and the gradient of mean_b loss[b] with
+ * respect to the network parameters */
+int fs_pc_loss_grad(fs_pc* pc, int kind, int nb, const double* e, double* loss, float* grad);
 /* Adam steps on the resident pairs (batches of `batch` <= 8 drawn with the
  * seed, random signs); losses[step] = mean batch loss before the update. */
-int fs_pc_train(fs_pc* pc, int steps, int batch, double lr, uint64_t seed, double* losses);
+int fs_pc_train(fs_pc* pc, int kind, int steps, int batch, double lr, uint64_t seed,
+                double* losses);
 
 #ifdef __cplusplus
 }

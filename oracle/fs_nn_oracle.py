@@ -12,14 +12,17 @@ this module restates the GPU library's own definitions from their mathematics.
   every stored activation and gradient.  Sums are float64.
 * ``Hierarchy``: Galerkin coarse operators ``P^T A P`` as sparse matrix
   products of box-aggregation prolongations.
-* ``Hierarchy.cycle``: the smoothing/restriction recursion of the cycle, and
-  ``loss_grad``: the energy-norm loss and its gradient by reverse mode.
+* ``Hierarchy.cycle``: the smoothing/restriction recursion of the cycle;
+  ``loss_grad``: the energy-norm loss of the whole cycle and its gradient by
+  reverse mode; ``bottom_loss_grad``: the energy functional of the bottom
+  system, which the network minimises at the exact bottom solve.
 
 Pinning (tests/test_nn_oracle.py): the backward passes against central finite
 differences in float64; the Galerkin operators against the dense triple
 product; the recursion of the cycle against its closed form
 ``M = omega D^-1 + S omega D^-1 + S P M_c P^T S^T`` (``S = I - omega D^-1 A``),
-including symmetry for a linear bottom.
+including symmetry for a linear bottom; the bottom loss against finite
+differences and its minimiser against a direct solve of the bottom system.
 
 Only ``tests/`` import this module.
 """
@@ -383,3 +386,22 @@ class Hierarchy:
             dY[b, ..., 0] = self.to_grid(s[b] * dh * gK[b])
         grad = net_backward(params, A, dY, bf)
         return loss, grad, z
+
+    def bottom_loss_grad(self, params, e0, bf=True):
+        """Bottom loss (y^T A_K y - 2 y^T r_K) / (r_K^T D_K^-1 r_K) per element,
+        r_K the down-sweep of r = A e, y the network's bottom error; and the
+        gradient of its batch mean."""
+        e0 = np.atleast_2d(e0)
+        r0 = (self.A[0] @ e0.T).T
+        rK = self.restrict_chain(r0)
+        y, (A, s, dh) = self.bottom_net(params, rK, bf)
+        AK = self.A[-1]
+        w = (AK @ y.T).T
+        c = np.einsum("bi,bi->b", rK * self.dinv[-1], rK)
+        loss = (np.einsum("bi,bi->b", y, w) - 2 * np.einsum("bi,bi->b", y, rK)) / c
+        nb = e0.shape[0]
+        g = 2.0 * (w - rK) / (c[:, None] * nb)
+        dY = np.zeros(A["Y"].shape)
+        for b in range(nb):
+            dY[b, ..., 0] = self.to_grid(s[b] * dh * g[b])
+        return loss, net_backward(params, A, dY, bf), y

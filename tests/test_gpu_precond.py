@@ -119,8 +119,12 @@ def test_loss_and_gradient_match_oracle():
     H = N.Hierarchy(A, free, g.shape_nodes, groups, 0.8)
     rng = np.random.default_rng(6)
     e = rng.normal(size=(3, g.n_nodes)) * free          # node-space errors
-    loss, grad = pc.loss_grad(e)
+    loss, grad = pc.loss_grad(e, kind=_lib.FS_LOSS_CYCLE)
     lref, gref, _ = H.loss_grad(p, e * dh, bf=True)
+    np.testing.assert_allclose(loss, lref, rtol=2e-3)
+    assert _rel(grad, gref) < 2e-2, _rel(grad, gref)
+    loss, grad = pc.loss_grad(e, kind=_lib.FS_LOSS_BOTTOM)
+    lref, gref, _ = H.bottom_loss_grad(p, e * dh, bf=True)
     np.testing.assert_allclose(loss, lref, rtol=2e-3)
     assert _rel(grad, gref) < 2e-2, _rel(grad, gref)
 
@@ -172,7 +176,8 @@ def test_training_lowers_loss_and_fcg_beats_the_untrained_cycle():
     losses = pc.train(300, batch=4, lr=2e-3, seed=1)
     print(f"loss first 30 {losses[:30].mean():.4f}  last 30 {losses[-30:].mean():.4f}", flush=True)
     assert np.all(np.isfinite(losses))
-    assert losses[-30:].mean() < 0.8 * losses[:30].mean(), (losses[:30].mean(), losses[-30:].mean())
+    # the bottom energy is negative and decreases towards the exact solve's
+    assert losses[-30:].mean() < losses[:30].mean() - 0.1 * abs(losses[:30].mean())
     # flexible CG with the trained network, against the cycle with one Jacobi
     # step at the bottom and plain FP64 CG, on lifted right-hand sides
     xD = np.zeros((2, g.n_nodes))

@@ -165,3 +165,25 @@ def test_loss_gradient_matches_finite_differences():
         h = 1e-6
         fd = (f(p + h * v) - f(p - h * v)) / (2 * h)
         assert abs(fd - grad @ v) <= 1e-5 * max(1.0, abs(fd)), (fd, grad @ v)
+
+
+def test_bottom_loss_gradient_and_minimiser():
+    g, A, free = _small_system((9, 8, 6), seed=3)
+    groups = hierarchy_groups((g.ax.hd, g.ay.hd, g.az.hd), max_bottom=60)
+    H = N.Hierarchy(A, free, g.shape_nodes, groups)
+    rng = np.random.default_rng(8)
+    p = N.init_params(2).astype(np.float64)
+    e = rng.normal(size=(2, A.shape[0])) * free
+    loss, grad, y = H.bottom_loss_grad(p, e, bf=False)
+    f = lambda q: float(np.mean(H.bottom_loss_grad(q, e, bf=False)[0]))
+    v = rng.normal(size=p.size)
+    h = 1e-6
+    fd = (f(p + h * v) - f(p - h * v)) / (2 * h)
+    assert abs(fd - grad @ v) <= 1e-5 * max(1.0, abs(fd)), (fd, grad @ v)
+    # the functional is bounded below by its value at the exact bottom solve
+    rK = H.restrict_chain((A @ e.T).T)
+    AK = H.A[-1].toarray()
+    yK = np.linalg.solve(AK, rK.T).T
+    c = np.einsum("bi,bi->b", rK * H.dinv[-1], rK)
+    best = (np.einsum("bi,bi->b", yK, (AK @ yK.T).T) - 2 * np.einsum("bi,bi->b", yK, rK)) / c
+    assert np.all(loss >= best - 1e-12) and np.all(best < 0)
