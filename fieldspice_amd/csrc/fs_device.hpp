@@ -128,4 +128,36 @@ __device__ __forceinline__ double raw_row_diff(const double* __restrict__ x,
   return v;
 }
 
+// --------------------------------------------------------------------------
+// Krylov vectors: NR = compile-time right-hand-side capacity, nr <= NR live;
+// the vectors of right-hand side r start at r*vs.
+// --------------------------------------------------------------------------
+struct VecArgs {
+  int64_t n;
+  int64_t vs;
+  int nr;
+};
+
+__device__ __forceinline__ bool rhs_live(const CgScalars* sc, int r, int mode) {
+  return mode == 0 ? sc[r].done == 0 : (sc[r].done == 0 && sc[r].cycle_done == 0);
+}
+
+__device__ __forceinline__ bool all_idle(const CgScalars* sc, int nr, int mode) {
+  for (int r = 0; r < nr; ++r)
+    if (rhs_live(sc, r, mode)) return false;
+  return true;
+}
+
+// Bind a tile of the XCD-remapped launch; returns false for padding blocks.
+__device__ __forceinline__ bool tile_of_block(const Tiling& tl, int& tx, int& ty, int& tz) {
+  const int bid = blockIdx.x;
+  const int per = tl.nblocks >> 3;
+  const int t = (bid & 7) * per + (bid >> 3);
+  if (t >= tl.ntiles) return false;
+  tx = t % tl.tiles_x;
+  ty = (t / tl.tiles_x) % tl.tiles_y;
+  tz = t / (tl.tiles_x * tl.tiles_y);
+  return true;
+}
+
 }  // namespace fs

@@ -352,33 +352,6 @@ int setup_system(fs_sys* sys, double* dsum, void* counts_dev, void* counts_host,
 // Krylov kernels.  NR = compile-time right-hand-side capacity, nr <= NR live.
 // Vectors of right-hand side r start at r*vs.
 // --------------------------------------------------------------------------
-struct VecArgs {
-  int64_t n;
-  int64_t vs;
-  int nr;
-};
-
-__device__ __forceinline__ bool rhs_live(const CgScalars* sc, int r, int mode) {
-  return mode == 0 ? sc[r].done == 0 : (sc[r].done == 0 && sc[r].cycle_done == 0);
-}
-
-__device__ __forceinline__ bool all_idle(const CgScalars* sc, int nr, int mode) {
-  for (int r = 0; r < nr; ++r)
-    if (rhs_live(sc, r, mode)) return false;
-  return true;
-}
-
-// Bind a tile of the XCD-remapped launch; returns false for padding blocks.
-__device__ __forceinline__ bool tile_of_block(const Tiling& tl, int& tx, int& ty, int& tz) {
-  const int bid = blockIdx.x;
-  const int per = tl.nblocks >> 3;
-  const int t = (bid & 7) * per + (bid >> 3);
-  if (t >= tl.ntiles) return false;
-  tx = t % tl.tiles_x;
-  ty = (t / tl.tiles_x) % tl.tiles_y;
-  tz = t / (tl.tiles_x * tl.tiles_y);
-  return true;
-}
 
 // q = A_s p for each live RHS (z-marching, couplings read once for all RHS),
 // fused with the block partials of p.q (alpha is set by k_finalize).
@@ -977,8 +950,9 @@ static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEve
                      s->q32, va, mode, s->sc, s->partials);
   finalize<NR>(st, s->partials, FIN_BETA, nr, vb, mode, s->sc);
   if (mode == 2) {
-    if (int rc = pc_apply_solver(s, nr)) return rc;
-    finalize<NR>(st, s->partials, FIN_FCG, nr, vb, mode, s->sc);
+    int nbp = 0;
+    if (int rc = pc_apply_solver(s, nr, &nbp)) return rc;
+    finalize<NR>(st, s->partials, FIN_FCG, nr, nbp, mode, s->sc);
   }
   return 0;
 }
@@ -1066,8 +1040,9 @@ static void launch_cycle_start_t(fs_sys* s, int nr, double inner_tol2) {
 }
 template <int NR>
 static int launch_fcg_start_t(fs_sys* s, int nr) {
-  if (int rc = pc_apply_solver(s, nr)) return rc;
-  finalize<NR>(s->ctx->stream, s->partials, FIN_FCG0, nr, s->vec_blocks, 2, s->sc);
+  int nbp = 0;
+  if (int rc = pc_apply_solver(s, nr, &nbp)) return rc;
+  finalize<NR>(s->ctx->stream, s->partials, FIN_FCG0, nr, nbp, 2, s->sc);
   return 0;
 }
 int launch_fcg_start(fs_sys* s, int nr) {

@@ -55,8 +55,9 @@ void launch_edge_power(hipStream_t st, const fs_op* A, const double* x, double* 
 
 // learned preconditioner (fs_pc.hip)
 int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, double* partials,
-             int nblocks);
-int pc_apply_solver(fs_sys* s, int nr);  // z32 = M(r32), FCG partials
+             int* nblocks, const CgScalars* sc, int mode);
+// z32 = M(r32) with the FCG partials ([2*nr][*nblocks])
+int pc_apply_solver(fs_sys* s, int nr, int* nblocks);
 // the system of pc is being destroyed: pc keeps only its own memory
 void pc_orphan(fs_pc* pc);
 // solver workspace of a system for nr right-hand sides (fs_api.hip)

@@ -370,6 +370,169 @@ __global__ __launch_bounds__(kBlock) void k_energy_grad(Level L, const float* __
 }
 
 // -------------------------------------------------------------------------
+// level 0 (the prepared system, unit diagonal) on the stencil's tiles: one
+// thread per (i, j) column of a 64x4 tile marching over z, couplings read
+// once per node, no index division.  A coupling that touches a Dirichlet or
+// pinned node is zero, so neighbour terms need no mask test.
+// -------------------------------------------------------------------------
+__device__ __forceinline__ bool live_of(const CgScalars* sc, int r, int mode) {
+  return sc == nullptr || rhs_live(sc, r, mode);
+}
+
+// e = z2 + omega (r - z2 + sum_m c_nm z2_m), z2 = omega r + ec[P(n)] on free
+// nodes; optional partials <e, r>, <e, q> ([2*nr][gridDim.x]).
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_prolong_l0(
+    const float* __restrict__ r, const float* __restrict__ ec, int64_t vsc,
+    float* __restrict__ e, const float* __restrict__ cx, const float* __restrict__ cy,
+    const float* __restrict__ cz, const uint8_t* __restrict__ mask, Dims d, Tiling tl, Map m,
+    int64_t vs, int nr, float omega, const float* __restrict__ q,
+    double* __restrict__ partials, const CgScalars* sc, int mode) {
+  double acc[2 * NR];
+#pragma unroll
+  for (int a = 0; a < 2 * NR; ++a) acc[a] = 0.0;
+  int tx, ty, tz;
+  if (tile_of_block(tl, tx, ty, tz)) {
+    const int i = tx * kTileX + (int)(threadIdx.x % kTileX);
+    const int j = ty * kTileY + (int)(threadIdx.x / kTileX);
+    const int k0 = tz * tl.kz;
+    const int k1 = min(d.nz, k0 + tl.kz);
+    if (i < d.nx && j < d.ny) {
+      const int64_t base = (int64_t)j * d.nx + i;
+      const bool has_e = i + 1 < d.nx, has_w = i > 0, has_n = j + 1 < d.ny, has_s = j > 0;
+      const int cnx = m.cn[0], cny = m.cn[1];
+      const int ci = m.f2c[0][i], cj = m.f2c[1][j];
+      const int cie = has_e ? m.f2c[0][i + 1] : ci, ciw = has_w ? m.f2c[0][i - 1] : ci;
+      const int cjn = has_n ? m.f2c[1][j + 1] : cj, cjs = has_s ? m.f2c[1][j - 1] : cj;
+#pragma unroll
+      for (int b = 0; b < NR; ++b) {
+        if (b >= nr || !live_of(sc, b, mode)) continue;
+        const float* __restrict__ rb = r + b * vs;
+        const float* __restrict__ eb = ec + b * vsc;
+        float* __restrict__ ob = e + b * vs;
+        int64_t nn = (int64_t)k0 * d.sxy + base;
+        // z2 and r along the column: below, current
+        float czm = 0.f, zm = 0.f;
+        if (k0 > 0) {
+          czm = cz[nn - d.sxy];
+          zm = omega * rb[nn - d.sxy] + eb[((int64_t)m.f2c[2][k0 - 1] * cny + cj) * cnx + ci];
+        }
+        float rc = rb[nn];
+        int ck = m.f2c[2][k0];
+        float zc = omega * rc + eb[((int64_t)ck * cny + cj) * cnx + ci];
+        double a0 = 0.0, a1 = 0.0;
+        for (int k = k0; k < k1; ++k, nn += d.sxy) {
+          const bool up = k + 1 < d.nz;
+          const int cku = up ? m.f2c[2][k + 1] : ck;
+          const float ru = up ? rb[nn + d.sxy] : 0.f;
+          const float zu = up ? omega * ru + eb[((int64_t)cku * cny + cj) * cnx + ci] : 0.f;
+          const int64_t plane = (int64_t)ck * cny;
+          const float czc = cz[nn];
+          float out = 0.f;
+          if (mask[nn] == 0) {
+            float s = czc * zu + czm * zm;
+            if (has_e) s += cx[nn] * (omega * rb[nn + 1] + eb[(plane + cj) * cnx + cie]);
+            if (has_w) s += cx[nn - 1] * (omega * rb[nn - 1] + eb[(plane + cj) * cnx + ciw]);
+            if (has_n) s += cy[nn] * (omega * rb[nn + d.nx] + eb[(plane + cjn) * cnx + ci]);
+            if (has_s) s += cy[nn - d.nx] * (omega * rb[nn - d.nx] + eb[(plane + cjs) * cnx + ci]);
+            out = zc + omega * (rc - zc + s);
+          }
+          ob[nn] = out;
+          if (partials) {
+            a0 += (double)out * (double)rc;
+            a1 += (double)out * (double)q[b * vs + nn];
+          }
+          zm = zc;
+          zc = zu;
+          rc = ru;
+          czm = czc;
+          ck = cku;
+        }
+        acc[b] = a0;
+        acc[NR + b] = a1;
+      }
+    }
+  } else if (!partials) {
+    return;  // padding block (uniform)
+  }
+  if (!partials) return;  // uniform over the launch
+  block_sum<2 * NR>(acc);
+  if (threadIdx.x == 0) {
+    for (int b = 0; b < nr; ++b) {
+      store_partial(partials + (int64_t)b * gridDim.x + blockIdx.x, acc[b]);
+      store_partial(partials + (int64_t)(nr + b) * gridDim.x + blockIdx.x, acc[NR + b]);
+    }
+  }
+}
+
+// rc = P^T (r - omega A r) when level 0 pairs nodes along x and y (groups
+// {2I, 2I+1}) and the z-tiles [kstart[t], kstart[t+1]) hold whole z-groups:
+// each thread sums its column over a z-group, x pairs meet by a lane shuffle,
+// y pairs through LDS, in a fixed order.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_restrict_l0(
+    const float* __restrict__ r, float* __restrict__ rc, int64_t vsc,
+    const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
+    Dims d, Tiling tl, const int* __restrict__ kstart, Map m, int64_t vs, int nr, float omega,
+    const CgScalars* sc, int mode) {
+  __shared__ float lds[kTileY][kTileX];
+  int tx, ty, tz;
+  if (!tile_of_block(tl, tx, ty, tz)) return;  // uniform over the block
+  const int lx = (int)(threadIdx.x % kTileX), ly = (int)(threadIdx.x / kTileX);
+  const int i = tx * kTileX + lx;
+  const int j = ty * kTileY + ly;
+  const int k0 = kstart[tz], k1 = kstart[tz + 1];
+  const bool valid = i < d.nx && j < d.ny;
+  const int64_t base = valid ? (int64_t)j * d.nx + i : 0;
+  const bool has_e = i + 1 < d.nx, has_w = i > 0, has_n = j + 1 < d.ny, has_s = j > 0;
+  const int cnx = m.cn[0], cny = m.cn[1];
+  for (int b = 0; b < nr; ++b) {
+    if (!live_of(sc, b, mode)) continue;  // uniform over the block
+    const float* __restrict__ rb = r + b * vs;
+    int64_t nn = (int64_t)k0 * d.sxy + base;
+    float rm = 0.f, czm = 0.f, rcur = 0.f;
+    if (valid) {
+      if (k0 > 0) {
+        rm = rb[nn - d.sxy];
+        czm = cz[nn - d.sxy];
+      }
+      rcur = rb[nn];
+    }
+    float col = 0.f;
+    for (int k = k0; k < k1; ++k, nn += d.sxy) {
+      const int ck = m.f2c[2][k];
+      if (valid) {
+        const bool up = k + 1 < d.nz;
+        const float ru = up ? rb[nn + d.sxy] : 0.f;
+        const float czc = cz[nn];
+        float s = czc * ru + czm * rm;
+        if (has_e) s += cx[nn] * rb[nn + 1];
+        if (has_w) s += cx[nn - 1] * rb[nn - 1];
+        if (has_n) s += cy[nn] * rb[nn + d.nx];
+        if (has_s) s += cy[nn - d.nx] * rb[nn - d.nx];
+        col += (1.f - omega) * rcur + omega * s;
+        rm = rcur;
+        rcur = ru;
+        czm = czc;
+      }
+      const bool group_end = k + 1 == k1 || m.f2c[2][k + 1] != ck;  // uniform
+      if (group_end) {
+        // x pairs: lanes 2I and 2I+1 of the wave
+        const float v = col + __shfl_down(col, 1, kWave);
+        lds[ly][lx] = v;
+        __syncthreads();
+        if ((lx & 1) == 0 && (ly & 1) == 0 && valid) {
+          const float w = v + (ly + 1 < kTileY ? lds[ly + 1][lx] : 0.f);
+          rc[b * vsc + ((int64_t)ck * cny + (j >> 1)) * cnx + (i >> 1)] = w;
+        }
+        __syncthreads();
+        col = 0.f;
+      }
+    }
+  }
+}
+
+// -------------------------------------------------------------------------
 // launchers
 // -------------------------------------------------------------------------
 void jacobi_norm(hipStream_t s, const Level& L, const float* r, int nr, double* partials,
@@ -381,6 +544,27 @@ void energy_grad(hipStream_t s, const Level& L, const float* y, const float* r, 
                  float* g, int nr, double* partials, int nblocks) {
   hipLaunchKernelGGL(k_energy_grad, dim3(nblocks), dim3(kBlock), 0, s, L, y, r, c, g, nr,
                      partials);
+}
+
+int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, int64_t vsc,
+               float* e, int nr, float omega, const float* q, double* partials,
+               const CgScalars* sc, int mode) {
+#define FS_L0P(NR_)                                                                        \
+  hipLaunchKernelGGL((k_prolong_l0<NR_>), dim3(a.tl.nblocks), dim3(kBlock), 0, s, r, ec, vsc, e, \
+                     a.c[0], a.c[1], a.c[2], a.mask, a.d, a.tl, a.m, a.vs, nr, omega, q,     \
+                     partials, sc, mode)
+  if (nr <= 1) FS_L0P(1);
+  else if (nr <= 2) FS_L0P(2);
+  else if (nr <= 4) FS_L0P(4);
+  else FS_L0P(8);
+#undef FS_L0P
+  return a.tl.nblocks;
+}
+
+void restrict_l0(hipStream_t s, const L0Args& a, const float* r, float* rc, int64_t vsc, int nr,
+                 float omega, const CgScalars* sc, int mode) {
+  hipLaunchKernelGGL((k_restrict_l0<8>), dim3(a.tlr.nblocks), dim3(kBlock), 0, s, r, rc, vsc,
+                     a.c[0], a.c[1], a.c[2], a.d, a.tlr, a.kstart, a.m, a.vs, nr, omega, sc, mode);
 }
 
 int vec_blocks(int64_t n) {

@@ -110,6 +110,25 @@ void loss_grad(hipStream_t s, const Level& L, const float* z, const float* r,
 // r = A e for FP64 e (pairs are made from exact errors), FP32 outputs of e and r
 void pair_make(hipStream_t s, const Level& L, const double* const c64[3], const double* e,
                float* e32, float* r32, double* partials, int nblocks);
+// Level 0 on the stencil tiles (fs_sys tiling for the prolongation; a
+// z-tiling aligned to the z-groups for the restriction, which needs the x
+// and y groupings of level 0 to be pairs {2I, 2I+1}).
+struct L0Args {
+  Dims d;
+  int64_t vs;
+  const float* c[3];
+  const uint8_t* mask;
+  Tiling tl;            // prolongation tiles (the system's)
+  Tiling tlr;           // restriction tiles
+  const int* kstart;    // tlr.tiles_z + 1 z-starts, each a z-group start
+  Map m;                // level 0 -> 1
+};
+// returns the number of blocks (the partials' stride)
+int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, int64_t vsc,
+               float* e, int nr, float omega, const float* q, double* partials,
+               const CgScalars* sc, int mode);
+void restrict_l0(hipStream_t s, const L0Args& a, const float* r, float* rc, int64_t vsc, int nr,
+                 float omega, const CgScalars* sc, int mode);
 // bottom-level training loss: c_b = r^T D^-1 r, then g = 2 (A y - r) / (c_b nr)
 void jacobi_norm(hipStream_t s, const Level& L, const float* r, int nr, double* partials,
                  int nblocks);

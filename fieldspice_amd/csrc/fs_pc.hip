@@ -278,6 +278,9 @@ struct fs_pc {
   float omega = 0.8f;
   std::vector<mg::Level> lv;  // K + 1 levels
   std::vector<mg::Map> maps;  // K maps
+  mg::L0Args l0{};            // level 0 on the stencil tiles
+  bool l0_pairs = false;      // level-0 x/y groupings are pairs: tiled restriction
+  int* kstart = nullptr;
   std::vector<float*> owned;  // coarse operator arrays
   std::vector<int32_t*> owned_i;
   std::vector<float*> rv, ev;  // per level >= 1: [kMaxRhs][vs]
@@ -304,6 +307,7 @@ void free_pc(fs_pc* pc) {
   if (!pc) return;
   for (auto& p : pc->owned) dfree(p);
   for (auto& p : pc->owned_i) dfree(p);
+  dfree(pc->kstart);
   for (auto& p : pc->rv) dfree(p);
   for (auto& p : pc->ev) dfree(p);
   dfree(pc->feat4);
@@ -337,14 +341,19 @@ namespace fs {
 
 // One application of the cycle to nr level-0 vectors r0 -> z0 (stride vs0).
 // With q and partials given, the level-0 prolongation also leaves the block
-// partials of <z,r> and <z,q> ([2*nr][nblocks]).
+// partials of <z,r> and <z,q> ([2*nr][*nblocks]).  Right-hand sides that sc
+// marks idle (mode as the Krylov kernels) are skipped on level 0.
 int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, double* partials,
-             int nblocks) {
+             int* nblocks, const CgScalars* sc, int mode) {
   hipStream_t s = pc->sys->ctx->stream;
   const int K = pc->K;
-  for (int l = 0; l < K; ++l)
-    mg::restrict_(s, pc->lv[l], pc->maps[l], l == 0 ? r0 : pc->rv[l], pc->rv[l + 1],
-                  pc->lv[l + 1].vs, nr, pc->omega);
+  for (int l = 0; l < K; ++l) {
+    if (l == 0 && pc->l0_pairs)
+      mg::restrict_l0(s, pc->l0, r0, pc->rv[1], pc->lv[1].vs, nr, pc->omega, sc, mode);
+    else
+      mg::restrict_(s, pc->lv[l], pc->maps[l], l == 0 ? r0 : pc->rv[l], pc->rv[l + 1],
+                    pc->lv[l + 1].vs, nr, pc->omega);
+  }
   const mg::Level& B = pc->lv[K];
   if (pc->net) {
     NetWs& w = pc->ws;
@@ -356,19 +365,19 @@ int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, doub
   } else {
     mg::bottom_jacobi(s, B, pc->rv[K], pc->ev[K], nr, pc->omega);
   }
-  for (int l = K - 1; l >= 0; --l) {
-    const bool top = l == 0;
-    mg::prolong(s, pc->lv[l], pc->maps[l], top ? r0 : pc->rv[l], pc->ev[l + 1],
-                pc->lv[l + 1].vs, top ? z0 : pc->ev[l], nr, pc->omega, top ? q : nullptr,
-                top ? partials : nullptr, top ? nblocks : 0);
-  }
+  for (int l = K - 1; l >= 1; --l)
+    mg::prolong(s, pc->lv[l], pc->maps[l], pc->rv[l], pc->ev[l + 1], pc->lv[l + 1].vs,
+                pc->ev[l], nr, pc->omega, nullptr, nullptr, 0);
+  const int nb = mg::prolong_l0(s, pc->l0, r0, pc->ev[1], pc->lv[1].vs, z0, nr, pc->omega, q,
+                                partials, sc, mode);
+  if (nblocks) *nblocks = nb;
   FS_HIP(hipGetLastError());
   return FS_OK;
 }
 
 // Solver hook (fs_kernels.hip, FS_PREC_FCG): z32 = M(r32) with FCG partials.
-int pc_apply_solver(fs_sys* s, int nr) {
-  return pc_cycle(s->pc, nr, s->r32, s->z32, s->q32, s->partials, s->vec_blocks);
+int pc_apply_solver(fs_sys* s, int nr, int* nblocks) {
+  return pc_cycle(s->pc, nr, s->r32, s->z32, s->q32, s->partials, nblocks, s->sc, 2);
 }
 
 void pc_orphan(fs_pc* pc) {
@@ -385,16 +394,21 @@ namespace {
 // ([2*nb]) and the parameter gradient in net->grad.
 int batch_loss_grad(fs_pc* pc, int nb, const float* gscale, int64_t sum_off) {
   hipStream_t s = pc->sys->ctx->stream;
-  if (int rc = fs::pc_cycle(pc, nb, pc->br, pc->bz, nullptr, nullptr, 0)) return rc;
+  if (int rc = fs::pc_cycle(pc, nb, pc->br, pc->bz, nullptr, nullptr, nullptr, nullptr, 0))
+    return rc;
   const mg::Level& L0 = pc->lv[0];
   mg::loss_grad(s, L0, pc->bz, pc->br, gscale, pc->bg, nb, pc->part, pc->nblk0);
   hipLaunchKernelGGL(k_sum_rows, dim3(2 * nb), dim3(256), 0, s, pc->part, pc->nblk0,
                      pc->sums + sum_off);
   // gradient with respect to the bottom error: the transpose of the up-sweep
   // (smoothing then prolongation) is the down-sweep restriction
-  for (int l = 0; l < pc->K; ++l)
-    mg::restrict_(s, pc->lv[l], pc->maps[l], l == 0 ? pc->bg : pc->rv[l], pc->rv[l + 1],
-                  pc->lv[l + 1].vs, nb, pc->omega);
+  for (int l = 0; l < pc->K; ++l) {
+    if (l == 0 && pc->l0_pairs)
+      mg::restrict_l0(s, pc->l0, pc->bg, pc->rv[1], pc->lv[1].vs, nb, pc->omega, nullptr, 0);
+    else
+      mg::restrict_(s, pc->lv[l], pc->maps[l], l == 0 ? pc->bg : pc->rv[l], pc->rv[l + 1],
+                    pc->lv[l + 1].vs, nb, pc->omega);
+  }
   NetWs& w = pc->ws;
   nn::put0(s, pc->rv[pc->K], pc->lv[pc->K].vs, pc->dh, pc->sc, w.g[1], w.t[T_DY]);
   if (int rc = net_backward(pc->net, w)) return rc;
@@ -409,9 +423,13 @@ int batch_loss_grad(fs_pc* pc, int nb, const float* gscale, int64_t sum_off) {
 int batch_bottom_loss_grad(fs_pc* pc, int nb, int64_t sum_off) {
   hipStream_t s = pc->sys->ctx->stream;
   const int K = pc->K;
-  for (int l = 0; l < K; ++l)
-    mg::restrict_(s, pc->lv[l], pc->maps[l], l == 0 ? pc->br : pc->rv[l], pc->rv[l + 1],
-                  pc->lv[l + 1].vs, nb, pc->omega);
+  for (int l = 0; l < K; ++l) {
+    if (l == 0 && pc->l0_pairs)
+      mg::restrict_l0(s, pc->l0, pc->br, pc->rv[1], pc->lv[1].vs, nb, pc->omega, nullptr, 0);
+    else
+      mg::restrict_(s, pc->lv[l], pc->maps[l], l == 0 ? pc->br : pc->rv[l], pc->rv[l + 1],
+                    pc->lv[l + 1].vs, nb, pc->omega);
+  }
   const mg::Level& B = pc->lv[K];
   NetWs& w = pc->ws;
   set_batch(w, nb);
@@ -721,6 +739,40 @@ int fs_pc_create(fs_sys* sys, fs_net* net, int nlevels, const int32_t* groups, d
     FS_HIP(hipMemset(pc->ev[l + 1], 0, sizeof(float) * fs::kMaxRhs * C.vs));
   }
   if (rc) return fail_pc(rc);
+  {
+    // level 0 on the stencil tiles; the tiled restriction needs x/y pairs and
+    // z-tiles made of whole z-groups
+    mg::L0Args& a = pc->l0;
+    a.d = sys->d;
+    a.vs = sys->vstride;
+    for (int q = 0; q < 3; ++q) a.c[q] = sys->c32[q];
+    a.mask = sys->mask;
+    a.tl = sys->tl;
+    a.m = pc->maps[0];
+    bool pairs = true;
+    for (int q = 0; q < 2; ++q) {
+      const std::vector<int32_t>& st = starts[0][q];
+      for (size_t I = 0; I + 1 < st.size(); ++I)
+        pairs = pairs && st[I] == (int32_t)(2 * I) && st[I + 1] == std::min<int32_t>(2 * I + 2, dims[0][q]);
+    }
+    pc->l0_pairs = pairs;
+    if (pairs) {
+      const std::vector<int32_t>& zs = starts[0][2];  // z-group starts + nz
+      std::vector<int> ks{0};
+      for (size_t G = 1; G < zs.size(); ++G)
+        if (zs[G] - ks.back() >= sys->tl.kz || G + 1 == zs.size()) ks.push_back(zs[G]);
+      if (ks.back() != sys->d.nz) ks.push_back(sys->d.nz);
+      a.tlr = sys->tl;
+      a.tlr.tiles_z = (int)ks.size() - 1;
+      a.tlr.ntiles = a.tlr.tiles_x * a.tlr.tiles_y * a.tlr.tiles_z;
+      a.tlr.nblocks = (a.tlr.ntiles + 7) / 8 * 8;
+      if ((rc = dalloc(&pc->kstart, ks.size()))) return fail_pc(rc);
+      if (hipMemcpy(pc->kstart, ks.data(), ks.size() * sizeof(int), hipMemcpyHostToDevice) !=
+          hipSuccess)
+        return fail_pc(fail(FS_EHIP, "hierarchy upload failed"));
+      a.kstart = pc->kstart;
+    }
+  }
   const mg::Level& B = pc->lv[nlevels];
   if (net && (B.ny > 65535 || (int64_t)B.nz * fs::kMaxRhs > 65535))
     return fail_pc(fail(FS_EINVAL, "bottom level %dx%dx%d too large for the network", B.nx,
@@ -797,7 +849,8 @@ int fs_pc_apply(fs_pc* pc, int nrhs, const double* r, double* z) {
       hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_of(d.n)), dim3(256), 0, ctx->stream, tmp, d.n,
                          pc->br + (size_t)b * sys->vstride);
     }
-    if (int rc = fs::pc_cycle(pc, nr, pc->br, pc->bz, nullptr, nullptr, 0)) return rc;
+    if (int rc = fs::pc_cycle(pc, nr, pc->br, pc->bz, nullptr, nullptr, nullptr, nullptr, 0))
+      return rc;
     for (int b = 0; b < nr; ++b) {
       hipLaunchKernelGGL(k_f32_to_f64, dim3(grid_of(d.n)), dim3(256), 0, ctx->stream,
                          pc->bz + (size_t)b * sys->vstride, d.n, tmp);

@@ -75,10 +75,10 @@ def test_net_backward_matches_oracle():
         off += nw + co
 
 
-@pytest.mark.parametrize("omega", [0.8])
-def test_cycle_with_jacobi_bottom_matches_oracle(omega):
-    print("[gpu case] cycle (Jacobi bottom) vs oracle", flush=True)
-    g, A, free, dh, op, sysd, L, fixed = _case()
+@pytest.mark.parametrize("shape,omega", [((20, 14, 11), 0.8), ((69, 13, 9), 0.7)])
+def test_cycle_with_jacobi_bottom_matches_oracle(shape, omega):
+    print(f"[gpu case] cycle (Jacobi bottom) vs oracle {shape}", flush=True)
+    g, A, free, dh, op, sysd, L, fixed = _case(shape)
     groups = hierarchy_groups((g.ax.hd, g.ay.hd, g.az.hd), max_bottom=150)
     pc = _lib.DevicePreconditioner(sysd, None, groups, omega)
     H = N.Hierarchy(A, free, g.shape_nodes, groups, omega)
