@@ -19,8 +19,14 @@ Methods timed on the same workload:
                  FP64 refresh (the reported value)
   hip_fcg_nonet  the same cycle with one Jacobi step in place of the network
 
+`--workload c5` instead times 64 seeded extractions of uneven size (node
+counts uniform in N over [32^3, 160^3], aspect up to 4:1, 2-6 terminals,
+layered eps_r in [2.5, 8]): plain FP64 PCG looped over the systems one at a
+time, against all systems solved together by learned FCG in one captured
+hipGraph (``fs_batch_solve``) with autotuned stencil tiles.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                       [--workload c4|c2] [--methods ...]
+                       [--workload c4|c2|c5] [--methods ...]
 Prints one JSON line.
 """
 
@@ -48,7 +54,9 @@ def _parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("c4", "c2"), default="c4")
+    ap.add_argument("--workload", choices=("c4", "c2", "c5"), default="c4")
+    ap.add_argument("--instances", type=int, default=64, help="c5: number of systems")
+    ap.add_argument("--chunk", type=int, default=4, help="c5: iterations per graph replay")
     ap.add_argument("--n", type=int, default=256, help="c4: nodes per side")
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
@@ -156,8 +164,96 @@ def cpu_baseline(ex, gpu_iters, budget_s=20.0):
     }
 
 
+def run_c5(args):
+    from fieldspice_amd import _lib, workloads
+    from fieldspice_amd.precond import LearnedPreconditioner
+    seed = 0 if args.seed is None else args.seed
+    log(f"building {args.instances} uneven extraction systems (seed {seed})")
+    t0 = time.perf_counter()
+    exs = workloads.batch_instances(args.instances, seed=seed)
+    lp = LearnedPreconditioner()
+    inst = []
+    for ex in exs:
+        g = ex.grid
+        L = _lib.DeviceOperator.from_cells(g, ex.eps_cell)
+        tid = np.full(g.n_nodes, -1, np.int32)
+        for m, t in enumerate(ex.terminals):
+            tid[t.nodes] = m
+        sysd = _lib.DeviceSystem(L, (tid >= 0).astype(np.uint8))
+        kz = sysd.autotune(len(ex.terminals))
+        pc = lp.attach(sysd, g)
+        cap = sysd.prepare_capacitance(tid, len(ex.terminals))
+        inst.append({"ex": ex, "L": L, "sys": sysd, "pc": pc, "cap": cap, "kz": kz,
+                     "nt": len(ex.terminals), "n": g.n_nodes})
+    setup = time.perf_counter() - t0
+    nrhs = sum(i["nt"] for i in inst)
+    log(f"setup {setup:.1f} s; {nrhs} right-hand sides, "
+        f"{sum(i['n'] for i in inst) / 1e6:.1f} M nodes in total")
+    p64 = _lib.DeviceSystem.params(1e-10)
+    pf = _lib.DeviceSystem.params(1e-10, precision=_lib.FS_PREC_FCG)
+    # plain FP64 PCG looped over the systems (the reference's algorithm, one
+    # extraction after the other)
+    for w in range(args.warmup):
+        inst[0]["cap"].solve(inst[0]["L"], p64)
+    loop_t, loop_C, loop_it = [], [], []
+    t1 = time.perf_counter()
+    for i in inst:
+        ta = time.perf_counter()
+        C, _, st, _ = i["cap"].solve(i["L"], p64)
+        loop_t.append(time.perf_counter() - ta)
+        loop_C.append(C)
+        loop_it.append(st["iterations"])
+    loop_total = time.perf_counter() - t1
+    log(f"looped FP64 PCG: {loop_total:.2f} s")
+    # every system at once: learned FCG in one captured graph
+    res = []
+    for step in range(args.warmup + args.steps):
+        batch = _lib.DeviceBatch(nstreams=16)
+        for i in inst:
+            batch.add(i["cap"], i["L"])
+        tb = time.perf_counter()
+        Cs, fin, its, st, rc = batch.solve(pf, args.chunk)
+        wall = time.perf_counter() - tb
+        batch.close()
+        if step >= args.warmup:
+            res.append((wall, Cs, fin, its, st))
+        log(f"batched learned FCG: {wall:.2f} s (step {step})")
+    wall, Cs, fin, its, st = min(res, key=lambda r: r[0])
+    dev = max(float(np.abs(C - R).max() / np.abs(R).max()) for C, R in zip(Cs, loop_C))
+    loop_fin = np.cumsum(loop_t)
+    out = {
+        "metric": "solves/s, batched uneven extraction: learned FCG in one graph vs looped PCG",
+        "value": nrhs / wall, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64 (bf16 network)",
+        "data": "synthetic",
+        "config": {"workload": f"{len(inst)} extraction systems, node counts uniform in N over "
+                               "[32^3, 160^3], aspect <= 4:1, 2-6 terminals, layered eps_r in "
+                               "[2.5, 8]", "seed": seed, "nodes_total": int(sum(i['n'] for i in inst)),
+                   "rhs_total": nrhs, "chunk": args.chunk, "streams": 16, "tol": 1e-10},
+        "batched_fcg": {"wall_s": wall, "solves_per_s": nrhs / wall,
+                        "p50_finish_s": float(np.percentile(fin, 50)),
+                        "p99_finish_s": float(np.percentile(fin, 99)),
+                        "iterations_max": int(its.max()), "converged": bool(st["converged"]),
+                        "fallbacks": st["fallbacks"], "residual_eq_max": st["residual_eq"]},
+        "looped_fp64_pcg": {"wall_s": loop_total, "solves_per_s": nrhs / loop_total,
+                            "p50_instance_s": float(np.percentile(loop_t, 50)),
+                            "p99_instance_s": float(np.percentile(loop_t, 99)),
+                            "p50_finish_s": float(np.percentile(loop_fin, 50)),
+                            "p99_finish_s": float(np.percentile(loop_fin, 99)),
+                            "iterations_max": int(max(loop_it))},
+        "speedup_vs_looped_pcg": loop_total / wall,
+        "C_rel_dev_vs_fp64": dev,
+        "autotuned_kz": sorted(set(int(i["kz"]) for i in inst)),
+        "setup_s": setup,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = _parse()
+    if args.workload == "c5" and args.gpus == 1:
+        return run_c5(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(sharding is described in DESIGN.md, not implemented)"}))

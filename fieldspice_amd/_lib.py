@@ -37,7 +37,8 @@ EXPORTS = (
     "fs_net_create", "fs_net_destroy", "fs_net_num_params", "fs_net_get_params",
     "fs_net_set_params", "fs_net_eval", "fs_pc_create", "fs_pc_destroy", "fs_pc_level_dims",
     "fs_sys_set_precond", "fs_pc_apply", "fs_pc_add_pairs", "fs_pc_num_pairs",
-    "fs_pc_clear_pairs", "fs_pc_loss_grad", "fs_pc_train",
+    "fs_pc_clear_pairs", "fs_pc_loss_grad", "fs_pc_train", "fs_sys_autotune",
+    "fs_batch_create", "fs_batch_destroy", "fs_batch_add", "fs_batch_solve",
 )
 
 
@@ -132,6 +133,12 @@ def _declare(lib):
     lib.fs_pc_clear_pairs.argtypes = [_vp]
     lib.fs_pc_loss_grad.argtypes = [_vp, C.c_int, C.c_int, _dp, _dp, _vp]
     lib.fs_pc_train.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]
+    lib.fs_sys_autotune.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    lib.fs_batch_create.argtypes = [_vp, C.c_int, C.POINTER(_vp)]
+    lib.fs_batch_destroy.argtypes = [_vp]
+    lib.fs_batch_add.argtypes = [_vp, _vp, _vp]
+    lib.fs_batch_solve.argtypes = [_vp, C.POINTER(SolveParams), C.c_int, _dp, _vp, _vp,
+                                   C.POINTER(SolveStats)]
     for name in EXPORTS:
         if name != "fs_last_error":
             getattr(lib, name).restype = C.c_int
@@ -381,6 +388,13 @@ class DeviceSystem:
     def set_profiling(self, on: bool) -> None:
         check(load().fs_sys_set_profiling(self._h, 1 if on else 0), "fs_sys_set_profiling")
 
+    def autotune(self, nr: int, reps: int = 5) -> int:
+        """Pick the stencil's z-march depth for ``nr`` right-hand sides by
+        timing; returns the chosen planes per tile."""
+        kz = C.c_int(0)
+        check(load().fs_sys_autotune(self._h, int(nr), int(reps), C.byref(kz)), "fs_sys_autotune")
+        return kz.value
+
     def set_precond(self, pc: "DevicePreconditioner | None") -> None:
         """Preconditioner of FS_PREC_FCG solves (the preconditioner keeps this
         system alive; the system keeps only the handle)."""
@@ -563,6 +577,51 @@ class DevicePreconditioner:
     def close(self):
         if getattr(self, "_h", None):
             load().fs_pc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBatch:
+    """Many capacitance extractions solved together (an ``fs_batch``): one
+    captured hipGraph of every system's iterations, replayed until all
+    converge."""
+
+    def __init__(self, nstreams: int = 16):
+        h = _vp()
+        check(load().fs_batch_create(context(), int(nstreams), C.byref(h)), "fs_batch_create")
+        self._h = h
+        self.items = []   # keeps the extractions and operators alive
+
+    def add(self, cap: DeviceCapacitance, L: DeviceOperator) -> int:
+        check(load().fs_batch_add(self._h, cap._h, L._h), "fs_batch_add")
+        self.items.append((cap, L))
+        return len(self.items) - 1
+
+    def solve(self, params: SolveParams, chunk: int = 4):
+        """Returns (list of C matrices, per-system finish times [s],
+        per-system iterations, stats dict, status)."""
+        sizes = [cap.nt for cap, _ in self.items]
+        Cflat = np.zeros(sum(n * n for n in sizes))
+        t = np.zeros(len(sizes))
+        it = np.zeros(len(sizes), dtype=np.int32)
+        st = SolveStats()
+        rc = load().fs_batch_solve(self._h, C.byref(params), int(chunk), Cflat,
+                                   t.ctypes.data_as(_vp), it.ctypes.data_as(_vp), C.byref(st))
+        check(rc, "fs_batch_solve")
+        out, off = [], 0
+        for n in sizes:
+            out.append(Cflat[off:off + n * n].reshape(n, n).copy())
+            off += n * n
+        return out, t, it, st.as_dict(), rc
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().fs_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

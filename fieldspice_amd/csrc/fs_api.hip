@@ -10,8 +10,12 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <chrono>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -627,6 +631,7 @@ int fs_sys_create(fs_op* op, const uint8_t* mask, int pin, fs_sys** out, fs_sys_
   FS_HIP(hipSetDevice(ctx->device));
   fs_sys* s = new fs_sys();
   s->ctx = ctx;
+  s->stream = ctx->stream;
   s->op = op;
   s->d = op->d;
   const fs::Dims& d = s->d;
@@ -1148,6 +1153,415 @@ int fs_sys_set_profiling(fs_sys* s, int on) {
   }
   s->profiling = on ? 1 : 0;
   return FS_OK;
+  FS_GUARD_END
+}
+
+
+// --------------------------------------------------------------------------
+// stencil tile autotuning
+// --------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+fs::Tiling tiling_for(const fs::Dims& d, int kz) {
+  fs::Tiling tl;
+  tl.tiles_x = (d.nx + fs::kTileX - 1) / fs::kTileX;
+  tl.tiles_y = (d.ny + fs::kTileY - 1) / fs::kTileY;
+  tl.kz = std::max(1, std::min(kz, d.nz));
+  tl.tiles_z = (d.nz + tl.kz - 1) / tl.kz;
+  tl.ntiles = tl.tiles_x * tl.tiles_y * tl.tiles_z;
+  tl.nblocks = (tl.ntiles + 7) / 8 * 8;
+  return tl;
+}
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, int> g_tune_cache;  // (nx, ny, nz, nr) -> kz
+
+}  // namespace
+
+extern "C" {
+
+int fs_sys_autotune(fs_sys* s, int nr, int reps, int* kz_out) {
+  FS_GUARD_BEGIN
+  if (!s) return fail(FS_EINVAL, "NULL argument");
+  if (nr < 1 || nr > fs::kMaxRhs) return fail(FS_EINVAL, "nr must be in 1..%d", fs::kMaxRhs);
+  if (reps < 1) reps = 5;
+  fs_ctx* ctx = s->ctx;
+  FS_HIP(hipSetDevice(ctx->device));
+  const fs::Dims& d = s->d;
+  const auto key = std::make_tuple(d.nx, d.ny, d.nz, nr);
+  int best_kz = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) best_kz = it->second;
+  }
+  if (best_kz == 0) {
+    if (int rc = ensure_ws(s, nr)) return rc;
+    // candidates: z-march depths from one plane to the whole column
+    std::vector<int> cand;
+    for (int kz = 1; kz <= 128; kz *= 2)
+      if (kz <= d.nz) cand.push_back(kz);
+    int64_t need = 0;
+    for (int kz : cand) need = std::max<int64_t>(need, tiling_for(d, kz).nblocks);
+    need = need * fs::kMaxRhs * 2 + 64 * (fs::kMaxRhs + 1) * fs::kMaxRhs;
+    if (need > s->partials_cap) {
+      double* np = nullptr;
+      if (int rc = dalloc(&np, (size_t)need)) return rc;
+      dfree(s->partials);
+      s->partials = np;
+      s->partials_cap = need;
+    }
+    // every right-hand side live for the timed launches; the next solve
+    // re-initialises the scalars
+    for (int r = 0; r < fs::kMaxRhs; ++r) {
+      s->sc_host[r] = fs::CgScalars{};
+    }
+    FS_HIP(hipMemcpy(s->sc, s->sc_host, sizeof(fs::CgScalars) * fs::kMaxRhs,
+                     hipMemcpyHostToDevice));
+    FS_HIP(hipMemsetAsync(s->p32, 0, sizeof(float) * nr * s->vstride, s->stream));
+    float best = 1e30f;
+    for (int kz : cand) {
+      const fs::Tiling tl = tiling_for(d, kz);
+      fs::launch_stencil_only(s, nr, tl);  // warm
+      FS_HIP(hipEventRecord(ctx->ev0, s->stream));
+      for (int k = 0; k < reps; ++k) fs::launch_stencil_only(s, nr, tl);
+      FS_HIP(hipEventRecord(ctx->ev1, s->stream));
+      FS_HIP(hipEventSynchronize(ctx->ev1));
+      float ms = 0.f;
+      FS_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+      if (ms < best) {
+        best = ms;
+        best_kz = kz;
+      }
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best_kz;
+  }
+  const fs::Tiling tl = tiling_for(d, best_kz);
+  const int64_t need = (int64_t)tl.nblocks * fs::kMaxRhs * 2 + 64 * (fs::kMaxRhs + 1) * fs::kMaxRhs;
+  if (need > s->partials_cap) {
+    double* np = nullptr;
+    if (int rc = dalloc(&np, (size_t)need)) return rc;
+    dfree(s->partials);
+    s->partials = np;
+    s->partials_cap = need;
+  }
+  s->tl = tl;
+  if (kz_out) *kz_out = best_kz;
+  return FS_OK;
+  FS_GUARD_END
+}
+
+}  // extern "C"
+
+// --------------------------------------------------------------------------
+// batched extraction: many prepared systems, one captured graph per solve
+// --------------------------------------------------------------------------
+struct fs_batch {
+  fs_ctx* ctx = nullptr;
+  std::vector<hipStream_t> streams;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins;
+  struct Inst {
+    fs_cap* cap;
+    const fs_op* L;
+  };
+  std::vector<Inst> inst;
+  fs::CgScalars* sc_all = nullptr;       // [ninst][kMaxRhs] device
+  fs::CgScalars* sc_all_host = nullptr;  // pinned mirror
+  int sc_cap = 0;
+};
+
+namespace {
+
+void free_batch(fs_batch* b) {
+  if (!b) return;
+  for (auto st : b->streams) (void)hipStreamDestroy(st);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  for (auto e : b->joins) (void)hipEventDestroy(e);
+  dfree(b->sc_all);
+  if (b->sc_all_host) (void)hipHostFree(b->sc_all_host);
+  delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fs_batch_create(fs_ctx* ctx, int nstreams, fs_batch** out) {
+  FS_GUARD_BEGIN
+  if (!ctx || !out) return fail(FS_EINVAL, "NULL argument");
+  if (nstreams < 1 || nstreams > 64) return fail(FS_EINVAL, "nstreams must be in 1..64");
+  *out = nullptr;
+  FS_HIP(hipSetDevice(ctx->device));
+  fs_batch* b = new fs_batch();
+  b->ctx = ctx;
+  for (int k = 0; k < nstreams; ++k) {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      free_batch(b);
+      return fail(FS_EHIP, "stream creation failed");
+    }
+    b->streams.push_back(st);
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      free_batch(b);
+      return fail(FS_EHIP, "event creation failed");
+    }
+    b->joins.push_back(ev);
+  }
+  if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+    free_batch(b);
+    return fail(FS_EHIP, "event creation failed");
+  }
+  *out = b;
+  return FS_OK;
+  FS_GUARD_END
+}
+
+int fs_batch_destroy(fs_batch* b) {
+  if (b) (void)hipSetDevice(b->ctx->device);
+  free_batch(b);
+  return FS_OK;
+}
+
+int fs_batch_add(fs_batch* b, fs_cap* cap, const fs_op* L) {
+  FS_GUARD_BEGIN
+  if (!b || !cap || !L) return fail(FS_EINVAL, "NULL argument");
+  fs_sys* s = cap->sys;
+  if (s->ctx != b->ctx) return fail(FS_EINVAL, "system on another context");
+  if (cap->nt > fs::kMaxRhs)
+    return fail(FS_EINVAL, "a batched extraction holds at most %d terminals", fs::kMaxRhs);
+  if (L->Nx != s->op->Nx || L->Ny != s->op->Ny || L->Nz != s->op->Nz)
+    return fail(FS_EINVAL, "L and the system differ in grid");
+  for (const auto& in : b->inst)
+    if (in.cap->sys == s) return fail(FS_EINVAL, "system already in the batch");
+  b->inst.push_back({cap, L});
+  return (int)b->inst.size() - 1 >= 0 ? FS_OK : FS_EINVAL;
+  FS_GUARD_END
+}
+
+int fs_batch_solve(fs_batch* b, const fs_solve_params* params, int chunk, double* C,
+                   double* inst_time, int* inst_iters, fs_solve_stats* stats) {
+  FS_GUARD_BEGIN
+  if (!b || !params || !C) return fail(FS_EINVAL, "NULL argument");
+  if (!(params->tol > 0.0)) return fail(FS_EINVAL, "tol must be > 0");
+  const int P = params->precision;
+  if (P != FS_PREC_FP64 && P != FS_PREC_MIXED && P != FS_PREC_FCG)
+    return fail(FS_EINVAL, "unknown precision %d", P);
+  if (chunk < 1 || chunk > 64) return fail(FS_EINVAL, "chunk must be in 1..64");
+  const int ni = (int)b->inst.size();
+  if (ni == 0) return fail(FS_EINVAL, "empty batch");
+  fs_ctx* ctx = b->ctx;
+  FS_HIP(hipSetDevice(ctx->device));
+  const hipStream_t main = ctx->stream;
+  const int mode = P == FS_PREC_FP64 ? 0 : (P == FS_PREC_MIXED ? 1 : 2);
+  const bool cyc = mode != 0;
+  const double inner_tol = params->inner_tol > 0 ? params->inner_tol : (mode == 2 ? 1e-5 : 1e-4);
+  for (const auto& in : b->inst)
+    if (mode == 2 && !in.cap->sys->pc)
+      return fail(FS_EINVAL, "FS_PREC_FCG needs a preconditioner on every system of the batch");
+  if (b->sc_cap < ni) {
+    dfree(b->sc_all);
+    if (b->sc_all_host) (void)hipHostFree(b->sc_all_host);
+    b->sc_all_host = nullptr;
+    b->sc_cap = 0;
+    if (int rc = dalloc(&b->sc_all, (size_t)ni * fs::kMaxRhs)) return rc;
+    if (hipHostMalloc((void**)&b->sc_all_host, sizeof(fs::CgScalars) * ni * fs::kMaxRhs) !=
+        hipSuccess)
+      return fail(FS_ENOMEM, "pinned allocation failed");
+    b->sc_cap = ni;
+  }
+  FS_HIP(hipMemset(b->sc_all, 0, sizeof(fs::CgScalars) * ni * fs::kMaxRhs));
+  // lend every system the batch's scalars and the main stream
+  std::vector<fs::CgScalars*> own_sc(ni);
+  std::vector<int> maxit(ni);
+  for (int i = 0; i < ni; ++i) {
+    fs_sys* s = b->inst[i].cap->sys;
+    own_sc[i] = s->sc;
+    s->sc = b->sc_all + (size_t)i * fs::kMaxRhs;
+    s->stream = main;
+    s->acc_fallbacks = 0;
+    s->acc_pc_applies = 0;
+    if (int rc = ensure_ws(s, b->inst[i].cap->nt)) return rc;
+    maxit[i] = default_maxiter(s, params);
+  }
+  auto restore = [&]() {
+    for (int i = 0; i < ni; ++i) {
+      fs_sys* s = b->inst[i].cap->sys;
+      s->sc = own_sc[i];
+      s->stream = ctx->stream;
+    }
+  };
+  enum { ITER, REFRESH, FINISHING, DONE };
+  std::vector<int> state(ni, ITER);
+  std::vector<std::vector<double>> hout(ni);
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int rc = FS_OK;
+  auto t0 = std::chrono::steady_clock::now();
+  auto start_cycle = [&](fs_sys* s, int nt) -> int {
+    fs::launch_cycle_start(s, nt, inner_tol * inner_tol);
+    if (mode == 2) {
+      if (int r = fs::launch_fcg_start(s, nt)) return r;
+      s->acc_pc_applies += 1;
+    }
+    return FS_OK;
+  };
+  auto body = [&]() -> int {
+    // initial state of every system (lift, zero guess, first cycle)
+    for (int i = 0; i < ni; ++i) {
+      fs_cap* cap = b->inst[i].cap;
+      fs_sys* s = cap->sys;
+      fs::launch_lift_cap(s, cap->nt, b->inst[i].L, cap->tid, 0);
+      fs::launch_init(s, cap->nt, params->tol, maxit[i]);
+      if (cyc)
+        if (int r = start_cycle(s, cap->nt)) return r;
+    }
+    FS_HIP(hipGetLastError());
+    // the graph: `chunk` iterations of every system, systems spread over the
+    // batch's streams (parallel branches), idle systems exit at once
+    const int nst = (int)b->streams.size();
+    FS_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
+    int crc = FS_OK;
+    if (hipEventRecord(b->fork, main) != hipSuccess) crc = fail(FS_EHIP, "capture fork failed");
+    for (int k = 0; k < nst && crc == FS_OK; ++k)
+      if (hipStreamWaitEvent(b->streams[k], b->fork, 0) != hipSuccess)
+        crc = fail(FS_EHIP, "capture fork failed");
+    for (int i = 0; i < ni && crc == FS_OK; ++i) {
+      fs_sys* s = b->inst[i].cap->sys;
+      s->stream = b->streams[i % nst];
+      for (int c = 0; c < chunk && crc == FS_OK; ++c)
+        crc = fs::launch_iteration(s, b->inst[i].cap->nt, mode, nullptr, nullptr);
+      s->stream = main;
+    }
+    for (int k = 0; k < nst && crc == FS_OK; ++k) {
+      if (hipEventRecord(b->joins[k], b->streams[k]) != hipSuccess ||
+          hipStreamWaitEvent(main, b->joins[k], 0) != hipSuccess)
+        crc = fail(FS_EHIP, "capture join failed");
+    }
+    const hipError_t ee = hipStreamEndCapture(main, &graph);
+    if (crc != FS_OK) return crc;
+    if (ee != hipSuccess) return fail(FS_EHIP, "graph capture failed: %s", hipGetErrorString(ee));
+    FS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    int max_rounds = 0;
+    for (int i = 0; i < ni; ++i) max_rounds = std::max(max_rounds, maxit[i]);
+    max_rounds = 4 * (max_rounds / chunk + 2) + 64;
+    for (int round = 0; round < max_rounds; ++round) {
+      bool any_iter = false, all_done = true;
+      for (int i = 0; i < ni; ++i) {
+        any_iter = any_iter || state[i] == ITER;
+        all_done = all_done && state[i] == DONE;
+      }
+      if (all_done) return FS_OK;
+      if (any_iter) FS_HIP(hipGraphLaunch(exec, main));
+      FS_HIP(hipMemcpyAsync(b->sc_all_host, b->sc_all, sizeof(fs::CgScalars) * ni * fs::kMaxRhs,
+                            hipMemcpyDeviceToHost, main));
+      FS_HIP(hipStreamSynchronize(main));
+      const double now =
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      for (int i = 0; i < ni; ++i) {
+        fs_cap* cap = b->inst[i].cap;
+        fs_sys* s = cap->sys;
+        const int nt = cap->nt;
+        const fs::CgScalars* c = b->sc_all_host + (size_t)i * fs::kMaxRhs;
+        if (state[i] == DONE) continue;
+        if (state[i] == FINISHING) {
+          if (inst_time) inst_time[i] = now;
+          state[i] = DONE;
+          continue;
+        }
+        if (any_iter && state[i] == ITER) s->acc_pc_applies += mode == 2 ? chunk : 0;
+        bool done = true, waiting = true, ok = true;
+        for (int r = 0; r < nt; ++r) {
+          done = done && c[r].done != 0;
+          waiting = waiting && (c[r].done != 0 || c[r].cycle_done != 0);
+          ok = ok && c[r].done == 1;
+        }
+        if (state[i] == ITER && cyc && !done && waiting) {
+          fs::launch_refresh(s, nt);
+          state[i] = REFRESH;
+          continue;
+        }
+        if (state[i] == REFRESH && !done) {
+          if (int r = start_cycle(s, nt)) return r;
+          state[i] = ITER;
+          continue;
+        }
+        if (!done) continue;
+        if (mode == 2 && !ok) {
+          // plain FP64 CG redoes this system (blocking)
+          fs_solve_params Q = *params;
+          Q.precision = FS_PREC_FP64;
+          Q.maxiter = 0;
+          s->acc_fallbacks += 1;
+          fs::launch_init(s, nt, params->tol, default_maxiter(s, &Q));
+          if (int r = run_krylov_mode(s, nt, Q, 0)) return r;
+          FS_HIP(hipMemcpyAsync(b->sc_all_host + (size_t)i * fs::kMaxRhs, s->sc,
+                                sizeof(fs::CgScalars) * nt, hipMemcpyDeviceToHost, main));
+        }
+        fs::launch_residual(s, nt, s->r);
+        fs::launch_expand(s, nt, nullptr, cap->tid, 0, s->q);
+        fs::launch_charge(s, nt, b->inst[i].L, s->q, cap->nodes, cap->offs, cap->nbins,
+                          cap->nbx, cap->out);
+        hout[i].assign((size_t)cap->nbins * fs::kMaxRhs, 0.0);
+        FS_HIP(hipMemcpyAsync(hout[i].data(), cap->out, sizeof(double) * cap->nbins * nt,
+                              hipMemcpyDeviceToHost, main));
+        state[i] = FINISHING;
+      }
+      FS_HIP(hipGetLastError());
+    }
+    return fail(FS_EINVAL, "batch driver exceeded its round guard");
+  };
+  rc = body();
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (rc == FS_OK) {
+    // final scalars (residuals of the finishing pass)
+    if (hipMemcpy(b->sc_all_host, b->sc_all, sizeof(fs::CgScalars) * ni * fs::kMaxRhs,
+                  hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(FS_EHIP, "scalar download failed");
+  }
+  restore();
+  if (rc != FS_OK) return rc;
+  const double total =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->converged = 1;
+    stats->solve_time = total;
+  }
+  int status = FS_OK;
+  size_t off = 0;
+  for (int i = 0; i < ni; ++i) {
+    fs_cap* cap = b->inst[i].cap;
+    fs_sys* s = cap->sys;
+    const int nt = cap->nt;
+    for (int r = 0; r < nt; ++r)
+      for (int q = 0; q < nt; ++q) C[off + (size_t)q * nt + r] = hout[i][(size_t)q * nt + r];
+    off += (size_t)nt * nt;
+    // per-system verdict from the batch's scalars
+    std::memcpy(s->sc_host, b->sc_all_host + (size_t)i * fs::kMaxRhs, sizeof(fs::CgScalars) * nt);
+    int st = FS_OK;
+    fs_solve_stats one{};
+    collect_stats(s, nt, &one, st);
+    if (inst_iters) inst_iters[i] = one.iterations;
+    if (stats) {
+      stats->iterations = std::max(stats->iterations, one.iterations);
+      stats->refreshes = std::max(stats->refreshes, one.refreshes);
+      stats->residual = std::max(stats->residual, one.residual);
+      stats->residual_eq = std::max(stats->residual_eq, one.residual_eq);
+      stats->fallbacks += one.fallbacks;
+      stats->pc_applies += one.pc_applies;
+      if (st != FS_OK) stats->converged = 0;
+    }
+    if (st != FS_OK) status = st;
+  }
+  if (status == FS_NOT_CONVERGED)
+    fail(FS_NOT_CONVERGED, "a batched extraction did not reach tol %.3g", params->tol);
+  return status;
   FS_GUARD_END
 }
 

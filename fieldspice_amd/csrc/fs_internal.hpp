@@ -85,6 +85,7 @@ struct fs_pc;
 
 struct fs_sys {
   fs_ctx* ctx = nullptr;
+  hipStream_t stream = nullptr;  // the context's stream; a batch solve lends its own
   const fs_op* op = nullptr;  // not owned; must outlive the system
   fs::Dims d{};
   fs::Tiling tl{};

@@ -311,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_couplings(const fs_op op,
 }
 
 int setup_system(fs_sys* sys, double* dsum, void* counts_dev, void* counts_host, int pin) {
-  hipStream_t s = sys->ctx->stream;
+  hipStream_t s = sys->stream;
   const Dims d = sys->d;
   const int nb = (int)((d.n + kBlock - 1) / kBlock);
   SetupCounts* cd = (SetupCounts*)counts_dev;
@@ -917,7 +917,7 @@ __global__ __launch_bounds__(kBlock) void k_charge_final(const double* __restric
 // --------------------------------------------------------------------------
 template <int NR>
 static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {
-  hipStream_t st = s->ctx->stream;
+  hipStream_t st = s->stream;
   VecArgs va{s->d.n, s->vstride, nr};
   const int vb = s->vec_blocks;
   const int sb = s->tl.nblocks;
@@ -975,9 +975,9 @@ int launch_iteration(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) 
 template <int NR>
 static void launch_init_t(fs_sys* s, int nr, double tol, int maxiter) {
   VecArgs va{s->d.n, s->vstride, nr};
-  hipLaunchKernelGGL((k_cg_init<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream,
+  hipLaunchKernelGGL((k_cg_init<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->stream,
                      s->bs, s->x, s->r, s->p, va, s->partials);
-  finalize<NR>(s->ctx->stream, s->partials, FIN_INIT, nr, s->vec_blocks, 0, s->sc, nullptr, tol,
+  finalize<NR>(s->stream, s->partials, FIN_INIT, nr, s->vec_blocks, 0, s->sc, nullptr, tol,
                maxiter);
 }
 void launch_init(fs_sys* s, int nr, double tol, int maxiter) {
@@ -988,8 +988,8 @@ template <int NR>
 static void launch_lift_general_t(fs_sys* s, int nr, const double* b) {
   VecArgs va{s->d.n, s->vstride, nr};
   hipLaunchKernelGGL((k_lift_general<NR>), dim3(s->vec_blocks), dim3(kBlock), 0,
-                     s->ctx->stream, b, s->mask, s->dinv, s->bs, va, s->partials);
-  finalize<NR>(s->ctx->stream, s->partials, FIN_BNORM, nr, s->vec_blocks, 0, s->sc);
+                     s->stream, b, s->mask, s->dinv, s->bs, va, s->partials);
+  finalize<NR>(s->stream, s->partials, FIN_BNORM, nr, s->vec_blocks, 0, s->sc);
 }
 void launch_lift_general(fs_sys* s, int nr, const double* b) {
   FS_DISPATCH_NR(nr, launch_lift_general_t, s, nr, b);
@@ -998,9 +998,9 @@ void launch_lift_general(fs_sys* s, int nr, const double* b) {
 template <int NR>
 static void launch_lift_cap_t(fs_sys* s, int nr, const fs_op* L, const int32_t* tid, int j0) {
   VecArgs va{s->d.n, s->vstride, nr};
-  hipLaunchKernelGGL((k_lift_cap<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream,
+  hipLaunchKernelGGL((k_lift_cap<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->stream,
                      *L, tid, s->mask, s->dinv, s->bs, va, j0, s->partials);
-  finalize<NR>(s->ctx->stream, s->partials, FIN_BNORM, nr, s->vec_blocks, 0, s->sc);
+  finalize<NR>(s->stream, s->partials, FIN_BNORM, nr, s->vec_blocks, 0, s->sc);
 }
 void launch_lift_cap(fs_sys* s, int nr, const fs_op* L, const int32_t* tid, int j0) {
   FS_DISPATCH_NR(nr, launch_lift_cap_t, s, nr, L, tid, j0);
@@ -1010,7 +1010,7 @@ template <int NR>
 static void launch_expand_t(fs_sys* s, int nr, const double* b, const int32_t* tid,
                             int j0, double* phi) {
   VecArgs va{s->d.n, s->vstride, nr};
-  hipLaunchKernelGGL((k_expand<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream,
+  hipLaunchKernelGGL((k_expand<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->stream,
                      s->x, s->mask, s->dinv, b, tid, phi, va, j0);
 }
 void launch_expand(fs_sys* s, int nr, const double* b, const int32_t* tid, int j0,
@@ -1021,10 +1021,10 @@ void launch_expand(fs_sys* s, int nr, const double* b, const int32_t* tid, int j
 template <int NR>
 static void launch_residual_t(fs_sys* s, int nr, double* res) {
   VecArgs va{s->d.n, s->vstride, nr};
-  hipLaunchKernelGGL((k_residual<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream,
+  hipLaunchKernelGGL((k_residual<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->stream,
                      s->bs, s->x, res, s->c64[0], s->c64[1], s->c64[2], s->dhalf, s->d, va,
                      s->partials);
-  finalize<NR>(s->ctx->stream, s->partials, FIN_RESID, nr, s->vec_blocks, 0, s->sc);
+  finalize<NR>(s->stream, s->partials, FIN_RESID, nr, s->vec_blocks, 0, s->sc);
 }
 void launch_residual(fs_sys* s, int nr, double* res) {
   FS_DISPATCH_NR(nr, launch_residual_t, s, nr, res);
@@ -1034,15 +1034,15 @@ template <int NR>
 static void launch_cycle_start_t(fs_sys* s, int nr, double inner_tol2) {
   VecArgs va{s->d.n, s->vstride, nr};
   hipLaunchKernelGGL((k_cycle_start<NR>), dim3(s->vec_blocks), dim3(kBlock), 0,
-                     s->ctx->stream, s->r, s->r32, s->x32, s->p32, va, s->sc, s->partials);
-  finalize<NR>(s->ctx->stream, s->partials, FIN_CYCLE, nr, s->vec_blocks, 1, s->sc, nullptr,
+                     s->stream, s->r, s->r32, s->x32, s->p32, va, s->sc, s->partials);
+  finalize<NR>(s->stream, s->partials, FIN_CYCLE, nr, s->vec_blocks, 1, s->sc, nullptr,
                0.0, 0, inner_tol2);
 }
 template <int NR>
 static int launch_fcg_start_t(fs_sys* s, int nr) {
   int nbp = 0;
   if (int rc = pc_apply_solver(s, nr, &nbp)) return rc;
-  finalize<NR>(s->ctx->stream, s->partials, FIN_FCG0, nr, nbp, 2, s->sc);
+  finalize<NR>(s->stream, s->partials, FIN_FCG0, nr, nbp, 2, s->sc);
   return 0;
 }
 int launch_fcg_start(fs_sys* s, int nr) {
@@ -1051,6 +1051,17 @@ int launch_fcg_start(fs_sys* s, int nr) {
   if (nr <= 4) return launch_fcg_start_t<4>(s, nr);
   return launch_fcg_start_t<8>(s, nr);
 }
+template <int NR>
+static void launch_stencil_only_t(fs_sys* s, int nr, const Tiling& tl) {
+  VecArgs va{s->d.n, s->vstride, nr};
+  hipLaunchKernelGGL((k_stencil_dot<float, NR>), dim3(tl.nblocks), dim3(kBlock), 0, s->stream,
+                     s->p32, s->q32, s->c32[0], s->c32[1], s->c32[2], s->d, tl, va, 1, s->sc,
+                     s->partials, nullptr);
+}
+void launch_stencil_only(fs_sys* s, int nr, const Tiling& tl) {
+  FS_DISPATCH_NR(nr, launch_stencil_only_t, s, nr, tl);
+}
+
 void launch_cycle_start(fs_sys* s, int nr, double inner_tol2) {
   FS_DISPATCH_NR(nr, launch_cycle_start_t, s, nr, inner_tol2);
 }
@@ -1059,9 +1070,9 @@ template <int NR>
 static void launch_refresh_t(fs_sys* s, int nr) {
   VecArgs va{s->d.n, s->vstride, nr};
   hipLaunchKernelGGL((k_cycle_accumulate<NR>), dim3(s->vec_blocks), dim3(kBlock), 0,
-                     s->ctx->stream, s->x, s->x32, va, s->sc);
+                     s->stream, s->x, s->x32, va, s->sc);
   launch_residual_t<NR>(s, nr, s->r);
-  hipLaunchKernelGGL(k_refresh_finish, dim3(1), dim3(64), 0, s->ctx->stream, nr, s->sc);
+  hipLaunchKernelGGL(k_refresh_finish, dim3(1), dim3(64), 0, s->stream, nr, s->sc);
 }
 void launch_refresh(fs_sys* s, int nr) { FS_DISPATCH_NR(nr, launch_refresh_t, s, nr); }
 
@@ -1070,9 +1081,9 @@ static void launch_charge_t(fs_sys* s, int nr, const fs_op* L, const double* phi
                             const int64_t* nodes, const int64_t* offs, int nbins, int nbx,
                             double* out) {
   VecArgs va{s->d.n, s->vstride, nr};
-  hipLaunchKernelGGL((k_charge<NR>), dim3(nbx, nbins), dim3(kBlock), 0, s->ctx->stream, *L,
+  hipLaunchKernelGGL((k_charge<NR>), dim3(nbx, nbins), dim3(kBlock), 0, s->stream, *L,
                      phi, nodes, offs, va, s->partials);
-  hipLaunchKernelGGL(k_charge_final, dim3(1), dim3(kBlock), 0, s->ctx->stream, s->partials,
+  hipLaunchKernelGGL(k_charge_final, dim3(1), dim3(kBlock), 0, s->stream, s->partials,
                      nbins, nr, nbx, out);
 }
 void launch_charge(fs_sys* s, int nr, const fs_op* L, const double* phi, const int64_t* nodes,

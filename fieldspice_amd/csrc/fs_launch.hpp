@@ -37,6 +37,8 @@ void launch_cycle_start(fs_sys* s, int nr, double inner_tol2);
 // flexible CG: z32 = M(r32) and <z,r> at the start of an FP32 cycle
 int launch_fcg_start(fs_sys* s, int nr);
 void launch_refresh(fs_sys* s, int nr);
+// one FP32 multi-RHS stencil launch (p32 -> q32) on tiling tl (autotuning)
+void launch_stencil_only(fs_sys* s, int nr, const Tiling& tl);
 void launch_charge(fs_sys* s, int nr, const fs_op* L, const double* phi, const int64_t* nodes,
                    const int64_t* offs, int nbins, int nbx, double* out);
 

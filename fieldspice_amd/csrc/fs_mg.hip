@@ -134,7 +134,9 @@ __device__ __forceinline__ float smooth_res(const Level& L, const float* __restr
 template <bool UNIT>
 __global__ __launch_bounds__(kBlock) void k_restrict(Level f, Map m, const float* __restrict__ r,
                                                      float* __restrict__ rc, int64_t vsc, int nr,
-                                                     float omega) {
+                                                     float omega,
+    const int* __restrict__ gate) {
+  if (gate && *gate) return;  // every right-hand side of the batch is idle
   const int64_t nc = (int64_t)m.cn[0] * m.cn[1] * m.cn[2];
   for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < nc;
        t += (int64_t)gridDim.x * kBlock) {
@@ -173,7 +175,9 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Level f, Map m, const float*
                                                     const float* __restrict__ ec, int64_t vsc,
                                                     float* __restrict__ e, int nr, float omega,
                                                     const float* __restrict__ q,
-                                                    double* __restrict__ partials) {
+                                                    double* __restrict__ partials,
+    const int* __restrict__ gate) {
+  if (gate && *gate) return;  // every right-hand side of the batch is idle
   double acc[2 * kMaxRhs];
 #pragma unroll
   for (int a = 0; a < 2 * kMaxRhs; ++a) acc[a] = 0.0;
@@ -220,7 +224,9 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Level f, Map m, const float*
 
 __global__ __launch_bounds__(kBlock) void k_bottom_jacobi(Level L, const float* __restrict__ r,
                                                           float* __restrict__ e, int nr,
-                                                          float omega) {
+                                                          float omega,
+    const int* __restrict__ gate) {
+  if (gate && *gate) return;  // every right-hand side of the batch is idle
   for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < L.n;
        n += (int64_t)gridDim.x * kBlock)
     for (int b = 0; b < nr; ++b) e[b * L.vs + n] = omega * L.dinv[n] * r[b * L.vs + n];
@@ -588,32 +594,40 @@ void bottom_features(hipStream_t s, const Level& L, float* feat4, float* dh) {
 }
 
 void restrict_(hipStream_t s, const Level& f, const Map& m, const float* r, float* rc,
-               int64_t vsc, int nr, float omega) {
+               int64_t vsc, int nr, float omega, const int* gate) {
   const int64_t nc = (int64_t)m.cn[0] * m.cn[1] * m.cn[2];
   if (f.dg == nullptr)
     hipLaunchKernelGGL(k_restrict<true>, dim3(vec_blocks(nc)), dim3(kBlock), 0, s, f, m, r, rc,
-                       vsc, nr, omega);
+                       vsc, nr, omega, gate);
   else
     hipLaunchKernelGGL(k_restrict<false>, dim3(vec_blocks(nc)), dim3(kBlock), 0, s, f, m, r, rc,
-                       vsc, nr, omega);
+                       vsc, nr, omega, gate);
 }
 
 void prolong(hipStream_t s, const Level& f, const Map& m, const float* r, const float* ec,
              int64_t vsc, float* e, int nr, float omega, const float* q, double* partials,
-             int nblocks) {
+             int nblocks, const int* gate) {
   const int nb = nblocks > 0 ? nblocks : vec_blocks(f.n);
   if (f.dg == nullptr)
     hipLaunchKernelGGL(k_prolong<true>, dim3(nb), dim3(kBlock), 0, s, f, m, r, ec, vsc, e, nr,
-                       omega, q, partials);
+                       omega, q, partials, gate);
   else
     hipLaunchKernelGGL(k_prolong<false>, dim3(nb), dim3(kBlock), 0, s, f, m, r, ec, vsc, e, nr,
-                       omega, q, partials);
+                       omega, q, partials, gate);
 }
 
 void bottom_jacobi(hipStream_t s, const Level& L, const float* r, float* e, int nr,
-                   float omega) {
+                   float omega, const int* gate) {
   hipLaunchKernelGGL(k_bottom_jacobi, dim3(vec_blocks(L.n)), dim3(kBlock), 0, s, L, r, e, nr,
-                     omega);
+                     omega, gate);
+}
+
+__global__ void k_gate(const CgScalars* sc, int nr, int mode, int* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = all_idle(sc, nr, mode) ? 1 : 0;
+}
+
+void gate(hipStream_t s, const CgScalars* sc, int nr, int mode, int* out) {
+  hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, s, sc, nr, mode, out);
 }
 
 void loss_grad(hipStream_t s, const Level& L, const float* z, const float* r,

@@ -55,7 +55,9 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X
                                                   const float* __restrict__ bias,
                                                   const uint16_t* __restrict__ skip,
                                                   const uint16_t* __restrict__ mask,
-                                                  uint16_t* __restrict__ Y, Grid3 g, int relu) {
+                                                  uint16_t* __restrict__ Y, Grid3 g, int relu,
+    const int* __restrict__ gate) {
+  if (gate && *gate) return;  // every right-hand side of the batch is idle
   constexpr int NT = COUT / 16;
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const int i0 = blockIdx.x * 64 + wave * 16;
@@ -188,7 +190,9 @@ __device__ __forceinline__ int child_count(const Grid3& gf, int I, int J, int K)
 }
 
 __global__ __launch_bounds__(256) void k_pool(const uint16_t* __restrict__ Xf, Grid3 gf,
-                                              uint16_t* __restrict__ Xc, Grid3 gc, int C) {
+                                              uint16_t* __restrict__ Xc, Grid3 gc, int C,
+    const int* __restrict__ gate) {
+  if (gate && *gate) return;  // every right-hand side of the batch is idle
   FS_GRID_STRIDE(t, gc.nb * gc.n * C) {
     const int c = (int)(t % C);
     const int64_t v = (t / C) % gc.n, b = t / (C * gc.n);
@@ -226,7 +230,9 @@ __global__ __launch_bounds__(256) void k_pool_bwd(const uint16_t* __restrict__ d
 }
 
 __global__ __launch_bounds__(256) void k_up(const uint16_t* __restrict__ Xc, Grid3 gc,
-                                            uint16_t* __restrict__ Xf, Grid3 gf, int C) {
+                                            uint16_t* __restrict__ Xf, Grid3 gf, int C,
+    const int* __restrict__ gate) {
+  if (gate && *gate) return;  // every right-hand side of the batch is idle
   FS_GRID_STRIDE(t, gf.nb * gf.n * C) {
     const int c = (int)(t % C);
     const int64_t v = (t / C) % gf.n, b = t / (C * gf.n);
@@ -264,7 +270,9 @@ __global__ __launch_bounds__(256) void k_up_bwd(const uint16_t* __restrict__ dXf
 // one block per batch element: s = sqrt(mean((r dh)^2)), sc = {s, 1/s}
 __global__ __launch_bounds__(256) void k_rms(const float* __restrict__ r, int64_t vs,
                                              const float* __restrict__ dh, int64_t n,
-                                             float* __restrict__ sc) {
+                                             float* __restrict__ sc,
+    const int* __restrict__ gate) {
+  if (gate && *gate) return;  // every right-hand side of the batch is idle
   const int b = blockIdx.x;
   double a[1] = {0.0};
   for (int64_t v = threadIdx.x; v < n; v += 256) {
@@ -283,7 +291,9 @@ __global__ __launch_bounds__(256) void k_features(const float* __restrict__ r, i
                                                   const float* __restrict__ dh,
                                                   const float* __restrict__ feat4,
                                                   const float* __restrict__ sc, Grid3 g,
-                                                  uint16_t* __restrict__ X) {
+                                                  uint16_t* __restrict__ X,
+    const int* __restrict__ gate) {
+  if (gate && *gate) return;  // every right-hand side of the batch is idle
   FS_GRID_STRIDE(t, g.nb * g.n * 16) {
     const int c = (int)(t % 16);
     const int64_t v = (t / 16) % g.n, b = t / (16 * g.n);
@@ -297,7 +307,9 @@ __global__ __launch_bounds__(256) void k_features(const float* __restrict__ r, i
 __global__ __launch_bounds__(256) void k_take0(const uint16_t* __restrict__ Y,
                                                const float* __restrict__ dh,
                                                const float* __restrict__ sc, Grid3 g,
-                                               float* __restrict__ e, int64_t vs) {
+                                               float* __restrict__ e, int64_t vs,
+    const int* __restrict__ gate) {
+  if (gate && *gate) return;  // every right-hand side of the batch is idle
   FS_GRID_STRIDE(t, g.nb * g.n) {
     const int64_t v = t % g.n, b = t / g.n;
     e[b * vs + v] = sc[2 * b] * dh[v] * bf2f(Y[t * 16]);
@@ -374,21 +386,21 @@ static int blocks_for(int64_t cnt) {
 template <int CIN, int COUT>
 static void conv_fwd_t(hipStream_t s, const uint16_t* X, const uint16_t* Wt, int kpad,
                        const float* bias, const uint16_t* skip, const uint16_t* mask,
-                       uint16_t* Y, const Grid3& g, int relu) {
+                       uint16_t* Y, const Grid3& g, int relu, const int* gate) {
   dim3 grid((g.nx + 63) / 64, g.ny, g.nz * g.nb);
   hipLaunchKernelGGL((k_conv_fwd<CIN, COUT>), grid, dim3(256), 0, s, X, Wt, kpad, bias, skip,
-                     mask, Y, g, relu);
+                     mask, Y, g, relu, gate);
 }
 
 #define FS_NN_SHAPES(X) X(16, 16) X(16, 32) X(32, 16) X(32, 32) X(32, 64) X(64, 32) X(64, 64)
 
 int conv_fwd(hipStream_t s, int cin, int cout, const uint16_t* X, const uint16_t* Wt, int kpad,
              const float* bias, const uint16_t* skip, const uint16_t* mask, uint16_t* Y,
-             const Grid3& g, int relu) {
+             const Grid3& g, int relu, const int* gate) {
   if (kpad != kpad_of(cin)) return -1;
 #define FS_CONV(CI, CO)                                                   \
   if (cin == CI && cout == CO) {                                          \
-    conv_fwd_t<CI, CO>(s, X, Wt, kpad, bias, skip, mask, Y, g, relu);     \
+    conv_fwd_t<CI, CO>(s, X, Wt, kpad, bias, skip, mask, Y, g, relu, gate);     \
     return 0;                                                             \
   }
   FS_NN_SHAPES(FS_CONV)
@@ -424,9 +436,9 @@ void bgrad(hipStream_t s, const uint16_t* dY, int64_t nvox, int cout, float* db)
   hipLaunchKernelGGL(k_bgrad, dim3(cout), dim3(256), 0, s, dY, nvox, cout, db);
 }
 void pool(hipStream_t s, const uint16_t* Xf, const Grid3& gf, uint16_t* Xc, const Grid3& gc,
-          int C) {
+          int C, const int* gate) {
   hipLaunchKernelGGL(k_pool, dim3(blocks_for(gc.nb * gc.n * C)), dim3(256), 0, s, Xf, gf, Xc, gc,
-                     C);
+                     C, gate);
 }
 void pool_bwd(hipStream_t s, const uint16_t* dXc, const Grid3& gc, uint16_t* dXf,
               const Grid3& gf, int C, const uint16_t* add, const uint16_t* mask) {
@@ -434,9 +446,9 @@ void pool_bwd(hipStream_t s, const uint16_t* dXc, const Grid3& gc, uint16_t* dXf
                      dXf, gf, C, add, mask);
 }
 void up(hipStream_t s, const uint16_t* Xc, const Grid3& gc, uint16_t* Xf, const Grid3& gf,
-        int C) {
+        int C, const int* gate) {
   hipLaunchKernelGGL(k_up, dim3(blocks_for(gf.nb * gf.n * C)), dim3(256), 0, s, Xc, gc, Xf, gf,
-                     C);
+                     C, gate);
 }
 void up_bwd(hipStream_t s, const uint16_t* dXf, const Grid3& gf, uint16_t* dXc,
             const Grid3& gc, int C, const uint16_t* mask) {
@@ -444,17 +456,18 @@ void up_bwd(hipStream_t s, const uint16_t* dXf, const Grid3& gf, uint16_t* dXc,
                      gc, C, mask);
 }
 void rms(hipStream_t s, const float* r, int64_t vs, const float* dh, const Grid3& g,
-         float* sc) {
-  hipLaunchKernelGGL(k_rms, dim3(g.nb), dim3(256), 0, s, r, vs, dh, g.n, sc);
+         float* sc, const int* gate) {
+  hipLaunchKernelGGL(k_rms, dim3(g.nb), dim3(256), 0, s, r, vs, dh, g.n, sc, gate);
 }
 void features(hipStream_t s, const float* r, int64_t vs, const float* dh, const float* feat4,
-              const float* sc, const Grid3& g, uint16_t* X) {
+              const float* sc, const Grid3& g, uint16_t* X, const int* gate) {
   hipLaunchKernelGGL(k_features, dim3(blocks_for(g.nb * g.n * 16)), dim3(256), 0, s, r, vs, dh,
-                     feat4, sc, g, X);
+                     feat4, sc, g, X, gate);
 }
 void take0(hipStream_t s, const uint16_t* Y, const float* dh, const float* sc, const Grid3& g,
-           float* e, int64_t vs) {
-  hipLaunchKernelGGL(k_take0, dim3(blocks_for(g.nb * g.n)), dim3(256), 0, s, Y, dh, sc, g, e, vs);
+           float* e, int64_t vs, const int* gate) {
+  hipLaunchKernelGGL(k_take0, dim3(blocks_for(g.nb * g.n)), dim3(256), 0, s, Y, dh, sc, g, e, vs,
+                     gate);
 }
 void put0(hipStream_t s, const float* ge, int64_t vs, const float* dh, const float* sc,
           const Grid3& g, uint16_t* dY) {

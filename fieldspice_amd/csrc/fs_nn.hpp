@@ -32,29 +32,30 @@ inline int kpad_of(int cin) { return (27 * cin + 31) / 32 * 32; }
 // [cout][kpad] with k = tap*cin + ci.  Returns -1 for an unsupported shape.
 int conv_fwd(hipStream_t s, int cin, int cout, const uint16_t* X, const uint16_t* Wt, int kpad,
              const float* bias, const uint16_t* skip, const uint16_t* mask, uint16_t* Y,
-             const Grid3& g, int relu);
+             const Grid3& g, int relu, const int* gate = nullptr);
 // dW[tap][ci][co] = sum over voxels of X(v + tap) dY(v); part holds
 // nranges * 27*(cin/16)*(cout/16) * 256 floats.
 int conv_wgrad(hipStream_t s, int cin, int cout, const uint16_t* X, const uint16_t* dY,
                const Grid3& g, float* part, int nranges, float* dW);
 void bgrad(hipStream_t s, const uint16_t* dY, int64_t nvox, int cout, float* db);
+// gate (optional, every forward kernel): a device flag; nonzero = skip the launch
 void pool(hipStream_t s, const uint16_t* Xf, const Grid3& gf, uint16_t* Xc, const Grid3& gc,
-          int C);
+          int C, const int* gate = nullptr);
 void pool_bwd(hipStream_t s, const uint16_t* dXc, const Grid3& gc, uint16_t* dXf,
               const Grid3& gf, int C, const uint16_t* add, const uint16_t* mask);
 void up(hipStream_t s, const uint16_t* Xc, const Grid3& gc, uint16_t* Xf, const Grid3& gf,
-        int C);
+        int C, const int* gate = nullptr);
 void up_bwd(hipStream_t s, const uint16_t* dXf, const Grid3& gf, uint16_t* dXc,
             const Grid3& gc, int C, const uint16_t* mask);
 // per batch element: s = rms(r*dh) over the level, sc[2b] = s, sc[2b+1] = 1/s
 void rms(hipStream_t s, const float* r, int64_t vs, const float* dh, const Grid3& g,
-         float* sc);
+         float* sc, const int* gate = nullptr);
 // X[b][v] = {r*dh/s, feat4[v][0..3], 0...} (16 channels)
 void features(hipStream_t s, const float* r, int64_t vs, const float* dh, const float* feat4,
-              const float* sc, const Grid3& g, uint16_t* X);
+              const float* sc, const Grid3& g, uint16_t* X, const int* gate = nullptr);
 // e[b][v] = s_b * dh[v] * Y[b][v][0]
 void take0(hipStream_t s, const uint16_t* Y, const float* dh, const float* sc, const Grid3& g,
-           float* e, int64_t vs);
+           float* e, int64_t vs, const int* gate = nullptr);
 // dY[b][v][c] = c == 0 ? s_b * dh[v] * ge[b][v] : 0
 void put0(hipStream_t s, const float* ge, int64_t vs, const float* dh, const float* sc,
           const Grid3& g, uint16_t* dY);
@@ -95,14 +96,16 @@ void coarsen(hipStream_t s, const Level& f, const Map& m, float* dg, float* dinv
 void bottom_features(hipStream_t s, const Level& L, float* feat4, float* dh);
 // rc = P^T (r - A (omega D^-1 r)) for nr vectors
 void restrict_(hipStream_t s, const Level& f, const Map& m, const float* r, float* rc,
-               int64_t vsc, int nr, float omega);
+               int64_t vsc, int nr, float omega, const int* gate = nullptr);
 // e = S (omega D^-1 r + P ec) + omega D^-1 r, S = I - omega D^-1 A; optional
 // FCG partials <e,r>, <e,q> per vector (partials [2*nr][nblocks])
 void prolong(hipStream_t s, const Level& f, const Map& m, const float* r, const float* ec,
              int64_t vsc, float* e, int nr, float omega, const float* q, double* partials,
-             int nblocks);
+             int nblocks, const int* gate = nullptr);
 void bottom_jacobi(hipStream_t s, const Level& L, const float* r, float* e, int nr,
-                   float omega);
+                   float omega, const int* gate = nullptr);
+// *out = 1 when every right-hand side of sc[0..nr) is idle (Krylov mode)
+void gate(hipStream_t s, const CgScalars* sc, int nr, int mode, int* out);
 // training loss on level 0: w = A z; g = 2 (w - r) * gscale[b]; partials
 // [2*nr][nblocks] of z.w and z.r
 void loss_grad(hipStream_t s, const Level& L, const float* z, const float* r,

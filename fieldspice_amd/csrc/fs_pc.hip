@@ -144,26 +144,27 @@ int repack(fs_net* net) {
     if ((expr) != 0) return fail(FS_EINVAL, "unsupported layer shape: %s", #expr); \
   } while (0)
 
-// forward: ws.t[T_X0] -> ws.t[T_Y]
-int net_forward(fs_net* net, NetWs& w) {
-  hipStream_t s = net->ctx->stream;
+// forward on stream s: ws.t[T_X0] -> ws.t[T_Y]; gate (optional device flag)
+// skips every launch
+int net_forward(fs_net* net, NetWs& w, hipStream_t s, const int* gate = nullptr) {
   auto conv = [&](int li, Tensor in, Tensor out, const uint16_t* skip, int relu) {
     const int lv = kLevel[li];
     return nn::conv_fwd(s, kCin[li], kCout[li], w.t[in], net->Wt[li], net->kpad[li],
-                        net->params + net->off_b[li], skip, nullptr, w.t[out], w.g[lv], relu);
+                        net->params + net->off_b[li], skip, nullptr, w.t[out], w.g[lv], relu,
+                        gate);
   };
   FS_NNCALL(conv(0, T_X0, T_A1, nullptr, 1));
   FS_NNCALL(conv(1, T_A1, T_A2, nullptr, 1));
-  nn::pool(s, w.t[T_A2], w.g[1], w.t[T_P2], w.g[2], 16);
+  nn::pool(s, w.t[T_A2], w.g[1], w.t[T_P2], w.g[2], 16, gate);
   FS_NNCALL(conv(2, T_P2, T_A3, nullptr, 1));
   FS_NNCALL(conv(3, T_A3, T_A4, nullptr, 1));
-  nn::pool(s, w.t[T_A4], w.g[2], w.t[T_P4], w.g[3], 32);
+  nn::pool(s, w.t[T_A4], w.g[2], w.t[T_P4], w.g[3], 32, gate);
   FS_NNCALL(conv(4, T_P4, T_A5, nullptr, 1));
   FS_NNCALL(conv(5, T_A5, T_A6, nullptr, 1));
-  nn::up(s, w.t[T_A6], w.g[3], w.t[T_U6], w.g[2], 64);
+  nn::up(s, w.t[T_A6], w.g[3], w.t[T_U6], w.g[2], 64, gate);
   FS_NNCALL(conv(6, T_U6, T_A7, w.t[T_A4], 1));
   FS_NNCALL(conv(7, T_A7, T_A8, nullptr, 1));
-  nn::up(s, w.t[T_A8], w.g[2], w.t[T_U8], w.g[1], 32);
+  nn::up(s, w.t[T_A8], w.g[2], w.t[T_U8], w.g[1], 32, gate);
   FS_NNCALL(conv(8, T_U8, T_A9, w.t[T_A2], 1));
   FS_NNCALL(conv(9, T_A9, T_A10, nullptr, 1));
   FS_NNCALL(conv(10, T_A10, T_Y, nullptr, 0));
@@ -173,8 +174,7 @@ int net_forward(fs_net* net, NetWs& w) {
 
 // backward: ws.t[T_DY] -> net->grad (overwritten); needs the activations of
 // the forward pass of the same batch
-int net_backward(fs_net* net, NetWs& w) {
-  hipStream_t s = net->ctx->stream;
+int net_backward(fs_net* net, NetWs& w, hipStream_t s) {
   // data gradient of layer li: out = mask(conv^T(in) + add)
   auto bwd = [&](int li, Tensor in, Tensor out, Tensor mask_t) {
     const int lv = kLevel[li];
@@ -286,6 +286,7 @@ struct fs_pc {
   std::vector<float*> rv, ev;  // per level >= 1: [kMaxRhs][vs]
   float *feat4 = nullptr, *dh = nullptr, *sc = nullptr;
   float* gK = nullptr;     // bottom-loss gradient [kMaxRhs][vs_K]
+  int* gate = nullptr;     // 1 while every right-hand side of the solve is idle
   double* cK = nullptr;    // bottom-loss normalisation per batch element
   NetWs ws;
   // training
@@ -314,6 +315,7 @@ void free_pc(fs_pc* pc) {
   dfree(pc->dh);
   dfree(pc->sc);
   dfree(pc->gK);
+  dfree(pc->gate);
   dfree(pc->cK);
   free_ws(pc->ws);
   dfree(pc->pe);
@@ -345,29 +347,36 @@ namespace fs {
 // marks idle (mode as the Krylov kernels) are skipped on level 0.
 int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, double* partials,
              int* nblocks, const CgScalars* sc, int mode) {
-  hipStream_t s = pc->sys->ctx->stream;
+  hipStream_t s = pc->sys->stream;
   const int K = pc->K;
+  // in a solve, every kernel below skips its work once all right-hand sides
+  // are idle (graph replays keep launching finished systems)
+  const int* g = nullptr;
+  if (sc) {
+    mg::gate(s, sc, nr, mode, pc->gate);
+    g = pc->gate;
+  }
   for (int l = 0; l < K; ++l) {
     if (l == 0 && pc->l0_pairs)
       mg::restrict_l0(s, pc->l0, r0, pc->rv[1], pc->lv[1].vs, nr, pc->omega, sc, mode);
     else
       mg::restrict_(s, pc->lv[l], pc->maps[l], l == 0 ? r0 : pc->rv[l], pc->rv[l + 1],
-                    pc->lv[l + 1].vs, nr, pc->omega);
+                    pc->lv[l + 1].vs, nr, pc->omega, g);
   }
   const mg::Level& B = pc->lv[K];
   if (pc->net) {
     NetWs& w = pc->ws;
     set_batch(w, nr);
-    nn::rms(s, pc->rv[K], B.vs, pc->dh, w.g[1], pc->sc);
-    nn::features(s, pc->rv[K], B.vs, pc->dh, pc->feat4, pc->sc, w.g[1], w.t[T_X0]);
-    if (int rc = net_forward(pc->net, w)) return rc;
-    nn::take0(s, w.t[T_Y], pc->dh, pc->sc, w.g[1], pc->ev[K], B.vs);
+    nn::rms(s, pc->rv[K], B.vs, pc->dh, w.g[1], pc->sc, g);
+    nn::features(s, pc->rv[K], B.vs, pc->dh, pc->feat4, pc->sc, w.g[1], w.t[T_X0], g);
+    if (int rc = net_forward(pc->net, w, s, g)) return rc;
+    nn::take0(s, w.t[T_Y], pc->dh, pc->sc, w.g[1], pc->ev[K], B.vs, g);
   } else {
-    mg::bottom_jacobi(s, B, pc->rv[K], pc->ev[K], nr, pc->omega);
+    mg::bottom_jacobi(s, B, pc->rv[K], pc->ev[K], nr, pc->omega, g);
   }
   for (int l = K - 1; l >= 1; --l)
     mg::prolong(s, pc->lv[l], pc->maps[l], pc->rv[l], pc->ev[l + 1], pc->lv[l + 1].vs,
-                pc->ev[l], nr, pc->omega, nullptr, nullptr, 0);
+                pc->ev[l], nr, pc->omega, nullptr, nullptr, 0, g);
   const int nb = mg::prolong_l0(s, pc->l0, r0, pc->ev[1], pc->lv[1].vs, z0, nr, pc->omega, q,
                                 partials, sc, mode);
   if (nblocks) *nblocks = nb;
@@ -393,7 +402,7 @@ namespace {
 // holding 1/(E_b nb).  Leaves per-element z.Az and z.r in pc->sums[sum_off..]
 // ([2*nb]) and the parameter gradient in net->grad.
 int batch_loss_grad(fs_pc* pc, int nb, const float* gscale, int64_t sum_off) {
-  hipStream_t s = pc->sys->ctx->stream;
+  hipStream_t s = pc->sys->stream;
   if (int rc = fs::pc_cycle(pc, nb, pc->br, pc->bz, nullptr, nullptr, nullptr, nullptr, 0))
     return rc;
   const mg::Level& L0 = pc->lv[0];
@@ -411,7 +420,7 @@ int batch_loss_grad(fs_pc* pc, int nb, const float* gscale, int64_t sum_off) {
   }
   NetWs& w = pc->ws;
   nn::put0(s, pc->rv[pc->K], pc->lv[pc->K].vs, pc->dh, pc->sc, w.g[1], w.t[T_DY]);
-  if (int rc = net_backward(pc->net, w)) return rc;
+  if (int rc = net_backward(pc->net, w, s)) return rc;
   FS_HIP(hipGetLastError());
   return FS_OK;
 }
@@ -421,7 +430,7 @@ int batch_loss_grad(fs_pc* pc, int nb, const float* gscale, int64_t sum_off) {
 // (the energy functional of A_K e = r_K, minimal at the exact bottom solve).
 // Leaves y.Ay, y.r and c per element in pc->sums[sum_off..] ([3*nb]).
 int batch_bottom_loss_grad(fs_pc* pc, int nb, int64_t sum_off) {
-  hipStream_t s = pc->sys->ctx->stream;
+  hipStream_t s = pc->sys->stream;
   const int K = pc->K;
   for (int l = 0; l < K; ++l) {
     if (l == 0 && pc->l0_pairs)
@@ -435,7 +444,7 @@ int batch_bottom_loss_grad(fs_pc* pc, int nb, int64_t sum_off) {
   set_batch(w, nb);
   nn::rms(s, pc->rv[K], B.vs, pc->dh, w.g[1], pc->sc);
   nn::features(s, pc->rv[K], B.vs, pc->dh, pc->feat4, pc->sc, w.g[1], w.t[T_X0]);
-  if (int rc = net_forward(pc->net, w)) return rc;
+  if (int rc = net_forward(pc->net, w, s)) return rc;
   nn::take0(s, w.t[T_Y], pc->dh, pc->sc, w.g[1], pc->ev[K], B.vs);
   const int nbk = mg::vec_blocks(B.n);
   mg::jacobi_norm(s, B, pc->rv[K], nb, pc->part, nbk);
@@ -446,7 +455,7 @@ int batch_bottom_loss_grad(fs_pc* pc, int nb, int64_t sum_off) {
   FS_HIP(hipMemcpyAsync(pc->sums + sum_off + 2 * nb, pc->cK, nb * sizeof(double),
                         hipMemcpyDeviceToDevice, s));
   nn::put0(s, pc->gK, B.vs, pc->dh, pc->sc, w.g[1], w.t[T_DY]);
-  if (int rc = net_backward(pc->net, w)) return rc;
+  if (int rc = net_backward(pc->net, w, s)) return rc;
   FS_HIP(hipGetLastError());
   return FS_OK;
 }
@@ -603,14 +612,14 @@ int fs_net_eval(fs_net* net, int nx, int ny, int nz, int nb, const float* X, flo
     hipStream_t s = ctx->stream;
     FS_HIP(hipMemcpyAsync(tmp, X, cnt * sizeof(float), hipMemcpyHostToDevice, s));
     nn::to_bf16(s, tmp, cnt, w.t[T_X0]);
-    if (int r = net_forward(net, w)) return r;
+    if (int r = net_forward(net, w, s)) return r;
     nn::from_bf16(s, w.t[T_Y], cnt, tmp);
     FS_HIP(hipMemcpyAsync(Y, tmp, cnt * sizeof(float), hipMemcpyDeviceToHost, s));
     FS_HIP(hipStreamSynchronize(s));
     if (dY) {
       FS_HIP(hipMemcpyAsync(tmp, dY, cnt * sizeof(float), hipMemcpyHostToDevice, s));
       nn::to_bf16(s, tmp, cnt, w.t[T_DY]);
-      if (int r = net_backward(net, w)) return r;
+      if (int r = net_backward(net, w, s)) return r;
       FS_HIP(hipMemcpyAsync(grad, net->grad, net->nparams * sizeof(float),
                             hipMemcpyDeviceToHost, s));
       FS_HIP(hipStreamSynchronize(s));
@@ -780,6 +789,7 @@ int fs_pc_create(fs_sys* sys, fs_net* net, int nlevels, const int32_t* groups, d
   if ((rc = dalloc(&pc->feat4, (size_t)B.n * 4)) || (rc = dalloc(&pc->dh, (size_t)B.n)) ||
       (rc = dalloc(&pc->sc, (size_t)2 * fs::kMaxRhs)) ||
       (rc = dalloc(&pc->gK, (size_t)fs::kMaxRhs * B.vs)) ||
+      (rc = dalloc(&pc->gate, 1)) ||
       (rc = dalloc(&pc->cK, (size_t)fs::kMaxRhs)) ||
       (rc = dalloc(&pc->part, (size_t)2 * fs::kMaxRhs * std::max(pc->nblk0, 2048) + 64)) ||
       (rc = ensure_sums(pc, 64)))

@@ -295,27 +295,27 @@ int grid_for(int64_t n, int cap) {
 void launch_explicit(fs_sys* s, const fs_op* A, const double* x, const double* src,
                      const double* scale, int nsrc, int64_t vs, double* rhs) {
   fs_op dummy = A ? *A : *s->op;
-  hipLaunchKernelGGL(k_explicit, dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream, dummy,
+  hipLaunchKernelGGL(k_explicit, dim3(s->vec_blocks), dim3(kBlock), 0, s->stream, dummy,
                      A ? 1 : 0, x, src, scale, nsrc, vs, rhs);
 }
 
 void launch_lift_groups(fs_sys* s, const double* rhs, const int32_t* group, const double* vals) {
-  hipLaunchKernelGGL(k_lift_groups, dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream,
+  hipLaunchKernelGGL(k_lift_groups, dim3(s->vec_blocks), dim3(kBlock), 0, s->stream,
                      *s->op, rhs, group, vals, s->mask, s->dinv, s->bs, s->partials);
-  hipLaunchKernelGGL(k_bnorm_final, dim3(1), dim3(kBlock), 0, s->ctx->stream, s->partials,
+  hipLaunchKernelGGL(k_bnorm_final, dim3(1), dim3(kBlock), 0, s->stream, s->partials,
                      s->vec_blocks, s->sc);
 }
 
 void launch_init_warm(fs_sys* s, const double* xprev, double tol, int maxiter) {
-  hipLaunchKernelGGL(k_init_warm, dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream, xprev,
+  hipLaunchKernelGGL(k_init_warm, dim3(s->vec_blocks), dim3(kBlock), 0, s->stream, xprev,
                      s->dhalf, s->c64[0], s->c64[1], s->c64[2], s->d, s->bs, s->x, s->r, s->p,
                      s->partials);
-  hipLaunchKernelGGL(k_init_warm_final, dim3(1), dim3(kBlock), 0, s->ctx->stream, s->partials,
+  hipLaunchKernelGGL(k_init_warm_final, dim3(1), dim3(kBlock), 0, s->stream, s->partials,
                      s->vec_blocks, tol, maxiter, s->sc);
 }
 
 void launch_expand_groups(fs_sys* s, const int32_t* group, const double* vals, double* x) {
-  hipLaunchKernelGGL(k_expand_groups, dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream,
+  hipLaunchKernelGGL(k_expand_groups, dim3(s->vec_blocks), dim3(kBlock), 0, s->stream,
                      s->x, s->mask, s->dinv, group, vals, s->d.n, x);
 }
 
@@ -323,18 +323,18 @@ void launch_probe(fs_sys* s, const fs_op* A1, const fs_op* A2, const double* x,
                   const double* xprev, const double* w, const int64_t* nodes,
                   const int64_t* offs, int nprobe, int nbx, double* out_i, double* out_v) {
   fs_op b = A2 ? *A2 : *A1;
-  hipLaunchKernelGGL(k_probe, dim3(nbx, nprobe), dim3(kBlock), 0, s->ctx->stream, *A1, b,
+  hipLaunchKernelGGL(k_probe, dim3(nbx, nprobe), dim3(kBlock), 0, s->stream, *A1, b,
                      A2 ? 1 : 0, x, xprev, w, nodes, offs, s->partials);
-  hipLaunchKernelGGL(k_probe_final, dim3(nprobe), dim3(kBlock), 0, s->ctx->stream, s->partials,
+  hipLaunchKernelGGL(k_probe_final, dim3(nprobe), dim3(kBlock), 0, s->stream, s->partials,
                      nbx, out_i, out_v);
 }
 
 void launch_step_stats(fs_sys* s, const double* x, const fs_op* Qa, const fs_op* Qb,
                        double* out) {
   fs_op a = Qa ? *Qa : *s->op, b = Qb ? *Qb : *s->op;
-  hipLaunchKernelGGL(k_step_stats, dim3(s->vec_blocks), dim3(kBlock), 0, s->ctx->stream, x,
+  hipLaunchKernelGGL(k_step_stats, dim3(s->vec_blocks), dim3(kBlock), 0, s->stream, x,
                      s->d, a, Qa ? 1 : 0, b, Qb ? 1 : 0, s->partials);
-  hipLaunchKernelGGL(k_step_stats_final, dim3(1), dim3(kBlock), 0, s->ctx->stream, s->partials,
+  hipLaunchKernelGGL(k_step_stats_final, dim3(1), dim3(kBlock), 0, s->stream, s->partials,
                      s->vec_blocks, out);
 }
 

@@ -19,7 +19,8 @@ from .grid import RectilinearGrid
 from .solvers.base import Terminal
 from .units import eps0
 
-__all__ = ["Extraction", "parallel_plate", "layered_boxes", "c2_geometry", "multi_conductor"]
+__all__ = ["Extraction", "parallel_plate", "layered_boxes", "c2_geometry", "multi_conductor",
+           "batch_instances"]
 
 
 @dataclass
@@ -130,3 +131,26 @@ def multi_conductor(nodes=(128, 128, 128), seed: int = 0, h: float = 20e-9,
     terms = [Terminal(f"c{m}", nid[i0:i1 + 1, j0:j1 + 1, k0:k1 + 1].ravel())
              for m, (i0, i1, j0, j1, k0, k1) in enumerate(boxes)]
     return Extraction(g, eps_r * eps0, terms, boxes, layer_eps, seed)
+
+
+def batch_instances(count: int = 64, seed: int = 0, n_min: int = 32 ** 3, n_max: int = 160 ** 3,
+                    max_aspect: float = 4.0, n_terms=(2, 6)):
+    """Seeded extraction problems of uneven size.  Node counts are drawn
+    uniformly in N (not in side length) from [n_min, n_max]; the three side
+    ratios are drawn log-uniformly in [1/2, 2] around the cube (so the
+    largest over the smallest side is at most ``max_aspect`` = 4); each
+    instance is a ``multi_conductor`` geometry with 2-6 terminals and layered
+    eps_r in [2.5, 8]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    half = np.sqrt(max_aspect)
+    for i in range(count):
+        n = float(rng.uniform(n_min, n_max))
+        a = np.exp(rng.uniform(np.log(1 / half), np.log(half), 3))
+        c = (n / np.prod(a)) ** (1.0 / 3.0)
+        sides = tuple(int(max(16, round(c * v))) for v in a)
+        m = min(sides)
+        ex = multi_conductor(sides, seed=int(rng.integers(1 << 30)), n_terms=n_terms,
+                             side=(3, max(4, m // 4)))
+        out.append(ex)
+    return out

@@ -266,6 +266,29 @@ int fs_pc_loss_grad(fs_pc* pc, int kind, int nb, const double* e, double* loss, 
 int fs_pc_train(fs_pc* pc, int kind, int steps, int batch, double lr, uint64_t seed,
                 double* losses);
 
+/* ---- tiles and batches ----------------------------------------------------
+ * No counterpart in the reference, which solves one system at a time on the
+ * CPU (fieldspice/solvers/poisson.py:1100-1249 per extraction). */
+typedef struct fs_batch fs_batch;
+/* Time the FP32 multi-RHS stencil of this system for every z-march depth
+ * (1, 2, 4, ... <= nz planes per tile) and keep the fastest (cached per grid
+ * shape and nr).  Run before fs_pc_create: the preconditioner copies the
+ * system's tiling. */
+int fs_sys_autotune(fs_sys* sys, int nr, int reps, int* kz);
+/* Many capacitance extractions (each <= 8 terminals) solved together: the
+ * iterations of every system, `chunk` at a time, run as one captured hipGraph
+ * whose parallel branches spread the systems over `nstreams` streams; systems
+ * that finish skip their kernels on later replays; FP64 refreshes, cycle
+ * starts and the final charge integration run per system between replays. */
+int fs_batch_create(fs_ctx* ctx, int nstreams, fs_batch** out);
+int fs_batch_destroy(fs_batch* batch);
+int fs_batch_add(fs_batch* batch, fs_cap* cap, const fs_op* L);
+/* C: the systems' nt_i x nt_i matrices concatenated (Maxwell form, not
+ * symmetrised); inst_time[i]: seconds from the start of the call until system
+ * i was seen finished; inst_iters[i]: its Krylov iterations (NULL to skip). */
+int fs_batch_solve(fs_batch* batch, const fs_solve_params* params, int chunk, double* C,
+                   double* inst_time, int* inst_iters, fs_solve_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif
