@@ -300,7 +300,9 @@ def main():
     fp64 = head == "hip_pcg"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_stencil_dot<%s,8>" % ("double" if fp64 else "float"),
+                "kernel": ("k_stencil_dot<double,8>" if fp64 else
+                           "k_stencil_dot4<8>" if g.shape_nodes[0] % 4 == 0 else
+                           "k_stencil_dot<float,8>"),
                 "avg_launch_us": round(avg_s * 1e6, 2),
                 "algorithmic_bytes_per_launch": int(bytes_per_launch)}
     cpu = None

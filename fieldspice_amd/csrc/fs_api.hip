@@ -649,7 +649,21 @@ int fs_sys_create(fs_op* op, const uint8_t* mask, int pin, fs_sys** out, fs_sys_
   s->vec_blocks = (int)std::min<int64_t>((d.n + fs::kBlock - 1) / fs::kBlock,
                                          (int64_t)std::max(ctx->n_cu, 1) * 8);
   s->vec_blocks = std::max(s->vec_blocks, 1);
-  const int64_t maxb = std::max<int64_t>(tl.nblocks, s->vec_blocks);
+  // FP32 stencil with 4 nodes per lane when rows are a multiple of 4
+  s->use_x4 = d.nx % 4 == 0 ? 1 : 0;
+  {
+    fs::Tiling& t4 = s->tl4;
+    t4.tiles_x = (d.nx / 4 + fs::kWave - 1) / fs::kWave;
+    t4.tiles_y = (d.ny + fs::kTileY - 1) / fs::kTileY;
+    const int64_t t4xy = (int64_t)t4.tiles_x * t4.tiles_y;
+    t4.kz = (int)std::min<int64_t>(16, std::max<int64_t>(4, (d.nz * t4xy + 2047) / 2048));
+    t4.kz = std::max(1, std::min(t4.kz, d.nz));
+    t4.tiles_z = (d.nz + t4.kz - 1) / t4.kz;
+    t4.ntiles = (int)(t4xy * t4.tiles_z);
+    t4.nblocks = (t4.ntiles + 7) / 8 * 8;
+  }
+  const int64_t maxb = std::max<int64_t>(std::max<int64_t>(tl.nblocks, s->tl4.nblocks),
+                                         s->vec_blocks);
   s->partials_cap = maxb * fs::kMaxRhs * 2 + 64 * (fs::kMaxRhs + 1) * fs::kMaxRhs;
   int rc = FS_OK;
   double* dsum = nullptr;
@@ -1164,9 +1178,9 @@ int fs_sys_set_profiling(fs_sys* s, int on) {
 
 namespace {
 
-fs::Tiling tiling_for(const fs::Dims& d, int kz) {
+fs::Tiling tiling_for(const fs::Dims& d, int kz, bool x4 = false) {
   fs::Tiling tl;
-  tl.tiles_x = (d.nx + fs::kTileX - 1) / fs::kTileX;
+  tl.tiles_x = x4 ? (d.nx / 4 + fs::kWave - 1) / fs::kWave : (d.nx + fs::kTileX - 1) / fs::kTileX;
   tl.tiles_y = (d.ny + fs::kTileY - 1) / fs::kTileY;
   tl.kz = std::max(1, std::min(kz, d.nz));
   tl.tiles_z = (d.nz + tl.kz - 1) / tl.kz;
@@ -1175,8 +1189,12 @@ fs::Tiling tiling_for(const fs::Dims& d, int kz) {
   return tl;
 }
 
+struct TuneResult {
+  int kz = 0, kz4 = 0, use_x4 = 0;
+};
+
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int, int>, int> g_tune_cache;  // (nx, ny, nz, nr) -> kz
+std::map<std::tuple<int, int, int, int>, TuneResult> g_tune_cache;  // (nx, ny, nz, nr)
 
 }  // namespace
 
@@ -1191,64 +1209,69 @@ int fs_sys_autotune(fs_sys* s, int nr, int reps, int* kz_out) {
   FS_HIP(hipSetDevice(ctx->device));
   const fs::Dims& d = s->d;
   const auto key = std::make_tuple(d.nx, d.ny, d.nz, nr);
-  int best_kz = 0;
+  TuneResult best;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_cache.find(key);
-    if (it != g_tune_cache.end()) best_kz = it->second;
+    if (it != g_tune_cache.end()) best = it->second;
   }
-  if (best_kz == 0) {
-    if (int rc = ensure_ws(s, nr)) return rc;
-    // candidates: z-march depths from one plane to the whole column
-    std::vector<int> cand;
-    for (int kz = 1; kz <= 128; kz *= 2)
-      if (kz <= d.nz) cand.push_back(kz);
-    int64_t need = 0;
-    for (int kz : cand) need = std::max<int64_t>(need, tiling_for(d, kz).nblocks);
-    need = need * fs::kMaxRhs * 2 + 64 * (fs::kMaxRhs + 1) * fs::kMaxRhs;
-    if (need > s->partials_cap) {
-      double* np = nullptr;
-      if (int rc = dalloc(&np, (size_t)need)) return rc;
-      dfree(s->partials);
-      s->partials = np;
-      s->partials_cap = need;
-    }
-    // every right-hand side live for the timed launches; the next solve
-    // re-initialises the scalars
-    for (int r = 0; r < fs::kMaxRhs; ++r) {
-      s->sc_host[r] = fs::CgScalars{};
-    }
-    FS_HIP(hipMemcpy(s->sc, s->sc_host, sizeof(fs::CgScalars) * fs::kMaxRhs,
-                     hipMemcpyHostToDevice));
-    FS_HIP(hipMemsetAsync(s->p32, 0, sizeof(float) * nr * s->vstride, s->stream));
-    float best = 1e30f;
-    for (int kz : cand) {
-      const fs::Tiling tl = tiling_for(d, kz);
-      fs::launch_stencil_only(s, nr, tl);  // warm
-      FS_HIP(hipEventRecord(ctx->ev0, s->stream));
-      for (int k = 0; k < reps; ++k) fs::launch_stencil_only(s, nr, tl);
-      FS_HIP(hipEventRecord(ctx->ev1, s->stream));
-      FS_HIP(hipEventSynchronize(ctx->ev1));
-      float ms = 0.f;
-      FS_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-      if (ms < best) {
-        best = ms;
-        best_kz = kz;
-      }
-    }
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_cache[key] = best_kz;
-  }
-  const fs::Tiling tl = tiling_for(d, best_kz);
-  const int64_t need = (int64_t)tl.nblocks * fs::kMaxRhs * 2 + 64 * (fs::kMaxRhs + 1) * fs::kMaxRhs;
-  if (need > s->partials_cap) {
+  auto grow_partials = [&](int64_t nblocks) -> int {
+    const int64_t need = nblocks * fs::kMaxRhs * 2 + 64 * (fs::kMaxRhs + 1) * fs::kMaxRhs;
+    if (need <= s->partials_cap) return FS_OK;
     double* np = nullptr;
     if (int rc = dalloc(&np, (size_t)need)) return rc;
     dfree(s->partials);
     s->partials = np;
     s->partials_cap = need;
+    return FS_OK;
+  };
+  const bool x4_ok = d.nx % 4 == 0;
+  if (best.kz == 0) {
+    if (int rc = ensure_ws(s, nr)) return rc;
+    // candidates: z-march depths from one plane to 128, for both kernels
+    std::vector<int> cand;
+    for (int kz = 1; kz <= 128; kz *= 2)
+      if (kz <= d.nz) cand.push_back(kz);
+    int64_t need = 0;
+    for (int kz : cand) need = std::max<int64_t>(need, tiling_for(d, kz).nblocks);
+    if (int rc = grow_partials(need)) return rc;
+    // every right-hand side live for the timed launches; the next solve
+    // re-initialises the scalars
+    for (int r = 0; r < fs::kMaxRhs; ++r) s->sc_host[r] = fs::CgScalars{};
+    FS_HIP(hipMemcpy(s->sc, s->sc_host, sizeof(fs::CgScalars) * fs::kMaxRhs,
+                     hipMemcpyHostToDevice));
+    FS_HIP(hipMemsetAsync(s->p32, 0, sizeof(float) * nr * s->vstride, s->stream));
+    float t_best[2] = {1e30f, 1e30f};
+    for (int x4 = 0; x4 < (x4_ok ? 2 : 1); ++x4) {
+      for (int kz : cand) {
+        const fs::Tiling tl = tiling_for(d, kz, x4 != 0);
+        fs::launch_stencil_only(s, nr, tl, x4 != 0);  // warm
+        FS_HIP(hipEventRecord(ctx->ev0, s->stream));
+        for (int k = 0; k < reps; ++k) fs::launch_stencil_only(s, nr, tl, x4 != 0);
+        FS_HIP(hipEventRecord(ctx->ev1, s->stream));
+        FS_HIP(hipEventSynchronize(ctx->ev1));
+        float ms = 0.f;
+        FS_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        if (ms < t_best[x4]) {
+          t_best[x4] = ms;
+          (x4 ? best.kz4 : best.kz) = kz;
+        }
+      }
+    }
+    best.use_x4 = x4_ok && t_best[1] < t_best[0] ? 1 : 0;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
   }
+  const fs::Tiling tl = tiling_for(d, best.kz);
+  if (int rc = grow_partials(tl.nblocks)) return rc;
   s->tl = tl;
+  if (x4_ok && best.kz4 > 0) {
+    const fs::Tiling t4 = tiling_for(d, best.kz4, true);
+    if (int rc = grow_partials(t4.nblocks)) return rc;
+    s->tl4 = t4;
+  }
+  s->use_x4 = best.use_x4;
+  const int best_kz = best.use_x4 ? best.kz4 : best.kz;
   if (kz_out) *kz_out = best_kz;
   return FS_OK;
   FS_GUARD_END

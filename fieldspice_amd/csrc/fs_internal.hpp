@@ -89,6 +89,8 @@ struct fs_sys {
   const fs_op* op = nullptr;  // not owned; must outlive the system
   fs::Dims d{};
   fs::Tiling tl{};
+  fs::Tiling tl4{};            // 4-nodes-per-lane FP32 stencil (x tiles of 256 nodes)
+  int use_x4 = 0;              // rows are a multiple of 4 and that kernel is selected
   int64_t vstride = 0;
   uint8_t* mask = nullptr;     // 0 free, 1 Dirichlet, 2 pinned (x-fast)
   double* dhalf = nullptr;     // sqrt(D) on free nodes, 0 elsewhere

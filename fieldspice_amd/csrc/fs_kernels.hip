@@ -427,6 +427,117 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot(const T* __restrict__ p,
   }
 }
 
+// FP32 variant of k_stencil_dot for rows whose length is a multiple of 4:
+// each lane owns 4 consecutive x nodes (one 16-byte load per array), a wave
+// covers 256 nodes of a row, and all right-hand sides advance together in
+// the z-march so the couplings are read once per node.  The x neighbours at
+// the ends of a lane's 4 nodes come from the adjacent lanes (shuffles), or
+// from memory at the edges of the wave.  Stores bypass the caches (q is not
+// read again by this kernel).
+typedef float f4n __attribute__((ext_vector_type(4)));
+
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_stencil_dot4(const float* __restrict__ p,
+                                                         float* __restrict__ q,
+                                                         const float* __restrict__ cx,
+                                                         const float* __restrict__ cy,
+                                                         const float* __restrict__ cz, Dims d,
+                                                         Tiling tl, VecArgs va, int mode,
+                                                         CgScalars* sc, double* partials,
+                                                         unsigned long long* prof) {
+  if (all_idle(sc, va.nr, mode)) return;
+  bool live[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) live[r] = r < va.nr && rhs_live(sc, r, mode);
+  if (prof && blockIdx.x == 0 && threadIdx.x == 0) {
+    unsigned long long nl = 0;
+    for (int r = 0; r < NR; ++r) nl += live[r] ? 1ull : 0ull;
+    prof[0] += 1ull;
+    prof[1] += nl;
+  }
+  double acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+  int tx, ty, tz;
+  if (tile_of_block(tl, tx, ty, tz)) {
+    const int lane = (int)(threadIdx.x % kWave);
+    const int i = (tx * kWave + lane) * 4;
+    const int j = ty * kTileY + (int)(threadIdx.x / kWave);
+    const int k0 = tz * tl.kz;
+    const int k1 = min(d.nz, k0 + tl.kz);
+    const bool valid = i < d.nx && j < d.ny;  // i .. i+3 inside: nx % 4 == 0
+    const int64_t base = valid ? (int64_t)j * d.nx + i : 0;
+    const bool has_w = i > 0, has_e = i + 4 < d.nx, has_n = j + 1 < d.ny, has_s = j > 0;
+    int64_t nn = (int64_t)k0 * d.sxy + base;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 pm[NR], pc[NR];
+    float4 czm = z4;
+    if (valid && k0 > 0) czm = *reinterpret_cast<const float4*>(cz + nn - d.sxy);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      pm[r] = z4;
+      pc[r] = z4;
+      if (valid && live[r]) {
+        if (k0 > 0) pm[r] = *reinterpret_cast<const float4*>(p + r * va.vs + nn - d.sxy);
+        pc[r] = *reinterpret_cast<const float4*>(p + r * va.vs + nn);
+      }
+    }
+    for (int k = k0; k < k1; ++k, nn += d.sxy) {
+      float4 cxc = z4, cyc = z4, czc = z4, cym = z4;
+      if (valid) {
+        cxc = *reinterpret_cast<const float4*>(cx + nn);
+        cyc = *reinterpret_cast<const float4*>(cy + nn);
+        czc = *reinterpret_cast<const float4*>(cz + nn);
+        if (has_s) cym = *reinterpret_cast<const float4*>(cy + nn - d.nx);
+      }
+      // coupling to the west of this lane's first node
+      const float cxw_l = __shfl_up(cxc.w, 1, kWave);
+      float cxw = 0.f;
+      if (valid && has_w) cxw = lane > 0 ? cxw_l : cx[nn - 1];
+      const bool up = k + 1 < d.nz;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        if (!live[r]) continue;  // uniform over the launch
+        const float* __restrict__ pr = p + r * va.vs;
+        float4 pu = z4, pn = z4, ps = z4;
+        if (valid) {
+          if (up) pu = *reinterpret_cast<const float4*>(pr + nn + d.sxy);
+          if (has_n) pn = *reinterpret_cast<const float4*>(pr + nn + d.nx);
+          if (has_s) ps = *reinterpret_cast<const float4*>(pr + nn - d.nx);
+        }
+        const float4 c = pc[r];
+        float pw = __shfl_up(c.w, 1, kWave), pe = __shfl_down(c.x, 1, kWave);
+        if (valid) {
+          if (!has_w) pw = 0.f;
+          else if (lane == 0) pw = pr[nn - 1];
+          if (!has_e) pe = 0.f;
+          else if (lane == kWave - 1) pe = pr[nn + 4];
+        }
+        const float4 m = pm[r];
+        f4n y;
+        y.x = c.x - (cxc.x * c.y + cxw * pw + cyc.x * pn.x + cym.x * ps.x + czc.x * pu.x + czm.x * m.x);
+        y.y = c.y - (cxc.y * c.z + cxc.x * c.x + cyc.y * pn.y + cym.y * ps.y + czc.y * pu.y + czm.y * m.y);
+        y.z = c.z - (cxc.z * c.w + cxc.y * c.y + cyc.z * pn.z + cym.z * ps.z + czc.z * pu.z + czm.z * m.z);
+        y.w = c.w - (cxc.w * pe + cxc.z * c.z + cyc.w * pn.w + cym.w * ps.w + czc.w * pu.w + czm.w * m.w);
+        if (valid) {
+          __builtin_nontemporal_store(y, reinterpret_cast<f4n*>(q + r * va.vs + nn));
+          acc[r] += (double)c.x * (double)y.x + (double)c.y * (double)y.y +
+                    (double)c.z * (double)y.z + (double)c.w * (double)y.w;
+        }
+        pm[r] = c;
+        pc[r] = pu;
+      }
+      czm = czc;
+    }
+  }
+  block_sum<NR>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (r < va.nr) store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
+  }
+}
+
 // x += alpha p; r -= alpha q; rho_new = r.r; beta = rho_new/rho; stop tests.
 // mode 0: FP64 main iteration (threshold thresh2, sets done).
 // mode 1: FP32 cycle of mixed precision (threshold aux1 on the cycle's own
@@ -941,11 +1052,19 @@ static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEve
   hipLaunchKernelGGL((k_pupdate<float, NR>), dim3(vb), dim3(kBlock), 0, st, s->p32, dir, va, mode,
                      s->sc);
   if (ea) (void)hipEventRecord(ea, st);
-  hipLaunchKernelGGL((k_stencil_dot<float, NR>), dim3(sb), dim3(kBlock), 0, st, s->p32, s->q32,
-                     s->c32[0], s->c32[1], s->c32[2], s->d, s->tl, va, mode, s->sc, s->partials,
-                     prof);
+  int sbf = sb;
+  if (s->use_x4) {
+    sbf = s->tl4.nblocks;
+    hipLaunchKernelGGL((k_stencil_dot4<NR>), dim3(sbf), dim3(kBlock), 0, st, s->p32, s->q32,
+                       s->c32[0], s->c32[1], s->c32[2], s->d, s->tl4, va, mode, s->sc,
+                       s->partials, prof);
+  } else {
+    hipLaunchKernelGGL((k_stencil_dot<float, NR>), dim3(sb), dim3(kBlock), 0, st, s->p32,
+                       s->q32, s->c32[0], s->c32[1], s->c32[2], s->d, s->tl, va, mode, s->sc,
+                       s->partials, prof);
+  }
   if (eb) (void)hipEventRecord(eb, st);
-  finalize<NR>(st, s->partials, FIN_ALPHA, nr, sb, mode, s->sc);
+  finalize<NR>(st, s->partials, FIN_ALPHA, nr, sbf, mode, s->sc);
   hipLaunchKernelGGL((k_update<float, NR>), dim3(vb), dim3(kBlock), 0, st, s->x32, s->r32, s->p32,
                      s->q32, va, mode, s->sc, s->partials);
   finalize<NR>(st, s->partials, FIN_BETA, nr, vb, mode, s->sc);
@@ -1052,14 +1171,19 @@ int launch_fcg_start(fs_sys* s, int nr) {
   return launch_fcg_start_t<8>(s, nr);
 }
 template <int NR>
-static void launch_stencil_only_t(fs_sys* s, int nr, const Tiling& tl) {
+static void launch_stencil_only_t(fs_sys* s, int nr, const Tiling& tl, bool x4) {
   VecArgs va{s->d.n, s->vstride, nr};
-  hipLaunchKernelGGL((k_stencil_dot<float, NR>), dim3(tl.nblocks), dim3(kBlock), 0, s->stream,
-                     s->p32, s->q32, s->c32[0], s->c32[1], s->c32[2], s->d, tl, va, 1, s->sc,
-                     s->partials, nullptr);
+  if (x4)
+    hipLaunchKernelGGL((k_stencil_dot4<NR>), dim3(tl.nblocks), dim3(kBlock), 0, s->stream,
+                       s->p32, s->q32, s->c32[0], s->c32[1], s->c32[2], s->d, tl, va, 1, s->sc,
+                       s->partials, nullptr);
+  else
+    hipLaunchKernelGGL((k_stencil_dot<float, NR>), dim3(tl.nblocks), dim3(kBlock), 0, s->stream,
+                       s->p32, s->q32, s->c32[0], s->c32[1], s->c32[2], s->d, tl, va, 1, s->sc,
+                       s->partials, nullptr);
 }
-void launch_stencil_only(fs_sys* s, int nr, const Tiling& tl) {
-  FS_DISPATCH_NR(nr, launch_stencil_only_t, s, nr, tl);
+void launch_stencil_only(fs_sys* s, int nr, const Tiling& tl, bool x4) {
+  FS_DISPATCH_NR(nr, launch_stencil_only_t, s, nr, tl, x4);
 }
 
 void launch_cycle_start(fs_sys* s, int nr, double inner_tol2) {

@@ -37,8 +37,9 @@ void launch_cycle_start(fs_sys* s, int nr, double inner_tol2);
 // flexible CG: z32 = M(r32) and <z,r> at the start of an FP32 cycle
 int launch_fcg_start(fs_sys* s, int nr);
 void launch_refresh(fs_sys* s, int nr);
-// one FP32 multi-RHS stencil launch (p32 -> q32) on tiling tl (autotuning)
-void launch_stencil_only(fs_sys* s, int nr, const Tiling& tl);
+// one FP32 multi-RHS stencil launch (p32 -> q32) on tiling tl (autotuning);
+// x4: the 4-nodes-per-lane kernel (tl in units of 256-node row segments)
+void launch_stencil_only(fs_sys* s, int nr, const Tiling& tl, bool x4);
 void launch_charge(fs_sys* s, int nr, const fs_op* L, const double* phi, const int64_t* nodes,
                    const int64_t* offs, int nbins, int nbx, double* out);
 
