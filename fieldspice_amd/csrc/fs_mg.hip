@@ -539,6 +539,255 @@ __global__ __launch_bounds__(kBlock) void k_restrict_l0(
 }
 
 // -------------------------------------------------------------------------
+// level 0 with 4 consecutive x nodes per lane (rows a multiple of 4): one
+// 16-byte load per array and plane, every right-hand side advanced in the
+// same z-march so couplings and masks are read once per node
+// -------------------------------------------------------------------------
+__device__ __forceinline__ float4 f4(float a, float b, float c, float d) {
+  return make_float4(a, b, c, d);
+}
+__device__ __forceinline__ float4 ldf4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
+    const float* __restrict__ r, const float* __restrict__ ec, int64_t vsc,
+    float* __restrict__ e, const float* __restrict__ cx, const float* __restrict__ cy,
+    const float* __restrict__ cz, const uint8_t* __restrict__ mask, Dims d, Tiling tl, Map m,
+    int64_t vs, int nr, float omega, const float* __restrict__ q,
+    double* __restrict__ partials, const CgScalars* sc, int mode) {
+  double acc[2 * NR];
+#pragma unroll
+  for (int a = 0; a < 2 * NR; ++a) acc[a] = 0.0;
+  bool live[NR];
+#pragma unroll
+  for (int b = 0; b < NR; ++b) live[b] = b < nr && live_of(sc, b, mode);
+  int tx, ty, tz;
+  if (tile_of_block(tl, tx, ty, tz)) {
+    const int lane = (int)(threadIdx.x % kWave);
+    const int i = (tx * kWave + lane) * 4;
+    const int j = ty * kTileY + (int)(threadIdx.x / kWave);
+    const int k0 = tz * tl.kz;
+    const int k1 = min(d.nz, k0 + tl.kz);
+    const bool valid = i < d.nx && j < d.ny;
+    const int64_t base = valid ? (int64_t)j * d.nx + i : 0;
+    const bool has_w = i > 0, has_e = i + 4 < d.nx, has_n = j + 1 < d.ny, has_s = j > 0;
+    const int cnx = m.cn[0], cny = m.cn[1];
+    int ci[4] = {0, 0, 0, 0}, ciw = 0, cie = 0, cj = 0, cjn = 0, cjs = 0;
+    if (valid) {
+      for (int a = 0; a < 4; ++a) ci[a] = m.f2c[0][i + a];
+      ciw = has_w ? m.f2c[0][i - 1] : 0;
+      cie = has_e ? m.f2c[0][i + 4] : 0;
+      cj = m.f2c[1][j];
+      cjn = has_n ? m.f2c[1][j + 1] : cj;
+      cjs = has_s ? m.f2c[1][j - 1] : cj;
+    }
+    // z2 = omega r + ec[P(n)] of the 4 nodes of row (jj, plane ck)
+    auto z2row = [&](const float* eb, float4 rr, int64_t rowbase) {
+      return f4(omega * rr.x + eb[rowbase + ci[0]], omega * rr.y + eb[rowbase + ci[1]],
+                omega * rr.z + eb[rowbase + ci[2]], omega * rr.w + eb[rowbase + ci[3]]);
+    };
+    int64_t nn = (int64_t)k0 * d.sxy + base;
+    const float4 z4 = f4(0.f, 0.f, 0.f, 0.f);
+    float4 zm[NR], zc[NR], rc[NR];
+    int ck = valid ? m.f2c[2][k0] : 0;
+    float4 czm = z4;
+    if (valid && k0 > 0) czm = ldf4(cz + nn - d.sxy);
+#pragma unroll
+    for (int b = 0; b < NR; ++b) {
+      zm[b] = z4;
+      zc[b] = z4;
+      rc[b] = z4;
+      if (valid && live[b]) {
+        const float* rb = r + b * vs;
+        const float* eb = ec + b * vsc;
+        if (k0 > 0)
+          zm[b] = z2row(eb, ldf4(rb + nn - d.sxy), ((int64_t)m.f2c[2][k0 - 1] * cny + cj) * cnx);
+        rc[b] = ldf4(rb + nn);
+        zc[b] = z2row(eb, rc[b], ((int64_t)ck * cny + cj) * cnx);
+      }
+    }
+    for (int k = k0; k < k1; ++k, nn += d.sxy) {
+      const bool up = k + 1 < d.nz;
+      const int cku = (valid && up) ? m.f2c[2][k + 1] : ck;
+      float4 cxc = z4, cyc = z4, czc = z4, cym = z4;
+      uchar4 mk = make_uchar4(1, 1, 1, 1);
+      if (valid) {
+        cxc = ldf4(cx + nn);
+        cyc = ldf4(cy + nn);
+        czc = ldf4(cz + nn);
+        if (has_s) cym = ldf4(cy + nn - d.nx);
+        mk = *reinterpret_cast<const uchar4*>(mask + nn);
+      }
+      const float cxw_l = __shfl_up(cxc.w, 1, kWave);
+      float cxw = 0.f;
+      if (valid && has_w) cxw = lane > 0 ? cxw_l : cx[nn - 1];
+      const int64_t rowc = ((int64_t)ck * cny + cj) * cnx;
+#pragma unroll
+      for (int b = 0; b < NR; ++b) {
+        if (!live[b]) continue;  // uniform over the launch
+        const float* __restrict__ rb = r + b * vs;
+        const float* __restrict__ eb = ec + b * vsc;
+        float4 ru = z4, zu = z4, zn = z4, zs = z4;
+        if (valid) {
+          if (up) {
+            ru = ldf4(rb + nn + d.sxy);
+            zu = z2row(eb, ru, ((int64_t)cku * cny + cj) * cnx);
+          }
+          if (has_n) zn = z2row(eb, ldf4(rb + nn + d.nx), ((int64_t)ck * cny + cjn) * cnx);
+          if (has_s) zs = z2row(eb, ldf4(rb + nn - d.nx), ((int64_t)ck * cny + cjs) * cnx);
+        }
+        const float4 c = zc[b];
+        float zw = __shfl_up(c.w, 1, kWave), ze = __shfl_down(c.x, 1, kWave);
+        if (valid) {
+          if (!has_w) zw = 0.f;
+          else if (lane == 0) zw = omega * rb[nn - 1] + eb[rowc + ciw];
+          if (!has_e) ze = 0.f;
+          else if (lane == kWave - 1) ze = omega * rb[nn + 4] + eb[rowc + cie];
+        }
+        const float4 mm = zm[b], rr = rc[b];
+        const float s0 = cxc.x * c.y + cxw * zw + cyc.x * zn.x + cym.x * zs.x + czc.x * zu.x + czm.x * mm.x;
+        const float s1 = cxc.y * c.z + cxc.x * c.x + cyc.y * zn.y + cym.y * zs.y + czc.y * zu.y + czm.y * mm.y;
+        const float s2 = cxc.z * c.w + cxc.y * c.y + cyc.z * zn.z + cym.z * zs.z + czc.z * zu.z + czm.z * mm.z;
+        const float s3 = cxc.w * ze + cxc.z * c.z + cyc.w * zn.w + cym.w * zs.w + czc.w * zu.w + czm.w * mm.w;
+        float4 out;
+        out.x = mk.x == 0 ? c.x + omega * (rr.x - c.x + s0) : 0.f;
+        out.y = mk.y == 0 ? c.y + omega * (rr.y - c.y + s1) : 0.f;
+        out.z = mk.z == 0 ? c.z + omega * (rr.z - c.z + s2) : 0.f;
+        out.w = mk.w == 0 ? c.w + omega * (rr.w - c.w + s3) : 0.f;
+        if (valid) {
+          *reinterpret_cast<float4*>(e + b * vs + nn) = out;
+          if (partials) {
+            const float4 qq = ldf4(q + b * vs + nn);
+            acc[b] += (double)out.x * rr.x + (double)out.y * rr.y + (double)out.z * rr.z +
+                      (double)out.w * rr.w;
+            acc[NR + b] += (double)out.x * qq.x + (double)out.y * qq.y + (double)out.z * qq.z +
+                           (double)out.w * qq.w;
+          }
+        }
+        zm[b] = c;
+        zc[b] = zu;
+        rc[b] = ru;
+      }
+      czm = czc;
+      ck = cku;
+    }
+  }
+  if (!partials) return;  // uniform over the launch
+  block_sum<2 * NR>(acc);
+  if (threadIdx.x == 0) {
+    for (int b = 0; b < nr; ++b) {
+      store_partial(partials + (int64_t)b * gridDim.x + blockIdx.x, acc[b]);
+      store_partial(partials + (int64_t)(nr + b) * gridDim.x + blockIdx.x, acc[NR + b]);
+    }
+  }
+}
+
+// rc = P^T (r - omega A r) with 4 nodes per lane: x pairs {2I, 2I+1} sum
+// inside the lane, y pairs through LDS at the end of each z-group
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_restrict_l0x4(
+    const float* __restrict__ r, float* __restrict__ rc, int64_t vsc,
+    const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
+    Dims d, Tiling tl, const int* __restrict__ kstart, Map m, int64_t vs, int nr, float omega,
+    const CgScalars* sc, int mode) {
+  __shared__ float2 lds[NR][kTileY][kWave];
+  int tx, ty, tz;
+  if (!tile_of_block(tl, tx, ty, tz)) return;  // uniform over the block
+  bool live[NR];
+#pragma unroll
+  for (int b = 0; b < NR; ++b) live[b] = b < nr && live_of(sc, b, mode);
+  const int lane = (int)(threadIdx.x % kWave), ly = (int)(threadIdx.x / kWave);
+  const int i = (tx * kWave + lane) * 4;
+  const int j = ty * kTileY + ly;
+  const int k0 = kstart[tz], k1 = kstart[tz + 1];
+  const bool valid = i < d.nx && j < d.ny;
+  const int64_t base = valid ? (int64_t)j * d.nx + i : 0;
+  const bool has_w = i > 0, has_e = i + 4 < d.nx, has_n = j + 1 < d.ny, has_s = j > 0;
+  const int cnx = m.cn[0], cny = m.cn[1];
+  const float4 z4 = f4(0.f, 0.f, 0.f, 0.f);
+  int64_t nn = (int64_t)k0 * d.sxy + base;
+  float4 rm[NR], rcur[NR];
+  float2 col[NR];
+  float4 czm = z4;
+  if (valid && k0 > 0) czm = ldf4(cz + nn - d.sxy);
+#pragma unroll
+  for (int b = 0; b < NR; ++b) {
+    rm[b] = z4;
+    rcur[b] = z4;
+    col[b] = make_float2(0.f, 0.f);
+    if (valid && live[b]) {
+      if (k0 > 0) rm[b] = ldf4(r + b * vs + nn - d.sxy);
+      rcur[b] = ldf4(r + b * vs + nn);
+    }
+  }
+  for (int k = k0; k < k1; ++k, nn += d.sxy) {
+    const int ck = m.f2c[2][k];
+    const bool up = k + 1 < d.nz;
+    float4 cxc = z4, cyc = z4, czc = z4, cym = z4;
+    if (valid) {
+      cxc = ldf4(cx + nn);
+      cyc = ldf4(cy + nn);
+      czc = ldf4(cz + nn);
+      if (has_s) cym = ldf4(cy + nn - d.nx);
+    }
+    const float cxw_l = __shfl_up(cxc.w, 1, kWave);
+    float cxw = 0.f;
+    if (valid && has_w) cxw = lane > 0 ? cxw_l : cx[nn - 1];
+#pragma unroll
+    for (int b = 0; b < NR; ++b) {
+      if (!live[b]) continue;  // uniform over the block
+      const float* __restrict__ rb = r + b * vs;
+      float4 ru = z4, rn = z4, rs = z4;
+      if (valid) {
+        if (up) ru = ldf4(rb + nn + d.sxy);
+        if (has_n) rn = ldf4(rb + nn + d.nx);
+        if (has_s) rs = ldf4(rb + nn - d.nx);
+      }
+      const float4 c = rcur[b];
+      float rw = __shfl_up(c.w, 1, kWave), re = __shfl_down(c.x, 1, kWave);
+      if (valid) {
+        if (!has_w) rw = 0.f;
+        else if (lane == 0) rw = rb[nn - 1];
+        if (!has_e) re = 0.f;
+        else if (lane == kWave - 1) re = rb[nn + 4];
+      }
+      const float4 mm = rm[b];
+      const float s0 = cxc.x * c.y + cxw * rw + cyc.x * rn.x + cym.x * rs.x + czc.x * ru.x + czm.x * mm.x;
+      const float s1 = cxc.y * c.z + cxc.x * c.x + cyc.y * rn.y + cym.y * rs.y + czc.y * ru.y + czm.y * mm.y;
+      const float s2 = cxc.z * c.w + cxc.y * c.y + cyc.z * rn.z + cym.z * rs.z + czc.z * ru.z + czm.z * mm.z;
+      const float s3 = cxc.w * re + cxc.z * c.z + cyc.w * rn.w + cym.w * rs.w + czc.w * ru.w + czm.w * mm.w;
+      if (valid) {
+        const float w1 = 1.f - omega;
+        col[b].x += (w1 * c.x + omega * s0) + (w1 * c.y + omega * s1);
+        col[b].y += (w1 * c.z + omega * s2) + (w1 * c.w + omega * s3);
+      }
+      rm[b] = c;
+      rcur[b] = ru;
+    }
+    czm = czc;
+    const bool group_end = k + 1 == k1 || m.f2c[2][k + 1] != ck;  // uniform
+    if (group_end) {
+#pragma unroll
+      for (int b = 0; b < NR; ++b) lds[b][ly][lane] = col[b];
+      __syncthreads();
+      if ((ly & 1) == 0 && valid) {
+#pragma unroll
+        for (int b = 0; b < NR; ++b) {
+          if (!live[b]) continue;
+          const float2 w = col[b], v = lds[b][ly + 1][lane];
+          float* out = rc + b * vsc + ((int64_t)ck * cny + (j >> 1)) * cnx + (i >> 1);
+          out[0] = w.x + v.x;
+          if ((i >> 1) + 1 < cnx) out[1] = w.y + v.y;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < NR; ++b) col[b] = make_float2(0.f, 0.f);
+    }
+  }
+}
+
+// -------------------------------------------------------------------------
 // launchers
 // -------------------------------------------------------------------------
 void jacobi_norm(hipStream_t s, const Level& L, const float* r, int nr, double* partials,
@@ -555,6 +804,18 @@ void energy_grad(hipStream_t s, const Level& L, const float* y, const float* r, 
 int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, int64_t vsc,
                float* e, int nr, float omega, const float* q, double* partials,
                const CgScalars* sc, int mode) {
+  if (a.x4) {
+#define FS_L0P4(NR_)                                                                         \
+  hipLaunchKernelGGL((k_prolong_l0x4<NR_>), dim3(a.tl4.nblocks), dim3(kBlock), 0, s, r, ec, vsc, \
+                     e, a.c[0], a.c[1], a.c[2], a.mask, a.d, a.tl4, a.m, a.vs, nr, omega, q,     \
+                     partials, sc, mode)
+    if (nr <= 1) FS_L0P4(1);
+    else if (nr <= 2) FS_L0P4(2);
+    else if (nr <= 4) FS_L0P4(4);
+    else FS_L0P4(8);
+#undef FS_L0P4
+    return a.tl4.nblocks;
+  }
 #define FS_L0P(NR_)                                                                        \
   hipLaunchKernelGGL((k_prolong_l0<NR_>), dim3(a.tl.nblocks), dim3(kBlock), 0, s, r, ec, vsc, e, \
                      a.c[0], a.c[1], a.c[2], a.mask, a.d, a.tl, a.m, a.vs, nr, omega, q,     \
@@ -569,6 +830,17 @@ int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, 
 
 void restrict_l0(hipStream_t s, const L0Args& a, const float* r, float* rc, int64_t vsc, int nr,
                  float omega, const CgScalars* sc, int mode) {
+  if (a.x4) {
+#define FS_L0R4(NR_)                                                                           \
+  hipLaunchKernelGGL((k_restrict_l0x4<NR_>), dim3(a.tlr4.nblocks), dim3(kBlock), 0, s, r, rc, vsc, \
+                     a.c[0], a.c[1], a.c[2], a.d, a.tlr4, a.kstart, a.m, a.vs, nr, omega, sc, mode)
+    if (nr <= 1) FS_L0R4(1);
+    else if (nr <= 2) FS_L0R4(2);
+    else if (nr <= 4) FS_L0R4(4);
+    else FS_L0R4(8);
+#undef FS_L0R4
+    return;
+  }
   hipLaunchKernelGGL((k_restrict_l0<8>), dim3(a.tlr.nblocks), dim3(kBlock), 0, s, r, rc, vsc,
                      a.c[0], a.c[1], a.c[2], a.d, a.tlr, a.kstart, a.m, a.vs, nr, omega, sc, mode);
 }

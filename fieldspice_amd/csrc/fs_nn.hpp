@@ -123,6 +123,8 @@ struct L0Args {
   const uint8_t* mask;
   Tiling tl;            // prolongation tiles (the system's)
   Tiling tlr;           // restriction tiles
+  int x4;               // rows a multiple of 4 (and x pairs): 4 nodes per lane
+  Tiling tl4, tlr4;     // the same tiles in units of 256-node row segments
   const int* kstart;    // tlr.tiles_z + 1 z-starts, each a z-group start
   Map m;                // level 0 -> 1
 };

@@ -780,6 +780,18 @@ int fs_pc_create(fs_sys* sys, fs_net* net, int nlevels, const int32_t* groups, d
           hipSuccess)
         return fail_pc(fail(FS_EHIP, "hierarchy upload failed"));
       a.kstart = pc->kstart;
+      // 4 nodes per lane when rows are a multiple of 4 (x pairs never cross a lane)
+      a.x4 = sys->d.nx % 4 == 0 ? 1 : 0;
+      if (a.x4) {
+        a.tl4 = sys->tl4;
+        a.tlr4 = sys->tl4;
+        a.tlr4.tiles_z = a.tlr.tiles_z;
+        a.tlr4.ntiles = a.tlr4.tiles_x * a.tlr4.tiles_y * a.tlr4.tiles_z;
+        a.tlr4.nblocks = (a.tlr4.ntiles + 7) / 8 * 8;
+        const int64_t need = (int64_t)std::max(a.tl4.nblocks, a.tlr4.nblocks) * fs::kMaxRhs * 2 +
+                             64 * (fs::kMaxRhs + 1) * fs::kMaxRhs;
+        if (need > sys->partials_cap) a.x4 = 0;  // keep within the system's partials
+      }
     }
   }
   const mg::Level& B = pc->lv[nlevels];
