@@ -77,6 +77,19 @@ def config_for(method):
     return SolverConfig(linear_solver=method, tol=1e-10)
 
 
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of the stencil from the committed counter profile
+    (two rocprofv3 --pmc passes calibrated on a streaming kernel), when this
+    run is the profiled configuration; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "stencil_traffic_256cube.md")
+    if kernel != "k_stencil_dot4<8>" or n != 256 ** 3 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("TRAFFIC_STENCIL_BYTES_PER_LAUNCH"):
+            return int(line.split("=")[1])
+    return None
+
+
 def run_method(ex, L, method, steps, warmup):
     from fieldspice_amd.solvers.base import check_terminals
     from fieldspice_amd.solvers.linear import GridLinearSystem
@@ -298,11 +311,14 @@ def main():
     bytes_per_launch = pr["stencil_bytes"] / max(pr["stencil_launches"], 1)
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
     fp64 = head == "hip_pcg"
+    kernel = ("k_stencil_dot<double,8>" if fp64 else
+              "k_stencil_dot4<8>" if g.shape_nodes[0] % 4 == 0 else "k_stencil_dot<float,8>")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": ("k_stencil_dot<double,8>" if fp64 else
-                           "k_stencil_dot4<8>" if g.shape_nodes[0] % 4 == 0 else
-                           "k_stencil_dot<float,8>"),
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(kernel, g.n_nodes),
+                "traffic_source": "profiles/stencil_traffic_256cube.md (FETCH_SIZE/WRITE_SIZE, "
+                                  "calibrated on k_pupdate)",
+                "kernel": kernel,
                 "avg_launch_us": round(avg_s * 1e6, 2),
                 "algorithmic_bytes_per_launch": int(bytes_per_launch)}
     cpu = None

@@ -656,7 +656,7 @@ int fs_sys_create(fs_op* op, const uint8_t* mask, int pin, fs_sys** out, fs_sys_
     t4.tiles_x = (d.nx / 4 + fs::kWave - 1) / fs::kWave;
     t4.tiles_y = (d.ny + fs::kTileY - 1) / fs::kTileY;
     const int64_t t4xy = (int64_t)t4.tiles_x * t4.tiles_y;
-    t4.kz = (int)std::min<int64_t>(16, std::max<int64_t>(4, (d.nz * t4xy + 2047) / 2048));
+    t4.kz = (int)std::min<int64_t>(16, std::max<int64_t>(4, (d.nz * t4xy + 1023) / 1024));
     t4.kz = std::max(1, std::min(t4.kz, d.nz));
     t4.tiles_z = (d.nz + t4.kz - 1) / t4.kz;
     t4.ntiles = (int)(t4xy * t4.tiles_z);
