@@ -54,7 +54,8 @@ def _parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("c4", "c2", "c5"), default="c4")
+    ap.add_argument("--workload", choices=("c4", "c2", "c5", "c3"), default="c4")
+    ap.add_argument("--nsteps", type=int, default=200, help="c3: time steps")
     ap.add_argument("--instances", type=int, default=64, help="c5: number of systems")
     ap.add_argument("--chunk", type=int, default=4, help="c5: iterations per graph replay")
     ap.add_argument("--n", type=int, default=256, help="c4: nodes per side")
@@ -263,10 +264,64 @@ def run_c5(args):
     print(json.dumps(out), flush=True)
 
 
+def run_c3(args):
+    """EQS backward-Euler transient, 256^3 two-phase copper/SiO2 (c3): time
+    per step and the per-step potential error of each method against the FP64
+    run (stored every 20 steps)."""
+    from fieldspice_amd import workloads
+    from fieldspice_amd.precond import LearnedPreconditioner
+    from fieldspice_amd.solvers.base import SolverConfig
+    from fieldspice_amd.solvers.eqs import EQSSolver
+    seed = 0 if args.seed is None else args.seed
+    n = args.n
+    log(f"building {n}^3 two-phase EQS geometry (seed {seed})")
+    g, eps, sigma, terms = workloads.eqs_two_phase(n, seed=seed)
+    dt, nsteps, every = 1e-12, args.nsteps, 20
+    out_m = {}
+    ref = None
+    for m in args.methods.split(","):
+        cfg = config_for(m)
+        cfg.tol = 1e-10
+        sol = EQSSolver(g, eps, sigma, cfg)
+        t0 = time.perf_counter()
+        r = sol.solve(terms, t_end=nsteps * dt, dt=dt, store=True, store_every=every)
+        wall = time.perf_counter() - t0
+        phi = r.fields["phi"].reshape(r.fields["phi"].shape[0], -1)
+        lin = r.meta.get("linear", {})
+        rec = {"wall_s": wall, "ms_per_step": 1e3 * wall / nsteps,
+               "iterations_total": lin.get("iterations"), "fallbacks": lin.get("fallbacks")}
+        if ref is None:
+            ref = phi
+        err = [float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)) for a, b in zip(phi, ref)]
+        rec["phi_rel_err_vs_fp64_max"] = max(err)
+        rec["phi_rel_err_vs_fp64_per_stored_step"] = err
+        out_m[m] = rec
+        log(f"  {m}: {wall:.2f} s ({rec['ms_per_step']:.2f} ms/step), iterations "
+            f"{rec['iterations_total']}, max rel err vs FP64 {max(err):.2e}")
+        del sol, r
+    head = "hip_mixed" if "hip_mixed" in out_m else list(out_m)[0]
+    fp = out_m.get("hip_pcg")
+    out = {
+        "metric": "s/step of the EQS implicit transient to 1e-10 FP64 residual; per-step error vs FP64",
+        "value": out_m[head]["ms_per_step"] / 1e3, "unit": "s/step", "n_gpus": 1,
+        "steps": nsteps, "warmup": 0, "ms_per_step": out_m[head]["ms_per_step"],
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"EQS backward Euler, {n}^3 nodes at 10 nm, copper boxes 20% "
+                               "volume in SiO2, 1 GHz 1 V drive, one grounded box, dt 1e-12 s",
+                   "seed": seed, "method": head, "steps": nsteps},
+        "methods": out_m,
+        "speedup_vs_fp64": {m: fp["wall_s"] / r["wall_s"] for m, r in out_m.items()} if fp else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = _parse()
     if args.workload == "c5" and args.gpus == 1:
         return run_c5(args)
+    if args.workload == "c3" and args.gpus == 1:
+        return run_c3(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(sharding is described in DESIGN.md, not implemented)"}))

@@ -20,7 +20,7 @@ from .solvers.base import Terminal
 from .units import eps0
 
 __all__ = ["Extraction", "parallel_plate", "layered_boxes", "c2_geometry", "multi_conductor",
-           "batch_instances"]
+           "batch_instances", "eqs_two_phase"]
 
 
 @dataclass
@@ -154,3 +154,48 @@ def batch_instances(count: int = 64, seed: int = 0, n_min: int = 32 ** 3, n_max:
                              side=(3, max(4, m // 4)))
         out.append(ex)
     return out
+
+
+def eqs_two_phase(n: int = 256, seed: int = 0, h: float = 10e-9, fill: float = 0.2,
+                  f0: float = 1e9, side=None, max_tries: int = 100000):
+    """Seeded two-phase EQS problem on a uniform n^3-node grid: copper boxes
+    (sigma 5.8e7 S/m, eps_r 3.9) placed at random until they fill ``fill`` of
+    the volume, in SiO2 (sigma 0, eps_r 3.9).  Two boxes with a free gap are
+    the terminals: "drv" at sin(2 pi f0 t) V and "gnd" at 0 V; the other
+    copper floats.  Returns (grid, eps_cell, sigma_cell, [drv, gnd])."""
+    rng = np.random.default_rng(seed)
+    N = n - 1
+    g = RectilinearGrid.uniform([(0, N * h)] * 3, (N, N, N))
+    cu = np.zeros(g.shape_cells, dtype=bool)
+    lo, hi = side if side is not None else (max(3, N // 24), max(4, N // 8))
+    boxes = []
+    tries = 0
+    while cu.mean() < fill:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("could not reach the copper fill")
+        s = [int(rng.integers(lo, hi + 1)) for _ in range(3)]
+        o = [int(rng.integers(1, N - sd)) for sd in s]
+        cu[o[0]:o[0] + s[0], o[1]:o[1] + s[1], o[2]:o[2] + s[2]] = True
+        boxes.append((o, s))
+    # terminals: two boxes whose closed node sets (plus a one-cell gap) are disjoint
+    nid = g.node_ids()
+    def nodes_of(b):
+        (i, j, k), (a, bb, c) = b
+        return nid[i:i + a + 1, j:j + bb + 1, k:k + c + 1].ravel()
+    drv_box = boxes[0]
+    gnd_box = None
+    for b in boxes[1:]:
+        (i, j, k), (a, bb, c) = b
+        (I, J, K), (A, B, C) = drv_box
+        if (i > I + A + 1 or I > i + a + 1 or j > J + B + 1 or J > j + bb + 1 or
+                k > K + C + 1 or K > k + c + 1):
+            gnd_box = b
+            break
+    if gnd_box is None:
+        raise RuntimeError("no disjoint ground box")
+    eps = np.full(g.shape_cells, 3.9 * eps0)
+    sigma = np.where(cu, 5.8e7, 0.0)
+    drv = Terminal("drv", nodes_of(drv_box), voltage=lambda t: float(np.sin(2 * np.pi * f0 * t)))
+    gnd = Terminal("gnd", nodes_of(gnd_box), voltage=0.0)
+    return g, eps, sigma, [drv, gnd]
