@@ -39,6 +39,7 @@ EXPORTS = (
     "fs_sys_set_precond", "fs_pc_apply", "fs_pc_add_pairs", "fs_pc_num_pairs",
     "fs_pc_clear_pairs", "fs_pc_loss_grad", "fs_pc_train", "fs_sys_autotune",
     "fs_batch_create", "fs_batch_destroy", "fs_batch_add", "fs_batch_solve",
+    "fs_sys_find_islands", "fs_sys_set_deflation", "fs_sys_get_deflation",
 )
 
 
@@ -139,6 +140,9 @@ def _declare(lib):
     lib.fs_batch_add.argtypes = [_vp, _vp, _vp]
     lib.fs_batch_solve.argtypes = [_vp, C.POINTER(SolveParams), C.c_int, _dp, _vp, _vp,
                                    C.POINTER(SolveStats)]
+    lib.fs_sys_find_islands.argtypes = [_vp, C.c_double, C.c_int, _i32p, C.POINTER(C.c_int)]
+    lib.fs_sys_set_deflation.argtypes = [_vp, C.c_int, _vp]
+    lib.fs_sys_get_deflation.argtypes = [_vp, C.POINTER(C.c_int), _vp]
     for name in EXPORTS:
         if name != "fs_last_error":
             getattr(lib, name).restype = C.c_int
@@ -402,6 +406,38 @@ class DeviceSystem:
             raise ValueError("the preconditioner was built for another system")
         check(load().fs_sys_set_precond(self._h, None if pc is None else pc._h),
               "fs_sys_set_precond")
+
+    def find_islands(self, contrast: float = 1e3, max_islands: int = 1024):
+        """Floating high-conductance components of the operator (node ids in
+        C order, -1 outside); returns (ids, count)."""
+        ids = np.empty(self.op.n_nodes, np.int32)
+        n = C.c_int(0)
+        check(load().fs_sys_find_islands(self._h, float(contrast), int(max_islands), ids,
+                                         C.byref(n)), "fs_sys_find_islands")
+        return ids, n.value
+
+    def set_deflation(self, ids=None, n_islands: int = 0) -> None:
+        """Attach the island coarse space (ids from find_islands), or remove it
+        (no ids)."""
+        if ids is None or n_islands == 0:
+            check(load().fs_sys_set_deflation(self._h, 0, None), "fs_sys_set_deflation")
+            return
+        a = np.ascontiguousarray(ids, dtype=np.int32).ravel()
+        if a.size != self.op.n_nodes:
+            raise ValueError("island ids must have one entry per node")
+        check(load().fs_sys_set_deflation(self._h, int(n_islands), a.ctypes.data_as(_vp)),
+              "fs_sys_set_deflation")
+
+    def deflation(self):
+        """(number of islands, E^-1 or None)."""
+        n = C.c_int(0)
+        check(load().fs_sys_get_deflation(self._h, C.byref(n), None), "fs_sys_get_deflation")
+        if n.value == 0:
+            return 0, None
+        einv = np.empty((n.value, n.value))
+        check(load().fs_sys_get_deflation(self._h, C.byref(n), einv.ctypes.data_as(_vp)),
+              "fs_sys_get_deflation")
+        return n.value, einv
 
     def close(self):
         if getattr(self, "_h", None):

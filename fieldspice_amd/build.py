@@ -18,7 +18,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB_NAME = "libfieldspice_hip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 SOURCES = ["fs_kernels.hip", "fs_transient.hip", "fs_mg.hip", "fs_nn.hip", "fs_pc.hip",
-           "fs_api.hip"]
+           "fs_defl.hip", "fs_api.hip"]
 ARCH = os.environ.get("FIELDSPICE_OFFLOAD_ARCH", "gfx950")
 
 

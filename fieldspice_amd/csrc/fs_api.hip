@@ -135,6 +135,7 @@ void free_sys(fs_sys* s) {
   if (!s) return;
   for (fs_pc* pc : s->pcs) fs::pc_orphan(pc);
   s->pcs.clear();
+  fs::free_defl(s);
   free_sys_ws(s);
   dfree(s->mask);
   dfree(s->dhalf);
@@ -230,8 +231,12 @@ int run_krylov_mode(fs_sys* s, int nr, const fs_solve_params& P, int mode) {
     FS_HIP(hipGetLastError());
     return FS_OK;
   };
-  if (cyc)
+  if (cyc) {
     if (int rc = start_cycle()) return rc;
+  } else if (s->defl_n > 0) {
+    fs::launch_defl_coef(s, nr, 0, 1);  // first search direction (I + Z E^-1 Z^T) r
+    FS_HIP(hipGetLastError());
+  }
   // a flexible-CG iteration costs several stencil applications: check often
   const int max_chunk = mode == 2 ? 32 : kMaxChunk;
   const int first_chunk = mode == 2 ? 4 : 16;
@@ -302,6 +307,9 @@ int run_krylov(fs_sys* s, int nr, const fs_solve_params& P) {
   if (P.precision != FS_PREC_FCG)
     return run_krylov_mode(s, nr, P, P.precision == FS_PREC_MIXED ? 1 : 0);
   if (!s->pc) return fail(FS_EINVAL, "FS_PREC_FCG needs a preconditioner (fs_sys_set_precond)");
+  if (s->defl_n > 0)
+    return fail(FS_EINVAL, "FS_PREC_FCG does not combine with an island space; use "
+                           "FS_PREC_FP64 or FS_PREC_MIXED");
   if (int rc = run_krylov_mode(s, nr, P, 2)) return rc;
   // plain FP64 CG redoes the batch when flexible CG broke down or stalled
   bool ok = true;
@@ -1361,6 +1369,7 @@ int fs_batch_add(fs_batch* b, fs_cap* cap, const fs_op* L) {
     return fail(FS_EINVAL, "L and the system differ in grid");
   for (const auto& in : b->inst)
     if (in.cap->sys == s) return fail(FS_EINVAL, "system already in the batch");
+  if (s->defl_n > 0) return fail(FS_EINVAL, "systems with an island space are not batched");
   b->inst.push_back({cap, L});
   return (int)b->inst.size() - 1 >= 0 ? FS_OK : FS_EINVAL;
   FS_GUARD_END

@@ -18,6 +18,8 @@ constexpr int kBlock = 256;       // threads per block for every kernel here
 constexpr int kMaxRhs = 8;        // right-hand sides one launch carries
 constexpr int kTileX = 64;        // stencil tile: one wavefront along x ...
 constexpr int kTileY = 4;         // ... times four rows
+constexpr int kMaxIslands = 1024; // island coarse space: islands per system
+constexpr int kIslChunk = 2048;   // island nodes per reduction block
 
 // Device layout: node (i, j, k) of an (nx, ny, nz) node grid lives at
 // n = (k*ny + j)*nx + i (x fastest).  Every node vector of a system is padded
@@ -122,6 +124,22 @@ struct fs_sys {
   std::vector<fs_pc*> pcs;    // every preconditioner built on this system
   int acc_fallbacks = 0;      // counters of the current solve call
   int acc_pc_applies = 0;
+  // island coarse space (fs_sys_set_deflation): Z = D^1/2 1_I per island I
+  // in the equilibrated space, E = Z^T A_s Z
+  int defl_n = 0;                  // islands (0: off)
+  int defl_nchunks = 0;            // chunks of the island node list
+  int32_t* defl_nodes = nullptr;   // island nodes (x-fast), grouped by island
+  int32_t* defl_cstart = nullptr;  // chunk c covers defl_nodes[cstart[c] .. cstart[c+1])
+  int32_t* defl_cisl = nullptr;    // island of chunk c
+  int32_t* defl_ichunk = nullptr;  // chunks of island I: ichunk[I] .. ichunk[I+1]
+  double* defl_einv = nullptr;     // E^-1 (defl_n x defl_n, row major)
+  double* defl_part = nullptr;     // chunk partials [kMaxRhs][defl_nchunks]
+  double* defl_coef = nullptr;     // E^-1 Z^T r per RHS [kMaxRhs][kMaxIslands]
+  // FP32 cycles of a deflated system run on the unscaled operator in
+  // difference form: y_n = e_n u_n + sum_e w_e (u_n - u_m), u = D^-1/2 x_s
+  float* u_w[3] = {nullptr, nullptr, nullptr};  // edge weights (FP32, x-fast)
+  float* u_extra = nullptr;        // node diagonal (FP32) or null
+  float* u_dinv = nullptr;         // 1/D on free rows, 0 elsewhere
 };
 
 struct fs_cap {

@@ -605,17 +605,6 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(T* __restrict__ p, const T* 
 // (a per-block release fence writes back the whole L2 and quadruples the
 // cost of a streaming kernel on this part).
 // --------------------------------------------------------------------------
-enum FinOp : int {
-  FIN_ALPHA = 0,   // q = p.q          -> alpha = rho / (p.q)
-  FIN_BETA = 1,    // q = r_new.r_new  -> beta, rho, stop tests
-  FIN_INIT = 2,    // q = ||b_s||^2    -> fresh Krylov state
-  FIN_BNORM = 3,   // q = ||b||^2      -> aux2
-  FIN_RESID = 4,   // q0 = ||r||^2, q1 = ||r dhalf||^2 -> aux3, aux1
-  FIN_CYCLE = 5,   // q = ||r32||^2    -> FP32 cycle state
-  FIN_FCG = 6,     // q0 = <z,r>, q1 = <z,q> -> flexible-CG beta, zr
-  FIN_FCG0 = 7,    // q0 = <z,r> at the start of a cycle -> zr
-};
-
 struct FinArgs {
   int op;
   int nr;
@@ -631,7 +620,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
                                                      FinArgs fa, CgScalars* sc,
                                                      unsigned long long* prof) {
   const int r = blockIdx.x;
-  if (fa.op == FIN_ALPHA || fa.op == FIN_BETA || fa.op == FIN_FCG) {
+  if (fa.op == FIN_ALPHA || fa.op == FIN_ALPHA_ZR || fa.op == FIN_BETA || fa.op == FIN_FCG) {
     if (!rhs_live(sc, r, fa.mode)) return;  // uniform over the block
   }
   if (fa.op == FIN_FCG0 && sc[r].done != 0) return;
@@ -641,9 +630,10 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
     s1 = sum_partials(partials + (int64_t)(fa.nr + r) * fa.nb, fa.nb);
   if (threadIdx.x != 0) return;
   CgScalars& c = sc[r];
-  if (fa.op == FIN_ALPHA) {
+  if (fa.op == FIN_ALPHA || fa.op == FIN_ALPHA_ZR) {
     const double pq = s0;
-    const double rho = fa.mode == 0 ? c.rho : (fa.mode == 1 ? c.aux0 : c.zr);
+    const double rho = fa.op == FIN_ALPHA_ZR ? c.zr
+                                              : (fa.mode == 0 ? c.rho : (fa.mode == 1 ? c.aux0 : c.zr));
     c.pq = pq;
     if (!(pq > 0.0) || !isfinite(pq)) {
       c.done = 3;  // breakdown: A_s not positive definite on p
@@ -709,6 +699,12 @@ static void finalize(hipStream_t st, const double* partials, int op, int nr, int
                      int maxiter = 0, double inner_tol2 = 0.0) {
   FinArgs fa{op, nr, nb, mode, tol, maxiter, inner_tol2};
   hipLaunchKernelGGL(k_finalize, dim3(nr), dim3(kBlock), 0, st, partials, fa, sc, prof);
+}
+
+void launch_finalize(hipStream_t st, const double* partials, int op, int nr, int nb, int mode,
+                     CgScalars* sc, double inner_tol2) {
+  FinArgs fa{op, nr, nb, mode, 0.0, 0, inner_tol2};
+  hipLaunchKernelGGL(k_finalize, dim3(nr), dim3(kBlock), 0, st, partials, fa, sc, nullptr);
 }
 
 // --------------------------------------------------------------------------
@@ -1034,17 +1030,21 @@ static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEve
   const int sb = s->tl.nblocks;
   unsigned long long* prof = ea ? s->prof_dev : nullptr;
   if (mode == 0) {
+    // with an island space: p = (I + Z E^-1 Z^T) r + beta p, alpha = zr/pq
+    const bool defl = s->defl_n > 0;
     hipLaunchKernelGGL((k_pupdate<double, NR>), dim3(vb), dim3(kBlock), 0, st, s->p, s->r, va,
                        mode, s->sc);
+    if (defl) launch_defl_scatter(s, nr, mode);
     if (ea) (void)hipEventRecord(ea, st);
     hipLaunchKernelGGL((k_stencil_dot<double, NR>), dim3(sb), dim3(kBlock), 0, st, s->p, s->q,
                        s->c64[0], s->c64[1], s->c64[2], s->d, s->tl, va, mode, s->sc,
                        s->partials, prof);
     if (eb) (void)hipEventRecord(eb, st);
-    finalize<NR>(st, s->partials, FIN_ALPHA, nr, sb, mode, s->sc);
+    finalize<NR>(st, s->partials, defl ? FIN_ALPHA_ZR : FIN_ALPHA, nr, sb, mode, s->sc);
     hipLaunchKernelGGL((k_update<double, NR>), dim3(vb), dim3(kBlock), 0, st, s->x, s->r, s->p,
                        s->q, va, mode, s->sc, s->partials);
     finalize<NR>(st, s->partials, FIN_BETA, nr, vb, mode, s->sc);
+    if (defl) launch_defl_coef(s, nr, mode, 0);
     return 0;
   }
   // FP32 cycle: CG searches along r32, flexible CG along z32 = M(r32)
@@ -1085,6 +1085,7 @@ static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEve
   } while (0)
 
 int launch_iteration(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {
+  if (s->defl_n > 0 && mode == 1) return launch_iteration_defl(s, nr, mode, ea, eb);
   if (nr <= 1) return launch_iteration_t<1>(s, nr, mode, ea, eb);
   if (nr <= 2) return launch_iteration_t<2>(s, nr, mode, ea, eb);
   if (nr <= 4) return launch_iteration_t<4>(s, nr, mode, ea, eb);
@@ -1187,6 +1188,10 @@ void launch_stencil_only(fs_sys* s, int nr, const Tiling& tl, bool x4) {
 }
 
 void launch_cycle_start(fs_sys* s, int nr, double inner_tol2) {
+  if (s->defl_n > 0) {
+    launch_cycle_start_defl(s, nr, inner_tol2);
+    return;
+  }
   FS_DISPATCH_NR(nr, launch_cycle_start_t, s, nr, inner_tol2);
 }
 
@@ -1198,7 +1203,16 @@ static void launch_refresh_t(fs_sys* s, int nr) {
   launch_residual_t<NR>(s, nr, s->r);
   hipLaunchKernelGGL(k_refresh_finish, dim3(1), dim3(64), 0, s->stream, nr, s->sc);
 }
-void launch_refresh(fs_sys* s, int nr) { FS_DISPATCH_NR(nr, launch_refresh_t, s, nr); }
+void launch_refresh_finish(fs_sys* s, int nr) {
+  hipLaunchKernelGGL(k_refresh_finish, dim3(1), dim3(64), 0, s->stream, nr, s->sc);
+}
+void launch_refresh(fs_sys* s, int nr) {
+  if (s->defl_n > 0) {
+    launch_refresh_defl(s, nr);
+    return;
+  }
+  FS_DISPATCH_NR(nr, launch_refresh_t, s, nr);
+}
 
 template <int NR>
 static void launch_charge_t(fs_sys* s, int nr, const fs_op* L, const double* phi,

@@ -25,6 +25,21 @@ void apply_raw(hipStream_t s, const fs_op* op, const double* x, double* y);
 int setup_system(fs_sys* sys, double* dsum, void* counts_dev, void* counts_host, int pin);
 constexpr size_t kSetupCountsBytes = 64;
 
+// Operations of the single-block finalisation kernel (one block per RHS).
+enum FinOp : int {
+  FIN_ALPHA = 0,     // q = p.q          -> alpha = rho / (p.q)
+  FIN_BETA = 1,      // q = r_new.r_new  -> beta, rho, stop tests
+  FIN_INIT = 2,      // q = ||b_s||^2    -> fresh Krylov state
+  FIN_BNORM = 3,     // q = ||b||^2      -> aux2
+  FIN_RESID = 4,     // q0 = ||r||^2, q1 = ||r dhalf||^2 -> aux3, aux1
+  FIN_CYCLE = 5,     // q = ||r32||^2    -> FP32 cycle state
+  FIN_FCG = 6,       // q0 = <z,r>, q1 = <z,q> -> flexible-CG beta, zr
+  FIN_FCG0 = 7,      // q0 = <z,r> at the start of a cycle -> zr
+  FIN_ALPHA_ZR = 8,  // q = p.q          -> alpha = zr / (p.q) (preconditioned CG)
+};
+void launch_finalize(hipStream_t st, const double* partials, int op, int nr, int nb, int mode,
+                     CgScalars* sc, double inner_tol2 = 0.0);
+
 // mode 0 FP64 CG, 1 FP32 CG cycle, 2 FP32 flexible CG cycle (returns a status)
 int launch_iteration(fs_sys* s, int nr, int mode, hipEvent_t ev_a, hipEvent_t ev_b);
 void launch_init(fs_sys* s, int nr, double tol, int maxiter);
@@ -37,6 +52,20 @@ void launch_cycle_start(fs_sys* s, int nr, double inner_tol2);
 // flexible CG: z32 = M(r32) and <z,r> at the start of an FP32 cycle
 int launch_fcg_start(fs_sys* s, int nr);
 void launch_refresh(fs_sys* s, int nr);
+
+// island coarse space (fs_defl.hip).  Mode 0 of a system with defl_n > 0
+// adds Z E^-1 Z^T r to each search direction (launch_defl_scatter after the
+// p update, launch_defl_coef after the residual update); its FP32 cycles
+// (mode 1) run launch_iteration_defl on the unscaled difference-form operator.
+int launch_iteration_defl(fs_sys* s, int nr, int mode, hipEvent_t ev_a, hipEvent_t ev_b);
+// Z^T r -> E^-1 Z^T r and zr (first: no beta, at the start of an iteration run)
+void launch_defl_coef(fs_sys* s, int nr, int mode, int first);
+void launch_defl_scatter(fs_sys* s, int nr, int mode);
+void launch_cycle_start_defl(fs_sys* s, int nr, double inner_tol2);
+void launch_refresh_defl(fs_sys* s, int nr);
+// new rho, refresh count and stop tests after the FP64 residual of a refresh
+void launch_refresh_finish(fs_sys* s, int nr);
+void free_defl(fs_sys* s);
 // one FP32 multi-RHS stencil launch (p32 -> q32) on tiling tl (autotuning);
 // x4: the 4-nodes-per-lane kernel (tl in units of 256-node row segments)
 void launch_stencil_only(fs_sys* s, int nr, const Tiling& tl, bool x4);

@@ -49,6 +49,13 @@ class SolverConfig:
         the trained network shipped with the package.
     maxiter
         Krylov cap; ``None`` is fieldspice's ``max(500, 10*sqrt(n))``.
+    deflate_islands
+        ``hip_pcg``/``hip_mixed``: add one coarse vector per floating
+        high-conductance component ("island": a copper box in SiO2 that touches
+        no terminal) to the preconditioner.  Islands are the components of the
+        edges stronger than ``sqrt(w_min*w_max)`` when ``w_max/w_min`` exceeds
+        ``island_contrast``; without them such systems stall Jacobi-CG (and
+        FP32 cycles cannot resolve them at all).
     """
     linear_solver: str = "auto"
     tol: float = 1e-10
@@ -62,6 +69,8 @@ class SolverConfig:
     maxiter: int | None = None
     check_every: int = 0
     preconditioner: object = None
+    deflate_islands: bool = False
+    island_contrast: float = 1e3
 
     def __post_init__(self):
         if self.linear_solver not in LINEAR_SOLVERS:

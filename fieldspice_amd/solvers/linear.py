@@ -95,6 +95,16 @@ class GridLinearSystem:
             "notes": (),
         }
         self.pc = None
+        self.info["islands"] = 0
+        if self.cfg.deflate_islands:
+            if self.precision == _lib.FS_PREC_FCG:
+                raise ValueError("deflate_islands applies to hip_pcg and hip_mixed, not "
+                                 "hip_fcg")
+            ids, nisl = self.sys.find_islands(self.cfg.island_contrast)
+            if nisl:
+                self.sys.set_deflation(ids, nisl)
+            self.info["islands"] = nisl
+            self.info["setup_time"] = time.perf_counter() - t0
         if self.precision == _lib.FS_PREC_FCG:
             from ..precond import LearnedPreconditioner, uniform_widths
             lp = self.cfg.preconditioner or LearnedPreconditioner()

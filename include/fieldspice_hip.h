@@ -289,6 +289,32 @@ int fs_batch_add(fs_batch* batch, fs_cap* cap, const fs_op* L);
 int fs_batch_solve(fs_batch* batch, const fs_solve_params* params, int chunk, double* C,
                    double* inst_time, int* inst_iters, fs_solve_stats* stats);
 
+/* ---- island coarse space (high-contrast systems) -------------------------
+ * No counterpart in the reference, whose Jacobi-CG (fieldspice/solvers/
+ * poisson.py:487-589) hits its max(500, 10 sqrt(n)) cap when a conductor
+ * floats in an insulator (an EQS step, eqs.py:279-300, on copper in SiO2).
+ * Each floating conductor ("island") is a near-null mode; the island space
+ * adds one coarse vector per island, Z_I = D^1/2 1_I (equilibrated space), to
+ * the preconditioner: M = I + Z E^-1 Z^T with E = Z^T A_s Z = 1_I^T L_ff 1_J.
+ * FS_PREC_FP64 runs PCG with M; FS_PREC_MIXED runs its FP32 cycles on the
+ * unscaled operator in difference form (y_n = e_n u_n + sum_e w_e (u_n - u_m))
+ * with M.  The answer is the same linear system's; only the iteration
+ * changes.  Not with FS_PREC_FCG or fs_batch. */
+/* Islands of the system's operator: components of its strong edges (w_e >
+ * sqrt(w_min w_max) over the positive weights) with >= 2 nodes and no
+ * Dirichlet or pinned row.  None when w_max < contrast * w_min.  island: C
+ * order, -1 or the island id; at most max_islands (<= 1024) are returned,
+ * the largest ones. */
+int fs_sys_find_islands(fs_sys* sys, double contrast, int max_islands, int32_t* island,
+                        int* n_islands);
+/* Attach (n_islands >= 1) or remove (0) the island space; island: C order,
+ * -1 or an id in [0, n_islands), every island non-empty and free of
+ * Dirichlet/pinned rows.  E must be positive definite (each island coupled
+ * to a Dirichlet row through the operator). */
+int fs_sys_set_deflation(fs_sys* sys, int n_islands, const int32_t* island);
+/* number of islands attached; einv (nullable): E^-1, row major */
+int fs_sys_get_deflation(fs_sys* sys, int* n_islands, double* einv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -480,3 +480,55 @@ def thermal_transient(grid, kappa_cell, rho_cp_cell, q_fn, t_end, dt, T0,
         T = prepared[key].solve(bb)
         Ts.append(T.copy())
     return times, Ts
+
+
+# --------------------------------------------------------------------------
+# island coarse space of high-contrast systems (no reference counterpart: the
+# library's own definition, restated here as the checker; pinned by a
+# constructed case in tests/test_oracle.py)
+# --------------------------------------------------------------------------
+def floating_islands(grid, w_edge, anchored, contrast=1e3):
+    """Components of the edges with ``w > sqrt(w_min*w_max)`` (positive
+    weights) holding >= 2 nodes and no anchored (Dirichlet or pinned) node,
+    when ``w_max >= contrast*w_min``.  Returns (labels, count): labels per node
+    (C order), -1 outside an island, islands numbered by their smallest node
+    in x-fastest order (i + nx*(j + ny*k))."""
+    from scipy.sparse.csgraph import connected_components
+    n = grid.n_nodes
+    w = np.asarray(w_edge, dtype=float)
+    pos = w[w > 0]
+    labels = np.full(n, -1, np.int64)
+    if pos.size == 0 or pos.max() < contrast * pos.min():
+        return labels, 0
+    thr = np.sqrt(pos.min() * pos.max())
+    G = grad_node_edge(grid).tocoo()
+    tail = np.empty(G.shape[0], np.int64)
+    head = np.empty(G.shape[0], np.int64)
+    tail[G.row[G.data < 0]] = G.col[G.data < 0]
+    head[G.row[G.data > 0]] = G.col[G.data > 0]
+    s = w > thr
+    S = sp.coo_matrix((np.ones(int(s.sum())), (tail[s], head[s])), shape=(n, n))
+    _, comp = connected_components(S, directed=False)
+    size = np.bincount(comp)
+    anch = np.zeros(size.size, bool)
+    anch[comp[np.asarray(anchored, bool)]] = True
+    keep = (size >= 2) & ~anch
+    nx, ny, nz = grid.shape_nodes
+    i, j, k = np.unravel_index(np.arange(n), (nx, ny, nz))
+    xfast = i + nx * (j + ny * k)
+    first = np.full(size.size, np.iinfo(np.int64).max)
+    np.minimum.at(first, comp, xfast)
+    ids = np.nonzero(keep)[0]
+    ids = ids[np.argsort(first[ids])]
+    remap = np.full(size.size, -1, np.int64)
+    remap[ids] = np.arange(ids.size)
+    return remap[comp], int(ids.size)
+
+
+def island_matrix(L_free, labels, count):
+    """E = 1_I^T L_ff 1_J for the masked (Dirichlet rows and columns removed)
+    operator ``L_free``."""
+    n = L_free.shape[0]
+    rows = np.nonzero(labels >= 0)[0]
+    Z = sp.csr_matrix((np.ones(rows.size), (rows, labels[rows])), shape=(n, count))
+    return (Z.T @ L_free @ Z).toarray()

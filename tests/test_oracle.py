@@ -326,3 +326,43 @@ def test_reference_ilu_branch_is_not_a_usable_oracle():
     assert rep["linear"]["iterations"] == 500
     assert abs(C[0, 0] / 3.4531332486813394e-16 - 1) > 1e-4
     assert rep["linear"]["residual"] < 1e-15
+
+
+def test_oracle_floating_islands_on_constructed_boxes():
+    # three copper boxes in an insulator: A floats, B touches an anchored node,
+    # C (one cell) floats; islands are the closed node boxes of A and C
+    g = RectilinearGrid.uniform([(0, 14e-9), (0, 6e-9), (0, 6e-9)], (14, 6, 6))
+    sig = np.zeros(g.shape_cells)
+    sig[1:4, 1:4, 1:4] = 5.8e7
+    sig[6:9, 1:4, 1:4] = 5.8e7
+    sig[11, 2, 2] = 5.8e7
+    w = O.edge_weights(g, O.cell_to_edge(g, np.full(g.shape_cells, 3.9 * eps0))) / 1e-12 \
+        + O.edge_weights(g, O.cell_to_edge(g, sig))
+    nid = g.node_ids()
+    anchored = np.zeros(g.n_nodes, bool)
+    anchored[nid[7, 2, 2]] = True
+    lab, n = O.floating_islands(g, w, anchored)
+    assert n == 2
+    boxA = np.zeros(g.n_nodes, bool)
+    boxA[nid[1:5, 1:5, 1:5].ravel()] = True
+    boxC = np.zeros(g.n_nodes, bool)
+    boxC[nid[11:13, 2:4, 2:4].ravel()] = True
+    np.testing.assert_array_equal(lab == 0, boxA)      # A's smallest node comes first
+    np.testing.assert_array_equal(lab == 1, boxC)
+    # E_II = total weight of the edges leaving island I (brute force over edges)
+    G = O.grad_node_edge(g).tocoo()
+    ends = {}
+    for e, node, v in zip(G.row, G.col, G.data):
+        ends.setdefault(e, {})[v > 0] = node
+    L = O.nodal_laplacian(g, O.cell_to_edge(g, np.full(g.shape_cells, 3.9 * eps0))) / 1e-12 \
+        + O.nodal_laplacian(g, O.cell_to_edge(g, sig))
+    free = ~anchored
+    Lf = sp.diags(free * 1.0) @ L @ sp.diags(free * 1.0)
+    E = O.island_matrix(Lf.tocsr(), lab, n)
+    for I in range(n):
+        leaving = sum(w[e] for e, ht in ends.items() if (lab[ht[False]] == I) != (lab[ht[True]] == I))
+        assert np.isclose(E[I, I], leaving, rtol=1e-12)
+    assert np.isclose(E[0, 1], 0.0) and np.allclose(E, E.T)
+    # without contrast there are no islands
+    lab0, n0 = O.floating_islands(g, np.ones_like(w), anchored)
+    assert n0 == 0 and np.all(lab0 == -1)
