@@ -87,10 +87,11 @@ def measured_traffic(kernel, n):
     (two rocprofv3 --pmc passes calibrated on a streaming kernel), when this
     run is the profiled configuration; None otherwise."""
     path = os.path.join(ROOT, "profiles", "stencil_traffic_256cube.md")
-    if kernel != "k_stencil_dot4<8>" or n != 256 ** 3 or not os.path.exists(path):
+    if n != 256 ** 3 or not os.path.exists(path):
         return None
+    key = f"TRAFFIC_BYTES_PER_LAUNCH[{kernel}]"
     for line in open(path):
-        if line.startswith("TRAFFIC_STENCIL_BYTES_PER_LAUNCH"):
+        if line.startswith(key):
             return int(line.split("=")[1])
     return None
 
@@ -411,8 +412,11 @@ def main():
     bytes_per_launch = pr["stencil_bytes"] / max(pr["stencil_launches"], 1)
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
     fp64 = head == "hip_pcg"
-    kernel = ("k_stencil_dot<double,8>" if fp64 else
-              "k_stencil_dot4<8>" if g.shape_nodes[0] % 4 == 0 else "k_stencil_dot<float,8>")
+    nt = len(ex.terminals)
+    bucket = 1 if nt <= 1 else 2 if nt <= 2 else 4 if nt <= 4 else 8
+    kernel = (f"k_stencil_dot<double,{bucket}>" if fp64 else
+              f"k_stencil_dot4<{nt}>" if g.shape_nodes[0] % 4 == 0
+              else f"k_stencil_dot<float,{bucket}>")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(kernel, g.n_nodes),

@@ -521,8 +521,8 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot4(const float* __restrict
         y.w = c.w - (cxc.w * pe + cxc.z * c.z + cyc.w * pn.w + cym.w * ps.w + czc.w * pu.w + czm.w * m.w);
         if (valid) {
           __builtin_nontemporal_store(y, reinterpret_cast<f4n*>(q + r * va.vs + nn));
-          acc[r] += (double)c.x * (double)y.x + (double)c.y * (double)y.y +
-                    (double)c.z * (double)y.z + (double)c.w * (double)y.w;
+          // the lane's 4 products in FP32, their sum accumulated in FP64
+          acc[r] += (double)(c.x * y.x + c.y * y.y + c.z * y.z + c.w * y.w);
         }
         pm[r] = c;
         pc[r] = pu;
@@ -1022,6 +1022,30 @@ __global__ __launch_bounds__(kBlock) void k_charge_final(const double* __restric
 // --------------------------------------------------------------------------
 // host launchers
 // --------------------------------------------------------------------------
+// The 4-wide FP32 stencil holds 8 registers per right-hand side and lane:
+// it is instantiated for the exact RHS count (occupancy 3 waves/SIMD at 8
+// RHS, more below), where the other kernels use the 1/2/4/8 buckets.
+template <int NR>
+static void dot4_t(hipStream_t st, fs_sys* s, const Tiling& tl, const VecArgs& va, int mode,
+                   unsigned long long* prof) {
+  hipLaunchKernelGGL((k_stencil_dot4<NR>), dim3(tl.nblocks), dim3(kBlock), 0, st, s->p32, s->q32,
+                     s->c32[0], s->c32[1], s->c32[2], s->d, tl, va, mode, s->sc, s->partials,
+                     prof);
+}
+static void launch_dot4(hipStream_t st, fs_sys* s, const Tiling& tl, const VecArgs& va, int mode,
+                        unsigned long long* prof) {
+  switch (va.nr) {
+    case 1: dot4_t<1>(st, s, tl, va, mode, prof); break;
+    case 2: dot4_t<2>(st, s, tl, va, mode, prof); break;
+    case 3: dot4_t<3>(st, s, tl, va, mode, prof); break;
+    case 4: dot4_t<4>(st, s, tl, va, mode, prof); break;
+    case 5: dot4_t<5>(st, s, tl, va, mode, prof); break;
+    case 6: dot4_t<6>(st, s, tl, va, mode, prof); break;
+    case 7: dot4_t<7>(st, s, tl, va, mode, prof); break;
+    default: dot4_t<8>(st, s, tl, va, mode, prof); break;
+  }
+}
+
 template <int NR>
 static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {
   hipStream_t st = s->stream;
@@ -1055,9 +1079,7 @@ static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEve
   int sbf = sb;
   if (s->use_x4) {
     sbf = s->tl4.nblocks;
-    hipLaunchKernelGGL((k_stencil_dot4<NR>), dim3(sbf), dim3(kBlock), 0, st, s->p32, s->q32,
-                       s->c32[0], s->c32[1], s->c32[2], s->d, s->tl4, va, mode, s->sc,
-                       s->partials, prof);
+    launch_dot4(st, s, s->tl4, va, mode, prof);
   } else {
     hipLaunchKernelGGL((k_stencil_dot<float, NR>), dim3(sb), dim3(kBlock), 0, st, s->p32,
                        s->q32, s->c32[0], s->c32[1], s->c32[2], s->d, s->tl, va, mode, s->sc,
@@ -1175,9 +1197,7 @@ template <int NR>
 static void launch_stencil_only_t(fs_sys* s, int nr, const Tiling& tl, bool x4) {
   VecArgs va{s->d.n, s->vstride, nr};
   if (x4)
-    hipLaunchKernelGGL((k_stencil_dot4<NR>), dim3(tl.nblocks), dim3(kBlock), 0, s->stream,
-                       s->p32, s->q32, s->c32[0], s->c32[1], s->c32[2], s->d, tl, va, 1, s->sc,
-                       s->partials, nullptr);
+    launch_dot4(s->stream, s, tl, va, 1, nullptr);
   else
     hipLaunchKernelGGL((k_stencil_dot<float, NR>), dim3(tl.nblocks), dim3(kBlock), 0, s->stream,
                        s->p32, s->q32, s->c32[0], s->c32[1], s->c32[2], s->d, tl, va, 1, s->sc,

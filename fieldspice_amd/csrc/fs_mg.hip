@@ -554,13 +554,15 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
     float* __restrict__ e, const float* __restrict__ cx, const float* __restrict__ cy,
     const float* __restrict__ cz, const uint8_t* __restrict__ mask, Dims d, Tiling tl, Map m,
     int64_t vs, int nr, float omega, const float* __restrict__ q,
-    double* __restrict__ partials, const CgScalars* sc, int mode) {
+    double* __restrict__ partials, const CgScalars* sc, int mode, int b0, int nrt) {
+  // this launch carries right-hand sides b0 .. b0+nr-1 of nrt (vector
+  // pointers already offset to b0); partials rows b0+b and nrt+b0+b
   double acc[2 * NR];
 #pragma unroll
   for (int a = 0; a < 2 * NR; ++a) acc[a] = 0.0;
   bool live[NR];
 #pragma unroll
-  for (int b = 0; b < NR; ++b) live[b] = b < nr && live_of(sc, b, mode);
+  for (int b = 0; b < NR; ++b) live[b] = b < nr && live_of(sc, b0 + b, mode);
   int tx, ty, tz;
   if (tile_of_block(tl, tx, ty, tz)) {
     const int lane = (int)(threadIdx.x % kWave);
@@ -571,20 +573,16 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
     const bool valid = i < d.nx && j < d.ny;
     const int64_t base = valid ? (int64_t)j * d.nx + i : 0;
     const bool has_w = i > 0, has_e = i + 4 < d.nx, has_n = j + 1 < d.ny, has_s = j > 0;
+    // x and y are grouped in pairs on this path (the level-0 set-up selects
+    // it only then): the lane's nodes i..i+3 lie in coarse columns i/2 and
+    // i/2+1, one aligned 8-byte load per coarse row (i/2 and cnx are even)
     const int cnx = m.cn[0], cny = m.cn[1];
-    int ci[4] = {0, 0, 0, 0}, ciw = 0, cie = 0, cj = 0, cjn = 0, cjs = 0;
-    if (valid) {
-      for (int a = 0; a < 4; ++a) ci[a] = m.f2c[0][i + a];
-      ciw = has_w ? m.f2c[0][i - 1] : 0;
-      cie = has_e ? m.f2c[0][i + 4] : 0;
-      cj = m.f2c[1][j];
-      cjn = has_n ? m.f2c[1][j + 1] : cj;
-      cjs = has_s ? m.f2c[1][j - 1] : cj;
-    }
+    const int ci0 = i >> 1, ciw = (i - 1) >> 1, cie = (i + 4) >> 1;
+    const int cj = j >> 1, cjn = has_n ? (j + 1) >> 1 : cj, cjs = has_s ? (j - 1) >> 1 : cj;
     // z2 = omega r + ec[P(n)] of the 4 nodes of row (jj, plane ck)
     auto z2row = [&](const float* eb, float4 rr, int64_t rowbase) {
-      return f4(omega * rr.x + eb[rowbase + ci[0]], omega * rr.y + eb[rowbase + ci[1]],
-                omega * rr.z + eb[rowbase + ci[2]], omega * rr.w + eb[rowbase + ci[3]]);
+      const float2 e = *reinterpret_cast<const float2*>(eb + rowbase + ci0);
+      return f4(omega * rr.x + e.x, omega * rr.y + e.x, omega * rr.z + e.y, omega * rr.w + e.y);
     };
     int64_t nn = (int64_t)k0 * d.sxy + base;
     const float4 z4 = f4(0.f, 0.f, 0.f, 0.f);
@@ -658,10 +656,9 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
           *reinterpret_cast<float4*>(e + b * vs + nn) = out;
           if (partials) {
             const float4 qq = ldf4(q + b * vs + nn);
-            acc[b] += (double)out.x * rr.x + (double)out.y * rr.y + (double)out.z * rr.z +
-                      (double)out.w * rr.w;
-            acc[NR + b] += (double)out.x * qq.x + (double)out.y * qq.y + (double)out.z * qq.z +
-                           (double)out.w * qq.w;
+            // the lane's 4 products in FP32, their sums accumulated in FP64
+            acc[b] += (double)(out.x * rr.x + out.y * rr.y + out.z * rr.z + out.w * rr.w);
+            acc[NR + b] += (double)(out.x * qq.x + out.y * qq.y + out.z * qq.z + out.w * qq.w);
           }
         }
         zm[b] = c;
@@ -676,8 +673,8 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
   block_sum<2 * NR>(acc);
   if (threadIdx.x == 0) {
     for (int b = 0; b < nr; ++b) {
-      store_partial(partials + (int64_t)b * gridDim.x + blockIdx.x, acc[b]);
-      store_partial(partials + (int64_t)(nr + b) * gridDim.x + blockIdx.x, acc[NR + b]);
+      store_partial(partials + (int64_t)(b0 + b) * gridDim.x + blockIdx.x, acc[b]);
+      store_partial(partials + (int64_t)(nrt + b0 + b) * gridDim.x + blockIdx.x, acc[NR + b]);
     }
   }
 }
@@ -805,15 +802,30 @@ int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, 
                float* e, int nr, float omega, const float* q, double* partials,
                const CgScalars* sc, int mode) {
   if (a.x4) {
-#define FS_L0P4(NR_)                                                                         \
-  hipLaunchKernelGGL((k_prolong_l0x4<NR_>), dim3(a.tl4.nblocks), dim3(kBlock), 0, s, r, ec, vsc, \
-                     e, a.c[0], a.c[1], a.c[2], a.mask, a.d, a.tl4, a.m, a.vs, nr, omega, q,     \
-                     partials, sc, mode)
-    if (nr <= 1) FS_L0P4(1);
-    else if (nr <= 2) FS_L0P4(2);
-    else if (nr <= 4) FS_L0P4(4);
-    else FS_L0P4(8);
+    // ~16 registers per RHS and lane: launched for the exact RHS count, and
+    // in two halves above 5 RHS (8 in one launch would run 1 wave per SIMD)
+#define FS_L0P4(NR_, B0, NB)                                                                  \
+  hipLaunchKernelGGL((k_prolong_l0x4<NR_>), dim3(a.tl4.nblocks), dim3(kBlock), 0, s,          \
+                     r + (B0) * a.vs, ec + (B0) * vsc, vsc, e + (B0) * a.vs, a.c[0], a.c[1],    \
+                     a.c[2], a.mask, a.d, a.tl4, a.m, a.vs, NB, omega,                        \
+                     q ? q + (B0) * a.vs : nullptr, partials, sc, mode, B0, nr)
+    auto part = [&](int b0, int nb) {
+      switch (nb) {
+        case 1: FS_L0P4(1, b0, nb); break;
+        case 2: FS_L0P4(2, b0, nb); break;
+        case 3: FS_L0P4(3, b0, nb); break;
+        case 4: FS_L0P4(4, b0, nb); break;
+        default: FS_L0P4(5, b0, nb); break;
+      }
+    };
 #undef FS_L0P4
+    if (nr <= 5) {
+      part(0, nr);
+    } else {
+      const int h = (nr + 1) / 2;
+      part(0, h);
+      part(h, nr - h);
+    }
     return a.tl4.nblocks;
   }
 #define FS_L0P(NR_)                                                                        \
@@ -834,10 +846,17 @@ void restrict_l0(hipStream_t s, const L0Args& a, const float* r, float* rc, int6
 #define FS_L0R4(NR_)                                                                           \
   hipLaunchKernelGGL((k_restrict_l0x4<NR_>), dim3(a.tlr4.nblocks), dim3(kBlock), 0, s, r, rc, vsc, \
                      a.c[0], a.c[1], a.c[2], a.d, a.tlr4, a.kstart, a.m, a.vs, nr, omega, sc, mode)
-    if (nr <= 1) FS_L0R4(1);
-    else if (nr <= 2) FS_L0R4(2);
-    else if (nr <= 4) FS_L0R4(4);
-    else FS_L0R4(8);
+    // exact RHS count: these kernels hold ~16 registers per RHS and lane
+    switch (nr) {
+      case 1: FS_L0R4(1); break;
+      case 2: FS_L0R4(2); break;
+      case 3: FS_L0R4(3); break;
+      case 4: FS_L0R4(4); break;
+      case 5: FS_L0R4(5); break;
+      case 6: FS_L0R4(6); break;
+      case 7: FS_L0R4(7); break;
+      default: FS_L0R4(8); break;
+    }
 #undef FS_L0R4
     return;
   }
