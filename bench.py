@@ -58,6 +58,8 @@ def _parse():
     ap.add_argument("--nsteps", type=int, default=200, help="c3: time steps")
     ap.add_argument("--store-every", type=int, default=10,
                     help="c3: steps between stored potentials (error samples)")
+    ap.add_argument("--tol", type=float, default=1e-10,
+                    help="c3: relative residual of every step (the reference's 1e-10)")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="c3: steps of the plain Jacobi-PCG run beside the island-space runs")
     ap.add_argument("--instances", type=int, default=64, help="c5: number of systems")
@@ -287,6 +289,7 @@ def run_c3(args):
 
     def run(method, islands, steps, store_every):
         cfg = config_for(method)
+        cfg.tol = args.tol
         cfg.deflate_islands = islands
         sol = EQSSolver(g, eps, sigma, cfg)
         t0 = time.perf_counter()
@@ -354,7 +357,7 @@ def run_c3(args):
         "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"EQS backward Euler, {n}^3 nodes at 10 nm, copper boxes 20% "
                                "volume in SiO2, 1 GHz 1 V drive, one grounded box, dt 1e-12 s",
-                   "seed": seed, "method": head, "steps": nsteps},
+                   "seed": seed, "method": head, "steps": nsteps, "tol": args.tol},
         "methods": out_m,
         "plain_reference_method": plain,
         "speedup_vs_fp64": {m: fp["wall_s"] / r["wall_s"] for m, r in out_m.items()} if fp else None,
