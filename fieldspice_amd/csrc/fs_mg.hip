@@ -548,11 +548,14 @@ __device__ __forceinline__ float4 f4(float a, float b, float c, float d) {
 }
 __device__ __forceinline__ float4 ldf4(const float* p) { return *reinterpret_cast<const float4*>(p); }
 
-template <int NR>
+// DIAG: a coarse level (diagonal dgv, inverse diagonal dinvv, activity
+// dinvv > 0) instead of level 0 (unit diagonal, Dirichlet mask)
+template <int NR, bool DIAG>
 __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
     const float* __restrict__ r, const float* __restrict__ ec, int64_t vsc,
     float* __restrict__ e, const float* __restrict__ cx, const float* __restrict__ cy,
-    const float* __restrict__ cz, const uint8_t* __restrict__ mask, Dims d, Tiling tl, Map m,
+    const float* __restrict__ cz, const uint8_t* __restrict__ mask,
+    const float* __restrict__ dgv, const float* __restrict__ dinvv, Dims d, Tiling tl, Map m,
     int64_t vs, int nr, float omega, const float* __restrict__ q,
     double* __restrict__ partials, const CgScalars* sc, int mode, int b0, int nrt) {
   // this launch carries right-hand sides b0 .. b0+nr-1 of nrt (vector
@@ -579,17 +582,30 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
     const int cnx = m.cn[0], cny = m.cn[1];
     const int ci0 = i >> 1, ciw = (i - 1) >> 1, cie = (i + 4) >> 1;
     const int cj = j >> 1, cjn = has_n ? (j + 1) >> 1 : cj, cjs = has_s ? (j - 1) >> 1 : cj;
-    // z2 = omega r + ec[P(n)] of the 4 nodes of row (jj, plane ck)
-    auto z2row = [&](const float* eb, float4 rr, int64_t rowbase) {
+    // z2 = omega D^-1 r + ec[P(n)] of the 4 nodes of row (jj, plane ck)
+    auto z2row = [&](const float* eb, float4 rr, float4 dd, int64_t rowbase) {
       const float2 e = *reinterpret_cast<const float2*>(eb + rowbase + ci0);
-      return f4(omega * rr.x + e.x, omega * rr.y + e.x, omega * rr.z + e.y, omega * rr.w + e.y);
+      if constexpr (DIAG)
+        return f4(omega * dd.x * rr.x + e.x, omega * dd.y * rr.y + e.x,
+                  omega * dd.z * rr.z + e.y, omega * dd.w * rr.w + e.y);
+      else
+        return f4(omega * rr.x + e.x, omega * rr.y + e.x, omega * rr.z + e.y, omega * rr.w + e.y);
     };
+    const float4 one4 = f4(1.f, 1.f, 1.f, 1.f);
+    // inverse diagonal of rows (plane k, row j) and (k-1, j); 1 on level 0
+    float4 dcur = one4, dprev = one4;
     int64_t nn = (int64_t)k0 * d.sxy + base;
     const float4 z4 = f4(0.f, 0.f, 0.f, 0.f);
     float4 zm[NR], zc[NR], rc[NR];
     int ck = valid ? m.f2c[2][k0] : 0;
     float4 czm = z4;
     if (valid && k0 > 0) czm = ldf4(cz + nn - d.sxy);
+    if constexpr (DIAG) {
+      if (valid) {
+        dcur = ldf4(dinvv + nn);
+        if (k0 > 0) dprev = ldf4(dinvv + nn - d.sxy);
+      }
+    }
 #pragma unroll
     for (int b = 0; b < NR; ++b) {
       zm[b] = z4;
@@ -599,9 +615,10 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
         const float* rb = r + b * vs;
         const float* eb = ec + b * vsc;
         if (k0 > 0)
-          zm[b] = z2row(eb, ldf4(rb + nn - d.sxy), ((int64_t)m.f2c[2][k0 - 1] * cny + cj) * cnx);
+          zm[b] = z2row(eb, ldf4(rb + nn - d.sxy), dprev,
+                        ((int64_t)m.f2c[2][k0 - 1] * cny + cj) * cnx);
         rc[b] = ldf4(rb + nn);
-        zc[b] = z2row(eb, rc[b], ((int64_t)ck * cny + cj) * cnx);
+        zc[b] = z2row(eb, rc[b], dcur, ((int64_t)ck * cny + cj) * cnx);
       }
     }
     for (int k = k0; k < k1; ++k, nn += d.sxy) {
@@ -609,12 +626,20 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
       const int cku = (valid && up) ? m.f2c[2][k + 1] : ck;
       float4 cxc = z4, cyc = z4, czc = z4, cym = z4;
       uchar4 mk = make_uchar4(1, 1, 1, 1);
+      float4 dup = one4, dnr = one4, dsr = one4, dgc = one4;
       if (valid) {
         cxc = ldf4(cx + nn);
         cyc = ldf4(cy + nn);
         czc = ldf4(cz + nn);
         if (has_s) cym = ldf4(cy + nn - d.nx);
-        mk = *reinterpret_cast<const uchar4*>(mask + nn);
+        if constexpr (DIAG) {
+          dgc = ldf4(dgv + nn);
+          if (up) dup = ldf4(dinvv + nn + d.sxy);
+          if (has_n) dnr = ldf4(dinvv + nn + d.nx);
+          if (has_s) dsr = ldf4(dinvv + nn - d.nx);
+        } else {
+          mk = *reinterpret_cast<const uchar4*>(mask + nn);
+        }
       }
       const float cxw_l = __shfl_up(cxc.w, 1, kWave);
       float cxw = 0.f;
@@ -629,18 +654,19 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
         if (valid) {
           if (up) {
             ru = ldf4(rb + nn + d.sxy);
-            zu = z2row(eb, ru, ((int64_t)cku * cny + cj) * cnx);
+            zu = z2row(eb, ru, dup, ((int64_t)cku * cny + cj) * cnx);
           }
-          if (has_n) zn = z2row(eb, ldf4(rb + nn + d.nx), ((int64_t)ck * cny + cjn) * cnx);
-          if (has_s) zs = z2row(eb, ldf4(rb + nn - d.nx), ((int64_t)ck * cny + cjs) * cnx);
+          if (has_n) zn = z2row(eb, ldf4(rb + nn + d.nx), dnr, ((int64_t)ck * cny + cjn) * cnx);
+          if (has_s) zs = z2row(eb, ldf4(rb + nn - d.nx), dsr, ((int64_t)ck * cny + cjs) * cnx);
         }
         const float4 c = zc[b];
         float zw = __shfl_up(c.w, 1, kWave), ze = __shfl_down(c.x, 1, kWave);
         if (valid) {
           if (!has_w) zw = 0.f;
-          else if (lane == 0) zw = omega * rb[nn - 1] + eb[rowc + ciw];
+          else if (lane == 0) zw = omega * (DIAG ? dinvv[nn - 1] : 1.f) * rb[nn - 1] + eb[rowc + ciw];
           if (!has_e) ze = 0.f;
-          else if (lane == kWave - 1) ze = omega * rb[nn + 4] + eb[rowc + cie];
+          else if (lane == kWave - 1)
+            ze = omega * (DIAG ? dinvv[nn + 4] : 1.f) * rb[nn + 4] + eb[rowc + cie];
         }
         const float4 mm = zm[b], rr = rc[b];
         const float s0 = cxc.x * c.y + cxw * zw + cyc.x * zn.x + cym.x * zs.x + czc.x * zu.x + czm.x * mm.x;
@@ -648,10 +674,17 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
         const float s2 = cxc.z * c.w + cxc.y * c.y + cyc.z * zn.z + cym.z * zs.z + czc.z * zu.z + czm.z * mm.z;
         const float s3 = cxc.w * ze + cxc.z * c.z + cyc.w * zn.w + cym.w * zs.w + czc.w * zu.w + czm.w * mm.w;
         float4 out;
-        out.x = mk.x == 0 ? c.x + omega * (rr.x - c.x + s0) : 0.f;
-        out.y = mk.y == 0 ? c.y + omega * (rr.y - c.y + s1) : 0.f;
-        out.z = mk.z == 0 ? c.z + omega * (rr.z - c.z + s2) : 0.f;
-        out.w = mk.w == 0 ? c.w + omega * (rr.w - c.w + s3) : 0.f;
+        if constexpr (DIAG) {  // e = z2 + omega D^-1 (r - D z2 + C z2) on active nodes
+          out.x = dcur.x > 0.f ? c.x + omega * dcur.x * (rr.x - dgc.x * c.x + s0) : 0.f;
+          out.y = dcur.y > 0.f ? c.y + omega * dcur.y * (rr.y - dgc.y * c.y + s1) : 0.f;
+          out.z = dcur.z > 0.f ? c.z + omega * dcur.z * (rr.z - dgc.z * c.z + s2) : 0.f;
+          out.w = dcur.w > 0.f ? c.w + omega * dcur.w * (rr.w - dgc.w * c.w + s3) : 0.f;
+        } else {
+          out.x = mk.x == 0 ? c.x + omega * (rr.x - c.x + s0) : 0.f;
+          out.y = mk.y == 0 ? c.y + omega * (rr.y - c.y + s1) : 0.f;
+          out.z = mk.z == 0 ? c.z + omega * (rr.z - c.z + s2) : 0.f;
+          out.w = mk.w == 0 ? c.w + omega * (rr.w - c.w + s3) : 0.f;
+        }
         if (valid) {
           *reinterpret_cast<float4*>(e + b * vs + nn) = out;
           if (partials) {
@@ -667,6 +700,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
       }
       czm = czc;
       ck = cku;
+      dcur = dup;
     }
   }
   if (!partials) return;  // uniform over the launch
@@ -681,11 +715,14 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
 
 // rc = P^T (r - omega A r) with 4 nodes per lane: x pairs {2I, 2I+1} sum
 // inside the lane, y pairs through LDS at the end of each z-group
-template <int NR>
+// DIAG: a coarse level, t_n = (1 - omega) r_n + omega sum_m c_nm dinv_m r_m
+// (level 0: dinv = 1 on free nodes, and r = 0 on the others)
+template <int NR, bool DIAG>
 __global__ __launch_bounds__(kBlock) void k_restrict_l0x4(
     const float* __restrict__ r, float* __restrict__ rc, int64_t vsc,
     const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
-    Dims d, Tiling tl, const int* __restrict__ kstart, Map m, int64_t vs, int nr, float omega,
+    const float* __restrict__ dinvv, Dims d, Tiling tl, const int* __restrict__ kstart, Map m,
+    int64_t vs, int nr, float omega,
     const CgScalars* sc, int mode) {
   __shared__ float2 lds[NR][kTileY][kWave];
   int tx, ty, tz;
@@ -707,6 +744,14 @@ __global__ __launch_bounds__(kBlock) void k_restrict_l0x4(
   float2 col[NR];
   float4 czm = z4;
   if (valid && k0 > 0) czm = ldf4(cz + nn - d.sxy);
+  const float4 one4 = f4(1.f, 1.f, 1.f, 1.f);
+  float4 dcur = one4, dprev = one4;  // inverse diagonal of rows (k, j), (k-1, j)
+  if constexpr (DIAG) {
+    if (valid) {
+      dcur = ldf4(dinvv + nn);
+      if (k0 > 0) dprev = ldf4(dinvv + nn - d.sxy);
+    }
+  }
 #pragma unroll
   for (int b = 0; b < NR; ++b) {
     rm[b] = z4;
@@ -730,6 +775,18 @@ __global__ __launch_bounds__(kBlock) void k_restrict_l0x4(
     const float cxw_l = __shfl_up(cxc.w, 1, kWave);
     float cxw = 0.f;
     if (valid && has_w) cxw = lane > 0 ? cxw_l : cx[nn - 1];
+    float4 dup = one4, dnr = one4, dsr = one4;
+    float dw = 1.f, de = 1.f;
+    if constexpr (DIAG) {
+      if (valid) {
+        if (up) dup = ldf4(dinvv + nn + d.sxy);
+        if (has_n) dnr = ldf4(dinvv + nn + d.nx);
+        if (has_s) dsr = ldf4(dinvv + nn - d.nx);
+        if (has_w && lane == 0) dw = dinvv[nn - 1];
+        if (has_e && lane == kWave - 1) de = dinvv[nn + 4];
+      }
+    }
+    auto mul = [](float4 a, float4 b) { return f4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w); };
 #pragma unroll
     for (int b = 0; b < NR; ++b) {
       if (!live[b]) continue;  // uniform over the block
@@ -740,28 +797,37 @@ __global__ __launch_bounds__(kBlock) void k_restrict_l0x4(
         if (has_n) rn = ldf4(rb + nn + d.nx);
         if (has_s) rs = ldf4(rb + nn - d.nx);
       }
-      const float4 c = rcur[b];
+      const float4 cr = rcur[b];
+      // neighbour values enter as D^-1 r (level 0: r)
+      const float4 c = DIAG ? mul(cr, dcur) : cr;
       float rw = __shfl_up(c.w, 1, kWave), re = __shfl_down(c.x, 1, kWave);
       if (valid) {
         if (!has_w) rw = 0.f;
-        else if (lane == 0) rw = rb[nn - 1];
+        else if (lane == 0) rw = dw * rb[nn - 1];
         if (!has_e) re = 0.f;
-        else if (lane == kWave - 1) re = rb[nn + 4];
+        else if (lane == kWave - 1) re = de * rb[nn + 4];
       }
-      const float4 mm = rm[b];
-      const float s0 = cxc.x * c.y + cxw * rw + cyc.x * rn.x + cym.x * rs.x + czc.x * ru.x + czm.x * mm.x;
-      const float s1 = cxc.y * c.z + cxc.x * c.x + cyc.y * rn.y + cym.y * rs.y + czc.y * ru.y + czm.y * mm.y;
-      const float s2 = cxc.z * c.w + cxc.y * c.y + cyc.z * rn.z + cym.z * rs.z + czc.z * ru.z + czm.z * mm.z;
-      const float s3 = cxc.w * re + cxc.z * c.z + cyc.w * rn.w + cym.w * rs.w + czc.w * ru.w + czm.w * mm.w;
+      const float4 ruS = DIAG ? mul(ru, dup) : ru;
+      if constexpr (DIAG) {
+        rn = mul(rn, dnr);
+        rs = mul(rs, dsr);
+      }
+      const float4 mm = DIAG ? mul(rm[b], dprev) : rm[b];
+      const float s0 = cxc.x * c.y + cxw * rw + cyc.x * rn.x + cym.x * rs.x + czc.x * ruS.x + czm.x * mm.x;
+      const float s1 = cxc.y * c.z + cxc.x * c.x + cyc.y * rn.y + cym.y * rs.y + czc.y * ruS.y + czm.y * mm.y;
+      const float s2 = cxc.z * c.w + cxc.y * c.y + cyc.z * rn.z + cym.z * rs.z + czc.z * ruS.z + czm.z * mm.z;
+      const float s3 = cxc.w * re + cxc.z * c.z + cyc.w * rn.w + cym.w * rs.w + czc.w * ruS.w + czm.w * mm.w;
       if (valid) {
         const float w1 = 1.f - omega;
-        col[b].x += (w1 * c.x + omega * s0) + (w1 * c.y + omega * s1);
-        col[b].y += (w1 * c.z + omega * s2) + (w1 * c.w + omega * s3);
+        col[b].x += (w1 * cr.x + omega * s0) + (w1 * cr.y + omega * s1);
+        col[b].y += (w1 * cr.z + omega * s2) + (w1 * cr.w + omega * s3);
       }
-      rm[b] = c;
+      rm[b] = cr;
       rcur[b] = ru;
     }
     czm = czc;
+    dprev = dcur;
+    dcur = dup;
     const bool group_end = k + 1 == k1 || m.f2c[2][k + 1] != ck;  // uniform
     if (group_end) {
 #pragma unroll
@@ -804,18 +870,28 @@ int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, 
   if (a.x4) {
     // ~16 registers per RHS and lane: launched for the exact RHS count, and
     // in two halves above 5 RHS (8 in one launch would run 1 wave per SIMD)
-#define FS_L0P4(NR_, B0, NB)                                                                  \
-  hipLaunchKernelGGL((k_prolong_l0x4<NR_>), dim3(a.tl4.nblocks), dim3(kBlock), 0, s,          \
+#define FS_L0P4(NR_, DG, B0, NB)                                                               \
+  hipLaunchKernelGGL((k_prolong_l0x4<NR_, DG>), dim3(a.tl4.nblocks), dim3(kBlock), 0, s,        \
                      r + (B0) * a.vs, ec + (B0) * vsc, vsc, e + (B0) * a.vs, a.c[0], a.c[1],    \
-                     a.c[2], a.mask, a.d, a.tl4, a.m, a.vs, NB, omega,                        \
+                     a.c[2], a.mask, a.dg, a.dinv, a.d, a.tl4, a.m, a.vs, NB, omega,           \
                      q ? q + (B0) * a.vs : nullptr, partials, sc, mode, B0, nr)
     auto part = [&](int b0, int nb) {
-      switch (nb) {
-        case 1: FS_L0P4(1, b0, nb); break;
-        case 2: FS_L0P4(2, b0, nb); break;
-        case 3: FS_L0P4(3, b0, nb); break;
-        case 4: FS_L0P4(4, b0, nb); break;
-        default: FS_L0P4(5, b0, nb); break;
+      if (a.dinv) {
+        switch (nb) {
+          case 1: FS_L0P4(1, true, b0, nb); break;
+          case 2: FS_L0P4(2, true, b0, nb); break;
+          case 3: FS_L0P4(3, true, b0, nb); break;
+          case 4: FS_L0P4(4, true, b0, nb); break;
+          default: FS_L0P4(5, true, b0, nb); break;
+        }
+      } else {
+        switch (nb) {
+          case 1: FS_L0P4(1, false, b0, nb); break;
+          case 2: FS_L0P4(2, false, b0, nb); break;
+          case 3: FS_L0P4(3, false, b0, nb); break;
+          case 4: FS_L0P4(4, false, b0, nb); break;
+          default: FS_L0P4(5, false, b0, nb); break;
+        }
       }
     };
 #undef FS_L0P4
@@ -843,19 +919,33 @@ int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, 
 void restrict_l0(hipStream_t s, const L0Args& a, const float* r, float* rc, int64_t vsc, int nr,
                  float omega, const CgScalars* sc, int mode) {
   if (a.x4) {
-#define FS_L0R4(NR_)                                                                           \
-  hipLaunchKernelGGL((k_restrict_l0x4<NR_>), dim3(a.tlr4.nblocks), dim3(kBlock), 0, s, r, rc, vsc, \
-                     a.c[0], a.c[1], a.c[2], a.d, a.tlr4, a.kstart, a.m, a.vs, nr, omega, sc, mode)
+#define FS_L0R4(NR_, DG)                                                                       \
+  hipLaunchKernelGGL((k_restrict_l0x4<NR_, DG>), dim3(a.tlr4.nblocks), dim3(kBlock), 0, s, r, rc, \
+                     vsc, a.c[0], a.c[1], a.c[2], a.dinv, a.d, a.tlr4, a.kstart, a.m, a.vs, nr,  \
+                     omega, sc, mode)
     // exact RHS count: these kernels hold ~16 registers per RHS and lane
-    switch (nr) {
-      case 1: FS_L0R4(1); break;
-      case 2: FS_L0R4(2); break;
-      case 3: FS_L0R4(3); break;
-      case 4: FS_L0R4(4); break;
-      case 5: FS_L0R4(5); break;
-      case 6: FS_L0R4(6); break;
-      case 7: FS_L0R4(7); break;
-      default: FS_L0R4(8); break;
+    if (a.dinv) {
+      switch (nr) {
+        case 1: FS_L0R4(1, true); break;
+        case 2: FS_L0R4(2, true); break;
+        case 3: FS_L0R4(3, true); break;
+        case 4: FS_L0R4(4, true); break;
+        case 5: FS_L0R4(5, true); break;
+        case 6: FS_L0R4(6, true); break;
+        case 7: FS_L0R4(7, true); break;
+        default: FS_L0R4(8, true); break;
+      }
+    } else {
+      switch (nr) {
+        case 1: FS_L0R4(1, false); break;
+        case 2: FS_L0R4(2, false); break;
+        case 3: FS_L0R4(3, false); break;
+        case 4: FS_L0R4(4, false); break;
+        case 5: FS_L0R4(5, false); break;
+        case 6: FS_L0R4(6, false); break;
+        case 7: FS_L0R4(7, false); break;
+        default: FS_L0R4(8, false); break;
+      }
     }
 #undef FS_L0R4
     return;

@@ -120,7 +120,9 @@ struct L0Args {
   Dims d;
   int64_t vs;
   const float* c[3];
-  const uint8_t* mask;
+  const uint8_t* mask;  // level 0: Dirichlet mask (unit diagonal)
+  const float* dg;      // a coarse level: diagonal and inverse diagonal (else null)
+  const float* dinv;
   Tiling tl;            // prolongation tiles (the system's)
   Tiling tlr;           // restriction tiles
   int x4;               // rows a multiple of 4 (and x pairs): 4 nodes per lane

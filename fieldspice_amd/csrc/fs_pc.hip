@@ -280,6 +280,8 @@ struct fs_pc {
   std::vector<mg::Map> maps;  // K maps
   mg::L0Args l0{};            // level 0 on the stencil tiles
   bool l0_pairs = false;      // level-0 x/y groupings are pairs: tiled restriction
+  // levels 1..K-1 on 4-wide z-march tiles (x4 set where the level qualifies)
+  std::vector<mg::L0Args> lx;
   int* kstart = nullptr;
   std::vector<float*> owned;  // coarse operator arrays
   std::vector<int32_t*> owned_i;
@@ -359,6 +361,9 @@ int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, doub
   for (int l = 0; l < K; ++l) {
     if (l == 0 && pc->l0_pairs)
       mg::restrict_l0(s, pc->l0, r0, pc->rv[1], pc->lv[1].vs, nr, pc->omega, sc, mode);
+    else if (l > 0 && pc->lx[l].x4)
+      mg::restrict_l0(s, pc->lx[l], pc->rv[l], pc->rv[l + 1], pc->lv[l + 1].vs, nr, pc->omega,
+                      sc, mode);
     else
       mg::restrict_(s, pc->lv[l], pc->maps[l], l == 0 ? r0 : pc->rv[l], pc->rv[l + 1],
                     pc->lv[l + 1].vs, nr, pc->omega, g);
@@ -374,9 +379,14 @@ int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, doub
   } else {
     mg::bottom_jacobi(s, B, pc->rv[K], pc->ev[K], nr, pc->omega, g);
   }
-  for (int l = K - 1; l >= 1; --l)
-    mg::prolong(s, pc->lv[l], pc->maps[l], pc->rv[l], pc->ev[l + 1], pc->lv[l + 1].vs,
-                pc->ev[l], nr, pc->omega, nullptr, nullptr, 0, g);
+  for (int l = K - 1; l >= 1; --l) {
+    if (pc->lx[l].x4)
+      mg::prolong_l0(s, pc->lx[l], pc->rv[l], pc->ev[l + 1], pc->lv[l + 1].vs, pc->ev[l], nr,
+                     pc->omega, nullptr, nullptr, sc, mode);
+    else
+      mg::prolong(s, pc->lv[l], pc->maps[l], pc->rv[l], pc->ev[l + 1], pc->lv[l + 1].vs,
+                  pc->ev[l], nr, pc->omega, nullptr, nullptr, 0, g);
+  }
   const int nb = mg::prolong_l0(s, pc->l0, r0, pc->ev[1], pc->lv[1].vs, z0, nr, pc->omega, q,
                                 partials, sc, mode);
   if (nblocks) *nblocks = nb;
@@ -793,6 +803,54 @@ int fs_pc_create(fs_sys* sys, fs_net* net, int nlevels, const int32_t* groups, d
         if (need > sys->partials_cap) a.x4 = 0;  // keep within the system's partials
       }
     }
+  }
+  // Coarse levels 1..K-1 on the same 4-wide z-march kernels (with their
+  // diagonal) where x and y are grouped in pairs and rows hold a multiple of
+  // 4 nodes, at least 128 (a wave spans 256 nodes of one row).
+  pc->lx.assign(nlevels, mg::L0Args{});
+  for (int l = 1; l < nlevels; ++l) {
+    const mg::Level& L = pc->lv[l];
+    bool ok = L.nx % 4 == 0 && L.nx >= 16;
+    for (int q = 0; q < 2 && ok; ++q) {
+      const std::vector<int32_t>& st = starts[l][q];
+      for (size_t I = 0; I + 1 < st.size(); ++I)
+        ok = ok && st[I] == (int32_t)(2 * I) &&
+             st[I + 1] == std::min<int32_t>(2 * I + 2, dims[l][q]);
+    }
+    if (!ok) continue;
+    mg::L0Args& a = pc->lx[l];
+    a.d = fs::Dims{L.nx, L.ny, L.nz, L.n, (int64_t)L.nx * L.ny};
+    a.vs = L.vs;
+    for (int q = 0; q < 3; ++q) a.c[q] = L.c[q];
+    a.mask = nullptr;
+    a.dg = L.dg;
+    a.dinv = L.dinv;
+    a.m = pc->maps[l];
+    fs::Tiling t{};
+    t.tiles_x = (L.nx + 255) / 256;
+    t.tiles_y = (L.ny + fs::kTileY - 1) / fs::kTileY;
+    const int64_t txy = (int64_t)t.tiles_x * t.tiles_y;
+    t.kz = (int)std::min<int64_t>(16, std::max<int64_t>(2, (L.nz * txy + 1023) / 1024));
+    t.tiles_z = (L.nz + t.kz - 1) / t.kz;
+    t.ntiles = t.tiles_x * t.tiles_y * t.tiles_z;
+    t.nblocks = (t.ntiles + 7) / 8 * 8;
+    a.tl4 = t;
+    const std::vector<int32_t>& zs = starts[l][2];
+    std::vector<int> ks{0};
+    for (size_t G = 1; G < zs.size(); ++G)
+      if (zs[G] - ks.back() >= t.kz || G + 1 == zs.size()) ks.push_back(zs[G]);
+    if (ks.back() != L.nz) ks.push_back(L.nz);
+    a.tlr4 = t;
+    a.tlr4.tiles_z = (int)ks.size() - 1;
+    a.tlr4.ntiles = a.tlr4.tiles_x * a.tlr4.tiles_y * a.tlr4.tiles_z;
+    a.tlr4.nblocks = (a.tlr4.ntiles + 7) / 8 * 8;
+    int32_t* kd = nullptr;
+    if ((rc = dalloc(&kd, ks.size()))) return fail_pc(rc);
+    pc->owned_i.push_back(kd);
+    if (hipMemcpy(kd, ks.data(), ks.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      return fail_pc(fail(FS_EHIP, "hierarchy upload failed"));
+    a.kstart = kd;
+    a.x4 = 1;
   }
   const mg::Level& B = pc->lv[nlevels];
   if (net && (B.ny > 65535 || (int64_t)B.nz * fs::kMaxRhs > 65535))

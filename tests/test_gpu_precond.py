@@ -16,9 +16,9 @@ from fieldspice_amd.precond import hierarchy_groups
 pytestmark = pytest.mark.gpu
 
 
-def _case(shape=(20, 14, 11), seed=0, plane=True):
+def _case(shape=(20, 14, 11), seed=0, plane=True, box=((0, 1), (0, 0.8), (0, 0.6))):
     rng = np.random.default_rng(seed)
-    g = RectilinearGrid.uniform([(0, 1), (0, 0.8), (0, 0.6)], tuple(s - 1 for s in shape))
+    g = RectilinearGrid.uniform(list(box), tuple(s - 1 for s in shape))
     eps = rng.uniform(1.0, 6.0, g.shape_cells)
     fixed = np.zeros(g.n_nodes, bool)
     if plane:
@@ -88,6 +88,28 @@ def test_cycle_with_jacobi_bottom_matches_oracle(shape, omega):
     R = rng.normal(size=(3, g.n_nodes)) * free
     Z = pc.apply(R)
     ref = H.cycle(R, lambda rK: omega * H.dinv[-1] * rK)
+    assert _rel(Z, ref) < 2e-6, _rel(Z, ref)
+    assert np.all(Z[:, fixed] == 0.0)
+
+
+@pytest.mark.parametrize("nx,nrhs", [(256, 5), (64, 7)])
+def test_cycle_coarse_levels_on_tiles_match_oracle(nx, nrhs):
+    # equal spacings: level 0 and level 1 (nx/2 x 6 x 5) are grouped in pairs,
+    # so both run the 4-wide z-march kernels (level 1 with its diagonal, and
+    # a 32-node row at nx = 64); 5 and 7 right-hand sides use the exact-count
+    # instantiations and the two half-width prolongation launches
+    print(f"[gpu case] cycle on 4-wide tiles at levels 0 and 1 vs oracle, nx {nx}", flush=True)
+    box = ((0, 1), (0, 11 / (nx - 1)), (0, 9 / (nx - 1)))
+    g, A, free, dh, op, sysd, L, fixed = _case((nx, 12, 10), seed=4, box=box)
+    groups = hierarchy_groups((g.ax.hd, g.ay.hd, g.az.hd), max_bottom=150)
+    pc = _lib.DevicePreconditioner(sysd, None, groups, 0.8)
+    H = N.Hierarchy(A, free, g.shape_nodes, groups, 0.8)
+    assert pc.level_dims(1) == (nx // 2, 6, 5)
+    assert pc.nlevels >= 2
+    rng = np.random.default_rng(5)
+    R = rng.normal(size=(nrhs, g.n_nodes)) * free
+    Z = pc.apply(R)
+    ref = H.cycle(R, lambda rK: 0.8 * H.dinv[-1] * rK)
     assert _rel(Z, ref) < 2e-6, _rel(Z, ref)
     assert np.all(Z[:, fixed] == 0.0)
 
