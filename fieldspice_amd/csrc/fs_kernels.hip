@@ -542,7 +542,9 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot4(const float* __restrict
 // mode 0: FP64 main iteration (threshold thresh2, sets done).
 // mode 1: FP32 cycle of mixed precision (threshold aux1 on the cycle's own
 //         rho kept in aux0, sets cycle_done; the iteration cap still ends it).
-template <typename T, int NR>
+// DX: x += alpha p is deferred to the next direction update (k_pupdate DX),
+// which reads p anyway; the refresh adds the pending term of the last step.
+template <typename T, int NR, bool DX = false>
 __global__ __launch_bounds__(kBlock) void k_update(T* __restrict__ x, T* __restrict__ r_,
                                                    const T* __restrict__ p,
                                                    const T* __restrict__ q, VecArgs va,
@@ -565,7 +567,7 @@ __global__ __launch_bounds__(kBlock) void k_update(T* __restrict__ x, T* __restr
     const T* qr = q + r * va.vs;
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride) {
-      xr[i] += a * pr[i];
+      if constexpr (!DX) xr[i] += a * pr[i];
       const T rn = rr[i] - a * qr[i];
       rr[i] = rn;
       s += (double)rn * (double)rn;
@@ -580,10 +582,12 @@ __global__ __launch_bounds__(kBlock) void k_update(T* __restrict__ x, T* __restr
   }
 }
 
-// p = r + beta p
-template <typename T, int NR>
+// p = r + beta p; DX: first x += alpha p with the previous step's alpha (the
+// deferred update of k_update DX; alpha is 0 at the start of a cycle)
+template <typename T, int NR, bool DX = false>
 __global__ __launch_bounds__(kBlock) void k_pupdate(T* __restrict__ p, const T* __restrict__ r_,
-                                                    VecArgs va, int mode, const CgScalars* sc) {
+                                                    VecArgs va, int mode, const CgScalars* sc,
+                                                    T* __restrict__ x = nullptr) {
   if (all_idle(sc, va.nr, mode)) return;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
 #pragma unroll
@@ -592,8 +596,18 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(T* __restrict__ p, const T* 
     const T b = (T)sc[r].beta;
     T* pr = p + r * va.vs;
     const T* rr = r_ + r * va.vs;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride)
-      pr[i] = rr[i] + b * pr[i];
+    if constexpr (DX) {
+      const T a = (T)sc[r].alpha;
+      T* xr = x + r * va.vs;
+      for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride) {
+        const T pv = pr[i];
+        xr[i] += a * pv;
+        pr[i] = rr[i] + b * pv;
+      }
+    } else {
+      for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride)
+        pr[i] = rr[i] + b * pr[i];
+    }
   }
 }
 
@@ -688,6 +702,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
     c.aux0 = s0;
     c.aux1 = fmax(fa.inner_tol2, 0.25 * c.thresh2 / c.rho) * s0;
     c.beta = 0.0;
+    c.alpha = 0.0;  // no deferred x update pending at a cycle start
     c.cycle_iter = 0;
     c.cycle_done = 0;
   }
@@ -943,18 +958,22 @@ __global__ __launch_bounds__(kBlock) void k_cycle_start(const double* __restrict
   }
 }
 
-// x64 += d32/scale for RHS whose cycle ended (then refreshed residual next)
+// x64 += (d32 + alpha p32)/scale for RHS whose cycle ended (then the
+// refreshed residual): alpha p32 is the cycle's last step, whose x update
+// was deferred to a direction update that did not come
 template <int NR>
 __global__ __launch_bounds__(kBlock) void k_cycle_accumulate(double* __restrict__ x64,
                                                              const float* __restrict__ d32,
+                                                             const float* __restrict__ p32,
                                                              VecArgs va, const CgScalars* sc) {
   const int64_t stride = (int64_t)gridDim.x * kBlock;
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     if (!(r < va.nr && sc[r].done == 0)) continue;
     const double inv = 1.0 / sc[r].scale;
+    const float a = (float)sc[r].alpha;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride)
-      x64[r * va.vs + i] += (double)d32[r * va.vs + i] * inv;
+      x64[r * va.vs + i] += (double)(d32[r * va.vs + i] + a * p32[r * va.vs + i]) * inv;
   }
 }
 
@@ -1073,8 +1092,8 @@ static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEve
   }
   // FP32 cycle: CG searches along r32, flexible CG along z32 = M(r32)
   const float* dir = mode == 2 ? s->z32 : s->r32;
-  hipLaunchKernelGGL((k_pupdate<float, NR>), dim3(vb), dim3(kBlock), 0, st, s->p32, dir, va, mode,
-                     s->sc);
+  hipLaunchKernelGGL((k_pupdate<float, NR, true>), dim3(vb), dim3(kBlock), 0, st, s->p32, dir, va,
+                     mode, s->sc, s->x32);
   if (ea) (void)hipEventRecord(ea, st);
   int sbf = sb;
   if (s->use_x4) {
@@ -1087,7 +1106,7 @@ static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEve
   }
   if (eb) (void)hipEventRecord(eb, st);
   finalize<NR>(st, s->partials, FIN_ALPHA, nr, sbf, mode, s->sc);
-  hipLaunchKernelGGL((k_update<float, NR>), dim3(vb), dim3(kBlock), 0, st, s->x32, s->r32, s->p32,
+  hipLaunchKernelGGL((k_update<float, NR, true>), dim3(vb), dim3(kBlock), 0, st, s->x32, s->r32, s->p32,
                      s->q32, va, mode, s->sc, s->partials);
   finalize<NR>(st, s->partials, FIN_BETA, nr, vb, mode, s->sc);
   if (mode == 2) {
@@ -1219,7 +1238,7 @@ template <int NR>
 static void launch_refresh_t(fs_sys* s, int nr) {
   VecArgs va{s->d.n, s->vstride, nr};
   hipLaunchKernelGGL((k_cycle_accumulate<NR>), dim3(s->vec_blocks), dim3(kBlock), 0,
-                     s->stream, s->x, s->x32, va, s->sc);
+                     s->stream, s->x, s->x32, s->p32, va, s->sc);
   launch_residual_t<NR>(s, nr, s->r);
   hipLaunchKernelGGL(k_refresh_finish, dim3(1), dim3(64), 0, s->stream, nr, s->sc);
 }
