@@ -385,11 +385,9 @@ static int iteration_defl_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent
   hipLaunchKernelGGL((k_pupdate_w<NR>), dim3(vb), dim3(kBlock), 0, st, s->p32, s->r32,
                      s->u_dinv, va, mode, s->sc);
   defl_scatter_t<NR>(s, nr, mode);
-  if (ea) (void)hipEventRecord(ea, st);
-  hipLaunchKernelGGL((k_stencil_diff<NR>), dim3(sb), dim3(kBlock), 0, st, s->p32, s->q32,
-                     s->u_w[0], s->u_w[1], s->u_w[2], s->u_extra, s->d, s->tl, va, mode, s->sc,
-                     s->partials, prof);
-  if (eb) (void)hipEventRecord(eb, st);
+  launch_timed(k_stencil_diff<NR>, dim3(sb), st, ea, eb, (const float*)s->p32, s->q32,
+               (const float*)s->u_w[0], (const float*)s->u_w[1], (const float*)s->u_w[2],
+               (const float*)s->u_extra, s->d, s->tl, va, mode, s->sc, s->partials, prof);
   launch_finalize(st, s->partials, FIN_ALPHA_ZR, nr, sb, mode, s->sc);
   hipLaunchKernelGGL((k_update_w<NR>), dim3(vb), dim3(kBlock), 0, st, s->x32, s->r32, s->p32,
                      s->q32, s->u_dinv, va, mode, s->sc, s->partials);

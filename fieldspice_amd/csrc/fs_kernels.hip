@@ -1046,22 +1046,23 @@ __global__ __launch_bounds__(kBlock) void k_charge_final(const double* __restric
 // RHS, more below), where the other kernels use the 1/2/4/8 buckets.
 template <int NR>
 static void dot4_t(hipStream_t st, fs_sys* s, const Tiling& tl, const VecArgs& va, int mode,
-                   unsigned long long* prof) {
-  hipLaunchKernelGGL((k_stencil_dot4<NR>), dim3(tl.nblocks), dim3(kBlock), 0, st, s->p32, s->q32,
-                     s->c32[0], s->c32[1], s->c32[2], s->d, tl, va, mode, s->sc, s->partials,
-                     prof);
+                   unsigned long long* prof, hipEvent_t ea, hipEvent_t eb) {
+  launch_timed(k_stencil_dot4<NR>, dim3(tl.nblocks), st, ea, eb, (const float*)s->p32, s->q32,
+               (const float*)s->c32[0], (const float*)s->c32[1], (const float*)s->c32[2], s->d,
+               tl, va, mode, s->sc, s->partials, prof);
 }
 static void launch_dot4(hipStream_t st, fs_sys* s, const Tiling& tl, const VecArgs& va, int mode,
-                        unsigned long long* prof) {
+                        unsigned long long* prof, hipEvent_t ea = nullptr,
+                        hipEvent_t eb = nullptr) {
   switch (va.nr) {
-    case 1: dot4_t<1>(st, s, tl, va, mode, prof); break;
-    case 2: dot4_t<2>(st, s, tl, va, mode, prof); break;
-    case 3: dot4_t<3>(st, s, tl, va, mode, prof); break;
-    case 4: dot4_t<4>(st, s, tl, va, mode, prof); break;
-    case 5: dot4_t<5>(st, s, tl, va, mode, prof); break;
-    case 6: dot4_t<6>(st, s, tl, va, mode, prof); break;
-    case 7: dot4_t<7>(st, s, tl, va, mode, prof); break;
-    default: dot4_t<8>(st, s, tl, va, mode, prof); break;
+    case 1: dot4_t<1>(st, s, tl, va, mode, prof, ea, eb); break;
+    case 2: dot4_t<2>(st, s, tl, va, mode, prof, ea, eb); break;
+    case 3: dot4_t<3>(st, s, tl, va, mode, prof, ea, eb); break;
+    case 4: dot4_t<4>(st, s, tl, va, mode, prof, ea, eb); break;
+    case 5: dot4_t<5>(st, s, tl, va, mode, prof, ea, eb); break;
+    case 6: dot4_t<6>(st, s, tl, va, mode, prof, ea, eb); break;
+    case 7: dot4_t<7>(st, s, tl, va, mode, prof, ea, eb); break;
+    default: dot4_t<8>(st, s, tl, va, mode, prof, ea, eb); break;
   }
 }
 
@@ -1078,11 +1079,9 @@ static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEve
     hipLaunchKernelGGL((k_pupdate<double, NR>), dim3(vb), dim3(kBlock), 0, st, s->p, s->r, va,
                        mode, s->sc);
     if (defl) launch_defl_scatter(s, nr, mode);
-    if (ea) (void)hipEventRecord(ea, st);
-    hipLaunchKernelGGL((k_stencil_dot<double, NR>), dim3(sb), dim3(kBlock), 0, st, s->p, s->q,
-                       s->c64[0], s->c64[1], s->c64[2], s->d, s->tl, va, mode, s->sc,
-                       s->partials, prof);
-    if (eb) (void)hipEventRecord(eb, st);
+    launch_timed(k_stencil_dot<double, NR>, dim3(sb), st, ea, eb, (const double*)s->p, s->q,
+                 (const double*)s->c64[0], (const double*)s->c64[1], (const double*)s->c64[2],
+                 s->d, s->tl, va, mode, s->sc, s->partials, prof);
     finalize<NR>(st, s->partials, defl ? FIN_ALPHA_ZR : FIN_ALPHA, nr, sb, mode, s->sc);
     hipLaunchKernelGGL((k_update<double, NR>), dim3(vb), dim3(kBlock), 0, st, s->x, s->r, s->p,
                        s->q, va, mode, s->sc, s->partials);
@@ -1094,17 +1093,15 @@ static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEve
   const float* dir = mode == 2 ? s->z32 : s->r32;
   hipLaunchKernelGGL((k_pupdate<float, NR, true>), dim3(vb), dim3(kBlock), 0, st, s->p32, dir, va,
                      mode, s->sc, s->x32);
-  if (ea) (void)hipEventRecord(ea, st);
   int sbf = sb;
   if (s->use_x4) {
     sbf = s->tl4.nblocks;
-    launch_dot4(st, s, s->tl4, va, mode, prof);
+    launch_dot4(st, s, s->tl4, va, mode, prof, ea, eb);
   } else {
-    hipLaunchKernelGGL((k_stencil_dot<float, NR>), dim3(sb), dim3(kBlock), 0, st, s->p32,
-                       s->q32, s->c32[0], s->c32[1], s->c32[2], s->d, s->tl, va, mode, s->sc,
-                       s->partials, prof);
+    launch_timed(k_stencil_dot<float, NR>, dim3(sb), st, ea, eb, (const float*)s->p32, s->q32,
+                 (const float*)s->c32[0], (const float*)s->c32[1], (const float*)s->c32[2],
+                 s->d, s->tl, va, mode, s->sc, s->partials, prof);
   }
-  if (eb) (void)hipEventRecord(eb, st);
   finalize<NR>(st, s->partials, FIN_ALPHA, nr, sbf, mode, s->sc);
   hipLaunchKernelGGL((k_update<float, NR, true>), dim3(vb), dim3(kBlock), 0, st, s->x32, s->r32, s->p32,
                      s->q32, va, mode, s->sc, s->partials);
