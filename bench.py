@@ -58,6 +58,8 @@ def _parse():
     ap.add_argument("--nsteps", type=int, default=200, help="c3: time steps")
     ap.add_argument("--store-every", type=int, default=10,
                     help="c3: steps between stored potentials (error samples)")
+    ap.add_argument("--chunk-steps", type=int, default=10,
+                    help="c3: time steps per solver call (a progress line each)")
     ap.add_argument("--tol", type=float, default=1e-10,
                     help="c3: relative residual of every step (the reference's 1e-10)")
     ap.add_argument("--plain-steps", type=int, default=2,
@@ -288,15 +290,36 @@ def run_c3(args):
     every = max(1, args.store_every)
 
     def run(method, islands, steps, store_every):
+        """The transient in chunks of --chunk-steps steps (each continuing from
+        the last potential), one log line per chunk; returns the stored
+        potentials (every store_every steps, step 0 first), terminal currents,
+        device solve seconds, wall seconds and the linear-solver info."""
         cfg = config_for(method)
         cfg.tol = args.tol
         cfg.deflate_islands = islands
         sol = EQSSolver(g, eps, sigma, cfg)
-        t0 = time.perf_counter()
-        r = sol.solve(terms, t_end=steps * dt, dt=dt, store=True, store_every=store_every)
-        wall = time.perf_counter() - t0
-        lin = r.meta.get("linear", {})
-        return r, wall, lin
+        chunk = max(store_every, (max(1, args.chunk_steps) // store_every) * store_every)
+        phis, curs, solve_s, wall, its = [], [], 0.0, 0.0, 0
+        phi, k0, lin = None, 0, {}
+        while k0 < steps:
+            k1 = min(steps, k0 + chunk)
+            t0 = time.perf_counter()
+            r = sol.solve(terms, t_end=k1 * dt, dt=dt, phi0=phi, t_start=k0 * dt, store=True,
+                          store_every=store_every)
+            wall += time.perf_counter() - t0
+            lin = r.meta.get("linear", {})
+            solve_s += float(lin.get("solve_time") or 0.0)
+            its += int(lin.get("iterations") or 0)
+            st = r.fields["phi"]
+            phis.extend(list(st if k0 == 0 else st[1:]))
+            cur = np.stack([r.terminals[t.name]["i"] for t in terms])
+            curs.append(cur if k0 == 0 else cur[:, 1:])
+            phi = r.meta["phi_final"]
+            log(f"    {method}{'+islands' if islands else ''}: steps {k0}-{k1} done, "
+                f"{wall:.1f} s wall, {solve_s:.1f} s device, {its} iterations")
+            k0 = k1
+        lin = dict(lin, iterations=its)
+        return np.array(phis), np.concatenate(curs, axis=1), solve_s, wall, lin
 
     methods = [m for m in args.methods.split(",") if m in ("hip_pcg", "hip_mixed")] or \
         ["hip_pcg", "hip_mixed"]
@@ -305,23 +328,20 @@ def run_c3(args):
     out_m, ref, ref_i = {}, None, None
     for m in methods:
         log(f"  {m} with the island space, {nsteps} steps")
-        r, wall, lin = run(m, True, nsteps, every)
-        phi = r.fields["phi"]
-        cur = np.stack([r.terminals[t.name]["i"] for t in terms])
+        phi, cur, solve_s, wall, lin = run(m, True, nsteps, every)
         if ref is None:
             ref, ref_i = phi, cur
         err = [float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)) for a, b in zip(phi, ref)]
         ierr = float(np.abs(cur - ref_i).max() / max(np.abs(ref_i).max(), 1e-300))
         out_m[m + "+islands"] = {
-            "wall_s": wall, "ms_per_step": 1e3 * wall / nsteps, "islands": lin.get("islands"),
-            "iterations_total": lin.get("iterations"), "refreshes": lin.get("refreshes"),
+            "solve_s": solve_s, "wall_s": wall, "ms_per_step": 1e3 * solve_s / nsteps,
+            "islands": lin.get("islands"), "iterations_total": lin.get("iterations"),
             "phi_rel_err_vs_fp64_max": max(err),
             "phi_rel_err_vs_fp64_stored_steps": err,
             "stored_every": every, "terminal_current_rel_err_vs_fp64_max": ierr}
-        log(f"    {wall:.1f} s ({1e3 * wall / nsteps:.1f} ms/step), {lin.get('islands')} islands, "
-            f"iterations {lin.get('iterations')}, max phi err vs FP64 {max(err):.2e}, "
-            f"current err {ierr:.2e}")
-        del r
+        log(f"    {solve_s:.1f} s device ({1e3 * solve_s / nsteps:.1f} ms/step), {wall:.1f} s wall, "
+            f"{lin.get('islands')} islands, iterations {lin.get('iterations')}, max phi err vs "
+            f"FP64 {max(err):.2e}, current err {ierr:.2e}")
     plain = None
     if args.plain_steps > 0:
         k = args.plain_steps
@@ -330,21 +350,21 @@ def run_c3(args):
         with warnings.catch_warnings(record=True) as wlist:
             warnings.simplefilter("always")
             try:
-                r, wall, lin = run("hip_pcg", False, k, 1)
-                phi_p = r.fields["phi"]
+                phi_p, _, solve_p, wall_p, lin = run("hip_pcg", False, k, 1)
                 status = "returned"
             except Exception as exc:           # ConvergenceError at the cap
-                phi_p, wall, lin, status = None, float("nan"), {}, f"raised {type(exc).__name__}"
-        r2, _, _ = run("hip_pcg", True, k, 1)
-        ref_p = r2.fields["phi"]
+                phi_p, solve_p, wall_p, lin = None, float("nan"), float("nan"), {}
+                status = f"raised {type(exc).__name__}"
+        ref_p = run("hip_pcg", True, k, 1)[0]
         err_p = None
         if phi_p is not None:
             err_p = [float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
                      for a, b in zip(phi_p, ref_p)]
-        plain = {"steps": k, "wall_s": wall, "ms_per_step": 1e3 * wall / k, "status": status,
+        plain = {"steps": k, "solve_s": solve_p, "wall_s": wall_p,
+                 "ms_per_step": 1e3 * solve_p / k, "status": status,
                  "iterations_total": lin.get("iterations"), "residual": lin.get("residual_eq"),
                  "warnings": len(wlist), "phi_rel_err_vs_fp64_islands": err_p}
-        log(f"    {status}: {wall:.1f} s, iterations {lin.get('iterations')}, "
+        log(f"    {status}: {solve_p:.1f} s device, iterations {lin.get('iterations')}, "
             f"residual {lin.get('residual_eq')}, phi err {err_p}")
     head = "hip_mixed+islands" if "hip_mixed+islands" in out_m else list(out_m)[0]
     fp = out_m.get("hip_pcg+islands")
@@ -360,7 +380,7 @@ def run_c3(args):
                    "seed": seed, "method": head, "steps": nsteps, "tol": args.tol},
         "methods": out_m,
         "plain_reference_method": plain,
-        "speedup_vs_fp64": {m: fp["wall_s"] / r["wall_s"] for m, r in out_m.items()} if fp else None,
+        "speedup_vs_fp64": {m: fp["solve_s"] / r["solve_s"] for m, r in out_m.items()} if fp else None,
     }
     print(json.dumps(out), flush=True)
 
