@@ -47,7 +47,10 @@ __device__ __forceinline__ void voxel_ijk(const Grid3& g, int64_t v, int& i, int
 
 // ---------------------------------------------------------------------------
 // forward convolution: Y = mask(act(conv(X, W) + b + skip))
-// block = 4 waves, each 16 voxels along x; grid (ceil(nx/64), ny, nz*nb)
+// block = 4 waves over 16 voxels along x each; for short rows (nx <= 32) the
+// waves of a block split the K loop (27*CIN) into ks slices over the same
+// voxels instead (ks = 64 / width of the block's x range), and the partial
+// tiles meet in LDS.  grid (ceil(nx/(64/ks)), ny, nz*nb)
 // ---------------------------------------------------------------------------
 template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X,
@@ -56,39 +59,59 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X
                                                   const uint16_t* __restrict__ skip,
                                                   const uint16_t* __restrict__ mask,
                                                   uint16_t* __restrict__ Y, Grid3 g, int relu,
-    const int* __restrict__ gate) {
-  if (gate && *gate) return;  // every right-hand side of the batch is idle
+    const int* __restrict__ gate, int ks) {
+  if (gate && *gate) return;  // every right-hand side of the batch is idle (uniform)
   constexpr int NT = COUT / 16;
+  __shared__ floatx4 red[4][NT][64];
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-  const int i0 = blockIdx.x * 64 + wave * 16;
+  const int vt = wave / ks, slice = wave % ks;
+  const int i0 = blockIdx.x * (64 / ks) + vt * 16;
   const int j = blockIdx.y;
   const int k = blockIdx.z % g.nz, b = blockIdx.z / g.nz;
-  if (i0 >= g.nx) return;  // whole wave outside (uniform per wave)
+  const bool active = i0 < g.nx;  // uniform per wave
   const int64_t xoff = (int64_t)b * g.n * CIN, yoff = (int64_t)b * g.n * COUT;
   const int m = lane % 16, kq = lane / 16;
   const int i = i0 + m;
   floatx4 acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-  for (int kb = 0; kb < kpad; kb += 32) {
-    const int kk = kb + 8 * kq;
-    const int tap = kk / CIN, ci0 = kk % CIN;
-    uint4 araw = make_uint4(0u, 0u, 0u, 0u);
-    if (tap < 27) {
-      const int si = i + tap % 3 - 1, sj = j + (tap / 3) % 3 - 1, sk = k + tap / 9 - 1;
-      if (i < g.nx && si >= 0 && si < g.nx && sj >= 0 && sj < g.ny && sk >= 0 && sk < g.nz) {
-        const int64_t sv = ((int64_t)sk * g.ny + sj) * g.nx + si;
-        araw = *reinterpret_cast<const uint4*>(X + xoff + sv * CIN + ci0);
+  const int nit = kpad / 32, per = (nit + ks - 1) / ks;
+  const int kb0 = slice * per * 32, kb1 = min(nit, (slice + 1) * per) * 32;
+  if (active) {
+    for (int kb = kb0; kb < kb1; kb += 32) {
+      const int kk = kb + 8 * kq;
+      const int tap = kk / CIN, ci0 = kk % CIN;
+      uint4 araw = make_uint4(0u, 0u, 0u, 0u);
+      if (tap < 27) {
+        const int si = i + tap % 3 - 1, sj = j + (tap / 3) % 3 - 1, sk = k + tap / 9 - 1;
+        if (i < g.nx && si >= 0 && si < g.nx && sj >= 0 && sj < g.ny && sk >= 0 && sk < g.nz) {
+          const int64_t sv = ((int64_t)sk * g.ny + sj) * g.nx + si;
+          araw = *reinterpret_cast<const uint4*>(X + xoff + sv * CIN + ci0);
+        }
+      }
+      const bf16x8 a = __builtin_bit_cast(bf16x8, araw);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const uint4 braw = *reinterpret_cast<const uint4*>(Wt + (int64_t)(16 * t + m) * kpad + kk);
+        const bf16x8 bw = __builtin_bit_cast(bf16x8, braw);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc[t], 0, 0, 0);
       }
     }
-    const bf16x8 a = __builtin_bit_cast(bf16x8, araw);
+  }
+  if (ks > 1) {  // uniform over the block
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const uint4 braw = *reinterpret_cast<const uint4*>(Wt + (int64_t)(16 * t + m) * kpad + kk);
-      const bf16x8 bw = __builtin_bit_cast(bf16x8, braw);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc[t], 0, 0, 0);
+    for (int t = 0; t < NT; ++t) red[wave][t][lane] = acc[t];
+    __syncthreads();
+    if (slice != 0) return;
+    for (int w = 1; w < ks; ++w) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const floatx4 o = red[wave + w][t][lane];
+        acc[t] = floatx4{acc[t][0] + o[0], acc[t][1] + o[1], acc[t][2] + o[2], acc[t][3] + o[3]};
+      }
     }
   }
+  if (!active) return;
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int co = 16 * t + m;
@@ -387,9 +410,11 @@ template <int CIN, int COUT>
 static void conv_fwd_t(hipStream_t s, const uint16_t* X, const uint16_t* Wt, int kpad,
                        const float* bias, const uint16_t* skip, const uint16_t* mask,
                        uint16_t* Y, const Grid3& g, int relu, const int* gate) {
-  dim3 grid((g.nx + 63) / 64, g.ny, g.nz * g.nb);
+  // short rows: the 4 waves of a block split K instead of covering 64 voxels
+  const int ks = g.nx <= 16 ? 4 : (g.nx <= 32 ? 2 : 1);
+  dim3 grid((g.nx + 64 / ks - 1) / (64 / ks), g.ny, g.nz * g.nb);
   hipLaunchKernelGGL((k_conv_fwd<CIN, COUT>), grid, dim3(256), 0, s, X, Wt, kpad, bias, skip,
-                     mask, Y, g, relu, gate);
+                     mask, Y, g, relu, gate, ks);
 }
 
 #define FS_NN_SHAPES(X) X(16, 16) X(16, 32) X(32, 16) X(32, 32) X(32, 64) X(64, 32) X(64, 64)
