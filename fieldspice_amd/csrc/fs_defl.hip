@@ -14,8 +14,9 @@
 // equilibrated space, and E = Z^T A_s Z (= 1_I^T L_ff 1_J of the unscaled
 // masked operator), inverted once on the host.  The preconditioner is the
 // additive M = I + Z E^-1 Z^T (symmetric positive definite, so plain PCG
-// applies): per iteration one segmented reduction over the island nodes,
-// one small dense product per right-hand side and one scatter.
+// applies): per iteration one segmented reduction over the island nodes
+// (chunk partials, then one wave per island), the dense E^-1 product (one
+// wave per row), a fixed-order sum for zr and beta, and one scatter.
 //
 // FP32 cycles of a deflated system run on the unscaled operator in
 // difference form, y_n = e_n u_n + sum_e w_e (u_n - u_m) with u = D^-1/2 x_s.
@@ -70,33 +71,66 @@ __global__ __launch_bounds__(kBlock) void k_isl_sums(const T* __restrict__ v,
   }
 }
 
-// One block per RHS: s = Z^T r from the chunk partials (fixed order), the
-// coefficients c = E^-1 s, and zr = <M r, r> = rho + c.s.  first: start of a
-// (FP64 or FP32-cycle) iteration sequence, beta stays 0; otherwise
-// beta = zr_new / zr_old (the stop tests of k_finalize already ran on rho).
-__global__ __launch_bounds__(kBlock) void k_defl_coef(const double* __restrict__ part,
-                                                      int nchunks,
-                                                      const int32_t* __restrict__ ichunk,
-                                                      const double* __restrict__ einv, int ni,
-                                                      double* __restrict__ coef, int mode,
-                                                      int first, CgScalars* sc) {
+// s = Z^T r: one wave per island (4 per block) sums its chunk partials in a
+// fixed order (lanes stride the chunks, then a wave reduction).
+// grid (ceil(ni / 4), nr).
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_isl_reduce(const double* __restrict__ part,
+                                                       int nchunks,
+                                                       const int32_t* __restrict__ ichunk,
+                                                       int ni, double* __restrict__ sv,
+                                                       int mode, int first,
+                                                       const CgScalars* sc) {
+  const int r = blockIdx.y;
+  if (first ? sc[r].done != 0 : !rhs_live(sc, r, mode)) return;  // uniform over the block
+  const int lane = threadIdx.x % kWave;
+  const int isl = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (isl >= ni) return;
+  double a = 0.0;
+  for (int c = ichunk[isl] + lane; c < ichunk[isl + 1]; c += kWave)
+    a += part[(int64_t)r * nchunks + c];
+  a = wave_sum_f64(a);
+  if (lane == 0) sv[(int64_t)r * kMaxIslands + isl] = a;
+}
+
+// c = E^-1 s, one wave per row (coalesced row reads), and the products c_I s_I.
+// grid (ceil(ni / 4), nr).
+__global__ __launch_bounds__(kBlock) void k_defl_matvec(const double* __restrict__ einv, int ni,
+                                                        const double* __restrict__ sv,
+                                                        double* __restrict__ coef,
+                                                        double* __restrict__ cs, int mode,
+                                                        int first, const CgScalars* sc) {
+  const int r = blockIdx.y;
+  if (first ? sc[r].done != 0 : !rhs_live(sc, r, mode)) return;  // uniform over the block
+  const int lane = threadIdx.x % kWave;
+  const int row = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (row >= ni) return;
+  const double* e = einv + (int64_t)row * ni;
+  const double* s = sv + (int64_t)r * kMaxIslands;
+  double a = 0.0;
+  for (int j = lane; j < ni; j += kWave) a += e[j] * s[j];
+  a = wave_sum_f64(a);
+  if (lane == 0) {
+    coef[(int64_t)r * kMaxIslands + row] = a;
+    cs[(int64_t)r * kMaxIslands + row] = a * s[row];
+  }
+}
+
+// One block per RHS: zr = <M r, r> = rho + sum_I c_I s_I (fixed order).
+// first: start of a (FP64 or FP32-cycle) iteration sequence, beta stays 0;
+// otherwise beta = zr_new / zr_old (the stop tests of k_finalize already ran
+// on rho).
+__global__ __launch_bounds__(kBlock) void k_defl_finish(const double* __restrict__ cs, int ni,
+                                                        int mode, int first, CgScalars* sc) {
   const int r = blockIdx.x;
   if (first ? sc[r].done != 0 : !rhs_live(sc, r, mode)) return;  // uniform over the block
-  __shared__ double s[kMaxIslands];
-  for (int i = threadIdx.x; i < ni; i += kBlock) {
-    double a = 0.0;
-    for (int c = ichunk[i]; c < ichunk[i + 1]; ++c) a += part[(int64_t)r * nchunks + c];
-    s[i] = a;
-  }
-  __syncthreads();
   double acc[1] = {0.0};
-  for (int i = threadIdx.x; i < ni; i += kBlock) {
-    const double* row = einv + (int64_t)i * ni;
-    double ci = 0.0;
-    for (int j = 0; j < ni; ++j) ci += row[j] * s[j];
-    coef[(int64_t)r * kMaxIslands + i] = ci;
-    acc[0] += ci * s[i];
-  }
+  for (int i = threadIdx.x; i < ni; i += kBlock) acc[0] += cs[(int64_t)r * kMaxIslands + i];
   block_sum<1>(acc);
   if (threadIdx.x != 0) return;
   CgScalars& c = sc[r];
@@ -357,8 +391,16 @@ static void defl_coef_t(fs_sys* s, int nr, int mode, int first) {
     hipLaunchKernelGGL((k_isl_sums<float, NR, false>), dim3(nc), dim3(kBlock), 0, s->stream,
                        s->r32, s->dhalf, s->defl_nodes, s->defl_cstart, nc, va, mode, s->sc,
                        s->defl_part);
-  hipLaunchKernelGGL(k_defl_coef, dim3(nr), dim3(kBlock), 0, s->stream, s->defl_part, nc,
-                     s->defl_ichunk, s->defl_einv, s->defl_n, s->defl_coef, mode, first, s->sc);
+  const int ni = s->defl_n;
+  const dim3 g2((ni + kBlock / kWave - 1) / (kBlock / kWave), nr);
+  double* sv = s->defl_coef + (size_t)kMaxRhs * kMaxIslands;      // s = Z^T r
+  double* cs = s->defl_coef + (size_t)2 * kMaxRhs * kMaxIslands;  // c_I s_I
+  hipLaunchKernelGGL(k_isl_reduce, g2, dim3(kBlock), 0, s->stream, s->defl_part, nc,
+                     s->defl_ichunk, ni, sv, mode, first, s->sc);
+  hipLaunchKernelGGL(k_defl_matvec, g2, dim3(kBlock), 0, s->stream, s->defl_einv, ni, sv,
+                     s->defl_coef, cs, mode, first, s->sc);
+  hipLaunchKernelGGL(k_defl_finish, dim3(nr), dim3(kBlock), 0, s->stream, cs, ni, mode, first,
+                     s->sc);
 }
 
 template <int NR>
@@ -707,7 +749,7 @@ int fs_sys_set_deflation(fs_sys* sys, int n_islands, const int32_t* island) {
       (rc = dalloc(&sys->defl_ichunk, ichunk.size())) ||
       (rc = dalloc(&sys->defl_einv, Einv.size())) ||
       (rc = dalloc(&sys->defl_part, (size_t)kMaxRhs * nchunks)) ||
-      (rc = dalloc(&sys->defl_coef, (size_t)kMaxRhs * kMaxIslands)) ||
+      (rc = dalloc(&sys->defl_coef, (size_t)3 * kMaxRhs * kMaxIslands)) ||
       (rc = dalloc(&sys->u_w[0], (size_t)n)) || (rc = dalloc(&sys->u_w[1], (size_t)n)) ||
       (rc = dalloc(&sys->u_w[2], (size_t)n)) || (rc = dalloc(&sys->u_dinv, (size_t)n)) ||
       (sys->op->extra && (rc = dalloc(&sys->u_extra, (size_t)n)))) {

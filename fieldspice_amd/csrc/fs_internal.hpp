@@ -134,7 +134,7 @@ struct fs_sys {
   int32_t* defl_ichunk = nullptr;  // chunks of island I: ichunk[I] .. ichunk[I+1]
   double* defl_einv = nullptr;     // E^-1 (defl_n x defl_n, row major)
   double* defl_part = nullptr;     // chunk partials [kMaxRhs][defl_nchunks]
-  double* defl_coef = nullptr;     // E^-1 Z^T r per RHS [kMaxRhs][kMaxIslands]
+  double* defl_coef = nullptr;     // [3][kMaxRhs][kMaxIslands]: E^-1 Z^T r, Z^T r, products
   // FP32 cycles of a deflated system run on the unscaled operator in
   // difference form: y_n = e_n u_n + sum_e w_e (u_n - u_m), u = D^-1/2 x_s
   float* u_w[3] = {nullptr, nullptr, nullptr};  // edge weights (FP32, x-fast)
