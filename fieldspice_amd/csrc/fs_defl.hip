@@ -249,11 +249,125 @@ __global__ __launch_bounds__(kBlock) void k_stencil_diff(const float* __restrict
   }
 }
 
-// x += alpha p; r -= alpha q; partial sum wdot r^2 (the equilibrated norm of
-// an unscaled residual, wdot = 1/D).
+// k_stencil_diff with 4 consecutive x nodes per lane (rows a multiple of 4):
+// one 16-byte load per array and plane, every right-hand side advanced in the
+// same z-march, x neighbours at a lane's ends from the adjacent lanes
+// (shuffles) or from memory at the edges of the wave; grid of 256-node row
+// segments.
+typedef float f4d __attribute__((ext_vector_type(4)));
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_update_w(float* __restrict__ x, float* __restrict__ r_,
-                                                     const float* __restrict__ p,
+__global__ __launch_bounds__(kBlock) void k_stencil_diff4(const float* __restrict__ p,
+                                                          float* __restrict__ q,
+                                                          const float* __restrict__ wx,
+                                                          const float* __restrict__ wy,
+                                                          const float* __restrict__ wz,
+                                                          const float* __restrict__ ex, Dims d,
+                                                          Tiling tl, VecArgs va, int mode,
+                                                          CgScalars* sc, double* partials,
+                                                          unsigned long long* prof) {
+  if (all_idle(sc, va.nr, mode)) return;
+  bool live[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) live[r] = r < va.nr && rhs_live(sc, r, mode);
+  if (prof && blockIdx.x == 0 && threadIdx.x == 0) {
+    unsigned long long nl = 0;
+    for (int r = 0; r < NR; ++r) nl += live[r] ? 1ull : 0ull;
+    prof[0] += 1ull;
+    prof[1] += nl;
+  }
+  double acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+  int tx, ty, tz;
+  if (tile_of_block(tl, tx, ty, tz)) {
+    const int lane = (int)(threadIdx.x % kWave);
+    const int i = (tx * kWave + lane) * 4;
+    const int j = ty * kTileY + (int)(threadIdx.x / kWave);
+    const int k0 = tz * tl.kz;
+    const int k1 = min(d.nz, k0 + tl.kz);
+    const bool valid = i < d.nx && j < d.ny;  // i .. i+3 inside: nx % 4 == 0
+    const int64_t base = valid ? (int64_t)j * d.nx + i : 0;
+    const bool has_w = i > 0, has_e = i + 4 < d.nx, has_n = j + 1 < d.ny, has_s = j > 0;
+    int64_t nn = (int64_t)k0 * d.sxy + base;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 pm[NR], pc[NR];
+    float4 wzm = z4;
+    if (valid && k0 > 0) wzm = *reinterpret_cast<const float4*>(wz + nn - d.sxy);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      pm[r] = z4;
+      pc[r] = z4;
+      if (valid && live[r]) {
+        if (k0 > 0) pm[r] = *reinterpret_cast<const float4*>(p + r * va.vs + nn - d.sxy);
+        pc[r] = *reinterpret_cast<const float4*>(p + r * va.vs + nn);
+      }
+    }
+    for (int k = k0; k < k1; ++k, nn += d.sxy) {
+      float4 wxc = z4, wyc = z4, wzc = z4, wym = z4, e4 = z4;
+      if (valid) {
+        wxc = *reinterpret_cast<const float4*>(wx + nn);
+        wyc = *reinterpret_cast<const float4*>(wy + nn);
+        wzc = *reinterpret_cast<const float4*>(wz + nn);
+        if (has_s) wym = *reinterpret_cast<const float4*>(wy + nn - d.nx);
+        if (ex) e4 = *reinterpret_cast<const float4*>(ex + nn);
+      }
+      // weight of the edge west of this lane's first node
+      const float wxw_l = __shfl_up(wxc.w, 1, kWave);
+      float wxw = 0.f;
+      if (valid && has_w) wxw = lane > 0 ? wxw_l : wx[nn - 1];
+      const bool up = k + 1 < d.nz;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        if (!live[r]) continue;  // uniform over the launch
+        const float* __restrict__ pr = p + r * va.vs;
+        float4 pu = z4, pn = z4, ps = z4;
+        if (valid) {
+          if (up) pu = *reinterpret_cast<const float4*>(pr + nn + d.sxy);
+          if (has_n) pn = *reinterpret_cast<const float4*>(pr + nn + d.nx);
+          if (has_s) ps = *reinterpret_cast<const float4*>(pr + nn - d.nx);
+        }
+        const float4 c = pc[r];
+        float pw = __shfl_up(c.w, 1, kWave), pe = __shfl_down(c.x, 1, kWave);
+        if (valid) {
+          if (!has_w) pw = 0.f;
+          else if (lane == 0) pw = pr[nn - 1];
+          if (!has_e) pe = 0.f;
+          else if (lane == kWave - 1) pe = pr[nn + 4];
+        }
+        const float4 m = pm[r];
+        // edges past the boundary carry weight 0 (node-padded arrays)
+        f4d y;
+        y.x = e4.x * c.x + wxc.x * (c.x - c.y) + wxw * (c.x - pw) + wyc.x * (c.x - pn.x) +
+              wym.x * (c.x - ps.x) + wzc.x * (c.x - pu.x) + wzm.x * (c.x - m.x);
+        y.y = e4.y * c.y + wxc.y * (c.y - c.z) + wxc.x * (c.y - c.x) + wyc.y * (c.y - pn.y) +
+              wym.y * (c.y - ps.y) + wzc.y * (c.y - pu.y) + wzm.y * (c.y - m.y);
+        y.z = e4.z * c.z + wxc.z * (c.z - c.w) + wxc.y * (c.z - c.y) + wyc.z * (c.z - pn.z) +
+              wym.z * (c.z - ps.z) + wzc.z * (c.z - pu.z) + wzm.z * (c.z - m.z);
+        y.w = e4.w * c.w + wxc.w * (c.w - pe) + wxc.z * (c.w - c.z) + wyc.w * (c.w - pn.w) +
+              wym.w * (c.w - ps.w) + wzc.w * (c.w - pu.w) + wzm.w * (c.w - m.w);
+        if (valid) {
+          __builtin_nontemporal_store(y, reinterpret_cast<f4d*>(q + r * va.vs + nn));
+          acc[r] += (double)(c.x * y.x + c.y * y.y + c.z * y.z + c.w * y.w);
+        }
+        pm[r] = c;
+        pc[r] = pu;
+      }
+      wzm = wzc;
+    }
+  }
+  block_sum<NR>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (r < va.nr) store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
+  }
+}
+
+// r -= alpha q; partial sum wdot r^2 (the equilibrated norm of an unscaled
+// residual, wdot = 1/D).  x += alpha p is deferred to the next k_pupdate_w
+// (which reads p anyway); the refresh adds the pending step of a cycle.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_update_w(float* __restrict__ r_,
                                                      const float* __restrict__ q,
                                                      const float* __restrict__ wdot, VecArgs va,
                                                      int mode, CgScalars* sc, double* partials) {
@@ -266,13 +380,10 @@ __global__ __launch_bounds__(kBlock) void k_update_w(float* __restrict__ x, floa
   for (int r = 0; r < NR; ++r) {
     if (!(r < va.nr && rhs_live(sc, r, mode))) continue;
     const float a = (float)sc[r].alpha;
-    float* xr = x + r * va.vs;
     float* rr = r_ + r * va.vs;
-    const float* pr = p + r * va.vs;
     const float* qr = q + r * va.vs;
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride) {
-      xr[i] += a * pr[i];
       const float rn = rr[i] - a * qr[i];
       rr[i] = rn;
       s += (double)wdot[i] * (double)rn * (double)rn;
@@ -287,9 +398,11 @@ __global__ __launch_bounds__(kBlock) void k_update_w(float* __restrict__ x, floa
   }
 }
 
-// p = wdir r + beta p
+// x += alpha p (the previous step's, deferred; alpha is 0 at a cycle start),
+// then p = wdir r + beta p
 template <int NR>
 __global__ __launch_bounds__(kBlock) void k_pupdate_w(float* __restrict__ p,
+                                                      float* __restrict__ x,
                                                       const float* __restrict__ r_,
                                                       const float* __restrict__ wdir,
                                                       VecArgs va, int mode,
@@ -299,11 +412,15 @@ __global__ __launch_bounds__(kBlock) void k_pupdate_w(float* __restrict__ p,
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     if (!(r < va.nr && rhs_live(sc, r, mode))) continue;
-    const float b = (float)sc[r].beta;
+    const float b = (float)sc[r].beta, a = (float)sc[r].alpha;
     float* pr = p + r * va.vs;
+    float* xr = x + r * va.vs;
     const float* rr = r_ + r * va.vs;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride)
-      pr[i] = wdir[i] * rr[i] + b * pr[i];
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride) {
+      const float pv = pr[i];
+      xr[i] += a * pv;
+      pr[i] = wdir[i] * rr[i] + b * pv;
+    }
   }
 }
 
@@ -343,10 +460,12 @@ __global__ __launch_bounds__(kBlock) void k_cycle_start_u(const double* __restri
   }
 }
 
-// x64 += dhalf * d32 / scale (the cycle solved for unscaled potentials)
+// x64 += dhalf * (d32 + alpha p32) / scale (the cycle solved for unscaled
+// potentials; alpha p32 is its last step, whose x update was deferred)
 template <int NR>
 __global__ __launch_bounds__(kBlock) void k_cycle_accumulate_u(double* __restrict__ x64,
                                                                const float* __restrict__ d32,
+                                                               const float* __restrict__ p32,
                                                                const double* __restrict__ dhalf,
                                                                VecArgs va, const CgScalars* sc) {
   const int64_t stride = (int64_t)gridDim.x * kBlock;
@@ -354,8 +473,10 @@ __global__ __launch_bounds__(kBlock) void k_cycle_accumulate_u(double* __restric
   for (int r = 0; r < NR; ++r) {
     if (!(r < va.nr && sc[r].done == 0)) continue;
     const double inv = 1.0 / sc[r].scale;
+    const float a = (float)sc[r].alpha;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride)
-      x64[r * va.vs + i] += dhalf[i] * (double)d32[r * va.vs + i] * inv;
+      x64[r * va.vs + i] +=
+          dhalf[i] * (double)(d32[r * va.vs + i] + a * p32[r * va.vs + i]) * inv;
   }
 }
 
@@ -424,15 +545,23 @@ static int iteration_defl_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent
   const int vb = s->vec_blocks, sb = s->tl.nblocks;
   unsigned long long* prof = ea ? s->prof_dev : nullptr;
   // FP32 cycle on unscaled potentials: p = (D^-1 + 1_I E^-1 1_I^T) r + beta p
-  hipLaunchKernelGGL((k_pupdate_w<NR>), dim3(vb), dim3(kBlock), 0, st, s->p32, s->r32,
+  hipLaunchKernelGGL((k_pupdate_w<NR>), dim3(vb), dim3(kBlock), 0, st, s->p32, s->x32, s->r32,
                      s->u_dinv, va, mode, s->sc);
   defl_scatter_t<NR>(s, nr, mode);
-  launch_timed(k_stencil_diff<NR>, dim3(sb), st, ea, eb, (const float*)s->p32, s->q32,
-               (const float*)s->u_w[0], (const float*)s->u_w[1], (const float*)s->u_w[2],
-               (const float*)s->u_extra, s->d, s->tl, va, mode, s->sc, s->partials, prof);
-  launch_finalize(st, s->partials, FIN_ALPHA_ZR, nr, sb, mode, s->sc);
-  hipLaunchKernelGGL((k_update_w<NR>), dim3(vb), dim3(kBlock), 0, st, s->x32, s->r32, s->p32,
-                     s->q32, s->u_dinv, va, mode, s->sc, s->partials);
+  int sbf = sb;
+  if (s->use_x4) {
+    sbf = s->tl4.nblocks;
+    launch_timed(k_stencil_diff4<NR>, dim3(sbf), st, ea, eb, (const float*)s->p32, s->q32,
+                 (const float*)s->u_w[0], (const float*)s->u_w[1], (const float*)s->u_w[2],
+                 (const float*)s->u_extra, s->d, s->tl4, va, mode, s->sc, s->partials, prof);
+  } else {
+    launch_timed(k_stencil_diff<NR>, dim3(sb), st, ea, eb, (const float*)s->p32, s->q32,
+                 (const float*)s->u_w[0], (const float*)s->u_w[1], (const float*)s->u_w[2],
+                 (const float*)s->u_extra, s->d, s->tl, va, mode, s->sc, s->partials, prof);
+  }
+  launch_finalize(st, s->partials, FIN_ALPHA_ZR, nr, sbf, mode, s->sc);
+  hipLaunchKernelGGL((k_update_w<NR>), dim3(vb), dim3(kBlock), 0, st, s->r32, s->q32,
+                     s->u_dinv, va, mode, s->sc, s->partials);
   launch_finalize(st, s->partials, FIN_BETA, nr, vb, mode, s->sc);
   defl_coef_t<NR>(s, nr, mode, 0);
   return 0;
@@ -476,7 +605,7 @@ template <int NR>
 static void refresh_defl_t(fs_sys* s, int nr) {
   VecArgs va{s->d.n, s->vstride, nr};
   hipLaunchKernelGGL((k_cycle_accumulate_u<NR>), dim3(s->vec_blocks), dim3(kBlock), 0,
-                     s->stream, s->x, s->x32, s->dhalf, va, s->sc);
+                     s->stream, s->x, s->x32, s->p32, s->dhalf, va, s->sc);
   launch_residual(s, nr, s->r);
   launch_refresh_finish(s, nr);
 }
