@@ -64,6 +64,9 @@ def _parse():
                     help="c3: relative residual of every step (the reference's 1e-10)")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="c3: steps of the plain Jacobi-PCG run beside the island-space runs")
+    ap.add_argument("--guess", choices=("extrapolate", "previous", "zero"), default="extrapolate",
+                    help="c3: initial guess of every step's solve (2 phi_{k-1} - phi_{k-2}, "
+                         "phi_{k-1}, or 0 as the reference's)")
     ap.add_argument("--instances", type=int, default=64, help="c5: number of systems")
     ap.add_argument("--chunk", type=int, default=4, help="c5: iterations per graph replay")
     ap.add_argument("--n", type=int, default=256, help="c4: nodes per side")
@@ -277,9 +280,10 @@ def run_c3(args):
     """EQS backward-Euler transient on the two-phase copper/SiO2 geometry:
     time per step and the per-step potential error of the mixed-precision
     solve against the FP64 solve, both with the island coarse space (one
-    coarse vector per floating copper island).  The reference's own method
-    (Jacobi-PCG, no island space) is run for --plain-steps steps beside them:
-    it stops at its iteration cap on this system."""
+    coarse vector per floating copper island), every step started from the
+    --guess field.  The reference's own method (Jacobi-PCG from a zero guess,
+    no island space) is run for --plain-steps steps beside them: it stops at
+    its iteration cap on this system."""
     from fieldspice_amd import workloads
     from fieldspice_amd.solvers.eqs import EQSSolver
     seed = 0 if args.seed is None else args.seed
@@ -289,7 +293,7 @@ def run_c3(args):
     dt, nsteps = 1e-12, args.nsteps
     every = max(1, args.store_every)
 
-    def run(method, islands, steps, store_every):
+    def run(method, islands, steps, store_every, guess=None):
         """The transient in chunks of --chunk-steps steps (each continuing from
         the last potential), one log line per chunk; returns the stored
         potentials (every store_every steps, step 0 first), terminal currents,
@@ -300,12 +304,13 @@ def run_c3(args):
         sol = EQSSolver(g, eps, sigma, cfg)
         chunk = max(store_every, (max(1, args.chunk_steps) // store_every) * store_every)
         phis, curs, solve_s, wall, its = [], [], 0.0, 0.0, 0
-        phi, k0, lin = None, 0, {}
+        phi, phi_prev, k0, lin = None, None, 0, {}
         while k0 < steps:
             k1 = min(steps, k0 + chunk)
             t0 = time.perf_counter()
             r = sol.solve(terms, t_end=k1 * dt, dt=dt, phi0=phi, t_start=k0 * dt, store=True,
-                          store_every=store_every)
+                          store_every=store_every, initial_guess=guess or args.guess,
+                          phi_prev=phi_prev)
             wall += time.perf_counter() - t0
             lin = r.meta.get("linear", {})
             solve_s += float(lin.get("solve_time") or 0.0)
@@ -314,7 +319,7 @@ def run_c3(args):
             phis.extend(list(st if k0 == 0 else st[1:]))
             cur = np.stack([r.terminals[t.name]["i"] for t in terms])
             curs.append(cur if k0 == 0 else cur[:, 1:])
-            phi = r.meta["phi_final"]
+            phi, phi_prev = r.meta["phi_final"], r.meta["phi_before_final"]
             log(f"    {method}{'+islands' if islands else ''}: steps {k0}-{k1} done, "
                 f"{wall:.1f} s wall, {solve_s:.1f} s device, {its} iterations")
             k0 = k1
@@ -350,7 +355,8 @@ def run_c3(args):
         with warnings.catch_warnings(record=True) as wlist:
             warnings.simplefilter("always")
             try:
-                phi_p, _, solve_p, wall_p, lin = run("hip_pcg", False, k, 1)
+                # as the reference: every step's CG starts from zero
+                phi_p, _, solve_p, wall_p, lin = run("hip_pcg", False, k, 1, guess="zero")
                 status = "returned"
             except Exception as exc:           # ConvergenceError at the cap
                 phi_p, solve_p, wall_p, lin = None, float("nan"), float("nan"), {}
@@ -377,7 +383,8 @@ def run_c3(args):
         "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"EQS backward Euler, {n}^3 nodes at 10 nm, copper boxes 20% "
                                "volume in SiO2, 1 GHz 1 V drive, one grounded box, dt 1e-12 s",
-                   "seed": seed, "method": head, "steps": nsteps, "tol": args.tol},
+                   "seed": seed, "method": head, "steps": nsteps, "tol": args.tol,
+                   "initial_guess": args.guess},
         "methods": out_m,
         "plain_reference_method": plain,
         "speedup_vs_fp64": {m: fp["solve_s"] / r["solve_s"] for m, r in out_m.items()} if fp else None,

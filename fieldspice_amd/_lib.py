@@ -64,13 +64,18 @@ class SysInfo(C.Structure):
                 ("n_empty", C.c_int64), ("gauge", C.c_int64), ("singular", C.c_int)]
 
 
+# fs_transient_desc.warm_start: the initial guess of each time step's solve
+GUESSES = {"zero": 0, "previous": 1, "extrapolate": 2}
+
+
 class TransientDesc(C.Structure):
     _fields_ = [("nsteps", C.c_int), ("ngroups", C.c_int), ("group", C.c_void_p),
                 ("values", C.c_void_p), ("nsrc", C.c_int), ("src", C.c_void_p),
                 ("src_scale", C.c_void_p), ("warm_start", C.c_int), ("nprobe", C.c_int),
                 ("probe", C.c_void_p), ("probe_w", C.c_void_p), ("cur_a", C.c_void_p),
                 ("cur_b", C.c_void_p), ("quad_a", C.c_void_p), ("quad_b", C.c_void_p),
-                ("store_every", C.c_int)]
+                ("store_every", C.c_int), ("x_before", C.c_void_p),
+                ("x_before_out", C.c_void_p)]
 
 
 class LibraryError(RuntimeError):
@@ -350,8 +355,17 @@ class DeviceSystem:
 
     def transient(self, Aexp, x0, nsteps, group=None, values=None, src=None, src_scale=None,
                   probe=None, probe_w=None, nprobe=0, cur_a=None, cur_b=None, quad_a=None,
-                  quad_b=None, store_every=0, warm_start=True, params=None):
-        """Device time loop (``fs_transient_run``); returns a dict of outputs."""
+                  quad_b=None, store_every=0, initial_guess="previous", x_before=None,
+                  params=None):
+        """Device time loop (``fs_transient_run``); returns a dict of outputs.
+
+        ``initial_guess`` of every step's solve: "zero" (as the reference's
+        solves start), "previous" (the last field) or "extrapolate" (from the
+        second step on, 2 x_{k-1} - x_{k-2}; from the first when ``x_before``, the
+        field one step before ``x0``, is given)."""
+        if initial_guess not in GUESSES:
+            raise ValueError(f"initial_guess must be one of {sorted(GUESSES)}, "
+                             f"got {initial_guess!r}")
         n = self.op.n_nodes
         keep = []
 
@@ -365,11 +379,15 @@ class DeviceSystem:
         nsrc = 0 if src is None else int(np.asarray(src).reshape(-1, n).shape[0])
         d = TransientDesc(int(nsteps), ng, ptr(group, np.int32), ptr(values, np.float64), nsrc,
                           ptr(src, np.float64), ptr(src_scale, np.float64),
-                          1 if warm_start else 0, int(nprobe), ptr(probe, np.int32),
+                          GUESSES[initial_guess], int(nprobe), ptr(probe, np.int32),
                           ptr(probe_w, np.float64),
                           None if cur_a is None else cur_a._h, None if cur_b is None else cur_b._h,
                           None if quad_a is None else quad_a._h,
-                          None if quad_b is None else quad_b._h, int(store_every))
+                          None if quad_b is None else quad_b._h, int(store_every),
+                          ptr(None if x_before is None else as_f64(x_before).ravel(), np.float64),
+                          None)
+        x_last_before = np.empty(n)
+        d.x_before_out = x_last_before.ctypes.data_as(_vp)
         x = as_f64(x0).ravel().copy()
         nstore = (nsteps // store_every + 1) if store_every > 0 else 0
         stored = np.empty((nstore, n)) if nstore else None
@@ -383,8 +401,8 @@ class DeviceSystem:
                                      None if pv is None else pv.ctypes.data_as(_vp),
                                      ss.ctypes.data_as(_vp), C.byref(params), C.byref(st))
         check(rc, "fs_transient_run")
-        return {"x": x, "stored": stored, "probe_i": pi, "probe_v": pv, "step_stats": ss,
-                "stats": st.as_dict(), "rc": rc}
+        return {"x": x, "x_before": x_last_before, "stored": stored, "probe_i": pi,
+                "probe_v": pv, "step_stats": ss, "stats": st.as_dict(), "rc": rc}
 
     def prepare_capacitance(self, term_id, nt: int) -> "DeviceCapacitance":
         return DeviceCapacitance(self, term_id, nt)

@@ -997,6 +997,8 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
     return fail(FS_EINVAL, "sources need vectors and scales");
   if (D->nprobe < 0 || (D->nprobe > 0 && (!D->probe || !D->probe_w)))
     return fail(FS_EINVAL, "probes need ids and weights");
+  if (D->warm_start < 0 || D->warm_start > 2)
+    return fail(FS_EINVAL, "warm_start must be 0, 1 or 2 (got %d)", D->warm_start);
   if (params->precision != FS_PREC_FP64 && params->precision != FS_PREC_MIXED &&
       params->precision != FS_PREC_FCG)
     return fail(FS_EINVAL, "unknown precision %d", params->precision);
@@ -1012,7 +1014,8 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
   const int nsteps = D->nsteps, ng = D->ngroups, np = D->nprobe;
   if (int rc = ensure_ws(s, 1)) return rc;
   DevBufs B;
-  double *xa = nullptr, *xb = nullptr, *vals = nullptr, *src = nullptr, *scale = nullptr;
+  double *xa = nullptr, *xb = nullptr, *xc = nullptr, *vals = nullptr, *src = nullptr;
+  double* scale = nullptr;
   double *pw = nullptr, *oi = nullptr, *ov = nullptr, *ost = nullptr;
   int32_t *grp = nullptr;
   int64_t *pnodes = nullptr, *poffs = nullptr;
@@ -1022,7 +1025,13 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
       (rc = B.get(&vals, (size_t)std::max(1, (nsteps + 1) * ng))) ||
       (rc = B.get(&ost, (size_t)(nsteps + 1) * 4)))
     return rc;
+  // the field two steps back, for the extrapolated initial guess
+  if (D->warm_start == 2 && (rc = B.get(&xc, (size_t)s->vstride))) return rc;
   if ((rc = upload_permuted(ctx, x, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz, xa))) return rc;
+  const bool have_before = D->warm_start == 2 && D->x_before;
+  if (have_before &&
+      (rc = upload_permuted(ctx, D->x_before, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz, xc)))
+    return rc;
   if (ng > 0) {
     // group ids must agree with the system's Dirichlet rows
     for (int64_t i = 0; i < d.n; ++i)
@@ -1096,7 +1105,11 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
     fs::launch_explicit(s, Aexp, xa, src, scale ? scale + (size_t)k * D->nsrc : nullptr, D->nsrc,
                         s->vstride, s->node_tmp);
     fs::launch_lift_groups(s, s->node_tmp, grp, vk);
-    if (D->warm_start) fs::launch_init_warm(s, xa, params->tol, maxiter);
+    // initial guess: 0, the previous field, or the linear extrapolation
+    // 2 x_{k-1} - x_{k-2} (from the second step on unless x_{-1} was given)
+    const bool extrap = D->warm_start == 2 && (k >= 2 || have_before);
+    if (D->warm_start)
+      fs::launch_init_warm(s, xa, extrap ? xc : nullptr, params->tol, maxiter);
     else fs::launch_init(s, 1, params->tol, maxiter);
     FS_HIP(hipGetLastError());
     if ((rc = run_krylov(s, 1, *params))) return rc;
@@ -1123,7 +1136,8 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
       status = FS_NOT_CONVERGED;
       if (stats) stats->converged = 0;
     }
-    std::swap(xa, xb);
+    if (D->warm_start == 2) std::swap(xc, xa);  // xc <- x_{k-1}
+    std::swap(xa, xb);                          // xa <- x_k
     if (store_every > 0 && k % store_every == 0 && (rc = store(xa))) return rc;
   }
   FS_HIP(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1131,6 +1145,13 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
   float ms = 0.0f;
   (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
   if ((rc = download_nodes(ctx, d, xa, x, s->r))) return rc;
+  // x_{nsteps-1}: the last field before xa (x_0 itself after one step)
+  if (D->x_before_out) {
+    const double* xl = nullptr;
+    if (D->warm_start == 2) xl = xc;        // rotated to hold x_{nsteps-1}
+    else if (nsteps >= 1) xl = xb;          // swapped out of xa
+    if ((rc = download_nodes(ctx, d, xl, D->x_before_out, s->r))) return rc;
+  }
   if (step_stats)
     FS_HIP(hipMemcpy(step_stats, ost, sizeof(double) * (nsteps + 1) * 4, hipMemcpyDeviceToHost));
   if (np > 0) {

@@ -89,7 +89,8 @@ void launch_charge(fs_sys* s, int nr, const fs_op* L, const double* phi, const i
 void launch_explicit(fs_sys* s, const fs_op* A, const double* x, const double* src,
                      const double* scale, int nsrc, int64_t vs, double* rhs);
 void launch_lift_groups(fs_sys* s, const double* rhs, const int32_t* group, const double* vals);
-void launch_init_warm(fs_sys* s, const double* xprev, double tol, int maxiter);
+void launch_init_warm(fs_sys* s, const double* xprev, const double* xprev2, double tol,
+                      int maxiter);
 void launch_expand_groups(fs_sys* s, const int32_t* group, const double* vals, double* x);
 void launch_probe(fs_sys* s, const fs_op* A1, const fs_op* A2, const double* x,
                   const double* xprev, const double* w, const int64_t* nodes,

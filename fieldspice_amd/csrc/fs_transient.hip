@@ -80,9 +80,12 @@ __global__ __launch_bounds__(kBlock) void k_lift_groups(const fs_op K,
   if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
 }
 
-// Warm start: x_s = x_prev*dhalf on free rows, r = b_s - A_s x_s, p = 0.
+// Warm start: x_s = g*dhalf on free rows, r = b_s - A_s x_s, p = 0, with the
+// guess g = x_prev, or with x_prev2 given the linear extrapolation
+// g = 2 x_prev - x_prev2 of the last two fields.
 // Partials: ||r||^2 (quantity 0) and ||b_s||^2 (quantity 1).
 __global__ __launch_bounds__(kBlock) void k_init_warm(const double* __restrict__ xprev,
+                                                      const double* __restrict__ xprev2,
                                                       const double* __restrict__ dhalf,
                                                       const double* __restrict__ cx,
                                                       const double* __restrict__ cy,
@@ -93,18 +96,23 @@ __global__ __launch_bounds__(kBlock) void k_init_warm(const double* __restrict__
                                                       double* __restrict__ p,
                                                       double* __restrict__ partials) {
   double acc[2] = {0.0, 0.0};
+  // dhalf = 0 off the free rows
+  auto guess = [&](int64_t m) {
+    const double g = xprev2 ? 2.0 * xprev[m] - xprev2[m] : xprev[m];
+    return g * dhalf[m];
+  };
   for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < d.n;
        n += (int64_t)gridDim.x * kBlock) {
     int i, j, k;
     node_ijk(d, n, i, j, k);
-    const double xc = xprev[n] * dhalf[n];  // dhalf = 0 off the free rows
+    const double xc = guess(n);
     double s = 0.0;
-    if (i + 1 < d.nx) s += cx[n] * (xprev[n + 1] * dhalf[n + 1]);
-    if (i > 0) s += cx[n - 1] * (xprev[n - 1] * dhalf[n - 1]);
-    if (j + 1 < d.ny) s += cy[n] * (xprev[n + d.nx] * dhalf[n + d.nx]);
-    if (j > 0) s += cy[n - d.nx] * (xprev[n - d.nx] * dhalf[n - d.nx]);
-    if (k + 1 < d.nz) s += cz[n] * (xprev[n + d.sxy] * dhalf[n + d.sxy]);
-    if (k > 0) s += cz[n - d.sxy] * (xprev[n - d.sxy] * dhalf[n - d.sxy]);
+    if (i + 1 < d.nx) s += cx[n] * guess(n + 1);
+    if (i > 0) s += cx[n - 1] * guess(n - 1);
+    if (j + 1 < d.ny) s += cy[n] * guess(n + d.nx);
+    if (j > 0) s += cy[n - d.nx] * guess(n - d.nx);
+    if (k + 1 < d.nz) s += cz[n] * guess(n + d.sxy);
+    if (k > 0) s += cz[n - d.sxy] * guess(n - d.sxy);
     const double b = bs[n];
     const double rv = b - (xc - s);
     xs[n] = xc;
@@ -306,9 +314,10 @@ void launch_lift_groups(fs_sys* s, const double* rhs, const int32_t* group, cons
                      s->vec_blocks, s->sc);
 }
 
-void launch_init_warm(fs_sys* s, const double* xprev, double tol, int maxiter) {
+void launch_init_warm(fs_sys* s, const double* xprev, const double* xprev2, double tol,
+                      int maxiter) {
   hipLaunchKernelGGL(k_init_warm, dim3(s->vec_blocks), dim3(kBlock), 0, s->stream, xprev,
-                     s->dhalf, s->c64[0], s->c64[1], s->c64[2], s->d, s->bs, s->x, s->r, s->p,
+                     xprev2, s->dhalf, s->c64[0], s->c64[1], s->c64[2], s->d, s->bs, s->x, s->r, s->p,
                      s->partials);
   hipLaunchKernelGGL(k_init_warm_final, dim3(1), dim3(kBlock), 0, s->stream, s->partials,
                      s->vec_blocks, tol, maxiter, s->sc);

@@ -7,7 +7,7 @@ Backward Euler / theta stepping of
 with voltage-driven terminals and Dirichlet walls imposed every step.  The
 whole time loop runs on the device (``fs_transient_run``): the explicit
 product, the Dirichlet lifting with that step's values, the CG solve
-(warm-started from the previous step), the terminal currents and voltages and
+(started from the previous step, or the extrapolation of the last two), the terminal currents and voltages and
 the stored energy / dissipation.  The matrix is prepared once, as the
 reference's cached factorisation is.
 """
@@ -110,8 +110,20 @@ class EQSSolver:
     # -- transient -------------------------------------------------------------
     def solve(self, terminals: Sequence[Terminal], t_end: float, dt: float, bc=None,
               theta: float = 1.0, monitors=None, phi0=None, store: bool = False,
-              store_every: int = 1, t_start: float = 0.0, warm_start: bool = True) -> Result:
-        """Step from ``t_start`` to ``t_end`` (``fieldspice/solvers/eqs.py:229-323``)."""
+              store_every: int = 1, t_start: float = 0.0,
+              initial_guess: str = "previous", phi_prev=None) -> Result:
+        """Step from ``t_start`` to ``t_end`` (``fieldspice/solvers/eqs.py:229-323``).
+
+        ``initial_guess`` of each step's solve: "previous" (the last field),
+        "extrapolate" (2 phi_{k-1} - phi_{k-2} from the second step on) or
+        "zero" (the reference's).  Every step stops at the same relative
+        residual whatever the guess; the better the guess, the fewer the
+        iterations, and the more of the residual left at the stop lies in
+        smooth modes, which is what terminal-current conservation measures
+        (the RC stack at tol 1e-10: |I_hi + I_lo|/|I_hi| < 1e-8 from the
+        previous field, 8e-8 extrapolated).  ``phi_prev`` (the field one step before
+        ``phi0``; ``meta["phi_before_final"]`` of a previous call) lets the
+        first step extrapolate too, so a run split into calls steps as one."""
         if dt <= 0.0:
             raise ValueError("dt must be positive")
         if t_end <= t_start:
@@ -130,6 +142,10 @@ class EQSSolver:
                else np.asarray(phi0, dtype=float).ravel().copy())
         if phi.shape != (n,):
             raise ValueError(f"phi0 must have shape ({n},)")
+        if phi_prev is not None:
+            phi_prev = np.asarray(phi_prev, dtype=float).ravel()
+            if phi_prev.shape != (n,):
+                raise ValueError(f"phi_prev must have shape ({n},)")
         nsteps = int(np.ceil((t_end - t_start) / dt))
         times = t_start + dt * np.arange(nsteps + 1)
         K = self.L_eps.combine(1.0 / dt, theta, self.L_sigma)
@@ -156,7 +172,7 @@ class EQSSolver:
             src=src, src_scale=scales, probe=probe if terms else None, probe_w=pw,
             nprobe=len(terms), cur_a=cur_a, cur_b=self.L_sigma, quad_a=self.L_eps,
             quad_b=self.L_sigma, store_every=max(1, store_every) if store else 0,
-            warm_start=warm_start, params=ls.params())
+            initial_guess=initial_guess, x_before=phi_prev, params=ls.params())
         ls._judge(out["stats"], out["rc"])
         res = Result(grid=g, t=times)
         if store:
@@ -171,6 +187,7 @@ class EQSSolver:
         res.scalars["dissipation"] = ss[:, 3]
         res.meta["linear"] = dict(ls.info)
         res.meta["phi_final"] = out["x"]
+        res.meta["phi_before_final"] = out["x_before"]
         return timer.finish(res, self.name, self.assumptions, g, mode="transient", theta=theta,
                             dt=dt, n_factorisations=1,
                             speedup_vs_courant=dt / g.courant_dt(eps_r_min=1.0))

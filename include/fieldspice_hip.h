@@ -180,7 +180,9 @@ typedef struct {
     int nsrc;                  /* source vectors */
     const double* src;         /* [nsrc * n_nodes] */
     const double* src_scale;   /* [(nsteps+1) * nsrc] */
-    int warm_start;            /* start each solve from the previous field */
+    int warm_start;            /* initial guess of each solve: 0 zero (the reference's),
+                                  1 the previous field, 2 from the second step on the
+                                  linear extrapolation 2 x_{k-1} - x_{k-2} */
     int nprobe;                /* recorded terminals */
     const int32_t* probe;      /* per node: -1 or probe id */
     const double* probe_w;     /* per node weight of the probe voltage (sum over probe = 1) */
@@ -189,6 +191,10 @@ typedef struct {
     const fs_op* quad_a;       /* per-step x^T quad_a x (may be NULL) */
     const fs_op* quad_b;       /* per-step x^T quad_b x (may be NULL) */
     int store_every;           /* copy x_k to `stored` every store_every steps (0: never) */
+    const double* x_before;    /* x_{-1}, the field one step before x_0 (host layout, may
+                                  be NULL): with warm_start 2 the first step extrapolates
+                                  from it too */
+    double* x_before_out;      /* out: x_{nsteps-1} (host layout, may be NULL) */
 } fs_transient_desc;
 
 /* x: in x_0, out x_nsteps.  Outputs (NULL to skip):

@@ -88,7 +88,7 @@ def test_eqs_steady_state_slab_resistance():
     assert R == pytest.approx(d / (sigma * W * H), rel=1e-9)
 
 
-def _eqs_parity_case(floating: bool, tol: float, rtol: float):
+def _two_phase(floating: bool):
     g = RectilinearGrid(np.linspace(0, 1.2e-7, 13), np.linspace(0, 1.0e-7, 11),
                         np.linspace(0, 0.8e-7, 9))
     cu = np.zeros(g.shape_cells, dtype=bool)
@@ -102,6 +102,11 @@ def _eqs_parity_case(floating: bool, tol: float, rtol: float):
     drv = Terminal("drv", nid[1:5, 1:5, 1:5].ravel(),
                    voltage=lambda t: np.sin(2 * np.pi * 1e9 * t))
     gnd = Terminal("gnd", nid[8:12, 6:10, 4:8].ravel(), voltage=0.0)
+    return g, eps, sig, drv, gnd
+
+
+def _eqs_parity_case(floating: bool, tol: float, rtol: float):
+    g, eps, sig, drv, gnd = _two_phase(floating)
     dt, nsteps = 1e-12, 40
     cfg = SolverConfig(tol=tol, maxiter=20000)
     r = EQSSolver(g, eps, sig, cfg).solve([drv, gnd], t_end=nsteps * dt, dt=dt, store=True,
@@ -135,6 +140,48 @@ def test_eqs_floating_island_transient_within_conditioning():
     # step (the oracle's own LU moves by ~1e-10 under refinement).  Both
     # solutions are checked to a few cond*u over the 40 steps.
     _eqs_parity_case(floating=True, tol=1e-14, rtol=1e-8)
+
+
+def test_eqs_initial_guesses_agree_and_split_run_matches():
+    _case("EQS initial guesses zero/previous/extrapolate, and a run split in two calls")
+    # Every guess stops each step at the same relative residual, so the
+    # fields agree to ~cond*tol; on the smooth 1 GHz drive the extrapolated
+    # guess needs the fewest iterations and the zero guess the most.  Two
+    # calls chained through phi_prev take the same steps as one call.
+    g, eps, sig, drv, gnd = _two_phase(False)
+    dt, nsteps = 1e-12, 24
+    cfg = SolverConfig(tol=1e-12, maxiter=20000)
+    s = EQSSolver(g, eps, sig, cfg)
+    runs = {}
+    for guess in ("zero", "previous", "extrapolate"):
+        r = s.solve([drv, gnd], t_end=nsteps * dt, dt=dt, store=True, store_every=1,
+                    initial_guess=guess)
+        runs[guess] = (r.fields["phi"], r.meta["linear"]["iterations"], r)
+    ref = runs["zero"][0]
+    scale = np.abs(ref).max()
+    for guess, (phi, its, _) in runs.items():
+        assert np.abs(phi - ref).max() <= 1e-9 * scale, guess
+    its = {k: v[1] for k, v in runs.items()}
+    print(f"iterations over {nsteps} steps: {its}", flush=True)
+    assert its["extrapolate"] < its["previous"] < its["zero"]
+    # the last two fields come back for chaining
+    r_full = runs["extrapolate"][2]
+    np.testing.assert_array_equal(r_full.meta["phi_final"], r_full.fields["phi"][-1])
+    np.testing.assert_array_equal(r_full.meta["phi_before_final"], r_full.fields["phi"][-2])
+    half = nsteps // 2
+    r1 = s.solve([drv, gnd], t_end=half * dt, dt=dt, store=True, store_every=1,
+                 initial_guess="extrapolate")
+    r2 = s.solve([drv, gnd], t_end=nsteps * dt, dt=dt, t_start=half * dt, store=True,
+                 store_every=1, phi0=r1.meta["phi_final"], phi_prev=r1.meta["phi_before_final"],
+                 initial_guess="extrapolate")
+    split = np.concatenate([r1.fields["phi"], r2.fields["phi"][1:]])
+    assert np.abs(split - runs["extrapolate"][0]).max() <= 1e-9 * scale
+    assert (r1.meta["linear"]["iterations"] + r2.meta["linear"]["iterations"]
+            == pytest.approx(runs["extrapolate"][1], rel=0.05))
+    with pytest.raises(ValueError, match="initial_guess"):
+        s.solve([drv, gnd], t_end=2 * dt, dt=dt, initial_guess="linear")
+    with pytest.raises(ValueError, match="phi_prev"):
+        s.solve([drv, gnd], t_end=2 * dt, dt=dt, phi_prev=np.zeros(3))
 
 
 def _bar(n=40, L=1e-3, k=150.0):
