@@ -58,15 +58,18 @@ def _parse():
     ap.add_argument("--nsteps", type=int, default=200, help="c3: time steps")
     ap.add_argument("--store-every", type=int, default=10,
                     help="c3: steps between stored potentials (error samples)")
-    ap.add_argument("--chunk-steps", type=int, default=10,
-                    help="c3: time steps per solver call (a progress line each)")
+    ap.add_argument("--chunk-steps", type=int, default=200,
+                    help="c3: time steps per solver call (a progress line each; the projection "
+                         "guess restarts its basis at every call)")
     ap.add_argument("--tol", type=float, default=1e-10,
                     help="c3: relative residual of every step (the reference's 1e-10)")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="c3: steps of the plain Jacobi-PCG run beside the island-space runs")
-    ap.add_argument("--guess", choices=("extrapolate", "previous", "zero"), default="extrapolate",
-                    help="c3: initial guess of every step's solve (2 phi_{k-1} - phi_{k-2}, "
-                         "phi_{k-1}, or 0 as the reference's)")
+    ap.add_argument("--guess", choices=("project", "extrapolate", "previous", "zero"),
+                    default="project",
+                    help="c3: initial guess of every step's solve (2 phi_{k-1} - phi_{k-2}; the "
+                         "projection onto earlier solutions of the chunk; phi_{k-1}; or 0 as the "
+                         "reference's)")
     ap.add_argument("--instances", type=int, default=64, help="c5: number of systems")
     ap.add_argument("--chunk", type=int, default=4, help="c5: iterations per graph replay")
     ap.add_argument("--n", type=int, default=256, help="c4: nodes per side")

@@ -65,7 +65,7 @@ class SysInfo(C.Structure):
 
 
 # fs_transient_desc.warm_start: the initial guess of each time step's solve
-GUESSES = {"zero": 0, "previous": 1, "extrapolate": 2}
+GUESSES = {"zero": 0, "previous": 1, "extrapolate": 2, "project": 3}
 
 
 class TransientDesc(C.Structure):
@@ -75,7 +75,7 @@ class TransientDesc(C.Structure):
                 ("probe", C.c_void_p), ("probe_w", C.c_void_p), ("cur_a", C.c_void_p),
                 ("cur_b", C.c_void_p), ("quad_a", C.c_void_p), ("quad_b", C.c_void_p),
                 ("store_every", C.c_int), ("x_before", C.c_void_p),
-                ("x_before_out", C.c_void_p)]
+                ("x_before_out", C.c_void_p), ("proj_dim", C.c_int)]
 
 
 class LibraryError(RuntimeError):
@@ -356,13 +356,15 @@ class DeviceSystem:
     def transient(self, Aexp, x0, nsteps, group=None, values=None, src=None, src_scale=None,
                   probe=None, probe_w=None, nprobe=0, cur_a=None, cur_b=None, quad_a=None,
                   quad_b=None, store_every=0, initial_guess="previous", x_before=None,
-                  params=None):
+                  proj_dim=0, params=None):
         """Device time loop (``fs_transient_run``); returns a dict of outputs.
 
         ``initial_guess`` of every step's solve: "zero" (as the reference's
         solves start), "previous" (the last field) or "extrapolate" (from the
         second step on, 2 x_{k-1} - x_{k-2}; from the first when ``x_before``, the
-        field one step before ``x0``, is given)."""
+        field one step before ``x0``, is given) or "project" (the A-norm
+        projection of each step's right-hand side onto a basis of ``proj_dim``
+        earlier solutions, 0 for 8)."""
         if initial_guess not in GUESSES:
             raise ValueError(f"initial_guess must be one of {sorted(GUESSES)}, "
                              f"got {initial_guess!r}")
@@ -385,7 +387,7 @@ class DeviceSystem:
                           None if quad_a is None else quad_a._h,
                           None if quad_b is None else quad_b._h, int(store_every),
                           ptr(None if x_before is None else as_f64(x_before).ravel(), np.float64),
-                          None)
+                          None, int(proj_dim))
         x_last_before = np.empty(n)
         d.x_before_out = x_last_before.ctypes.data_as(_vp)
         x = as_f64(x0).ravel().copy()

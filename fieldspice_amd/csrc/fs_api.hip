@@ -997,8 +997,11 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
     return fail(FS_EINVAL, "sources need vectors and scales");
   if (D->nprobe < 0 || (D->nprobe > 0 && (!D->probe || !D->probe_w)))
     return fail(FS_EINVAL, "probes need ids and weights");
-  if (D->warm_start < 0 || D->warm_start > 2)
-    return fail(FS_EINVAL, "warm_start must be 0, 1 or 2 (got %d)", D->warm_start);
+  if (D->warm_start < 0 || D->warm_start > 3)
+    return fail(FS_EINVAL, "warm_start must be 0, 1, 2 or 3 (got %d)", D->warm_start);
+  if (D->warm_start == 3 && (D->proj_dim < 0 || D->proj_dim == 1 || D->proj_dim > fs::kProjMax))
+    return fail(FS_EINVAL, "proj_dim must be 0 (default 8) or 2..%d (got %d)", fs::kProjMax,
+                D->proj_dim);
   if (params->precision != FS_PREC_FP64 && params->precision != FS_PREC_MIXED &&
       params->precision != FS_PREC_FCG)
     return fail(FS_EINVAL, "unknown precision %d", params->precision);
@@ -1026,9 +1029,31 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
       (rc = B.get(&ost, (size_t)(nsteps + 1) * 4)))
     return rc;
   // the field two steps back, for the extrapolated initial guess
-  if (D->warm_start == 2 && (rc = B.get(&xc, (size_t)s->vstride))) return rc;
+  const bool rot = D->warm_start >= 2;
+  if (rot && (rc = B.get(&xc, (size_t)s->vstride))) return rc;
+  // projection guesses: A_s-orthonormal basis X, AX = A_s X, the guess x0 of
+  // the step, the previous step's solution, coefficients and a norm slot
+  const int pm = D->warm_start == 3 ? (D->proj_dim > 0 ? D->proj_dim : 8) : 0;
+  double *PX = nullptr, *PAX = nullptr, *px0 = nullptr, *pxs = nullptr, *pcoef = nullptr;
+  if (pm > 0) {
+    if ((rc = B.get(&PX, (size_t)pm * s->vstride)) || (rc = B.get(&PAX, (size_t)pm * s->vstride)) ||
+        (rc = B.get(&px0, (size_t)s->vstride)) || (rc = B.get(&pxs, (size_t)s->vstride)) ||
+        (rc = B.get(&pcoef, (size_t)fs::kProjMax + 2)))
+      return rc;
+    const int64_t need = (int64_t)fs::kProjMax * s->vec_blocks;
+    if (need > s->partials_cap) {
+      dfree(s->partials);
+      if ((rc = dalloc(&s->partials, (size_t)need))) {
+        s->partials_cap = 0;
+        return rc;
+      }
+      s->partials_cap = need;
+    }
+  }
+  int pnb = 0;            // basis vectors held
+  bool have_pxs = false;  // pxs holds the previous step's x_s
   if ((rc = upload_permuted(ctx, x, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz, xa))) return rc;
-  const bool have_before = D->warm_start == 2 && D->x_before;
+  const bool have_before = rot && D->x_before;
   if (have_before &&
       (rc = upload_permuted(ctx, D->x_before, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz, xc)))
     return rc;
@@ -1105,10 +1130,14 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
     fs::launch_explicit(s, Aexp, xa, src, scale ? scale + (size_t)k * D->nsrc : nullptr, D->nsrc,
                         s->vstride, s->node_tmp);
     fs::launch_lift_groups(s, s->node_tmp, grp, vk);
-    // initial guess: 0, the previous field, or the linear extrapolation
-    // 2 x_{k-1} - x_{k-2} (from the second step on unless x_{-1} was given)
-    const bool extrap = D->warm_start == 2 && (k >= 2 || have_before);
-    if (D->warm_start)
+    // initial guess: 0, the previous field, the linear extrapolation
+    // 2 x_{k-1} - x_{k-2} (from the second step on unless x_{-1} was given),
+    // or the projection onto the basis of earlier solutions
+    const bool extrap = rot && (k >= 2 || have_before);
+    const bool from_basis = pm > 0 && pnb > 0;
+    if (from_basis)
+      fs::launch_proj_guess(s, PX, PAX, s->vstride, pnb, pcoef, px0, params->tol, maxiter);
+    else if (D->warm_start)
       fs::launch_init_warm(s, xa, extrap ? xc : nullptr, params->tol, maxiter);
     else fs::launch_init(s, 1, params->tol, maxiter);
     FS_HIP(hipGetLastError());
@@ -1136,8 +1165,28 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
       status = FS_NOT_CONVERGED;
       if (stats) stats->converged = 0;
     }
-    if (D->warm_start == 2) std::swap(xc, xa);  // xc <- x_{k-1}
-    std::swap(xa, xb);                          // xa <- x_k
+    if (pm > 0) {
+      // extend the basis with the new direction x_s - x0 (x_s itself when
+      // the guess was not from the basis); a full basis restarts from x_s
+      // and x_s - x_s(previous step), keeping the extrapolation direction
+      const bool restart = pnb == pm;
+      if (restart) pnb = 0;
+      const int ncand = restart && have_pxs ? 2 : 1;
+      for (int c = 0; c < ncand; ++c) {
+        double* slot = PX + (int64_t)pnb * s->vstride;
+        fs::launch_diff(s, s->x, c == 1 ? pxs : (from_basis && !restart ? px0 : nullptr), slot);
+        fs::launch_proj_append(s, PX, PAX, s->vstride, pnb, pcoef, pcoef + fs::kProjMax);
+        double q = 0.0;
+        FS_HIP(hipMemcpyAsync(&q, pcoef + fs::kProjMax, sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        FS_HIP(hipStreamSynchronize(ctx->stream));
+        if (q > 0.0 && q < INFINITY) ++pnb;
+      }
+      fs::launch_diff(s, s->x, nullptr, pxs);
+      have_pxs = true;
+    }
+    if (rot) std::swap(xc, xa);  // xc <- x_{k-1}
+    std::swap(xa, xb);           // xa <- x_k
     if (store_every > 0 && k % store_every == 0 && (rc = store(xa))) return rc;
   }
   FS_HIP(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1148,7 +1197,7 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
   // x_{nsteps-1}: the last field before xa (x_0 itself after one step)
   if (D->x_before_out) {
     const double* xl = nullptr;
-    if (D->warm_start == 2) xl = xc;        // rotated to hold x_{nsteps-1}
+    if (rot) xl = xc;                       // rotated to hold x_{nsteps-1}
     else if (nsteps >= 1) xl = xb;          // swapped out of xa
     if ((rc = download_nodes(ctx, d, xl, D->x_before_out, s->r))) return rc;
   }

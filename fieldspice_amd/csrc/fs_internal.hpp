@@ -20,6 +20,7 @@ constexpr int kTileX = 64;        // stencil tile: one wavefront along x ...
 constexpr int kTileY = 4;         // ... times four rows
 constexpr int kMaxIslands = 1024; // island coarse space: islands per system
 constexpr int kIslChunk = 2048;   // island nodes per reduction block
+constexpr int kProjMax = 16;      // time loops: basis of the projection initial guess
 
 // Device layout: node (i, j, k) of an (nx, ny, nz) node grid lives at
 // n = (k*ny + j)*nx + i (x fastest).  Every node vector of a system is padded

@@ -90,7 +90,13 @@ void launch_explicit(fs_sys* s, const fs_op* A, const double* x, const double* s
                      const double* scale, int nsrc, int64_t vs, double* rhs);
 void launch_lift_groups(fs_sys* s, const double* rhs, const int32_t* group, const double* vals);
 void launch_init_warm(fs_sys* s, const double* xprev, const double* xprev2, double tol,
-                      int maxiter);
+                      int maxiter, int scaled = 0);
+// projection initial guesses of the time loops (basis of at most kProjMax)
+void launch_proj_guess(fs_sys* s, const double* X, const double* AX, int64_t vs, int nb,
+                       double* coef, double* x0, double tol, int maxiter);
+void launch_proj_append(fs_sys* s, double* X, double* AX, int64_t vs, int nb, double* coef,
+                        double* norm2);
+void launch_diff(fs_sys* s, const double* a, const double* b, double* out);
 void launch_expand_groups(fs_sys* s, const int32_t* group, const double* vals, double* x);
 void launch_probe(fs_sys* s, const fs_op* A1, const fs_op* A2, const double* x,
                   const double* xprev, const double* w, const int64_t* nodes,

@@ -596,10 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
     float4 dcur = one4, dprev = one4;
     int64_t nn = (int64_t)k0 * d.sxy + base;
     const float4 z4 = f4(0.f, 0.f, 0.f, 0.f);
-    // carried per RHS: z2 of plane k-1 and r of plane k (z2 of plane k is
-    // recomputed from r and one coarse load rather than carried: 4 fewer
-    // registers per RHS)
-    float4 zm[NR], rc[NR];
+    float4 zm[NR], zc[NR], rc[NR];
     int ck = valid ? m.f2c[2][k0] : 0;
     float4 czm = z4;
     if (valid && k0 > 0) czm = ldf4(cz + nn - d.sxy);
@@ -612,6 +609,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
 #pragma unroll
     for (int b = 0; b < NR; ++b) {
       zm[b] = z4;
+      zc[b] = z4;
       rc[b] = z4;
       if (valid && live[b]) {
         const float* rb = r + b * vs;
@@ -620,6 +618,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
           zm[b] = z2row(eb, ldf4(rb + nn - d.sxy), dprev,
                         ((int64_t)m.f2c[2][k0 - 1] * cny + cj) * cnx);
         rc[b] = ldf4(rb + nn);
+        zc[b] = z2row(eb, rc[b], dcur, ((int64_t)ck * cny + cj) * cnx);
       }
     }
     for (int k = k0; k < k1; ++k, nn += d.sxy) {
@@ -660,7 +659,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
           if (has_n) zn = z2row(eb, ldf4(rb + nn + d.nx), dnr, ((int64_t)ck * cny + cjn) * cnx);
           if (has_s) zs = z2row(eb, ldf4(rb + nn - d.nx), dsr, ((int64_t)ck * cny + cjs) * cnx);
         }
-        const float4 c = valid ? z2row(eb, rc[b], dcur, rowc) : z4;
+        const float4 c = zc[b];
         float zw = __shfl_up(c.w, 1, kWave), ze = __shfl_down(c.x, 1, kWave);
         if (valid) {
           if (!has_w) zw = 0.f;
@@ -696,6 +695,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
           }
         }
         zm[b] = c;
+        zc[b] = zu;
         rc[b] = ru;
       }
       czm = czc;

@@ -82,10 +82,12 @@ __global__ __launch_bounds__(kBlock) void k_lift_groups(const fs_op K,
 
 // Warm start: x_s = g*dhalf on free rows, r = b_s - A_s x_s, p = 0, with the
 // guess g = x_prev, or with x_prev2 given the linear extrapolation
-// g = 2 x_prev - x_prev2 of the last two fields.
+// g = 2 x_prev - x_prev2 of the last two fields.  scaled: x_prev is already
+// x_s (scaled variables, zero off the free rows).
 // Partials: ||r||^2 (quantity 0) and ||b_s||^2 (quantity 1).
 __global__ __launch_bounds__(kBlock) void k_init_warm(const double* __restrict__ xprev,
                                                       const double* __restrict__ xprev2,
+                                                      int scaled,
                                                       const double* __restrict__ dhalf,
                                                       const double* __restrict__ cx,
                                                       const double* __restrict__ cy,
@@ -98,6 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_init_warm(const double* __restrict__
   double acc[2] = {0.0, 0.0};
   // dhalf = 0 off the free rows
   auto guess = [&](int64_t m) {
+    if (scaled) return xprev[m];
     const double g = xprev2 ? 2.0 * xprev[m] - xprev2[m] : xprev[m];
     return g * dhalf[m];
   };
@@ -298,6 +301,115 @@ int grid_for(int64_t n, int cap) {
   return (int)(nb < cap ? (nb > 0 ? nb : 1) : cap);
 }
 
+
+// ----------------------------------------------------------------------------
+// Initial guesses from earlier solutions (projection method of P. F. Fischer,
+// "Projection techniques for iterative solution of Ax = b with successive
+// right-hand sides", CMAME 163, 1998): X holds nb A_s-orthonormal vectors of
+// the scaled space (zero off the free rows) and AX = A_s X.  The guess for a
+// right-hand side b is x0 = X X^T b, the A_s-norm best approximation of the
+// solution in span(X).
+// ----------------------------------------------------------------------------
+
+// partials[i * gridDim.x + block] = V_i . w  for i < nb
+__global__ __launch_bounds__(kBlock) void k_proj_dots(const double* __restrict__ V, int64_t vs,
+                                                      int nb, const double* __restrict__ w,
+                                                      int64_t n, double* __restrict__ partials) {
+  double acc[kProjMax];
+#pragma unroll
+  for (int i = 0; i < kProjMax; ++i) acc[i] = 0.0;
+  for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < n;
+       m += (int64_t)gridDim.x * kBlock) {
+    const double wv = w[m];
+#pragma unroll
+    for (int i = 0; i < kProjMax; ++i)
+      if (i < nb) acc[i] += V[i * vs + m] * wv;
+  }
+  block_sum<kProjMax>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < kProjMax; ++i)
+      if (i < nb) partials[(int64_t)i * gridDim.x + blockIdx.x] = acc[i];
+  }
+}
+
+// coef[i] = sum of the nblocks partials of quantity i (one block per i)
+__global__ __launch_bounds__(kBlock) void k_proj_coef(const double* __restrict__ partials,
+                                                      int nblocks, double* __restrict__ coef) {
+  const double v = sum_partials(partials + (int64_t)blockIdx.x * nblocks, nblocks);
+  if (threadIdx.x == 0) coef[blockIdx.x] = v;
+}
+
+// out = sum_i coef[i] X_i   (sub: out -= sum_i coef[i] X_i)
+__global__ __launch_bounds__(kBlock) void k_proj_combine(const double* __restrict__ X, int64_t vs,
+                                                         int nb, const double* __restrict__ coef,
+                                                         int sub, int64_t n,
+                                                         double* __restrict__ out) {
+  double c[kProjMax];
+#pragma unroll
+  for (int i = 0; i < kProjMax; ++i) c[i] = i < nb ? coef[i] : 0.0;
+  for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < n;
+       m += (int64_t)gridDim.x * kBlock) {
+    double v = 0.0;
+#pragma unroll
+    for (int i = 0; i < kProjMax; ++i)
+      if (i < nb) v += c[i] * X[i * vs + m];
+    out[m] = sub ? out[m] - v : v;
+  }
+}
+
+// out = a - b (b may be null: out = a)
+__global__ __launch_bounds__(kBlock) void k_diff(const double* __restrict__ a,
+                                                 const double* __restrict__ b, int64_t n,
+                                                 double* __restrict__ out) {
+  for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < n;
+       m += (int64_t)gridDim.x * kBlock)
+    out[m] = b ? a[m] - b[m] : a[m];
+}
+
+// y = A_s v (equilibrated: unit diagonal on free rows, masked couplings;
+// v is zero off the free rows) and partials of v . y
+__global__ __launch_bounds__(kBlock) void k_apply_s(const double* __restrict__ cx,
+                                                    const double* __restrict__ cy,
+                                                    const double* __restrict__ cz, Dims d,
+                                                    const double* __restrict__ v,
+                                                    double* __restrict__ y,
+                                                    double* __restrict__ partials) {
+  double acc[1] = {0.0};
+  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < d.n;
+       n += (int64_t)gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(d, n, i, j, k);
+    double s = 0.0;
+    if (i + 1 < d.nx) s += cx[n] * v[n + 1];
+    if (i > 0) s += cx[n - 1] * v[n - 1];
+    if (j + 1 < d.ny) s += cy[n] * v[n + d.nx];
+    if (j > 0) s += cy[n - d.nx] * v[n - d.nx];
+    if (k + 1 < d.nz) s += cz[n] * v[n + d.sxy];
+    if (k > 0) s += cz[n - d.sxy] * v[n - d.sxy];
+    const double yv = v[n] - s;
+    y[n] = yv;
+    acc[0] += v[n] * yv;
+  }
+  block_sum<1>(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
+}
+
+// A_s-normalise a new basis vector in place: x /= sqrt(norm2), ax likewise,
+// when norm2 = x . A_s x (device slot) is positive and finite
+__global__ __launch_bounds__(kBlock) void k_proj_normalise(const double* __restrict__ norm2,
+                                                           int64_t n, double* __restrict__ x,
+                                                           double* __restrict__ ax) {
+  const double q = *norm2;
+  if (!(q > 0.0) || !(q < INFINITY)) return;  // uniform over the launch
+  const double f = 1.0 / sqrt(q);
+  for (int64_t m = (int64_t)blockIdx.x * kBlock + threadIdx.x; m < n;
+       m += (int64_t)gridDim.x * kBlock) {
+    x[m] *= f;
+    ax[m] *= f;
+  }
+}
+
 }  // namespace
 
 void launch_explicit(fs_sys* s, const fs_op* A, const double* x, const double* src,
@@ -315,12 +427,58 @@ void launch_lift_groups(fs_sys* s, const double* rhs, const int32_t* group, cons
 }
 
 void launch_init_warm(fs_sys* s, const double* xprev, const double* xprev2, double tol,
-                      int maxiter) {
+                      int maxiter, int scaled) {
   hipLaunchKernelGGL(k_init_warm, dim3(s->vec_blocks), dim3(kBlock), 0, s->stream, xprev,
-                     xprev2, s->dhalf, s->c64[0], s->c64[1], s->c64[2], s->d, s->bs, s->x, s->r, s->p,
+                     xprev2, scaled, s->dhalf, s->c64[0], s->c64[1], s->c64[2], s->d, s->bs, s->x, s->r, s->p,
                      s->partials);
   hipLaunchKernelGGL(k_init_warm_final, dim3(1), dim3(kBlock), 0, s->stream, s->partials,
                      s->vec_blocks, tol, maxiter, s->sc);
+}
+
+
+// x0 = X X^T b_s, then the usual start from x0 (r = b_s - A_s x0 computed
+// from the operator, not from AX)
+void launch_proj_guess(fs_sys* s, const double* X, const double* AX, int64_t vs, int nb,
+                       double* coef, double* x0, double tol, int maxiter) {
+  (void)AX;
+  const int vb = s->vec_blocks;
+  hipLaunchKernelGGL(k_proj_dots, dim3(vb), dim3(kBlock), 0, s->stream, X, vs, nb,
+                     (const double*)s->bs, s->d.n, s->partials);
+  hipLaunchKernelGGL(k_proj_coef, dim3(nb), dim3(kBlock), 0, s->stream,
+                     (const double*)s->partials, vb, coef);
+  hipLaunchKernelGGL(k_proj_combine, dim3(vb), dim3(kBlock), 0, s->stream, X, vs, nb,
+                     (const double*)coef, 0, s->d.n, x0);
+  launch_init_warm(s, x0, nullptr, tol, maxiter, 1);
+}
+
+// Append v (overwritten) to the basis as slot nb: two classical Gram-Schmidt
+// passes in the A_s inner product (h = AX^T v), then AX[nb] = A_s v from the
+// operator and norm2 = v . A_s v into *norm2; the vector is normalised on
+// the device when norm2 > 0.  The caller reads norm2 to count the slot.
+void launch_proj_append(fs_sys* s, double* X, double* AX, int64_t vs, int nb, double* coef,
+                        double* norm2) {
+  const int vb = s->vec_blocks;
+  double* v = X + (int64_t)nb * vs;
+  double* av = AX + (int64_t)nb * vs;
+  for (int pass = 0; pass < 2 && nb > 0; ++pass) {
+    hipLaunchKernelGGL(k_proj_dots, dim3(vb), dim3(kBlock), 0, s->stream, (const double*)AX, vs,
+                       nb, (const double*)v, s->d.n, s->partials);
+    hipLaunchKernelGGL(k_proj_coef, dim3(nb), dim3(kBlock), 0, s->stream,
+                       (const double*)s->partials, vb, coef);
+    hipLaunchKernelGGL(k_proj_combine, dim3(vb), dim3(kBlock), 0, s->stream, (const double*)X,
+                       vs, nb, (const double*)coef, 1, s->d.n, v);
+  }
+  hipLaunchKernelGGL(k_apply_s, dim3(vb), dim3(kBlock), 0, s->stream, (const double*)s->c64[0],
+                     (const double*)s->c64[1], (const double*)s->c64[2], s->d, (const double*)v,
+                     av, s->partials);
+  hipLaunchKernelGGL(k_proj_coef, dim3(1), dim3(kBlock), 0, s->stream, (const double*)s->partials,
+                     vb, norm2);
+  hipLaunchKernelGGL(k_proj_normalise, dim3(vb), dim3(kBlock), 0, s->stream,
+                     (const double*)norm2, s->d.n, v, av);
+}
+
+void launch_diff(fs_sys* s, const double* a, const double* b, double* out) {
+  hipLaunchKernelGGL(k_diff, dim3(s->vec_blocks), dim3(kBlock), 0, s->stream, a, b, s->d.n, out);
 }
 
 void launch_expand_groups(fs_sys* s, const int32_t* group, const double* vals, double* x) {

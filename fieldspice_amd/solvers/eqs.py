@@ -115,8 +115,9 @@ class EQSSolver:
         """Step from ``t_start`` to ``t_end`` (``fieldspice/solvers/eqs.py:229-323``).
 
         ``initial_guess`` of each step's solve: "previous" (the last field),
-        "extrapolate" (2 phi_{k-1} - phi_{k-2} from the second step on) or
-        "zero" (the reference's).  Every step stops at the same relative
+        "extrapolate" (2 phi_{k-1} - phi_{k-2} from the second step on),
+        "project" (the energy-norm best combination of up to 8 earlier
+        solutions of this call) or "zero" (the reference's).  Every step stops at the same relative
         residual whatever the guess; the better the guess, the fewer the
         iterations, and the more of the residual left at the stop lies in
         smooth modes, which is what terminal-current conservation measures

@@ -182,7 +182,9 @@ typedef struct {
     const double* src_scale;   /* [(nsteps+1) * nsrc] */
     int warm_start;            /* initial guess of each solve: 0 zero (the reference's),
                                   1 the previous field, 2 from the second step on the
-                                  linear extrapolation 2 x_{k-1} - x_{k-2} */
+                                  linear extrapolation 2 x_{k-1} - x_{k-2}, 3 the
+                                  A-norm projection onto a basis of earlier solutions
+                                  (proj_dim vectors; the first step as 2) */
     int nprobe;                /* recorded terminals */
     const int32_t* probe;      /* per node: -1 or probe id */
     const double* probe_w;     /* per node weight of the probe voltage (sum over probe = 1) */
@@ -195,6 +197,7 @@ typedef struct {
                                   be NULL): with warm_start 2 the first step extrapolates
                                   from it too */
     double* x_before_out;      /* out: x_{nsteps-1} (host layout, may be NULL) */
+    int proj_dim;              /* warm_start 3: basis vectors, 2..16 (0: 8) */
 } fs_transient_desc;
 
 /* x: in x_0, out x_nsteps.  Outputs (NULL to skip):

@@ -153,7 +153,7 @@ def test_eqs_initial_guesses_agree_and_split_run_matches():
     cfg = SolverConfig(tol=1e-12, maxiter=20000)
     s = EQSSolver(g, eps, sig, cfg)
     runs = {}
-    for guess in ("zero", "previous", "extrapolate"):
+    for guess in ("zero", "previous", "extrapolate", "project"):
         r = s.solve([drv, gnd], t_end=nsteps * dt, dt=dt, store=True, store_every=1,
                     initial_guess=guess)
         runs[guess] = (r.fields["phi"], r.meta["linear"]["iterations"], r)
@@ -163,7 +163,7 @@ def test_eqs_initial_guesses_agree_and_split_run_matches():
         assert np.abs(phi - ref).max() <= 1e-9 * scale, guess
     its = {k: v[1] for k, v in runs.items()}
     print(f"iterations over {nsteps} steps: {its}", flush=True)
-    assert its["extrapolate"] < its["previous"] < its["zero"]
+    assert its["project"] < its["extrapolate"] < its["previous"] < its["zero"]
     # the last two fields come back for chaining
     r_full = runs["extrapolate"][2]
     np.testing.assert_array_equal(r_full.meta["phi_final"], r_full.fields["phi"][-1])
