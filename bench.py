@@ -63,6 +63,9 @@ def _parse():
                          "guess restarts its basis at every call)")
     ap.add_argument("--tol", type=float, default=1e-10,
                     help="c3: relative residual of every step (the reference's 1e-10)")
+    ap.add_argument("--ref-tol", type=float, default=None,
+                    help="c3: also solve every step in FP64 to this tolerance and report each "
+                         "method's potential error against it")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="c3: steps of the plain Jacobi-PCG run beside the island-space runs")
     ap.add_argument("--guess", choices=("project", "extrapolate", "previous", "zero"),
@@ -296,13 +299,13 @@ def run_c3(args):
     dt, nsteps = 1e-12, args.nsteps
     every = max(1, args.store_every)
 
-    def run(method, islands, steps, store_every, guess=None):
+    def run(method, islands, steps, store_every, guess=None, tol=None):
         """The transient in chunks of --chunk-steps steps (each continuing from
         the last potential), one log line per chunk; returns the stored
         potentials (every store_every steps, step 0 first), terminal currents,
         device solve seconds, wall seconds and the linear-solver info."""
         cfg = config_for(method)
-        cfg.tol = args.tol
+        cfg.tol = tol or args.tol
         cfg.deflate_islands = islands
         sol = EQSSolver(g, eps, sigma, cfg)
         chunk = max(store_every, (max(1, args.chunk_steps) // store_every) * store_every)
@@ -334,6 +337,14 @@ def run_c3(args):
     if methods[0] != "hip_pcg":
         methods.insert(0, "hip_pcg")
     out_m, ref, ref_i = {}, None, None
+    tight = None
+    if args.ref_tol:
+        # the accuracy reference: FP64 with the island space at a tighter
+        # tolerance, every step from the previous field
+        log(f"  hip_pcg with the island space at tol {args.ref_tol:g} (accuracy reference)")
+        tight, _, tsolve, _, tlin = run("hip_pcg", True, nsteps, every, guess="previous",
+                                        tol=args.ref_tol)
+        log(f"    {tsolve:.1f} s device, iterations {tlin.get('iterations')}")
     for m in methods:
         log(f"  {m} with the island space, {nsteps} steps")
         phi, cur, solve_s, wall, lin = run(m, True, nsteps, every)
@@ -341,15 +352,22 @@ def run_c3(args):
             ref, ref_i = phi, cur
         err = [float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)) for a, b in zip(phi, ref)]
         ierr = float(np.abs(cur - ref_i).max() / max(np.abs(ref_i).max(), 1e-300))
+        terr = None
+        if tight is not None:
+            terr = [float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+                    for a, b in zip(phi, tight)]
         out_m[m + "+islands"] = {
             "solve_s": solve_s, "wall_s": wall, "ms_per_step": 1e3 * solve_s / nsteps,
             "islands": lin.get("islands"), "iterations_total": lin.get("iterations"),
             "phi_rel_err_vs_fp64_max": max(err),
             "phi_rel_err_vs_fp64_stored_steps": err,
-            "stored_every": every, "terminal_current_rel_err_vs_fp64_max": ierr}
+            "stored_every": every, "terminal_current_rel_err_vs_fp64_max": ierr,
+            "phi_rel_err_vs_tight_stored_steps": terr,
+            "phi_rel_err_vs_tight_max": max(terr) if terr else None}
         log(f"    {solve_s:.1f} s device ({1e3 * solve_s / nsteps:.1f} ms/step), {wall:.1f} s wall, "
             f"{lin.get('islands')} islands, iterations {lin.get('iterations')}, max phi err vs "
-            f"FP64 {max(err):.2e}, current err {ierr:.2e}")
+            f"FP64 {max(err):.2e}, current err {ierr:.2e}"
+            + (f", vs tol {args.ref_tol:g} {max(terr):.2e}" if terr else ""))
     plain = None
     if args.plain_steps > 0:
         k = args.plain_steps
@@ -387,7 +405,7 @@ def run_c3(args):
         "config": {"workload": f"EQS backward Euler, {n}^3 nodes at 10 nm, copper boxes 20% "
                                "volume in SiO2, 1 GHz 1 V drive, one grounded box, dt 1e-12 s",
                    "seed": seed, "method": head, "steps": nsteps, "tol": args.tol,
-                   "initial_guess": args.guess},
+                   "initial_guess": args.guess, "ref_tol": args.ref_tol},
         "methods": out_m,
         "plain_reference_method": plain,
         "speedup_vs_fp64": {m: fp["solve_s"] / r["solve_s"] for m, r in out_m.items()} if fp else None,

@@ -355,7 +355,7 @@ class DeviceSystem:
 
     def transient(self, Aexp, x0, nsteps, group=None, values=None, src=None, src_scale=None,
                   probe=None, probe_w=None, nprobe=0, cur_a=None, cur_b=None, quad_a=None,
-                  quad_b=None, store_every=0, initial_guess="previous", x_before=None,
+                  quad_b=None, store_every=0, initial_guess="project", x_before=None,
                   proj_dim=0, params=None):
         """Device time loop (``fs_transient_run``); returns a dict of outputs.
 
