@@ -325,17 +325,25 @@ class DeviceSystem:
         return SolveParams(float(tol), int(maxiter or 0), int(precision), float(inner_tol),
                            int(check_every))
 
-    def solve(self, b, params: SolveParams):
-        """Solve for one (n,) or several (k, n) right-hand sides."""
+    def solve(self, b, params: SolveParams, x0=None):
+        """Solve for one (n,) or several (k, n) right-hand sides; ``x0`` (same
+        shape as ``b``) is an initial guess."""
         b = as_f64(b)
         single = b.ndim == 1
         b2 = b.reshape(1, -1) if single else b
         n = self.op.n_nodes
         if b2.shape[1] != n:
             raise ValueError(f"right-hand side must have shape ({n},), got {b.shape}")
+        g = None
+        if x0 is not None:
+            g = as_f64(x0)
+            if g.shape != b.shape:
+                raise ValueError(f"x0 must have the shape of b {b.shape}, got {g.shape}")
+            g = np.ascontiguousarray(g.reshape(b2.shape))
         x = np.empty_like(b2)
         st = SolveStats()
-        rc = load().fs_sys_solve(self._h, b2.shape[0], np.ascontiguousarray(b2), None, x,
+        rc = load().fs_sys_solve(self._h, b2.shape[0], np.ascontiguousarray(b2),
+                                 None if g is None else g.ctypes.data_as(_vp), x,
                                  C.byref(params), C.byref(st))
         check(rc, "fs_sys_solve")
         return (x[0] if single else x), st.as_dict(), rc

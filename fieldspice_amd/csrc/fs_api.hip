@@ -739,7 +739,6 @@ int fs_sys_solve(fs_sys* s, int nrhs, const double* b, const double* x0, double*
   FS_GUARD_BEGIN
   if (!s || !b || !x || !params) return fail(FS_EINVAL, "NULL argument");
   if (nrhs < 1) return fail(FS_EINVAL, "nrhs must be >= 1");
-  if (x0) return fail(FS_EINVAL, "an initial guess is not supported by this build");
   if (!(params->tol > 0.0)) return fail(FS_EINVAL, "tol must be > 0");
   if (params->precision != FS_PREC_FP64 && params->precision != FS_PREC_MIXED &&
       params->precision != FS_PREC_FCG)
@@ -755,12 +754,21 @@ int fs_sys_solve(fs_sys* s, int nrhs, const double* b, const double* x0, double*
   for (int r0 = 0; r0 < nrhs; r0 += fs::kMaxRhs) {
     const int nr = std::min(fs::kMaxRhs, nrhs - r0);
     if (int rc = ensure_ws(s, nr)) return rc;
-    // full right-hand sides (x-fast) into q
+    // full right-hand sides (x-fast) into q, initial guesses into x0d
     for (int r = 0; r < nr; ++r) {
       const double* hb = b + (size_t)(r0 + r) * d.n;
       if (int rc = upload_permuted(ctx, hb, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz,
                                    s->q + (size_t)r * s->vstride))
         return rc;
+    }
+    DevBufs B;
+    double* x0d = nullptr;
+    if (x0) {
+      if (int rc = B.get(&x0d, (size_t)nr * s->vstride)) return rc;
+      for (int r = 0; r < nr; ++r)
+        if (int rc = upload_permuted(ctx, x0 + (size_t)(r0 + r) * d.n, d.nx, d.ny, d.nz, d.nx,
+                                     d.ny, d.nz, x0d + (size_t)r * s->vstride))
+          return rc;
     }
     prof_begin(s);
     FS_HIP(hipEventRecord(ctx->ev0, ctx->stream));
@@ -769,6 +777,11 @@ int fs_sys_solve(fs_sys* s, int nrhs, const double* b, const double* x0, double*
     FS_HIP(hipMemcpyAsync(s->node_tmp, s->q, (size_t)nr * s->vstride * sizeof(double),
                           hipMemcpyDeviceToDevice, ctx->stream));
     fs::launch_init(s, nr, params->tol, maxiter);
+    // LinearSystem.solve(b, x0): x_s = x0 * D^1/2 on the free rows
+    // (fieldspice/solvers/poisson.py:543-547), r = b_s - A_s x_s
+    if (x0d)
+      for (int r = 0; r < nr; ++r)
+        fs::launch_init_warm(s, x0d + (size_t)r * s->vstride, nullptr, params->tol, maxiter, 0, r);
     FS_HIP(hipGetLastError());
     if (int rc = run_krylov(s, nr, *params)) return rc;
     fs::launch_residual(s, nr, s->r);

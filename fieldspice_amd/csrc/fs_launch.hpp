@@ -89,8 +89,9 @@ void launch_charge(fs_sys* s, int nr, const fs_op* L, const double* phi, const i
 void launch_explicit(fs_sys* s, const fs_op* A, const double* x, const double* src,
                      const double* scale, int nsrc, int64_t vs, double* rhs);
 void launch_lift_groups(fs_sys* s, const double* rhs, const int32_t* group, const double* vals);
+// start right-hand side `rhs` from a guess (node layout, or scaled)
 void launch_init_warm(fs_sys* s, const double* xprev, const double* xprev2, double tol,
-                      int maxiter, int scaled = 0);
+                      int maxiter, int scaled = 0, int rhs = 0);
 // projection initial guesses of the time loops (basis of at most kProjMax)
 void launch_proj_guess(fs_sys* s, const double* X, const double* AX, int64_t vs, int nb,
                        double* coef, double* x0, double tol, int maxiter);

@@ -427,12 +427,13 @@ void launch_lift_groups(fs_sys* s, const double* rhs, const int32_t* group, cons
 }
 
 void launch_init_warm(fs_sys* s, const double* xprev, const double* xprev2, double tol,
-                      int maxiter, int scaled) {
+                      int maxiter, int scaled, int rhs) {
+  const int64_t o = (int64_t)rhs * s->vstride;
   hipLaunchKernelGGL(k_init_warm, dim3(s->vec_blocks), dim3(kBlock), 0, s->stream, xprev,
-                     xprev2, scaled, s->dhalf, s->c64[0], s->c64[1], s->c64[2], s->d, s->bs, s->x, s->r, s->p,
-                     s->partials);
+                     xprev2, scaled, s->dhalf, s->c64[0], s->c64[1], s->c64[2], s->d,
+                     s->bs + o, s->x + o, s->r + o, s->p + o, s->partials);
   hipLaunchKernelGGL(k_init_warm_final, dim3(1), dim3(kBlock), 0, s->stream, s->partials,
-                     s->vec_blocks, tol, maxiter, s->sc);
+                     s->vec_blocks, tol, maxiter, s->sc + rhs);
 }
 
 

@@ -143,12 +143,19 @@ class GridLinearSystem:
 
     def solve(self, b, x0=None) -> np.ndarray:
         """``LinearSystem.solve``: ``b`` carries the Dirichlet values on
-        Dirichlet rows; several right-hand sides may be stacked as (k, n)."""
-        if x0 is not None:
-            raise NotImplementedError("initial guesses are not supported on the GPU path")
+        Dirichlet rows; several right-hand sides may be stacked as (k, n).
+        ``x0`` (the shape of ``b``) is an initial guess: its free entries,
+        equilibrated, start the iteration (``fieldspice/solvers/poisson.py:543-547``);
+        the Dirichlet values still come from ``b``."""
         b = np.asarray(b, dtype=np.float64)
         if b.shape[-1] != self.n or b.ndim > 2:
             raise ValueError(f"right-hand side must have shape ({self.n},), got {b.shape}")
-        x, st, rc = self.sys.solve(b, self.params())
+        if x0 is not None:
+            x0 = np.asarray(x0, dtype=np.float64)
+            if x0.shape != b.shape:
+                raise ValueError(f"x0 must have shape {b.shape}, got {x0.shape}")
+            if not np.all(np.isfinite(x0)):
+                raise ValueError("x0 must be finite")
+        x, st, rc = self.sys.solve(b, self.params(), x0=x0)
         self._judge(st, rc)
         return x

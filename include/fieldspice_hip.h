@@ -138,7 +138,8 @@ int fs_sys_destroy(fs_sys* sys);
 /* LinearSystem.solve (fieldspice/solvers/poisson.py:487-589) for nrhs
  * right-hand sides at once: b and x are nrhs node vectors back to back.
  * b holds the Dirichlet values on Dirichlet rows (the output of
- * apply_dirichlet); x0 (may be NULL) is an initial guess.  Returns
+ * apply_dirichlet); x0 (may be NULL) holds nrhs initial guesses, of which
+ * the free entries start the iteration (poisson.py:543-547).  Returns
  * FS_NOT_CONVERGED when maxiter is reached. */
 int fs_sys_solve(fs_sys* sys, int nrhs, const double* b, const double* x0,
                  double* x, const fs_solve_params* params, fs_solve_stats* stats);

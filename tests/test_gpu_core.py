@@ -107,6 +107,45 @@ def test_iteration_count_matches_scipy_cg(gpu_lib):
     assert abs(ls.info["iterations"] - ols.info["iterations"]) <= 2
 
 
+@pytest.mark.parametrize("solver", ["hip_pcg", "hip_mixed"])
+def test_initial_guess_solve(gpu_lib, solver):
+    _case(f"general solve from an initial guess, {solver}")
+    g, prop, fixed, vals, A, b = _dirichlet_problem()
+    x_ref = O.LinearSystem(A, method="direct").solve(b)
+    op = _lib.DeviceOperator.from_cells(g, prop)
+    ls = GridLinearSystem(op, fixed, SolverConfig(linear_solver=solver))
+    ls.solve(b)
+    cold = ls.info["iterations"]
+    # the exact solution: converged before the first iteration
+    x = ls.solve(b, x0=x_ref)
+    assert ls.info["iterations"] == 0
+    assert np.abs(x - x_ref).max() <= 1e-9 * np.abs(x_ref).max()
+    # a guess 1e-4 off on the free rows (Dirichlet entries of x0 are ignored:
+    # the values come from b): fewer iterations, the same answer
+    rng = np.random.default_rng(4)
+    x0 = x_ref * (1.0 + 1e-4 * rng.standard_normal(x_ref.size))
+    x0[fixed] = -7.0
+    x = ls.solve(b, x0=x0)
+    warm = ls.info["iterations"]
+    assert warm < cold
+    assert np.abs(x - x_ref).max() <= 1e-9 * np.abs(x_ref).max()
+    assert np.all(x[fixed] == vals)
+    if solver == "hip_pcg":
+        # the reference's CG from the same guess (scipy cg, x0 scaled as
+        # fieldspice/solvers/poisson.py:543-547) takes as many iterations
+        ols = O.LinearSystem(A, method="cg_jacobi", tol=1e-10)
+        ols.solve(b, x0=x0)
+        assert abs(warm - ols.info["iterations"]) <= 2
+    # two right-hand sides, one with a guess per row
+    X = ls.solve(np.stack([b, b]), x0=np.stack([x0, np.zeros_like(x0)]))
+    for k in range(2):
+        assert np.abs(X[k] - x_ref).max() <= 1e-9 * np.abs(x_ref).max()
+    with pytest.raises(ValueError, match="x0"):
+        ls.solve(b, x0=x0[:-1])
+    with pytest.raises(ValueError, match="x0"):
+        ls.solve(b, x0=np.full_like(x0, np.nan))
+
+
 def test_multi_rhs_solve(gpu_lib):
     _case("multi-RHS general solve, 5 right-hand sides")
     g, prop, fixed, vals, A, b = _dirichlet_problem()
