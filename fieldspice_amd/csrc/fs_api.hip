@@ -670,8 +670,22 @@ int fs_sys_create(fs_op* op, const uint8_t* mask, int pin, fs_sys** out, fs_sys_
     t4.ntiles = (int)(t4xy * t4.tiles_z);
     t4.nblocks = (t4.ntiles + 7) / 8 * 8;
   }
-  const int64_t maxb = std::max<int64_t>(std::max<int64_t>(tl.nblocks, s->tl4.nblocks),
-                                         s->vec_blocks);
+  // FP64 stencil with 2 nodes per lane when rows are even
+  s->use_x2 = d.nx % 2 == 0 ? 1 : 0;
+  {
+    fs::Tiling& t2 = s->tl2;
+    t2.tiles_x = (d.nx / 2 + fs::kWave - 1) / fs::kWave;
+    t2.tiles_y = (d.ny + fs::kTileY - 1) / fs::kTileY;
+    const int64_t t2xy = (int64_t)t2.tiles_x * t2.tiles_y;
+    t2.kz = (int)std::min<int64_t>(16, std::max<int64_t>(4, (d.nz * t2xy + 1023) / 1024));
+    t2.kz = std::max(1, std::min(t2.kz, d.nz));
+    t2.tiles_z = (d.nz + t2.kz - 1) / t2.kz;
+    t2.ntiles = (int)(t2xy * t2.tiles_z);
+    t2.nblocks = (t2.ntiles + 7) / 8 * 8;
+  }
+  const int64_t maxb = std::max<int64_t>(
+      std::max<int64_t>(std::max<int64_t>(tl.nblocks, s->tl4.nblocks), s->tl2.nblocks),
+      s->vec_blocks);
   s->partials_cap = maxb * fs::kMaxRhs * 2 + 64 * (fs::kMaxRhs + 1) * fs::kMaxRhs;
   int rc = FS_OK;
   double* dsum = nullptr;

@@ -94,6 +94,8 @@ struct fs_sys {
   fs::Tiling tl{};
   fs::Tiling tl4{};            // 4-nodes-per-lane FP32 stencil (x tiles of 256 nodes)
   int use_x4 = 0;              // rows are a multiple of 4 and that kernel is selected
+  fs::Tiling tl2{};            // 2-nodes-per-lane FP64 stencil (x tiles of 128 nodes)
+  int use_x2 = 0;              // rows are a multiple of 2
   int64_t vstride = 0;
   uint8_t* mask = nullptr;     // 0 free, 1 Dirichlet, 2 pinned (x-fast)
   double* dhalf = nullptr;     // sqrt(D) on free nodes, 0 elsewhere

@@ -538,6 +538,112 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot4(const float* __restrict
   }
 }
 
+// FP64 q = A_s p with 2 consecutive x nodes per lane (16-byte loads; rows a
+// multiple of 2): the layout of k_stencil_dot4 for doubles, 128 x nodes per
+// wavefront, every right-hand side advanced in the same z-march so the
+// couplings are read once per node.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_stencil_dot2(const double* __restrict__ p,
+                                                         double* __restrict__ q,
+                                                         const double* __restrict__ cx,
+                                                         const double* __restrict__ cy,
+                                                         const double* __restrict__ cz, Dims d,
+                                                         Tiling tl, VecArgs va, int mode,
+                                                         CgScalars* sc, double* partials,
+                                                         unsigned long long* prof) {
+  if (all_idle(sc, va.nr, mode)) return;
+  bool live[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) live[r] = r < va.nr && rhs_live(sc, r, mode);
+  if (prof && blockIdx.x == 0 && threadIdx.x == 0) {
+    unsigned long long nl = 0;
+    for (int r = 0; r < NR; ++r) nl += live[r] ? 1ull : 0ull;
+    prof[0] += 1ull;
+    prof[1] += nl;
+  }
+  double acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+  int tx, ty, tz;
+  if (tile_of_block(tl, tx, ty, tz)) {
+    const int lane = (int)(threadIdx.x % kWave);
+    const int i = (tx * kWave + lane) * 2;
+    const int j = ty * kTileY + (int)(threadIdx.x / kWave);
+    const int k0 = tz * tl.kz;
+    const int k1 = min(d.nz, k0 + tl.kz);
+    const bool valid = i < d.nx && j < d.ny;  // i, i+1 inside: nx % 2 == 0
+    const int64_t base = valid ? (int64_t)j * d.nx + i : 0;
+    const bool has_w = i > 0, has_e = i + 2 < d.nx, has_n = j + 1 < d.ny, has_s = j > 0;
+    int64_t nn = (int64_t)k0 * d.sxy + base;
+    const double2 z2 = make_double2(0.0, 0.0);
+    auto ld2 = [](const double* a) { return *reinterpret_cast<const double2*>(a); };
+    double2 pm[NR], pc[NR];
+    double2 czm = z2;
+    if (valid && k0 > 0) czm = ld2(cz + nn - d.sxy);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      pm[r] = z2;
+      pc[r] = z2;
+      if (valid && live[r]) {
+        if (k0 > 0) pm[r] = ld2(p + r * va.vs + nn - d.sxy);
+        pc[r] = ld2(p + r * va.vs + nn);
+      }
+    }
+    for (int k = k0; k < k1; ++k, nn += d.sxy) {
+      double2 cxc = z2, cyc = z2, czc = z2, cym = z2;
+      if (valid) {
+        cxc = ld2(cx + nn);
+        cyc = ld2(cy + nn);
+        czc = ld2(cz + nn);
+        if (has_s) cym = ld2(cy + nn - d.nx);
+      }
+      // coupling to the west of this lane's first node
+      const double cxw_l = __shfl_up(cxc.y, 1, kWave);
+      double cxw = 0.0;
+      if (valid && has_w) cxw = lane > 0 ? cxw_l : cx[nn - 1];
+      const bool up = k + 1 < d.nz;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        if (!live[r]) continue;  // uniform over the launch
+        const double* __restrict__ pr = p + r * va.vs;
+        double2 pu = z2, pn = z2, ps = z2;
+        if (valid) {
+          if (up) pu = ld2(pr + nn + d.sxy);
+          if (has_n) pn = ld2(pr + nn + d.nx);
+          if (has_s) ps = ld2(pr + nn - d.nx);
+        }
+        const double2 c = pc[r];
+        double pw = __shfl_up(c.y, 1, kWave), pe = __shfl_down(c.x, 1, kWave);
+        if (valid) {
+          if (!has_w) pw = 0.0;
+          else if (lane == 0) pw = pr[nn - 1];
+          if (!has_e) pe = 0.0;
+          else if (lane == kWave - 1) pe = pr[nn + 2];
+        }
+        const double2 m = pm[r];
+        double2 y;
+        y.x = c.x - (cxc.x * c.y + cxw * pw + cyc.x * pn.x + cym.x * ps.x + czc.x * pu.x +
+                     czm.x * m.x);
+        y.y = c.y - (cxc.y * pe + cxc.x * c.x + cyc.y * pn.y + cym.y * ps.y + czc.y * pu.y +
+                     czm.y * m.y);
+        if (valid) {
+          *reinterpret_cast<double2*>(q + r * va.vs + nn) = y;
+          acc[r] += c.x * y.x + c.y * y.y;
+        }
+        pm[r] = c;
+        pc[r] = pu;
+      }
+      czm = czc;
+    }
+  }
+  block_sum<NR>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (r < va.nr) store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
+  }
+}
+
 // x += alpha p; r -= alpha q; rho_new = r.r; beta = rho_new/rho; stop tests.
 // mode 0: FP64 main iteration (threshold thresh2, sets done).
 // mode 1: FP32 cycle of mixed precision (threshold aux1 on the cycle's own
@@ -1067,6 +1173,28 @@ static void launch_dot4(hipStream_t st, fs_sys* s, const Tiling& tl, const VecAr
 }
 
 template <int NR>
+static void dot2_t(hipStream_t st, fs_sys* s, const VecArgs& va, int mode,
+                   unsigned long long* prof, hipEvent_t ea, hipEvent_t eb) {
+  launch_timed(k_stencil_dot2<NR>, dim3(s->tl2.nblocks), st, ea, eb, (const double*)s->p, s->q,
+               (const double*)s->c64[0], (const double*)s->c64[1], (const double*)s->c64[2],
+               s->d, s->tl2, va, mode, s->sc, s->partials, prof);
+}
+// FP64 stencil of the Krylov loop, instantiated for the exact RHS count
+static void launch_dot2(hipStream_t st, fs_sys* s, const VecArgs& va, int mode,
+                        unsigned long long* prof, hipEvent_t ea, hipEvent_t eb) {
+  switch (va.nr) {
+    case 1: dot2_t<1>(st, s, va, mode, prof, ea, eb); break;
+    case 2: dot2_t<2>(st, s, va, mode, prof, ea, eb); break;
+    case 3: dot2_t<3>(st, s, va, mode, prof, ea, eb); break;
+    case 4: dot2_t<4>(st, s, va, mode, prof, ea, eb); break;
+    case 5: dot2_t<5>(st, s, va, mode, prof, ea, eb); break;
+    case 6: dot2_t<6>(st, s, va, mode, prof, ea, eb); break;
+    case 7: dot2_t<7>(st, s, va, mode, prof, ea, eb); break;
+    default: dot2_t<8>(st, s, va, mode, prof, ea, eb); break;
+  }
+}
+
+template <int NR>
 static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {
   hipStream_t st = s->stream;
   VecArgs va{s->d.n, s->vstride, nr};
@@ -1079,10 +1207,16 @@ static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEve
     hipLaunchKernelGGL((k_pupdate<double, NR>), dim3(vb), dim3(kBlock), 0, st, s->p, s->r, va,
                        mode, s->sc);
     if (defl) launch_defl_scatter(s, nr, mode);
-    launch_timed(k_stencil_dot<double, NR>, dim3(sb), st, ea, eb, (const double*)s->p, s->q,
-                 (const double*)s->c64[0], (const double*)s->c64[1], (const double*)s->c64[2],
-                 s->d, s->tl, va, mode, s->sc, s->partials, prof);
-    finalize<NR>(st, s->partials, defl ? FIN_ALPHA_ZR : FIN_ALPHA, nr, sb, mode, s->sc);
+    int sbd = sb;
+    if (s->use_x2) {
+      sbd = s->tl2.nblocks;
+      launch_dot2(st, s, va, mode, prof, ea, eb);
+    } else {
+      launch_timed(k_stencil_dot<double, NR>, dim3(sb), st, ea, eb, (const double*)s->p, s->q,
+                   (const double*)s->c64[0], (const double*)s->c64[1], (const double*)s->c64[2],
+                   s->d, s->tl, va, mode, s->sc, s->partials, prof);
+    }
+    finalize<NR>(st, s->partials, defl ? FIN_ALPHA_ZR : FIN_ALPHA, nr, sbd, mode, s->sc);
     hipLaunchKernelGGL((k_update<double, NR>), dim3(vb), dim3(kBlock), 0, st, s->x, s->r, s->p,
                        s->q, va, mode, s->sc, s->partials);
     finalize<NR>(st, s->partials, FIN_BETA, nr, vb, mode, s->sc);
