@@ -11,7 +11,7 @@ from .boundaries import BoundarySpec, Dirichlet, Neumann
 from .grid import RectilinearGrid, graded_1d
 from .materials import MaterialMap
 from .solvers.base import ConvergenceError, Result, SolverConfig, Terminal
-from .solvers.poisson import PoissonSolver
+from .solvers.poisson import NonlinearPoissonSolver, PoissonSolver, boltzmann_equilibrium_charge
 from .solvers.eqs import EQSSolver
 from .solvers.thermal import ThermalSolver, joule_heating_nodes
 
@@ -19,5 +19,5 @@ __version__ = "0.1.0"
 
 __all__ = ["RectilinearGrid", "graded_1d", "MaterialMap", "BoundarySpec", "Dirichlet",
            "Neumann", "Terminal", "SolverConfig", "Result", "ConvergenceError",
-           "PoissonSolver", "EQSSolver", "ThermalSolver", "joule_heating_nodes", "extraction",
+           "PoissonSolver", "NonlinearPoissonSolver", "boltzmann_equilibrium_charge", "EQSSolver", "ThermalSolver", "joule_heating_nodes", "extraction",
            "units", "boundaries"]

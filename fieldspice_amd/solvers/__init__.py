@@ -3,9 +3,10 @@ EQS / thermal implicit steps."""
 
 from .base import ConvergenceError, Result, SolverConfig, Terminal
 from .linear import GridLinearSystem
-from .poisson import PoissonSolver
+from .poisson import NonlinearPoissonSolver, PoissonSolver, boltzmann_equilibrium_charge
 from .eqs import EQSSolver
 from .thermal import ThermalSolver, joule_heating_nodes
 
 __all__ = ["ConvergenceError", "Result", "SolverConfig", "Terminal", "GridLinearSystem",
-           "PoissonSolver", "EQSSolver", "ThermalSolver", "joule_heating_nodes"]
+           "PoissonSolver", "NonlinearPoissonSolver", "boltzmann_equilibrium_charge",
+           "EQSSolver", "ThermalSolver", "joule_heating_nodes"]

@@ -31,6 +31,11 @@ How each part is pinned (see the tests named in brackets):
   [tests/test_oracle.py]
 * ``thermal_*``: parabolic profile, slab R_th, convective R_th, an RC ladder
   built independently from the 1D network  [tests/test_oracle.py]
+* ``boltzmann_charge``/``nonlinear_poisson``: the built-in potential of an
+  abrupt pn junction, Vt ln(Na Nd/ni^2) = 0.833370 V, to 1e-9 (the
+  reference's docs/CONTRACTS.md:455,480-481), a charge-neutral guess, Gauss's
+  law over the junction, and a wrong derivative rejected by the line search
+  [tests/test_oracle.py]
 """
 
 from __future__ import annotations
@@ -532,3 +537,86 @@ def island_matrix(L_free, labels, count):
     rows = np.nonzero(labels >= 0)[0]
     Z = sp.csr_matrix((np.ones(rows.size), (rows, labels[rows])), shape=(n, count))
     return (Z.T @ L_free @ Z).toarray()
+
+
+# --------------------------------------------------------------------------
+# nonlinear Poisson (fieldspice/solvers/poisson.py:1380-1524, 1550-1617)
+# --------------------------------------------------------------------------
+Q_ELEM = 1.602_176_634e-19
+KB = 1.380_649e-23
+
+
+def boltzmann_charge(doping_node, ni, T=300.0):
+    """rho = q (p - n + N) with n, p = ni exp(+-phi/Vt), exponent clipped at
+    400; drho = -q (n + p)/Vt; guess Vt asinh(N/2ni)
+    (fieldspice/solvers/poisson.py:1550-1617).  Returns (rho, drho, guess, Vt)."""
+    dop = np.asarray(doping_node, dtype=float).ravel()
+    Vt = KB * T / Q_ELEM
+
+    def carriers(phi):
+        z = np.clip(np.asarray(phi, dtype=float) / Vt, -400.0, 400.0)
+        return ni * np.exp(z), ni * np.exp(-z)
+
+    def rho(phi):
+        n, p = carriers(phi)
+        return Q_ELEM * (p - n + dop)
+
+    def drho(phi):
+        n, p = carriers(phi)
+        return -Q_ELEM * (n + p) / Vt
+
+    return rho, drho, Vt * np.arcsinh(dop / (2.0 * ni)), Vt
+
+
+def nonlinear_poisson(grid, eps_cell, rho, drho, fixed=None, vals=None, rho_node=None,
+                      phi0=None, v_scale=None, step_limit=5.0, armijo=1e-4, newton_tol=1e-8,
+                      max_newton=60, damping=1.0):
+    """Damped Newton for L phi - rho(phi) V_node = rho_node V_node with
+    Dirichlet rows ``fixed`` at ``vals`` (fieldspice/solvers/poisson.py:1429-1511):
+    the Jacobian L - diag(drho V_node) is eliminated and solved by SuperLU each
+    step, the first trial step is clamped to step_limit*v_scale, then Armijo
+    backtracking.  Returns (phi, history of |F|, damping factors)."""
+    L = nodal_laplacian(grid, cell_to_edge(grid, eps_cell))
+    vnode = np.asarray(grid.node_volumes(), dtype=float).ravel()
+    n = grid.n_nodes
+    fixed = np.empty(0, dtype=np.intp) if fixed is None else np.asarray(fixed, dtype=np.intp)
+    vals = np.empty(0) if vals is None else np.asarray(vals, dtype=float)
+    b_ext = np.zeros(n) if rho_node is None else np.asarray(rho_node, float).ravel() * vnode
+    v_scale = KB * 300.0 / Q_ELEM if v_scale is None else v_scale
+    phi = np.zeros(n) if phi0 is None else np.asarray(phi0, dtype=float).ravel().copy()
+    phi[fixed] = vals
+
+    def residual(p):
+        f = L @ p - rho(p) * vnode - b_ext
+        f[fixed] = 0.0
+        return f
+
+    F = residual(phi)
+    hist = [float(np.linalg.norm(F))]
+    lams = []
+    for it in range(max_newton):
+        if hist[-1] <= 1e-14 * max(hist[0], 1e-300):
+            return phi, hist, lams
+        J = (L - sp.diags(drho(phi) * vnode)).tocsr()
+        A, b = apply_dirichlet(J, -F, fixed, np.zeros(fixed.size))
+        dphi = spla.splu(A.tocsc()).solve(b)
+        big = float(np.abs(dphi).max())
+        lam = float(damping)
+        if big > 0.0:
+            lam = min(lam, step_limit * v_scale / big)
+        f0 = hist[-1]
+        while True:
+            Fn = residual(phi + lam * dphi)
+            fn = float(np.linalg.norm(Fn))
+            if np.isfinite(fn) and fn < (1.0 - armijo * lam) * f0:
+                break
+            lam *= 0.5
+            if lam < 1e-12:
+                raise OracleConvergenceError("line search collapsed")
+        phi = phi + lam * dphi
+        F = Fn
+        hist.append(fn)
+        lams.append(lam)
+        if lam * big <= newton_tol * v_scale:
+            return phi, hist, lams
+    raise OracleConvergenceError("Newton did not converge")

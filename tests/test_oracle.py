@@ -366,3 +366,35 @@ def test_oracle_floating_islands_on_constructed_boxes():
     # without contrast there are no islands
     lab0, n0 = O.floating_islands(g, np.ones_like(w), anchored)
     assert n0 == 0 and np.all(lab0 == -1)
+
+
+def _pn_junction(ni=1e16, N=1e23, L=1e-6, cells=1000):
+    """Abrupt Si pn junction (Na = Nd = 1e17 cm^-3, 300 K), one ohmic contact
+    at x = 0 held at the p-side neutral potential, Neumann elsewhere."""
+    g = RectilinearGrid(np.linspace(0.0, L, cells + 1), None, None)
+    x = np.broadcast_to(g.xn[:, None, None], g.shape_nodes).ravel()
+    dop = np.where(x < L / 2, -N, N)
+    eps = np.full(g.shape_cells, 11.7 * eps0)
+    nid = g.node_ids()
+    return g, eps, dop, nid[0].ravel(), nid[-1].ravel()
+
+
+def test_oracle_newton_pn_junction_built_in_potential():
+    # docs/CONTRACTS.md:455,480-481 of the reference: Vbi = Vt ln(Na Nd / ni^2)
+    # = 0.833370 V to 1e-9 relative
+    g, eps, dop, lo, hi = _pn_junction()
+    rho, drho, guess, Vt = O.boltzmann_charge(dop, 1e16)
+    assert np.abs(rho(guess)).max() < 1e-9 * O.Q_ELEM * 1e23    # the guess is neutral
+    phi, hist, lams = O.nonlinear_poisson(g, eps, rho, drho, fixed=lo, vals=guess[lo],
+                                          phi0=guess)
+    vbi = float(phi[hi].mean() - phi[lo].mean())
+    assert vbi == pytest.approx(Vt * np.log(1e23 * 1e23 / 1e32), rel=1e-9)
+    assert vbi == pytest.approx(0.833370, abs=5e-7)
+    assert hist[-1] < 1e-9 * hist[0]
+    # Gauss: the junction's net charge vanishes (Neumann on both far faces)
+    q = rho(phi) * g.node_volumes().ravel()
+    assert abs(q.sum()) < 1e-9 * np.abs(q).sum()
+    # a wrong derivative is caught by the line search, not silently accepted
+    with pytest.raises(O.OracleConvergenceError):
+        O.nonlinear_poisson(g, eps, rho, lambda p: -drho(p), fixed=lo, vals=guess[lo],
+                            phi0=guess, max_newton=30)
