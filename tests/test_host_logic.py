@@ -195,3 +195,23 @@ def test_nonlinear_poisson_argument_checks_and_edge_gradient():
     phi = np.random.default_rng(0).standard_normal(g.n_nodes)
     np.testing.assert_allclose(edge_gradient(g, phi), O.grad_node_edge(g) @ phi, rtol=0,
                                atol=1e-15)
+
+
+def test_electrothermal_host_checks():
+    from fieldspice_amd.physics import PhysicsOptions
+    from fieldspice_amd.solvers.electrothermal import ElectroThermalSolver, runaway_threshold
+    # tests/test_thermal.py:329-340 of the reference
+    assert runaway_threshold(1e3, 1.0, 1e-3) == pytest.approx(1.0)
+    assert runaway_threshold(1e3, 1.0, 0.0) == float("inf")
+    assert runaway_threshold(1e3, 1.0, -1e-3) == float("inf")
+    g = _g()
+    sig, kap = np.ones(g.shape_cells), np.ones(g.shape_cells)
+    with pytest.raises(ValueError, match="self_heating"):
+        ElectroThermalSolver(g, sig, kap, 3.93e-3, PhysicsOptions())
+    with pytest.raises(ValueError, match="tcr_cell"):
+        ElectroThermalSolver(g, sig, kap, np.zeros(3))
+    with pytest.raises(ValueError, match="coupling_tolerance"):
+        PhysicsOptions(coupling_tolerance=0.0)
+    o = PhysicsOptions(self_heating=True)
+    assert o.sigma_varies_with_T and "A6" in o.relaxed_assumptions()
+    assert o.remaining_assumptions(("A1a", "A6")) == ("A1a",)
