@@ -215,3 +215,29 @@ def test_electrothermal_host_checks():
     o = PhysicsOptions(self_heating=True)
     assert o.sigma_varies_with_T and "A6" in o.relaxed_assumptions()
     assert o.remaining_assumptions(("A1a", "A6")) == ("A1a",)
+
+
+def test_field_region_host_checks_and_stamps():
+    import scipy.sparse as sp
+    from fieldspice_amd.circuit import FieldRegion, stamp_g, stamp_i
+    g = _g()
+    nid = g.node_ids()
+    eps = np.full(g.shape_cells, 3.9 * eps0)
+    a, b = Terminal("a", nid[0].ravel()), Terminal("b", nid[-1].ravel())
+    with pytest.raises(ValueError, match="at least one terminal"):
+        FieldRegion("FR", g, eps, np.zeros(g.shape_cells), [])
+    with pytest.raises(ValueError, match="circuit_nodes"):
+        FieldRegion("FR", g, eps, np.zeros(g.shape_cells), [a, b], circuit_nodes=["x"])
+    with pytest.raises(ValueError, match="ABSOLUTE"):
+        FieldRegion("FR", g, -eps, np.zeros(g.shape_cells), [a, b])
+    with pytest.raises(ValueError, match="overlap"):
+        FieldRegion("FR", g, eps, np.zeros(g.shape_cells), [a, Terminal("c", nid[0, 0])])
+    G = np.zeros((2, 2))
+    stamp_g(G, 0, 1, 2.0)
+    stamp_g(G, -1, 1, 5.0)             # ground dropped
+    L = sp.dok_matrix((2, 2))
+    stamp_g(L, 1, 1, 3.0)
+    I = np.zeros(2)
+    stamp_i(I, 1, 4.0)
+    stamp_i(I, -1, 9.0)
+    assert G[0, 1] == 2.0 and G.sum() == 2.0 and L[1, 1] == 3.0 and I.tolist() == [0.0, 4.0]
