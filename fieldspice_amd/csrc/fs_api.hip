@@ -205,7 +205,20 @@ int default_maxiter(const fs_sys* s, const fs_solve_params* P) {
 // Krylov loop after init: FP64 CG (mode 0), or FP32 cycles with FP64
 // refreshes: CG (mode 1) or flexible CG preconditioned by the attached fs_pc
 // (mode 2).  Host checks the device flags between chunks of iterations.
+static int run_krylov_loop(fs_sys* s, int nr, const fs_solve_params& P, int mode);
+
 int run_krylov_mode(fs_sys* s, int nr, const fs_solve_params& P, int mode) {
+  const int rc = run_krylov_loop(s, nr, P, mode);
+  if (mode == 0) {
+    // FP64 CG leaves the last x += alpha p pending (deferred into the next
+    // direction update, which a finished right-hand side never runs)
+    fs::launch_flush_x(s, nr);
+    if (rc == FS_OK) FS_HIP(hipGetLastError());
+  }
+  return rc;
+}
+
+static int run_krylov_loop(fs_sys* s, int nr, const fs_solve_params& P, int mode) {
   hipStream_t st = s->ctx->stream;
   const bool cyc = mode != 0;
   const double inner_tol = P.inner_tol > 0 ? P.inner_tol : (mode == 2 ? 1e-5 : 1e-4);

@@ -688,6 +688,22 @@ __global__ __launch_bounds__(kBlock) void k_update(T* __restrict__ x, T* __restr
   }
 }
 
+// The pending step of a DX loop: x += alpha p for every right-hand side,
+// with the alpha of its last iteration (0 at the start and after a breakdown).
+__global__ __launch_bounds__(kBlock) void k_flush_x(double* __restrict__ x,
+                                                    const double* __restrict__ p, VecArgs va,
+                                                    const CgScalars* sc) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int r = 0; r < va.nr; ++r) {
+    const double a = sc[r].alpha;
+    if (a == 0.0 || !isfinite(a)) continue;  // uniform over the launch
+    double* xr = x + r * va.vs;
+    const double* pr = p + r * va.vs;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride)
+      xr[i] += a * pr[i];
+  }
+}
+
 // p = r + beta p; DX: first x += alpha p with the previous step's alpha (the
 // deferred update of k_update DX; alpha is 0 at the start of a cycle)
 template <typename T, int NR, bool DX = false>
@@ -1204,8 +1220,10 @@ static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEve
   if (mode == 0) {
     // with an island space: p = (I + Z E^-1 Z^T) r + beta p, alpha = zr/pq
     const bool defl = s->defl_n > 0;
-    hipLaunchKernelGGL((k_pupdate<double, NR>), dim3(vb), dim3(kBlock), 0, st, s->p, s->r, va,
-                       mode, s->sc);
+    // x += alpha p of the previous step rides on the direction update (DX);
+    // the last pending step is added by launch_flush_x when the loop ends
+    hipLaunchKernelGGL((k_pupdate<double, NR, true>), dim3(vb), dim3(kBlock), 0, st, s->p, s->r,
+                       va, mode, s->sc, s->x);
     if (defl) launch_defl_scatter(s, nr, mode);
     int sbd = sb;
     if (s->use_x2) {
@@ -1217,8 +1235,8 @@ static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEve
                    s->d, s->tl, va, mode, s->sc, s->partials, prof);
     }
     finalize<NR>(st, s->partials, defl ? FIN_ALPHA_ZR : FIN_ALPHA, nr, sbd, mode, s->sc);
-    hipLaunchKernelGGL((k_update<double, NR>), dim3(vb), dim3(kBlock), 0, st, s->x, s->r, s->p,
-                       s->q, va, mode, s->sc, s->partials);
+    hipLaunchKernelGGL((k_update<double, NR, true>), dim3(vb), dim3(kBlock), 0, st, s->x, s->r,
+                       s->p, s->q, va, mode, s->sc, s->partials);
     finalize<NR>(st, s->partials, FIN_BETA, nr, vb, mode, s->sc);
     if (defl) launch_defl_coef(s, nr, mode, 0);
     return 0;
@@ -1255,6 +1273,12 @@ static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEve
     else if ((nr) <= 4) FN<4>(__VA_ARGS__); \
     else FN<8>(__VA_ARGS__);               \
   } while (0)
+
+void launch_flush_x(fs_sys* s, int nr) {
+  VecArgs va{s->d.n, s->vstride, nr};
+  hipLaunchKernelGGL(k_flush_x, dim3(s->vec_blocks), dim3(kBlock), 0, s->stream, s->x,
+                     (const double*)s->p, va, (const CgScalars*)s->sc);
+}
 
 int launch_iteration(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {
   if (s->defl_n > 0 && mode == 1) return launch_iteration_defl(s, nr, mode, ea, eb);

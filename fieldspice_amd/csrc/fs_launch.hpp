@@ -55,6 +55,9 @@ void launch_finalize(hipStream_t st, const double* partials, int op, int nr, int
 
 // mode 0 FP64 CG, 1 FP32 CG cycle, 2 FP32 flexible CG cycle (returns a status)
 int launch_iteration(fs_sys* s, int nr, int mode, hipEvent_t ev_a, hipEvent_t ev_b);
+// FP64 CG (mode 0) defers x += alpha p into the next direction update; this adds
+// the last pending step, once, when the loop ends
+void launch_flush_x(fs_sys* s, int nr);
 void launch_init(fs_sys* s, int nr, double tol, int maxiter);
 void launch_lift_general(fs_sys* s, int nr, const double* b);
 void launch_lift_cap(fs_sys* s, int nr, const fs_op* L, const int32_t* tid, int j0);
