@@ -1042,6 +1042,107 @@ __global__ __launch_bounds__(kBlock) void k_residual(const double* __restrict__ 
   }
 }
 
+// FP64 residual r = b_s - A_s x with 2 consecutive x nodes per lane (the
+// tiles and z-march of k_stencil_dot2; rows a multiple of 2) and the two
+// partial sums of k_residual: ||r||^2 and ||r D^1/2||^2 per right-hand side.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_residual2(const double* __restrict__ bs,
+                                                      const double* __restrict__ x,
+                                                      double* __restrict__ res,
+                                                      const double* __restrict__ cx,
+                                                      const double* __restrict__ cy,
+                                                      const double* __restrict__ cz,
+                                                      const double* __restrict__ dhalf, Dims d,
+                                                      Tiling tl, VecArgs va, double* partials) {
+  double acc[2 * NR];
+#pragma unroll
+  for (int a = 0; a < 2 * NR; ++a) acc[a] = 0.0;
+  int tx, ty, tz;
+  if (tile_of_block(tl, tx, ty, tz)) {
+    const int lane = (int)(threadIdx.x % kWave);
+    const int i = (tx * kWave + lane) * 2;
+    const int j = ty * kTileY + (int)(threadIdx.x / kWave);
+    const int k0 = tz * tl.kz;
+    const int k1 = min(d.nz, k0 + tl.kz);
+    const bool valid = i < d.nx && j < d.ny;  // i, i+1 inside: nx % 2 == 0
+    const int64_t base = valid ? (int64_t)j * d.nx + i : 0;
+    const bool has_w = i > 0, has_e = i + 2 < d.nx, has_n = j + 1 < d.ny, has_s = j > 0;
+    int64_t nn = (int64_t)k0 * d.sxy + base;
+    const double2 z2 = make_double2(0.0, 0.0);
+    auto ld2 = [](const double* a) { return *reinterpret_cast<const double2*>(a); };
+    double2 xm[NR], xc[NR];
+    double2 czm = z2;
+    if (valid && k0 > 0) czm = ld2(cz + nn - d.sxy);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      xm[r] = z2;
+      xc[r] = z2;
+      if (valid && r < va.nr) {
+        if (k0 > 0) xm[r] = ld2(x + r * va.vs + nn - d.sxy);
+        xc[r] = ld2(x + r * va.vs + nn);
+      }
+    }
+    for (int k = k0; k < k1; ++k, nn += d.sxy) {
+      double2 cxc = z2, cyc = z2, czc = z2, cym = z2, dh = z2;
+      if (valid) {
+        cxc = ld2(cx + nn);
+        cyc = ld2(cy + nn);
+        czc = ld2(cz + nn);
+        dh = ld2(dhalf + nn);
+        if (has_s) cym = ld2(cy + nn - d.nx);
+      }
+      const double cxw_l = __shfl_up(cxc.y, 1, kWave);
+      double cxw = 0.0;
+      if (valid && has_w) cxw = lane > 0 ? cxw_l : cx[nn - 1];
+      const bool up = k + 1 < d.nz;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        if (r >= va.nr) continue;  // uniform over the launch
+        const double* __restrict__ xr = x + r * va.vs;
+        double2 xu = z2, xn = z2, xs = z2, b = z2;
+        if (valid) {
+          if (up) xu = ld2(xr + nn + d.sxy);
+          if (has_n) xn = ld2(xr + nn + d.nx);
+          if (has_s) xs = ld2(xr + nn - d.nx);
+          b = ld2(bs + r * va.vs + nn);
+        }
+        const double2 c = xc[r];
+        double xw = __shfl_up(c.y, 1, kWave), xe = __shfl_down(c.x, 1, kWave);
+        if (valid) {
+          if (!has_w) xw = 0.0;
+          else if (lane == 0) xw = xr[nn - 1];
+          if (!has_e) xe = 0.0;
+          else if (lane == kWave - 1) xe = xr[nn + 2];
+        }
+        const double2 m = xm[r];
+        // Dirichlet and pinned rows are identity rows with x_s = b_s = 0 there
+        double2 rv;
+        rv.x = b.x - (c.x - (cxc.x * c.y + cxw * xw + cyc.x * xn.x + cym.x * xs.x +
+                             czc.x * xu.x + czm.x * m.x));
+        rv.y = b.y - (c.y - (cxc.y * xe + cxc.x * c.x + cyc.y * xn.y + cym.y * xs.y +
+                             czc.y * xu.y + czm.y * m.y));
+        if (valid) {
+          *reinterpret_cast<double2*>(res + r * va.vs + nn) = rv;
+          acc[r] += rv.x * rv.x + rv.y * rv.y;
+          acc[NR + r] += (rv.x * dh.x) * (rv.x * dh.x) + (rv.y * dh.y) * (rv.y * dh.y);
+        }
+        xm[r] = c;
+        xc[r] = xu;
+      }
+      czm = czc;
+    }
+  }
+  block_sum<2 * NR>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      if (r >= va.nr) continue;
+      store_partial(partials + (int64_t)r * gridDim.x + blockIdx.x, acc[r]);
+      store_partial(partials + (int64_t)(va.nr + r) * gridDim.x + blockIdx.x, acc[NR + r]);
+    }
+  }
+}
+
 // --------------------------------------------------------------------------
 // mixed precision: FP32 cycles with FP64 residual refresh
 // --------------------------------------------------------------------------
@@ -1335,8 +1436,31 @@ void launch_expand(fs_sys* s, int nr, const double* b, const int32_t* tid, int j
 }
 
 template <int NR>
+static void residual2_t(fs_sys* s, const VecArgs& va, double* res) {
+  hipLaunchKernelGGL((k_residual2<NR>), dim3(s->tl2.nblocks), dim3(kBlock), 0, s->stream,
+                     (const double*)s->bs, (const double*)s->x, res, (const double*)s->c64[0],
+                     (const double*)s->c64[1], (const double*)s->c64[2],
+                     (const double*)s->dhalf, s->d, s->tl2, va, s->partials);
+}
+
+template <int NR>
 static void launch_residual_t(fs_sys* s, int nr, double* res) {
   VecArgs va{s->d.n, s->vstride, nr};
+  if (s->use_x2) {
+    // 2 nodes per lane on the stencil tiles, instantiated for the exact count
+    switch (nr) {
+      case 1: residual2_t<1>(s, va, res); break;
+      case 2: residual2_t<2>(s, va, res); break;
+      case 3: residual2_t<3>(s, va, res); break;
+      case 4: residual2_t<4>(s, va, res); break;
+      case 5: residual2_t<5>(s, va, res); break;
+      case 6: residual2_t<6>(s, va, res); break;
+      case 7: residual2_t<7>(s, va, res); break;
+      default: residual2_t<8>(s, va, res); break;
+    }
+    finalize<NR>(s->stream, s->partials, FIN_RESID, nr, s->tl2.nblocks, 0, s->sc);
+    return;
+  }
   hipLaunchKernelGGL((k_residual<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->stream,
                      s->bs, s->x, res, s->c64[0], s->c64[1], s->c64[2], s->dhalf, s->d, va,
                      s->partials);
