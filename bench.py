@@ -420,7 +420,7 @@ def main():
     if args.workload == "c3" and args.gpus == 1:
         return run_c3(args)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "fieldspice_amd runs on one GPU; --gpus must be 1 "
                           "(sharding across GPUs is not implemented)"}))
         sys.exit(2)
     from fieldspice_amd import _lib, workloads
