@@ -148,6 +148,22 @@ __device__ __forceinline__ bool all_idle(const CgScalars* sc, int nr, int mode) 
   return true;
 }
 
+// (i, j, k) of node n = (k*ny + j)*nx + i: 32-bit divisions when the node
+// count fits (64-bit integer division is a long emulated sequence)
+__device__ __forceinline__ void node_coords(const Dims& d, int64_t n, int& i, int& j, int& k) {
+  if (d.n <= 0xffffffffLL) {
+    const uint32_t u = (uint32_t)n, nx = (uint32_t)d.nx, ny = (uint32_t)d.ny;
+    const uint32_t row = u / nx;
+    i = (int)(u - row * nx);
+    k = (int)(row / ny);
+    j = (int)(row - (uint32_t)k * ny);
+  } else {
+    i = (int)(n % d.nx);
+    j = (int)((n / d.nx) % d.ny);
+    k = (int)(n / d.sxy);
+  }
+}
+
 // Bind a tile of the XCD-remapped launch; returns false for padding blocks.
 __device__ __forceinline__ bool tile_of_block(const Tiling& tl, int& tx, int& ty, int& tz) {
   const int bid = blockIdx.x;

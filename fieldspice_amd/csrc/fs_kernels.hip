@@ -185,7 +185,8 @@ __global__ __launch_bounds__(kBlock) void k_apply_raw(const double* __restrict__
                                                       Dims d) {
   const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (n >= d.n) return;
-  const int i = (int)(n % d.nx), j = (int)((n / d.nx) % d.ny), k = (int)(n / d.sxy);
+  int i, j, k;
+  node_coords(d, n, i, j, k);
   y[n] = raw_row(x, wx, wy, wz, extra, d, i, j, k, n);
 }
 
@@ -216,7 +217,8 @@ __global__ __launch_bounds__(kBlock) void k_setup_diag(const fs_op op, uint8_t* 
   bool is_fixed = false, is_empty = false, nonzero_sum = false;
   unsigned long long dbits = 0ull;
   if (n < d.n) {
-    const int i = (int)(n % d.nx), j = (int)((n / d.nx) % d.ny), k = (int)(n / d.sxy);
+    int i, j, k;
+    node_coords(d, n, i, j, k);
     double ws = op.w[0][n] + op.w[1][n] + op.w[2][n];
     if (i > 0) ws += op.w[0][n - 1];
     if (j > 0) ws += op.w[1][n - d.nx];
@@ -297,7 +299,8 @@ __global__ __launch_bounds__(kBlock) void k_setup_couplings(const fs_op op,
   const Dims d = op.d;
   const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (n >= d.n) return;
-  const int i = (int)(n % d.nx), j = (int)((n / d.nx) % d.ny), k = (int)(n / d.sxy);
+  int i, j, k;
+  node_coords(d, n, i, j, k);
   const double s = dinv[n];
   const double vx = (i + 1 < d.nx) ? s * (op.w[0][n] * dinv[n + 1]) : 0.0;
   const double vy = (j + 1 < d.ny) ? s * (op.w[1][n] * dinv[n + d.nx]) : 0.0;
@@ -893,7 +896,8 @@ __global__ __launch_bounds__(kBlock) void k_lift_cap(const fs_op op, const int32
   for (int r = 0; r < NR; ++r) acc[r] = 0.0;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < va.n; n += stride) {
-    const int i = (int)(n % d.nx), j = (int)((n / d.nx) % d.ny), k = (int)(n / d.sxy);
+    int i, j, k;
+    node_coords(d, n, i, j, k);
     const int t_self = tid[n];
     if (mask[n] != 0) {
 #pragma unroll
@@ -1007,7 +1011,8 @@ __global__ __launch_bounds__(kBlock) void k_residual(const double* __restrict__ 
   for (int r = 0; r < 2 * NR; ++r) acc[r] = 0.0;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < va.n; n += stride) {
-    const int i = (int)(n % d.nx), j = (int)((n / d.nx) % d.ny), k = (int)(n / d.sxy);
+    int i, j, k;
+    node_coords(d, n, i, j, k);
     const double cxc = cx[n], cyc = cy[n], czc = cz[n];
     const double cxm = i > 0 ? cx[n - 1] : 0.0;
     const double cym = j > 0 ? cy[n - d.nx] : 0.0;
@@ -1233,7 +1238,8 @@ __global__ __launch_bounds__(kBlock) void k_charge(const fs_op op, const double*
   for (int64_t t = lo + (int64_t)blockIdx.x * kBlock + threadIdx.x; t < hi;
        t += (int64_t)gridDim.x * kBlock) {
     const int64_t n = nodes[t];
-    const int i = (int)(n % d.nx), j = (int)((n / d.nx) % d.ny), k = (int)(n / d.sxy);
+    int i, j, k;
+    node_coords(d, n, i, j, k);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       if (r >= va.nr) continue;

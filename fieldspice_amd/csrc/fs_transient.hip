@@ -14,9 +14,7 @@ namespace fs {
 namespace {
 
 __device__ __forceinline__ void node_ijk(const Dims& d, int64_t n, int& i, int& j, int& k) {
-  i = (int)(n % d.nx);
-  j = (int)((n / d.nx) % d.ny);
-  k = (int)(n / d.sxy);
+  node_coords(d, n, i, j, k);
 }
 
 // rhs = Aexp x + sum_s scale[s] * src[s]   (Aexp may be absent)
