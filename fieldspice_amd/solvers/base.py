@@ -43,7 +43,7 @@ class SolverConfig:
         refresh), ``"hip_fcg"`` (flexible CG with the learned preconditioner).
     inner_tol
         Residual reduction of one FP32 cycle before the FP64 refresh
-        (``None``: 1e-4 for ``hip_mixed``, 1e-5 for ``hip_fcg``).
+        (``None``: 1e-4 for ``hip_mixed``, 1e-3 for ``hip_fcg``).
     preconditioner
         ``hip_fcg`` only: a ``precond.LearnedPreconditioner``; ``None`` uses
         the trained network shipped with the package.

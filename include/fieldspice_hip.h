@@ -65,7 +65,7 @@ typedef struct {
     int maxiter;         /* <= 0: max(500, floor(10*sqrt(n_free))) as the reference */
     int precision;       /* FS_PREC_FP64, FS_PREC_MIXED or FS_PREC_FCG */
     double inner_tol;    /* MIXED/FCG: residual drop per FP32 cycle before an FP64 refresh
-                            (<= 0: 1e-4 MIXED, 1e-5 FCG) */
+                            (<= 0: 1e-4 MIXED, 1e-3 FCG) */
     int check_every;     /* iterations between host convergence checks (<= 0: adaptive) */
 } fs_solve_params;
 
