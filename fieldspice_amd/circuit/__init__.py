@@ -4,6 +4,6 @@ reference's device protocol, so its MNA solver can take the GPU region in
 place of its own; the netlist and MNA machinery themselves are not part of
 this package."""
 
-from .coupling import FieldRegion, stamp_g, stamp_i
+from .coupling import FieldRegion, presb_fgmres, stamp_g, stamp_i
 
-__all__ = ["FieldRegion", "stamp_g", "stamp_i"]
+__all__ = ["FieldRegion", "stamp_g", "stamp_i", "presb_fgmres"]

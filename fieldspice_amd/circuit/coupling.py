@@ -11,6 +11,17 @@ capacitance path with ``K`` in place of ``L_eps``: all ``nE`` interior solves
 run as one multi-right-hand-side device solve and the electrode sums are
 integrated on the device.  The history term and the interior back-substitution
 of an accepted step are single device solves of the same system.
+
+The AC stamp needs ``K = L_sigma + j omega L_eps``, complex symmetric and not
+Hermitian, which CG does not solve.  Written as the real 2x2 block system
+``[[B, -C], [C, B]] [x; y] = [f; g]`` (``B = L_sigma``, ``C = omega L_eps`` on
+the interior), it is solved by flexible GMRES preconditioned with PRESB
+(Axelsson, Neytcheva and Ahmad, Numer. Algorithms 66 (2014) 811-841): the
+preconditioner ``[[B, -C], [C, B + 2C]]`` is applied exactly by two solves with
+``H = B + C``, a real SPD grid Laplacian that the device solves like any
+other, and bounds the preconditioned spectrum to ``[1/2, 1]`` for every
+``omega``, so the iteration count does not grow with the conductivity
+contrast or the frequency.
 """
 
 from __future__ import annotations
@@ -23,7 +34,7 @@ from .. import _lib
 from ..solvers.base import SolverConfig, Terminal
 from ..solvers.linear import GridLinearSystem
 
-__all__ = ["FieldRegion", "stamp_g", "stamp_i"]
+__all__ = ["FieldRegion", "stamp_g", "stamp_i", "presb_fgmres"]
 
 
 def stamp_g(G: Any, i: int, j: int, value: float) -> None:
@@ -43,6 +54,102 @@ def stamp_i(I: Any, i: int, value: float) -> None:
     if i < 0:
         return
     I[i] += value
+
+
+def presb_fgmres(apply_B, apply_C, solve_H, f, g, tol: float = 1e-10, restart: int = 30,
+                 max_restarts: int = 10):
+    """Solve ``(B + jC)(x + jy) = f + jg`` for real symmetric positive
+    semi-definite ``B``, ``C`` with ``H = B + C`` positive definite.
+
+    ``f``, ``g``: real ``(k, n)`` right-hand sides, solved together: every
+    callable takes and returns a ``(k, n)`` array (``solve_H`` one multi-RHS
+    solve).  Flexible GMRES on the real block form, right-preconditioned with
+    PRESB; the columns iterate in lockstep, each with its own Hessenberg
+    matrix, until every column's residual is below ``tol`` times its
+    right-hand side.  Returns ``(x, y, info)``; raises ``RuntimeError`` when
+    ``max_restarts`` cycles of ``restart`` iterations do not reach ``tol``.
+    """
+    f = np.asarray(f, dtype=np.float64)
+    g = np.asarray(g, dtype=np.float64)
+    k = f.shape[0]
+
+    def A(u):                               # u: (k, 2, n)
+        bx, by = apply_B(u[:, 0]), apply_B(u[:, 1])
+        cx, cy = apply_C(u[:, 0]), apply_C(u[:, 1])
+        return np.stack([bx - cy, cx + by], axis=1)
+
+    def P_inv(v):                           # PRESB: two solves with H
+        h = solve_H(v[:, 0] + v[:, 1])
+        y = solve_H(apply_B(h) - v[:, 0])
+        return np.stack([h - y, y], axis=1)
+
+    def norms(u):
+        return np.sqrt(np.einsum("kij,kij->k", u, u))
+
+    rhs = np.stack([f, g], axis=1)
+    bnorm = norms(rhs)
+    target = tol * bnorm
+    sol = np.zeros_like(rhs)
+    r = rhs.copy()
+    total = 0
+    for cycle in range(max_restarts + 1):
+        beta = norms(r)
+        if np.all(beta <= target):
+            return sol[:, 0], sol[:, 1], {"iterations": total, "restarts": cycle,
+                                          "residual": float(np.max(beta / np.maximum(bnorm,
+                                                                                    1e-300)))}
+        if cycle == max_restarts:
+            break
+        m = restart
+        V = [r / np.where(beta > 0, beta, 1.0)[:, None, None]]
+        Z = []
+        Hm = np.zeros((k, m + 1, m))
+        cs = np.zeros((k, m))
+        sn = np.zeros((k, m))
+        s = np.zeros((k, m + 1))
+        s[:, 0] = beta
+        j_end = 0
+        for j in range(m):
+            z = P_inv(V[j])
+            Z.append(z)
+            w = A(z)
+            for i in range(j + 1):          # modified Gram-Schmidt, per column
+                hij = np.einsum("kij,kij->k", w, V[i])
+                w -= hij[:, None, None] * V[i]
+                Hm[:, i, j] = hij
+            hn = norms(w)
+            Hm[:, j + 1, j] = hn
+            V.append(w / np.where(hn > 0, hn, 1.0)[:, None, None])
+            for i in range(j):              # earlier rotations on the new column
+                a, b = Hm[:, i, j].copy(), Hm[:, i + 1, j].copy()
+                Hm[:, i, j] = cs[:, i] * a + sn[:, i] * b
+                Hm[:, i + 1, j] = -sn[:, i] * a + cs[:, i] * b
+            a, b = Hm[:, j, j], Hm[:, j + 1, j]
+            rho = np.hypot(a, b)
+            ok = rho > 0
+            cs[:, j] = np.where(ok, a / np.where(ok, rho, 1.0), 1.0)
+            sn[:, j] = np.where(ok, b / np.where(ok, rho, 1.0), 0.0)
+            Hm[:, j, j] = rho
+            Hm[:, j + 1, j] = 0.0
+            s[:, j + 1] = -sn[:, j] * s[:, j]
+            s[:, j] = cs[:, j] * s[:, j]
+            j_end = j + 1
+            total += 1
+            if np.all(np.abs(s[:, j + 1]) <= 0.5 * target):
+                break
+        for c in range(k):                  # back substitution per column
+            if beta[c] <= target[c]:
+                continue
+            Rm = Hm[c, :j_end, :j_end]
+            d = np.abs(np.diag(Rm)) > 0
+            jj = int(np.argmin(d)) if not np.all(d) else j_end
+            yv = np.linalg.solve(Rm[:jj, :jj], s[c, :jj]) if jj else np.zeros(0)
+            for i in range(jj):
+                sol[c] += yv[i] * Z[i][c]
+        r = rhs - A(sol)
+    raise RuntimeError(
+        f"PRESB-preconditioned GMRES did not reach relative residual {tol:.1e} in "
+        f"{total} iterations (reached {float(np.max(norms(r) / np.maximum(bnorm, 1e-300))):.2e})")
 
 
 class FieldRegion:
@@ -87,6 +194,8 @@ class FieldRegion:
         self.L_eps = _lib.DeviceOperator.from_cells(grid, eps_cell)
         self.L_sigma = _lib.DeviceOperator.from_cells(grid, sigma_cell)
         self._cache: dict[float, dict[str, Any]] = {}
+        self._ac_cache: dict[float, np.ndarray] = {}
+        self.ac_info: dict[str, Any] = {}
         self.phi = np.zeros(n)
         self.asymmetry: float = 0.0
 
@@ -153,10 +262,74 @@ class FieldRegion:
                 stamp_g(G, idx[a], idx[b], Y[a, b])
             stamp_i(I, idx[a], f[a])
 
+    def ac_admittance(self, omega: float) -> np.ndarray:
+        """The complex ``nE x nE`` admittance ``K_tt - K_ti K_ii^-1 K_it`` of
+        ``K = L_sigma + j omega L_eps`` [S] (cached per ``omega``;
+        ``fieldspice/circuit/coupling.py:216-230``).
+
+        Column ``b`` is the unit excitation of electrode ``b``: the interior
+        potential ``x + jy`` solves ``(B + jC)(x + jy) = -(B + jC) e_b`` on the
+        interior (``presb_fgmres``, all columns together, every ``H`` solve one
+        multi-RHS device solve), and ``Y[a, b]`` is the electrode-``a`` sum of
+        ``K phi``.  ``Y(-omega) = conj(Y(omega))``; ``omega = 0`` is the DC
+        stamp's admittance."""
+        omega = float(omega)
+        if omega == 0.0:
+            return self._reduce(None)["Y"].astype(complex)
+        if omega < 0.0:
+            return np.conj(self.ac_admittance(-omega))
+        if omega in self._ac_cache:
+            return self._ac_cache[omega]
+        nE = len(self.terminals)
+        n = self.grid.n_nodes
+        interior = (self._tid < 0).astype(np.float64)
+        Ls, Le = self.L_sigma, self.L_eps
+        sig = not self._sigma_zero
+        H = Le.combine(omega, 1.0, Ls) if sig else Le.combine(omega)
+        ls = GridLinearSystem(H, self._tid >= 0, self.cfg, grid=self.grid)
+
+        def rows(op, X, scale=1.0):
+            return np.stack([scale * op.apply(x) for x in X]) * interior
+
+        def apply_B(X):
+            return rows(Ls, X) if sig else np.zeros_like(X)
+
+        def apply_C(X):
+            return rows(Le, X, omega)
+
+        def solve_H(X):
+            return ls.solve(np.ascontiguousarray(X * interior)).reshape(X.shape)
+
+        E = np.zeros((nE, n))                # unit excitations on the electrodes
+        for k, t in enumerate(self.terminals):
+            E[k, t.nodes] = 1.0
+        x = np.zeros((nE, n))
+        y = np.zeros((nE, n))
+        if interior.any():
+            x, y, info = presb_fgmres(apply_B, apply_C, solve_H, -apply_B(E), -apply_C(E),
+                                      tol=self.cfg.tol)
+            self.ac_info = dict(info, omega=omega)
+        phr = E + x
+        Y = np.zeros((nE, nE), dtype=complex)
+        for b in range(nE):
+            kr = -omega * Le.apply(y[b])         # K phi = (B + jC)(phr + j y)
+            ki = omega * Le.apply(phr[b])
+            if sig:
+                kr += Ls.apply(phr[b])
+                ki += Ls.apply(y[b])
+            for a, t in enumerate(self.terminals):
+                Y[a, b] = complex(kr[t.nodes].sum(), ki[t.nodes].sum())
+        self._ac_cache[omega] = Y
+        return Y
+
     def stamp_ac(self, Y_mat, J, x_op, omega: float, nmap: Mapping[str, int]) -> None:
-        raise NotImplementedError(
-            "the AC stamp needs complex K = L_sigma + j omega L_eps; the GPU path solves real "
-            "systems only")
+        """Frequency domain: ``K = L_sigma + j omega L_eps`` needs no history
+        (``fieldspice/circuit/coupling.py:216-237``)."""
+        Y = self.ac_admittance(omega)
+        idx = [nmap.get(n, -1) for n in self.nodes]
+        for a in range(len(idx)):
+            for b in range(len(idx)):
+                stamp_g(Y_mat, idx[a], idx[b], Y[a, b])
 
     def accept_timestep(self, x, x_prev, dt: float, nmap: Mapping[str, int]) -> None:
         """Back-substitute the interior field of an accepted step

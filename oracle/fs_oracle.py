@@ -36,6 +36,9 @@ How each part is pinned (see the tests named in brackets):
   reference's docs/CONTRACTS.md:455,480-481), a charge-neutral guess, Gauss's
   law over the junction, and a wrong derivative rejected by the line search
   [tests/test_oracle.py]
+* ``field_region_ac``: the uniform slab's (sigma + j omega eps) A/d, omega C
+  of an insulator (the capacitance extraction above), the DC conductance as
+  omega -> 0, complex symmetry and zero column sums  [tests/test_oracle.py]
 """
 
 from __future__ import annotations
@@ -620,3 +623,33 @@ def nonlinear_poisson(grid, eps_cell, rho, drho, fixed=None, vals=None, rho_node
         if lam * big <= newton_tol * v_scale:
             return phi, hist, lams
     raise OracleConvergenceError("Newton did not converge")
+
+
+# --------------------------------------------------------------------------
+# field region AC admittance (fieldspice/circuit/coupling.py:216-234)
+# --------------------------------------------------------------------------
+def field_region_ac(grid, eps_cell, sigma_cell, terminal_nodes, omega):
+    """``Y = K_tt - K_ti K_ii^-1 K_it`` of ``K = L_sigma + j omega L_eps``
+    with ``K_tt``/``K_ti`` the electrode sums of ``K`` and ``K_ii`` the
+    interior block, factorised by complex ``splu`` as the reference does."""
+    Ls = nodal_laplacian(grid, cell_to_edge(grid, sigma_cell, "parallel"))
+    Le = nodal_laplacian(grid, cell_to_edge(grid, eps_cell, "parallel"))
+    K = (Ls.astype(complex) + 1j * float(omega) * Le).tocsr()
+    n = grid.n_nodes
+    nt = len(terminal_nodes)
+    rows, cols = [], []
+    for k, t in enumerate(terminal_nodes):
+        rows.append(np.full(len(t), k))
+        cols.append(np.asarray(t, dtype=np.intp))
+    P = sp.csr_matrix((np.ones(sum(len(t) for t in terminal_nodes)),
+                       (np.concatenate(rows), np.concatenate(cols))), shape=(nt, n))
+    interior = np.setdiff1d(np.arange(n), np.concatenate(cols))
+    S = sp.csr_matrix((np.ones(interior.size), (np.arange(interior.size), interior)),
+                      shape=(interior.size, n))
+    K_tt = (P @ K @ P.T).toarray()
+    if interior.size == 0:
+        return K_tt
+    K_ti = (P @ K @ S.T).tocsr()
+    lu = spla.splu((S @ K @ S.T).tocsc())
+    W = lu.solve(np.asarray((K_ti.T).todense()))
+    return K_tt - K_ti @ W

@@ -94,5 +94,46 @@ def test_coupled_rc_transient_matches_analytic():
         errs.append(np.abs(vs[m] - (1.0 - np.exp(-(ts[m] - t0) / tau))).max())
     assert errs[-1] < 2.4e-2
     assert 3.0 < errs[0] / errs[1] < 5.0      # first order: 4x the steps, 4x smaller error
-    with pytest.raises(NotImplementedError):
-        fr.stamp_ac(np.zeros((1, 1), complex), np.zeros(1, complex), None, 1e9, nmap)
+
+
+def test_ac_stamp_of_a_lossy_slab_is_the_parallel_rc():
+    _case("field region: AC stamp of a uniform lossy slab = (sigma + j w eps) A/d")
+    d, W, H, er, sigma = 2e-6, 5e-6, 4e-6, 3.9, 1e-1
+    g, terms = _plates(d, W, H)
+    fr = FieldRegion("FR", g, np.full(g.shape_cells, er * eps0), np.full(g.shape_cells, sigma),
+                     terms, circuit_nodes=["in", "0"])
+    w = 2e9 * np.pi
+    y = (sigma + 1j * w * er * eps0) * W * H / d
+    Ym, J = np.zeros((1, 1), complex), np.zeros(1, complex)
+    fr.stamp_ac(Ym, J, None, w, {"in": 0, "0": -1})     # ground dropped
+    assert Ym[0, 0] == pytest.approx(y, rel=1e-9)
+    assert not np.any(J)
+    np.testing.assert_allclose(fr.ac_admittance(-w), np.conj(fr.ac_admittance(w)), rtol=0)
+    np.testing.assert_allclose(fr.ac_admittance(0.0), fr.terminal_admittance(None), rtol=0)
+    assert fr.ac_info["iterations"] <= 30
+
+
+@pytest.mark.parametrize("sig_scale", [1e-4, 1.0])
+def test_ac_admittance_of_an_inhomogeneous_region_vs_oracle(sig_scale):
+    _case(f"field region: AC admittance, 3 electrodes, random eps/sigma x{sig_scale} vs oracle")
+    import fs_oracle as O
+    g = RectilinearGrid(np.linspace(0, 3e-6, 25), np.linspace(0, 2e-6, 17),
+                        np.linspace(0, 1e-6, 9))
+    rng = np.random.default_rng(21)
+    eps = rng.uniform(1.0, 8.0, g.shape_cells) * eps0
+    sig = sig_scale * rng.uniform(1e3, 5e4, g.shape_cells)
+    sig[10:14, 4:12, :] = 0.0                    # an insulating block
+    nid = g.node_ids()
+    terms = [Terminal("a", nid[0].ravel()), Terminal("b", nid[-1, :8].ravel()),
+             Terminal("c", nid[-1, 10:].ravel())]
+    fr = FieldRegion("FR", g, eps, sig, terms)
+    w = 2e9 * np.pi
+    Y = fr.ac_admittance(w)
+    Yr = O.field_region_ac(g, eps, sig, [t.nodes for t in terms], w)
+    np.testing.assert_allclose(Y, Yr, rtol=1e-8, atol=1e-8 * np.abs(Yr).max())
+    assert fr.ac_info["iterations"] <= 40
+    # the MNA view: stamped into a 2-node system with "c" grounded
+    fr2 = FieldRegion("FR2", g, eps, sig, terms, circuit_nodes=["n1", "n2", "0"])
+    Ym = np.zeros((2, 2), complex)
+    fr2.stamp_ac(Ym, np.zeros(2, complex), None, w, {"n1": 0, "n2": 1, "0": -1})
+    np.testing.assert_allclose(Ym, Yr[:2, :2], rtol=1e-8, atol=1e-8 * np.abs(Yr).max())

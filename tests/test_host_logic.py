@@ -241,3 +241,43 @@ def test_field_region_host_checks_and_stamps():
     stamp_i(I, 1, 4.0)
     stamp_i(I, -1, 9.0)
     assert G[0, 1] == 2.0 and G.sum() == 2.0 and L[1, 1] == 3.0 and I.tolist() == [0.0, 4.0]
+
+
+def test_presb_fgmres_solves_complex_symmetric_systems():
+    """The AC stamp's Krylov solver on SciPy operators: the block system
+    ``(B + jC) u = f + jg`` against a dense complex solve, for a conductor
+    (B dominant), an insulator (B = 0) and a mixed case, several right-hand
+    sides in lockstep."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    from fieldspice_amd.circuit import presb_fgmres
+    g = RectilinearGrid(np.linspace(0, 1, 9), np.linspace(0, 1, 7), np.linspace(0, 1, 5))
+    rng = np.random.default_rng(5)
+    nid = g.node_ids()
+    fixed = np.concatenate([nid[0].ravel(), nid[-1].ravel()])
+    inner = np.setdiff1d(np.arange(g.n_nodes), fixed)
+    Le = O.nodal_laplacian(g, O.cell_to_edge(g, rng.uniform(1, 5, g.shape_cells)))
+    for sig_scale, w in ((1e3, 1.0), (0.0, 1.0), (1.0, 1.0), (1.0, 1e-6)):
+        sig = sig_scale * rng.uniform(0, 2, g.shape_cells) * (rng.random(g.shape_cells) < 0.6)
+        Ls = O.nodal_laplacian(g, O.cell_to_edge(g, sig))
+        B = Ls[inner][:, inner].tocsc()
+        Cm = (w * Le)[inner][:, inner].tocsc()
+        lu = spl.splu((B + Cm).tocsc())
+        f = rng.standard_normal((3, inner.size))
+        gg = rng.standard_normal((3, inner.size))
+        x, y, info = presb_fgmres(lambda X: (B @ X.T).T, lambda X: (Cm @ X.T).T,
+                                  lambda X: lu.solve(np.ascontiguousarray(X.T)).T, f, gg,
+                                  tol=1e-11)
+        K = (B + 1j * Cm).toarray()
+        u = np.linalg.solve(K, (f + 1j * gg).T).T
+        np.testing.assert_allclose(x + 1j * y, u, rtol=0, atol=1e-9 * np.abs(u).max())
+        assert info["iterations"] <= 30 and info["residual"] <= 1e-11
+    # a zero right-hand side column stays zero; the others still converge
+    f[1] = 0.0
+    gg[1] = 0.0
+    x, y, _ = presb_fgmres(lambda X: (B @ X.T).T, lambda X: (Cm @ X.T).T,
+                           lambda X: lu.solve(np.ascontiguousarray(X.T)).T, f, gg)
+    assert not np.any(x[1]) and not np.any(y[1])
+    with pytest.raises(RuntimeError, match="did not reach"):
+        presb_fgmres(lambda X: (B @ X.T).T, lambda X: (Cm @ X.T).T, lambda X: X, f, gg,
+                     restart=2, max_restarts=1)

@@ -398,3 +398,41 @@ def test_oracle_newton_pn_junction_built_in_potential():
     with pytest.raises(O.OracleConvergenceError):
         O.nonlinear_poisson(g, eps, rho, lambda p: -drho(p), fixed=lo, vals=guess[lo],
                             phi0=guess, max_newton=30)
+
+
+def test_field_region_ac_admittance_pins():
+    """``field_region_ac``: a dropped Schur term fails the slab, a wrong sign
+    of ``j omega`` or a transposed coupling fails the insulator/conductor
+    limits and the symmetry."""
+    d, W, H = 2e-6, 5e-6, 4e-6
+    g = RectilinearGrid(np.linspace(0, d, 9), np.linspace(0, W, 5), np.linspace(0, H, 4))
+    nid = g.node_ids()
+    ends = [nid[0].ravel(), nid[-1].ravel()]
+    sigma, er, w = 1e4, 3.9, 2e9 * np.pi
+    Y = O.field_region_ac(g, np.full(g.shape_cells, er * eps0), np.full(g.shape_cells, sigma),
+                          ends, w)
+    y = (sigma + 1j * w * er * eps0) * W * H / d
+    np.testing.assert_allclose(Y, y * np.array([[1, -1], [-1, 1]]), rtol=1e-12)
+    # an inhomogeneous 3-electrode region
+    g = RectilinearGrid(np.linspace(0, 3e-6, 10), np.linspace(0, 2e-6, 8),
+                        np.linspace(0, 1e-6, 5))
+    rng = np.random.default_rng(3)
+    eps = rng.uniform(1.0, 8.0, g.shape_cells) * eps0
+    nid = g.node_ids()
+    terms = [nid[0].ravel(), nid[-1, :4].ravel(), nid[-1, 5:].ravel()]
+    # insulator: Y = j omega C of the capacitance extraction
+    C, _ = O.capacitance_matrix(g, eps, terms, method="direct")
+    Yi = O.field_region_ac(g, eps, np.zeros(g.shape_cells), terms, w)
+    np.testing.assert_allclose(Yi, 1j * w * C, rtol=1e-10, atol=1e-10 * np.abs(w * C).max())
+    # conductor at a negligible frequency: Y = the DC conductance matrix
+    sig = rng.uniform(1e3, 5e4, g.shape_cells)
+    G, _ = O.capacitance_matrix(g, sig, terms, method="direct")
+    Y0 = O.field_region_ac(g, eps, sig, terms, 1e-3)
+    np.testing.assert_allclose(Y0.real, G, rtol=1e-9, atol=1e-9 * np.abs(G).max())
+    # both terms: complex symmetric, zero column sums (the charge is
+    # conserved), and passive (Re Y positive semi-definite)
+    Yb = O.field_region_ac(g, eps, sig * 1e-4, terms, w)
+    assert np.abs(Yb.imag).max() > 0.1 * np.abs(Yb.real).max() > 0
+    np.testing.assert_allclose(Yb, Yb.T, rtol=1e-10, atol=1e-12 * np.abs(Yb).max())
+    assert np.abs(Yb.sum(axis=0)).max() < 1e-10 * np.abs(Yb).max()
+    assert np.linalg.eigvalsh(0.5 * (Yb.real + Yb.real.T)).min() > -1e-10 * np.abs(Yb).max()
