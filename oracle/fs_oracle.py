@@ -39,6 +39,10 @@ How each part is pinned (see the tests named in brackets):
 * ``field_region_ac``: the uniform slab's (sigma + j omega eps) A/d, omega C
   of an insulator (the capacitance extraction above), the DC conductance as
   omega -> 0, complex symmetry and zero column sums  [tests/test_oracle.py]
+* ``poisson_floating``: a floating plate between driven plates at the
+  series-capacitor potential (Q + C1 V)/(C1 + C2), with and without charge;
+  the electrode's charge equals the prescribed one; a zero-charge electrode
+  in a uniform field leaves the field unchanged  [tests/test_oracle.py]
 """
 
 from __future__ import annotations
@@ -653,3 +657,31 @@ def field_region_ac(grid, eps_cell, sigma_cell, terminal_nodes, omega):
     lu = spla.splu((S @ K @ S.T).tocsc())
     W = lu.solve(np.asarray((K_ti.T).todense()))
     return K_tt - K_ti @ W
+
+
+# --------------------------------------------------------------------------
+# Poisson with floating / charge-driven electrodes
+# (fieldspice/solvers/poisson.py:1039-1063, node merge :663-720)
+# --------------------------------------------------------------------------
+def poisson_floating(grid, eps_cell, fixed, vals, floating, charges, b_free=None):
+    """Each node set in ``floating`` merged into one unknown (``phi = P x``,
+    ``P^T L P x = P^T b``), its prescribed charge added to the merged row,
+    the Dirichlet nodes ``fixed`` eliminated, solved directly.  Returns the
+    node potential."""
+    L = nodal_laplacian(grid, cell_to_edge(grid, eps_cell, "parallel"))
+    n = grid.n_nodes
+    label = np.arange(n)
+    for g in floating:
+        g = np.asarray(g, dtype=np.intp)
+        label[g] = g[0]
+    _, group = np.unique(label, return_inverse=True)
+    nred = int(group.max()) + 1
+    P = sp.csr_matrix((np.ones(n), (np.arange(n), group)), shape=(n, nred))
+    b = np.zeros(n) if b_free is None else np.asarray(b_free, dtype=float).copy()
+    for g, q in zip(floating, charges):
+        b[np.asarray(g, dtype=np.intp)[0]] += float(q)
+    A_r = sp.csr_matrix(P.T @ L @ P)
+    b_r = P.T @ b
+    A_bc, b_bc = apply_dirichlet(A_r, b_r, group[np.asarray(fixed, dtype=np.intp)],
+                                 np.asarray(vals, dtype=float))
+    return P @ spla.spsolve(A_bc.tocsc(), b_bc)

@@ -436,3 +436,30 @@ def test_field_region_ac_admittance_pins():
     np.testing.assert_allclose(Yb, Yb.T, rtol=1e-10, atol=1e-12 * np.abs(Yb).max())
     assert np.abs(Yb.sum(axis=0)).max() < 1e-10 * np.abs(Yb).max()
     assert np.linalg.eigvalsh(0.5 * (Yb.real + Yb.real.T)).min() > -1e-10 * np.abs(Yb).max()
+
+
+def test_poisson_floating_electrode_pins():
+    """``poisson_floating`` (the reference's node merge): a full-plane plate
+    between driven plates sits at the series-capacitor potential
+    ``(Q + C1 V)/(C1 + C2)`` and carries exactly the prescribed charge; with
+    no charge it leaves the linear field unchanged."""
+    d, W, H, er = 3e-6, 2e-6, 1e-6, 3.0
+    xs = np.linspace(0, d, 13)
+    g = RectilinearGrid(xs, np.linspace(0, W, 5), np.linspace(0, H, 4))
+    nid = g.node_ids()
+    eps = np.full(g.shape_cells, er * eps0)
+    fixed = np.concatenate([nid[0].ravel(), nid[-1].ravel()])
+    vals = np.concatenate([np.ones(nid[0].size), np.zeros(nid[-1].size)])
+    ia = 4
+    plate = nid[ia].ravel()
+    C1 = er * eps0 * W * H / xs[ia]
+    C2 = er * eps0 * W * H / (d - xs[ia])
+    L = O.nodal_laplacian(g, O.cell_to_edge(g, eps))
+    for Q in (0.0, 2.5e-17, -4e-17):
+        phi = O.poisson_floating(g, eps, fixed, vals, [plate], [Q])
+        vf = (Q + C1) / (C1 + C2)
+        np.testing.assert_allclose(phi[plate], vf, rtol=1e-12)
+        assert float((L @ phi)[plate].sum()) == pytest.approx(Q, abs=1e-12 * C1)
+        if Q == 0.0:
+            xn = np.broadcast_to(xs[:, None, None], g.shape_nodes).ravel()
+            np.testing.assert_allclose(phi, 1.0 - xn / d, atol=1e-12)
