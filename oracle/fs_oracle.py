@@ -34,8 +34,9 @@ How each part is pinned (see the tests named in brackets):
 * ``boltzmann_charge``/``nonlinear_poisson``: the built-in potential of an
   abrupt pn junction, Vt ln(Na Nd/ni^2) = 0.833370 V, to 1e-9 (the
   reference's docs/CONTRACTS.md:455,480-481), a charge-neutral guess, Gauss's
-  law over the junction, and a wrong derivative rejected by the line search
-  [tests/test_oracle.py]
+  law over the junction, and a wrong derivative rejected by the line search;
+  with merged electrodes, equal to ``poisson_floating`` without space charge
+  and the prescribed electrode charge met exactly  [tests/test_oracle.py]
 * ``field_region_ac``: the uniform slab's (sigma + j omega eps) A/d, omega C
   of an insulator (the capacitance extraction above), the DC conductance as
   omega -> 0, complex symmetry and zero column sums  [tests/test_oracle.py]
@@ -577,25 +578,36 @@ def boltzmann_charge(doping_node, ni, T=300.0):
 
 def nonlinear_poisson(grid, eps_cell, rho, drho, fixed=None, vals=None, rho_node=None,
                       phi0=None, v_scale=None, step_limit=5.0, armijo=1e-4, newton_tol=1e-8,
-                      max_newton=60, damping=1.0):
+                      max_newton=60, damping=1.0, floating=(), charges=()):
     """Damped Newton for L phi - rho(phi) V_node = rho_node V_node with
     Dirichlet rows ``fixed`` at ``vals`` (fieldspice/solvers/poisson.py:1429-1511):
     the Jacobian L - diag(drho V_node) is eliminated and solved by SuperLU each
     step, the first trial step is clamped to step_limit*v_scale, then Armijo
-    backtracking.  Returns (phi, history of |F|, damping factors)."""
+    backtracking.  Node sets in ``floating`` are merged into one unknown each,
+    carrying ``charges`` (the reference's reduction, :1432-1451).  Returns
+    (phi, history of |F|, damping factors)."""
     L = nodal_laplacian(grid, cell_to_edge(grid, eps_cell))
     vnode = np.asarray(grid.node_volumes(), dtype=float).ravel()
     n = grid.n_nodes
     fixed = np.empty(0, dtype=np.intp) if fixed is None else np.asarray(fixed, dtype=np.intp)
     vals = np.empty(0) if vals is None else np.asarray(vals, dtype=float)
     b_ext = np.zeros(n) if rho_node is None else np.asarray(rho_node, float).ravel() * vnode
+    b_ext = b_ext.copy()
+    label = np.arange(n)
+    for g, q in zip(floating, charges):
+        g = np.asarray(g, dtype=np.intp)
+        label[g] = g[0]
+        b_ext[g[0]] += float(q)
+    _, group = np.unique(label, return_inverse=True)
+    P = sp.csr_matrix((np.ones(n), (np.arange(n), group)), shape=(n, int(group.max()) + 1))
+    fixed_r = group[fixed]
     v_scale = KB * 300.0 / Q_ELEM if v_scale is None else v_scale
     phi = np.zeros(n) if phi0 is None else np.asarray(phi0, dtype=float).ravel().copy()
     phi[fixed] = vals
 
     def residual(p):
-        f = L @ p - rho(p) * vnode - b_ext
-        f[fixed] = 0.0
+        f = P.T @ (L @ p - rho(p) * vnode - b_ext)
+        f[fixed_r] = 0.0
         return f
 
     F = residual(phi)
@@ -604,9 +616,9 @@ def nonlinear_poisson(grid, eps_cell, rho, drho, fixed=None, vals=None, rho_node
     for it in range(max_newton):
         if hist[-1] <= 1e-14 * max(hist[0], 1e-300):
             return phi, hist, lams
-        J = (L - sp.diags(drho(phi) * vnode)).tocsr()
-        A, b = apply_dirichlet(J, -F, fixed, np.zeros(fixed.size))
-        dphi = spla.splu(A.tocsc()).solve(b)
+        J = sp.csr_matrix(P.T @ (L - sp.diags(drho(phi) * vnode)) @ P)
+        A, b = apply_dirichlet(J, -F, fixed_r, np.zeros(fixed_r.size))
+        dphi = P @ spla.splu(A.tocsc()).solve(b)
         big = float(np.abs(dphi).max())
         lam = float(damping)
         if big > 0.0:

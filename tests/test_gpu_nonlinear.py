@@ -87,3 +87,36 @@ def test_wrong_derivative_raises_convergence_error():
                                lambda p: -ec.drho_dphi(p), SolverConfig(max_newton=30))
     with pytest.raises(ConvergenceError):
         s.solve([Terminal("p", lo, voltage=float(ec.phi_guess[lo][0]))], phi0=ec.phi_guess)
+
+
+@pytest.mark.parametrize("Qw", [0.0, 2e-18])
+def test_3d_well_with_floating_contact_matches_oracle_merge(Qw):
+    _case(f"nonlinear Poisson: 3D n-well, floating well contact Q={Qw}, vs oracle merge")
+    g = RectilinearGrid(np.linspace(0, 0.6e-6, 31), np.linspace(0, 0.4e-6, 21),
+                        np.linspace(0, 0.3e-6, 16))
+    xc = np.broadcast_to(g.xn[:, None, None], g.shape_nodes)
+    yc = np.broadcast_to(g.yn[None, :, None], g.shape_nodes)
+    zc = np.broadcast_to(g.zn[None, None, :], g.shape_nodes)
+    well = (xc > 0.2e-6) & (xc < 0.45e-6) & (yc > 0.1e-6) & (yc < 0.3e-6) & (zc > 0.15e-6)
+    dop = np.where(well, 5e22, -2e22).ravel()
+    ec = boltzmann_equilibrium_charge(dop, 1e16)
+    nid = g.node_ids()
+    sub = nid[:, :, 0].ravel()
+    wc = nid[15:19, 8:13, -1].ravel()
+    guess = ec.phi_guess.copy()
+    guess[wc] = guess[wc].mean()                 # the contact starts as one equipotential
+    eps = np.full(g.shape_cells, 11.7 * eps0)
+    s = NonlinearPoissonSolver(g, eps, ec.rho, ec.drho_dphi, SolverConfig(tol=1e-12))
+    r = s.solve([Terminal("sub", sub, voltage=float(guess[sub][0])),
+                 Terminal("well", wc, current=(Qw or None))], phi0=guess)
+    phi = r.fields["phi"][0].ravel()
+    rho, drho, _, _ = O.boltzmann_charge(dop, 1e16)
+    po, _, _ = O.nonlinear_poisson(g, eps, rho, drho, fixed=sub, vals=guess[sub], phi0=guess,
+                                   floating=[wc], charges=[Qw])
+    assert np.abs(phi - po).max() <= 1e-8 * np.abs(po).max()
+    assert np.ptp(phi[wc]) < 1e-12
+    Lo = O.nodal_laplacian(g, O.cell_to_edge(g, eps))
+    qo = Lo @ po - rho(po) * g.node_volumes().ravel()
+    scale = np.abs(qo[wc]).sum()
+    assert r.terminals["well"]["q"][0] == pytest.approx(Qw, abs=1e-6 * scale)
+    assert r.meta["n_unknowns"] == g.n_nodes - wc.size + 1

@@ -463,3 +463,33 @@ def test_poisson_floating_electrode_pins():
         if Q == 0.0:
             xn = np.broadcast_to(xs[:, None, None], g.shape_nodes).ravel()
             np.testing.assert_allclose(phi, 1.0 - xn / d, atol=1e-12)
+
+
+def test_nonlinear_poisson_floating_merge_pins():
+    """The Newton oracle's merged electrodes: with no space charge it is the
+    linear node merge (``poisson_floating``); with Boltzmann charge the
+    floating electrode still carries exactly its prescribed charge and is
+    one equipotential."""
+    g = RectilinearGrid(np.linspace(0, 0.4e-6, 17), np.linspace(0, 0.3e-6, 13),
+                        np.linspace(0, 0.2e-6, 9))
+    nid = g.node_ids()
+    eps = np.full(g.shape_cells, 11.7 * eps0)
+    drv = nid[0].ravel()
+    flt = nid[8:11, 4:8, 3:6].ravel()
+    fixed, vals = drv, np.full(drv.size, 0.2)
+    zero = lambda p: np.zeros_like(p)
+    for Q in (0.0, 1e-18):
+        po, _, _ = O.nonlinear_poisson(g, eps, zero, zero, fixed=fixed, vals=vals,
+                                       floating=[flt], charges=[Q])
+        pl = O.poisson_floating(g, eps, fixed, vals, [flt], [Q])
+        np.testing.assert_allclose(po, pl, rtol=0, atol=1e-12)
+    dop = np.where(np.arange(g.n_nodes) % 2 == 0, 1e22, -1e22).astype(float)
+    rho, drho, guess, _ = O.boltzmann_charge(dop, 1e16)
+    Qf = 3e-19
+    po, hist, _ = O.nonlinear_poisson(g, eps, rho, drho, fixed=fixed, vals=guess[fixed],
+                                      phi0=guess, floating=[flt], charges=[Qf])
+    L = O.nodal_laplacian(g, O.cell_to_edge(g, eps))
+    q = L @ po - rho(po) * g.node_volumes().ravel()
+    assert float(q[flt].sum()) == pytest.approx(Qf, rel=1e-6)
+    assert np.ptp(po[flt] - guess[flt]) < 1e-12 * max(1.0, np.abs(po).max())
+    assert hist[-1] < 1e-8 * hist[0]
