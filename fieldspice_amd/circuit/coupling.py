@@ -195,6 +195,7 @@ class FieldRegion:
         self.L_sigma = _lib.DeviceOperator.from_cells(grid, sigma_cell)
         self._cache: dict[float, dict[str, Any]] = {}
         self._ac_cache: dict[float, np.ndarray] = {}
+        self.last_iterations = 0
         self.ac_info: dict[str, Any] = {}
         self.phi = np.zeros(n)
         self.asymmetry: float = 0.0
@@ -223,15 +224,16 @@ class FieldRegion:
         self._cache[key] = entry
         return entry
 
-    def _interior_solve(self, entry, f, v) -> np.ndarray:
-        """``phi`` with the electrodes at ``v`` and ``K_ii phi_i = f_i - K_it v``."""
+    def _interior_solve(self, entry, f, v, guess=None) -> np.ndarray:
+        """``phi`` with the electrodes at ``v`` and ``K_ii phi_i = f_i - K_it v``;
+        ``guess`` (a full potential) starts the iteration."""
         K, ls = entry["K"], entry["ls"]
         x0 = np.zeros(self.grid.n_nodes)
         vals = np.concatenate([np.full(t.nodes.size, v[k]) for k, t in enumerate(self.terminals)])
         x0[self._flat] = vals
         b = f - K.apply(x0)
         b[self._flat] = vals
-        return ls.solve(b)
+        return ls.solve(b, x0=guess)
 
     def _history(self, entry, dt) -> np.ndarray:
         """``f_eff`` [A]: the history current source of the previous step."""
@@ -338,7 +340,9 @@ class FieldRegion:
         v = np.array([0.0 if nmap.get(n, -1) < 0 else float(x[nmap[n]]) for n in self.nodes])
         if entry["ls"] is not None and dt is not None:
             f = self.L_eps.apply(self.phi) / dt
-            self.phi = self._interior_solve(entry, f, v)
+            # the previous field starts the solve: a step changes it little
+            self.phi = self._interior_solve(entry, f, v, guess=self.phi)
+            self.last_iterations = int(entry["ls"].info.get("iterations", 0))
         else:
             phi = np.zeros(self.grid.n_nodes)
             phi[self._flat] = np.concatenate(

@@ -78,7 +78,7 @@ def test_coupled_rc_transient_matches_analytic():
         nmap = {"out": 0, "0": -1}
         dt = tau / spt
         x = np.zeros(1)
-        ts, vs = [], []
+        ts, vs, its = [], [], []
         for k in range(1, 10 * spt + 1):
             t = k * dt
             vin = 1.0 if t >= t0 else 0.0
@@ -86,6 +86,7 @@ def test_coupled_rc_transient_matches_analytic():
             fr.stamp_tran(G, I, x, x, dt, nmap)
             x_new = I / G[0]
             fr.accept_timestep(x_new, x, dt, nmap)
+            its.append(fr.last_iterations)
             x = x_new
             ts.append(t)
             vs.append(float(x[0]))
@@ -94,6 +95,19 @@ def test_coupled_rc_transient_matches_analytic():
         errs.append(np.abs(vs[m] - (1.0 - np.exp(-(ts[m] - t0) / tau))).max())
     assert errs[-1] < 2.4e-2
     assert 3.0 < errs[0] / errs[1] < 5.0      # first order: 4x the steps, 4x smaller error
+    # a settled step's back-substitution started from the previous field
+    # needs fewer iterations than the same solve started from zero
+    entry = fr._reduce(dt)
+    f = fr.L_eps.apply(fr.phi) / dt
+    v = np.array([float(x[0]), 0.0])            # electrode voltages: "out", ground
+    cold = fr._interior_solve(entry, f, v)
+    n_cold = entry["ls"].info["iterations"]
+    warm = fr._interior_solve(entry, f, v, guess=fr.phi)
+    n_warm = entry["ls"].info["iterations"]
+    print(f"[gpu case] back-substitution iterations: cold {n_cold}, warm {n_warm}, "
+          f"in the run {its[:2]}...{its[-2:]}", flush=True)
+    assert n_warm < n_cold
+    np.testing.assert_allclose(warm, cold, rtol=0, atol=1e-9 * np.abs(cold).max())
 
 
 def test_ac_stamp_of_a_lossy_slab_is_the_parallel_rc():
