@@ -218,7 +218,7 @@ class FieldRegion:
         K = self.L_sigma if dt is None else self.L_eps.combine(1.0 / dt, 1.0, self.L_sigma)
         ls = GridLinearSystem(K, self._tid >= 0, self.cfg, grid=self.grid)
         Y, _, _, st, rc = ls.sys.capacitance(K, self._tid, nE, ls.params())
-        ls._judge(st, rc)
+        ls._verdict(st, rc)
         self.asymmetry = float(np.abs(Y - Y.T).max() / max(np.abs(Y).max(), 1e-300))
         entry = {"K": K, "Y": 0.5 * (Y + Y.T), "ls": ls, "dt": dt}
         self._cache[key] = entry

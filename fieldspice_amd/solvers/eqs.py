@@ -175,7 +175,7 @@ class EQSSolver:
             nprobe=len(terms), cur_a=cur_a, cur_b=self.L_sigma, quad_a=self.L_eps,
             quad_b=self.L_sigma, store_every=max(1, store_every) if store else 0,
             initial_guess=initial_guess, x_before=phi_prev, params=ls.params())
-        ls._judge(out["stats"], out["rc"])
+        ls._verdict(out["stats"], out["rc"])
         res = Result(grid=g, t=times)
         if store:
             res.fields["phi"] = out["stored"]

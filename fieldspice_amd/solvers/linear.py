@@ -118,7 +118,7 @@ class GridLinearSystem:
         return _lib.DeviceSystem.params(self.cfg.tol, self.cfg.maxiter, self.precision,
                                         self.cfg.inner_tol or 0.0, self.cfg.check_every)
 
-    def _judge(self, st: dict, rc: int, what: str = "cg"):
+    def _verdict(self, st: dict, rc: int, what: str = "cg"):
         """Convergence verdict of ``fieldspice/solvers/poisson.py:554-587``,
         applied to both residual norms."""
         self.info.update(iterations=st["iterations"], residual=st["residual"],
@@ -157,5 +157,5 @@ class GridLinearSystem:
             if not np.all(np.isfinite(x0)):
                 raise ValueError("x0 must be finite")
         x, st, rc = self.sys.solve(b, self.params(), x0=x0)
-        self._judge(st, rc)
+        self._verdict(st, rc)
         return x
