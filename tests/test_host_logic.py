@@ -281,3 +281,19 @@ def test_presb_fgmres_solves_complex_symmetric_systems():
     with pytest.raises(RuntimeError, match="did not reach"):
         presb_fgmres(lambda X: (B @ X.T).T, lambda X: (Cm @ X.T).T, lambda X: X, f, gg,
                      restart=2, max_restarts=1)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """``bench.py --gpus N`` with N > 1 exits with status 2 and says why, for
+    every workload, before touching a device."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for wl in ("c4", "c3", "c5"):
+        p = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2",
+                            "--workload", wl], capture_output=True, text=True, timeout=120)
+        assert p.returncode == 2, (wl, p.returncode, p.stderr[-500:])
+        msg = json.loads(p.stdout.strip().splitlines()[-1])
+        assert "one GPU" in msg["error"]
