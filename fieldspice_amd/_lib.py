@@ -53,7 +53,9 @@ class SolveStats(C.Structure):
                 ("residual", C.c_double), ("residual_eq", C.c_double),
                 ("solve_time", C.c_double), ("stencil_time", C.c_double),
                 ("stencil_launches", C.c_int64), ("stencil_bytes", C.c_double),
-                ("fallbacks", C.c_int), ("pc_applies", C.c_int)]
+                ("fallbacks", C.c_int), ("pc_applies", C.c_int), ("stop_reason", C.c_int),
+                ("maxiter", C.c_int), ("replacements", C.c_int), ("stencil_width", C.c_int),
+                ("cycle_width", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

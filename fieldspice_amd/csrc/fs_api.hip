@@ -213,15 +213,54 @@ int default_maxiter(const fs_sys* s, const fs_solve_params* P) {
 // (mode 2).  Host checks the device flags between chunks of iterations.
 static int run_krylov_loop(fs_sys* s, int nr, const fs_solve_params& P, int mode);
 
+// Residual replacements FP64 CG may make per solve.  A restart that does not
+// halve the true residual means it sits at the rounding floor of the
+// problem (a solution dominated by a near-null mode); the solve then stops
+// and the caller sees the true residual.
+constexpr int kMaxReplace = 4;
+
+// End of an FP64 CG run: add the pending x += alpha p (deferred into the next
+// direction update, which a finished right-hand side never runs), recompute
+// b_s - A_s x in FP64, and restart from that residual every right-hand side
+// whose recursive residual met tol while the true one did not (the
+// recursion drifts from b_s - A_s x by rounding).
+int fp64_finish(fs_sys* s, int nr, const fs_solve_params& P) {
+  fs::launch_flush_x(s, nr);
+  FS_HIP(hipGetLastError());
+  const size_t scb = sizeof(fs::CgScalars) * (size_t)nr;
+  std::vector<double> prev(nr, INFINITY);
+  for (int rep = 0; rep <= kMaxReplace; ++rep) {
+    fs::launch_residual(s, nr, s->r);
+    FS_HIP(hipGetLastError());
+    FS_HIP(hipMemcpyAsync(s->sc_host, s->sc, scb, hipMemcpyDeviceToHost, s->stream));
+    FS_HIP(hipStreamSynchronize(s->stream));
+    unsigned restart = 0;
+    for (int r = 0; r < nr; ++r) {
+      const fs::CgScalars& c = s->sc_host[r];
+      // (thresh2 == 0: a zero right-hand side, already exact)
+      if (rep < kMaxReplace && c.done == 1 && std::isfinite(c.aux3) && c.thresh2 > 0.0 &&
+          c.aux3 >= c.thresh2 && c.aux3 < 0.25 * prev[r] && c.iter < c.maxiter)
+        restart |= 1u << r;
+      prev[r] = c.aux3;
+    }
+    fs::launch_replace(s, nr, restart);
+    FS_HIP(hipGetLastError());
+    if (!restart) return FS_OK;
+    s->acc_replacements += __builtin_popcount(restart);
+    if (int rc = run_krylov_loop(s, nr, P, 0)) return rc;
+    fs::launch_flush_x(s, nr);
+    FS_HIP(hipGetLastError());
+  }
+  return FS_OK;
+}
+
 int run_krylov_mode(fs_sys* s, int nr, const fs_solve_params& P, int mode) {
   const int rc = run_krylov_loop(s, nr, P, mode);
-  if (mode == 0) {
-    // FP64 CG leaves the last x += alpha p pending (deferred into the next
-    // direction update, which a finished right-hand side never runs)
-    fs::launch_flush_x(s, nr);
-    if (rc == FS_OK) FS_HIP(hipGetLastError());
+  if (mode != 0 || rc != FS_OK) {
+    if (mode == 0) fs::launch_flush_x(s, nr);
+    return rc;
   }
-  return rc;
+  return fp64_finish(s, nr, P);
 }
 
 static int run_krylov_loop(fs_sys* s, int nr, const fs_solve_params& P, int mode) {
@@ -345,18 +384,29 @@ int run_krylov(fs_sys* s, int nr, const fs_solve_params& P) {
   return run_krylov_mode(s, nr, Q, 0);
 }
 
+// which kernels a solve in `precision` runs on this system (fs_solve_stats)
+void kernel_widths(const fs_sys* s, int precision, fs_solve_stats* st) {
+  if (!st) return;
+  st->stencil_width = precision == FS_PREC_FP64 ? (s->use_x2 ? 2 : 1) : (s->use_x4 ? 4 : 1);
+  st->cycle_width = precision == FS_PREC_FCG && s->pc ? fs::pc_level0_width(s->pc) : 0;
+}
+
 void collect_stats(fs_sys* s, int nr, fs_solve_stats* st, int& status) {
   if (st) {
     st->fallbacks += s->acc_fallbacks;
     st->pc_applies += s->acc_pc_applies;
+    st->replacements += s->acc_replacements;
   }
   s->acc_fallbacks = 0;
   s->acc_pc_applies = 0;
+  s->acc_replacements = 0;
   for (int r = 0; r < nr; ++r) {
     const fs::CgScalars& c = s->sc_host[r];
     if (st) {
       st->iterations = std::max(st->iterations, c.iter);
       st->refreshes = std::max(st->refreshes, c.refreshes);
+      st->stop_reason = std::max(st->stop_reason, c.done);
+      st->maxiter = std::max(st->maxiter, c.maxiter);
       double res = c.aux2 > 0 ? std::sqrt(c.aux1 / c.aux2) : 0.0;
       double req = c.bnorm2 > 0 ? std::sqrt(c.aux3 / c.bnorm2) : 0.0;
       // a non-finite residual (NaN input, breakdown) must not read as converged
@@ -784,6 +834,7 @@ int fs_sys_solve(fs_sys* s, int nrhs, const double* b, const double* x0, double*
   int status = FS_OK;
   float ms_total = 0.0f;
   if (stats) stats->converged = 1;
+  kernel_widths(s, params->precision, stats);
   for (int r0 = 0; r0 < nrhs; r0 += fs::kMaxRhs) {
     const int nr = std::min(fs::kMaxRhs, nrhs - r0);
     if (int rc = ensure_ws(s, nr)) return rc;
@@ -944,6 +995,7 @@ int fs_cap_solve(fs_cap* cap, const fs_op* L, double* C, double* q_shield, doubl
     std::memset(stats, 0, sizeof(*stats));
     stats->converged = 1;
   }
+  kernel_widths(s, params->precision, stats);
   const int maxiter = default_maxiter(s, params);
   const bool mixed = params->precision != FS_PREC_FP64;
   int status = FS_OK;
@@ -1152,10 +1204,13 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
     std::memset(stats, 0, sizeof(*stats));
     stats->converged = 1;
   }
+  kernel_widths(s, params->precision, stats);
   const int maxiter = default_maxiter(s, params);
   const int store_every = D->store_every;
   s->acc_fallbacks = 0;
   s->acc_pc_applies = 0;
+  s->acc_replacements = 0;
+  int worst_done = 1;
   int64_t nstored = 0;
   auto store = [&](const double* dev) -> int {
     if (!stored) return FS_OK;
@@ -1207,6 +1262,7 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
     worst_res = std::max(worst_res, res);
     worst_req = std::max(worst_req, req);
     if (stats) stats->refreshes += c.refreshes;
+    worst_done = std::max(worst_done, c.done);
     if (c.done != 1) {
       status = FS_NOT_CONVERGED;
       if (stats) stats->converged = 0;
@@ -1264,9 +1320,13 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
     stats->solve_time = ms * 1e-3;
     stats->fallbacks = s->acc_fallbacks;
     stats->pc_applies = s->acc_pc_applies;
+    stats->replacements = s->acc_replacements;
+    stats->stop_reason = worst_done;
+    stats->maxiter = maxiter;
   }
   s->acc_fallbacks = 0;
   s->acc_pc_applies = 0;
+  s->acc_replacements = 0;
   if (status == FS_NOT_CONVERGED)
     fail(FS_NOT_CONVERGED, "a time step did not reach tol %.3g within %d iterations", params->tol,
          maxiter);
@@ -1523,9 +1583,15 @@ int fs_batch_solve(fs_batch* b, const fs_solve_params* params, int chunk, double
     b->sc_cap = ni;
   }
   FS_HIP(hipMemset(b->sc_all, 0, sizeof(fs::CgScalars) * ni * fs::kMaxRhs));
-  // lend every system the batch's scalars and the main stream
+  // workspaces first (an allocation failure returns before any system is
+  // changed), then lend every system the batch's scalars and the main stream
   std::vector<fs::CgScalars*> own_sc(ni);
   std::vector<int> maxit(ni);
+  for (int i = 0; i < ni; ++i) {
+    fs_sys* s = b->inst[i].cap->sys;
+    if (int rc = ensure_ws(s, b->inst[i].cap->nt)) return rc;
+    maxit[i] = default_maxiter(s, params);
+  }
   for (int i = 0; i < ni; ++i) {
     fs_sys* s = b->inst[i].cap->sys;
     own_sc[i] = s->sc;
@@ -1533,8 +1599,7 @@ int fs_batch_solve(fs_batch* b, const fs_solve_params* params, int chunk, double
     s->stream = main;
     s->acc_fallbacks = 0;
     s->acc_pc_applies = 0;
-    if (int rc = ensure_ws(s, b->inst[i].cap->nt)) return rc;
-    maxit[i] = default_maxiter(s, params);
+    s->acc_replacements = 0;
   }
   auto restore = [&]() {
     for (int i = 0; i < ni; ++i) {
@@ -1649,6 +1714,9 @@ int fs_batch_solve(fs_batch* b, const fs_solve_params* params, int chunk, double
           if (int r = run_krylov_mode(s, nt, Q, 0)) return r;
           FS_HIP(hipMemcpyAsync(b->sc_all_host + (size_t)i * fs::kMaxRhs, s->sc,
                                 sizeof(fs::CgScalars) * nt, hipMemcpyDeviceToHost, main));
+        } else if (mode == 0) {
+          // the pending x += alpha p, and the true-residual check (blocking)
+          if (int r = fp64_finish(s, nt, *params)) return r;
         }
         fs::launch_residual(s, nt, s->r);
         fs::launch_expand(s, nt, nullptr, cap->tid, 0, s->q);
@@ -1680,6 +1748,8 @@ int fs_batch_solve(fs_batch* b, const fs_solve_params* params, int chunk, double
     std::memset(stats, 0, sizeof(*stats));
     stats->converged = 1;
     stats->solve_time = total;
+    stats->stencil_width = 4;
+    stats->cycle_width = 4;
   }
   int status = FS_OK;
   size_t off = 0;
@@ -1703,6 +1773,13 @@ int fs_batch_solve(fs_batch* b, const fs_solve_params* params, int chunk, double
       stats->residual_eq = std::max(stats->residual_eq, one.residual_eq);
       stats->fallbacks += one.fallbacks;
       stats->pc_applies += one.pc_applies;
+      stats->replacements += one.replacements;
+      stats->stop_reason = std::max(stats->stop_reason, one.stop_reason);
+      stats->maxiter = std::max(stats->maxiter, one.maxiter);
+      // the narrowest kernels any system of the batch ran
+      kernel_widths(s, P, &one);
+      stats->stencil_width = std::min(stats->stencil_width, one.stencil_width);
+      stats->cycle_width = std::min(stats->cycle_width, one.cycle_width);
       if (st != FS_OK) stats->converged = 0;
     }
     if (st != FS_OK) status = st;

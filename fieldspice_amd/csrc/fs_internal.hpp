@@ -127,6 +127,7 @@ struct fs_sys {
   std::vector<fs_pc*> pcs;    // every preconditioner built on this system
   int acc_fallbacks = 0;      // counters of the current solve call
   int acc_pc_applies = 0;
+  int acc_replacements = 0;   // FP64 residual replacements of the current solve call
   // island coarse space (fs_sys_set_deflation): Z = D^1/2 1_I per island I
   // in the equilibrated space, E = Z^T A_s Z
   int defl_n = 0;                  // islands (0: off)

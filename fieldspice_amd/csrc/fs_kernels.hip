@@ -212,6 +212,7 @@ struct SetupCounts {
 __global__ __launch_bounds__(kBlock) void k_setup_diag(const fs_op op, uint8_t* __restrict__ mask,
                                                        double* __restrict__ dsum,
                                                        SetupCounts* cnt) {
+#pragma clang fp contract(off)
   const Dims d = op.d;
   const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   bool is_fixed = false, is_empty = false, nonzero_sum = false;
@@ -219,10 +220,17 @@ __global__ __launch_bounds__(kBlock) void k_setup_diag(const fs_op op, uint8_t* 
   if (n < d.n) {
     int i, j, k;
     node_coords(d, n, i, j, k);
-    double ws = op.w[0][n] + op.w[1][n] + op.w[2][n];
-    if (i > 0) ws += op.w[0][n - 1];
-    if (j > 0) ws += op.w[1][n - d.nx];
-    if (k > 0) ws += op.w[2][n - d.sxy];
+    // summed in fieldspice's edge order (x, y, z families; the lower edge of
+    // each first), as the diagonal of its G^T W G product accumulates, so
+    // the diagonal, its maximum (the gauge node) and the equilibration
+    // match the reference to the last bit
+    double ws = 0.0;
+    if (i > 0) ws = ws + op.w[0][n - 1];
+    ws = ws + op.w[0][n];
+    if (j > 0) ws = ws + op.w[1][n - d.nx];
+    ws = ws + op.w[1][n];
+    if (k > 0) ws = ws + op.w[2][n - d.sxy];
+    ws = ws + op.w[2][n];
     const double ex = op.extra ? op.extra[n] : 0.0;
     const double D = ws + ex;
     dsum[n] = D;
@@ -1385,6 +1393,27 @@ void launch_flush_x(fs_sys* s, int nr) {
   VecArgs va{s->d.n, s->vstride, nr};
   hipLaunchKernelGGL(k_flush_x, dim3(s->vec_blocks), dim3(kBlock), 0, s->stream, s->x,
                      (const double*)s->p, va, (const CgScalars*)s->sc);
+}
+
+// After k_flush_x and an FP64 residual r = b_s - A_s x (aux3 = ||r||^2):
+// the pending step is consumed (alpha = 0 everywhere, so a later flush adds
+// nothing), and the right-hand sides in `restart` continue from the true
+// residual with a fresh search direction (beta = 0: p = r).
+__global__ void k_replace(int nr, unsigned restart, CgScalars* sc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int r = 0; r < nr; ++r) {
+    CgScalars& c = sc[r];
+    c.alpha = 0.0;
+    if (!((restart >> r) & 1u)) continue;
+    c.rho = c.aux3;
+    c.beta = 0.0;
+    c.done = 0;
+    c.refreshes += 1;
+  }
+}
+
+void launch_replace(fs_sys* s, int nr, unsigned restart) {
+  hipLaunchKernelGGL(k_replace, dim3(1), dim3(64), 0, s->stream, nr, restart, s->sc);
 }
 
 int launch_iteration(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {

@@ -58,6 +58,11 @@ int launch_iteration(fs_sys* s, int nr, int mode, hipEvent_t ev_a, hipEvent_t ev
 // FP64 CG (mode 0) defers x += alpha p into the next direction update; this adds
 // the last pending step, once, when the loop ends
 void launch_flush_x(fs_sys* s, int nr);
+// FP64 residual replacement: consume the flushed step on every RHS and
+// restart the RHS whose bit is set in `restart` from r (rho = aux3, beta = 0)
+void launch_replace(fs_sys* s, int nr, unsigned restart);
+// nodes per lane of the level-0 cycle kernels of a preconditioner (1 or 4)
+int pc_level0_width(const fs_pc* pc);
 void launch_init(fs_sys* s, int nr, double tol, int maxiter);
 void launch_lift_general(fs_sys* s, int nr, const double* b);
 void launch_lift_cap(fs_sys* s, int nr, const fs_op* L, const int32_t* tid, int j0);

@@ -394,6 +394,10 @@ int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, doub
   return FS_OK;
 }
 
+int pc_level0_width(const fs_pc* pc) {
+  return pc && pc->l0_pairs && pc->l0.x4 ? 4 : 1;
+}
+
 // Solver hook (fs_kernels.hip, FS_PREC_FCG): z32 = M(r32) with FCG partials.
 int pc_apply_solver(fs_sys* s, int nr, int* nblocks) {
   return pc_cycle(s->pc, nr, s->r32, s->z32, s->q32, s->partials, nblocks, s->sc, 2);

@@ -24,6 +24,15 @@ from .base import ConvergenceError, SolverConfig
 __all__ = ["GridLinearSystem", "method_for"]
 
 
+def _scipy_flag(st: dict) -> str:
+    """The flag ``fieldspice/solvers/poisson.py:554-557`` prints: SciPy's
+    ``cg`` returns its iteration cap as ``info`` whenever it stops without
+    converging.  SciPy has no breakdown exit; the device stops on one
+    (non-finite or non-positive curvature) and says so."""
+    flag = str(int(st["maxiter"]))
+    return flag + " (breakdown)" if st["stop_reason"] == 3 else flag
+
+
 def method_for(cfg: SolverConfig) -> tuple[str, int]:
     """(method name, precision code) the GPU runs for ``cfg.linear_solver``."""
     want = cfg.linear_solver
@@ -124,12 +133,13 @@ class GridLinearSystem:
         self.info.update(iterations=st["iterations"], residual=st["residual"],
                          residual_eq=st["residual_eq"], refreshes=st["refreshes"],
                          solve_time=st["solve_time"], fallbacks=st["fallbacks"],
-                         precond_applies=st["pc_applies"])
+                         precond_applies=st["pc_applies"], replacements=st["replacements"],
+                         stencil_width=st["stencil_width"], cycle_width=st["cycle_width"])
         if st["fallbacks"]:
             warnings.warn("flexible CG did not converge; the solve was redone with FP64 CG",
                           RuntimeWarning, stacklevel=3)
         if rc == _lib.FS_NOT_CONVERGED:
-            warnings.warn(f"{self.method} returned flag {st['iterations']} after "
+            warnings.warn(f"{self.method} returned flag {_scipy_flag(st)} after "
                           f"{st['iterations']} iterations", RuntimeWarning, stacklevel=3)
         worst = max(st["residual"], st["residual_eq"])
         tol = self.cfg.tol

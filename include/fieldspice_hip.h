@@ -82,6 +82,15 @@ typedef struct {
     double stencil_bytes;     /* algorithmic bytes: couplings once + p read + q write per live RHS */
     int fallbacks;       /* FS_PREC_FCG runs redone by FP64 CG */
     int pc_applies;      /* preconditioner applications (FS_PREC_FCG) */
+    int stop_reason;     /* worst over right-hand sides: 1 converged, 2 iteration cap,
+                            3 breakdown (non-finite or non-positive curvature) */
+    int maxiter;         /* iteration cap the solve ran with */
+    int replacements;    /* FS_PREC_FP64: restarts from the recomputed FP64 residual after
+                            the recursive residual met tol but the true one did not */
+    int stencil_width;   /* nodes per lane of the Krylov stencil kernel that ran:
+                            FP64 1 or 2 (even rows), FP32 1 or 4 (rows a multiple of 4) */
+    int cycle_width;     /* FS_PREC_FCG: nodes per lane of the level-0 cycle kernels (1 or 4);
+                            0 without a preconditioner */
 } fs_solve_stats;
 
 typedef struct {

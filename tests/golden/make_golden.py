@@ -8,9 +8,13 @@ Run in the development container only (it imports the reference package from
 Covers what no closed form pins: Maxwell capacitance and potentials of a
 graded, layered multi-conductor geometry (with and without a grounded shield),
 cell_to_edge on graded multi-material grids, an EQS transient through a
-two-phase medium, and a thermal transient with convection.  Outputs are small
-.npz files beside this script; inputs are regenerated from the stored seeds and
-arrays.
+two-phase medium, a thermal transient with convection, cell_to_node on graded
+multi-material grids, the gauge node LinearSystem pins on singular
+(all-Neumann) systems with and without ties in the diagonal, and a thermal
+transient through a medium of two heat capacities.  Outputs are small .npz
+files beside this script; inputs are regenerated from the stored seeds and
+arrays.  ``python tests/golden/make_golden.py [set ...]`` writes only the named
+sets (``main``, ``nodes``).
 """
 
 from __future__ import annotations
@@ -114,5 +118,71 @@ def main():
     print("golden fixtures written to", HERE)
 
 
+def nodes_gauge_thermal():
+    """cell_to_node on graded multi-material grids, gauge pins of singular
+    systems, and a thermal transient with heterogeneous rho*c_p."""
+    sys.path.insert(0, REF)
+    from fieldspice import operators as ops
+    from fieldspice.boundaries import BoundarySpec, Dirichlet
+    from fieldspice.grid import RectilinearGrid, graded_1d
+    from fieldspice.solvers.base import SolverConfig
+    from fieldspice.solvers.poisson import LinearSystem
+    from fieldspice.solvers.thermal import ThermalSolver
+
+    rng = np.random.default_rng(20261204)
+    # -- cell_to_node: graded axes, blocky materials plus random cells ------
+    x = graded_1d(0, 3.0e-6, 9, 1.3)
+    y = graded_1d(0, 2.0e-6, 7, 0.8)
+    z = graded_1d(0, 1.5e-6, 5, 1.5)
+    g = RectilinearGrid(x, y, z)
+    prop = np.full(g.shape_cells, 2.0)
+    prop[3:7, 1:4, :2] = 11.0
+    prop[:, 5:, 3:] = 0.5
+    prop[rng.random(g.shape_cells) < 0.2] = rng.uniform(20.0, 40.0)
+    c2n = ops.cell_to_node(g, prop)
+    # -- gauge pins: a heterogeneous graded Laplacian and a uniform one -----
+    sig = rng.uniform(1.0, 3.0, g.shape_cells)
+    L1 = ops.nodal_laplacian(g, ops.cell_to_edge(g, sig))
+    b1 = rng.standard_normal(g.n_nodes)
+    b1 -= b1.mean()
+    gu = RectilinearGrid(np.linspace(0, 1.0, 7), np.linspace(0, 1.0, 6), np.linspace(0, 1.0, 5))
+    L2 = ops.nodal_laplacian(gu, ops.cell_to_edge(gu, np.ones(gu.shape_cells)))
+    b2 = rng.standard_normal(gu.n_nodes)
+    b2 -= b2.mean()
+    cfg = SolverConfig(verbose=0, linear_solver="direct")
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        ls1 = LinearSystem(L1, cfg)
+        x1 = ls1.solve(b1)
+        ls2 = LinearSystem(L2, cfg)
+        x2 = ls2.solve(b2)
+    # -- thermal transient, two heat capacities and two conductivities -------
+    gt = RectilinearGrid(np.linspace(0, 4e-5, 13), graded_1d(0, 3e-5, 8, 1.2),
+                         np.linspace(0, 2e-5, 6))
+    kap = np.full(gt.shape_cells, 1.4)
+    kap[4:9, 2:6, :3] = 148.0
+    rcp = np.full(gt.shape_cells, 2200.0 * 730.0)
+    rcp[4:9, 2:6, :3] = 2329.0 * 700.0
+    rcp[:, :, 4:] = 8960.0 * 385.0
+    q = np.zeros(gt.n_nodes)
+    nid = np.arange(gt.n_nodes).reshape(gt.shape_nodes)
+    q[nid[5:8, 3:5, 0].ravel()] = 2e-4
+    bcT = BoundarySpec(xlo=Dirichlet(300.0))
+    ts = ThermalSolver(gt, kap, rcp)
+    rT = ts.transient(q, 3e-6, 1e-7, T0=300.0, bc=bcT, store=True)
+    np.savez(os.path.join(HERE, "nodes_gauge_thermal.npz"),
+             x=x, y=y, z=z, prop=prop, cell_to_node=c2n,
+             sigma=sig, b1=b1, x1=x1, gauge1=ls1.info["pinned"],
+             b2=b2, x2=x2, gauge2=ls2.info["pinned"],
+             tx=gt.xn, ty=gt.yn, tz=gt.zn, kappa=kap, rho_cp=rcp, q=q, T=rT.fields["T"],
+             T_max=rT.scalars["T_max"], T_mean=rT.scalars["T_mean"], t_end=3e-6, dt=1e-7,
+             C_th=ops.cell_to_node(gt, rcp).ravel() * gt.node_volumes().ravel())
+    print("node/gauge/thermal fixtures written to", HERE)
+
+
 if __name__ == "__main__":
-    main()
+    sets = sys.argv[1:] or ["main", "nodes"]
+    if "main" in sets:
+        main()
+    if "nodes" in sets:
+        nodes_gauge_thermal()

@@ -48,6 +48,25 @@ def _thermal_case():
     return f, g
 
 
+def _nodes_case():
+    f = _load("nodes_gauge_thermal.npz")
+    return f, RectilinearGrid(f["x"], f["y"], f["z"])
+
+
+def _gauge_systems(f, g):
+    """(operator cells, grid, rhs, reference solution, reference gauge) of the
+    two singular systems of the fixture."""
+    gu = RectilinearGrid(np.linspace(0, 1.0, 7), np.linspace(0, 1.0, 6), np.linspace(0, 1.0, 5))
+    return [(f["sigma"], g, f["b1"], f["x1"], int(f["gauge1"])),
+            (np.ones(gu.shape_cells), gu, f["b2"], f["x2"], int(f["gauge2"]))]
+
+
+def _hetero_thermal(f):
+    g = RectilinearGrid(f["tx"], f["ty"], f["tz"])
+    nid = g.node_ids()
+    return g, nid[0].ravel()
+
+
 # -------------------------------------------------------------------- oracle
 def test_oracle_cell_to_edge_matches_reference_bitwise():
     f, g, eps, _ = _cap_case()
@@ -82,9 +101,40 @@ def test_oracle_eqs_matches_reference():
                                atol=1e-9 * np.abs(f["i_drv"]).max())
 
 
+def test_oracle_and_host_cell_to_node_match_reference():
+    f, g = _nodes_case()
+    from fieldspice_amd.operators import cell_to_node
+    np.testing.assert_allclose(O.cell_to_node(g, f["prop"]), f["cell_to_node"], rtol=1e-15)
+    np.testing.assert_allclose(cell_to_node(g, f["prop"]), f["cell_to_node"], rtol=1e-15)
+    gt, _ = _hetero_thermal(f)
+    _, Cth = O.thermal_matrices(gt, f["kappa"], f["rho_cp"])
+    np.testing.assert_allclose(Cth, f["C_th"], rtol=1e-15)
+
+
+def test_oracle_gauge_matches_reference():
+    f, g = _nodes_case()
+    for cells, grid, b, x, gauge in _gauge_systems(f, g):
+        L = O.nodal_laplacian(grid, O.cell_to_edge(grid, cells))
+        with pytest.warns(RuntimeWarning, match="gauge"):
+            ls = O.LinearSystem(L, method="direct")
+        assert ls.gauge == gauge
+        assert np.abs(ls.solve(b) - x).max() <= 1e-10 * np.abs(x).max()
+
+
+def test_oracle_heterogeneous_thermal_matches_reference():
+    f, _ = _nodes_case()
+    g, wall = _hetero_thermal(f)
+    q = f["q"]
+    times, Ts = O.thermal_transient(g, f["kappa"], f["rho_cp"], lambda t: q, float(f["t_end"]),
+                                    float(f["dt"]), 300.0,
+                                    fixed_fn=lambda t: (wall, np.full(wall.size, 300.0)))
+    rise = np.abs(f["T"] - 300.0).max()
+    assert np.abs(np.array(Ts) - f["T"]).max() <= 1e-10 * rise
+
+
 # -------------------------------------------------------------------- GPU
 @pytest.mark.gpu
-@pytest.mark.parametrize("solver", ["hip_pcg", "hip_mixed"])
+@pytest.mark.parametrize("solver", ["hip_pcg", "hip_mixed", "hip_fcg"])
 def test_gpu_capacitance_matches_reference(solver):
     print(f"[gpu case] golden graded layered capacitance {solver}", flush=True)
     from fieldspice_amd.solvers.poisson import PoissonSolver
@@ -152,3 +202,35 @@ def test_gpu_thermal_matches_reference():
         warnings.simplefilter("ignore")
         Ts = ts.steady(f["q"], bc=bc, convection=conv).fields["T"][0]
     assert np.abs(Ts - f["T_steady"]).max() <= 1e-9 * np.abs(f["T_steady"] - 300.0).max()
+
+
+@pytest.mark.gpu
+def test_gpu_gauge_matches_reference():
+    print("[gpu case] golden gauge pins of singular systems", flush=True)
+    from fieldspice_amd import _lib
+    from fieldspice_amd.solvers.linear import GridLinearSystem
+    f, g = _nodes_case()
+    for cells, grid, b, x, gauge in _gauge_systems(f, g):
+        op = _lib.DeviceOperator.from_cells(grid, cells)
+        with pytest.warns(RuntimeWarning, match="gauge"):
+            ls = GridLinearSystem(op, None, SolverConfig(tol=1e-12))
+        assert ls.info["pinned"] == gauge
+        assert np.abs(ls.solve(b) - x).max() <= 1e-9 * np.abs(x).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("solver", ["hip_pcg", "hip_mixed"])
+def test_gpu_heterogeneous_thermal_matches_reference(solver):
+    print(f"[gpu case] golden thermal transient, two heat capacities, {solver}", flush=True)
+    from fieldspice_amd.solvers.thermal import ThermalSolver
+    f, _ = _nodes_case()
+    g, _ = _hetero_thermal(f)
+    # tol as in the convective case: ||b_s|| carries the 300 K baseline
+    ts = ThermalSolver(g, f["kappa"], f["rho_cp"], SolverConfig(tol=1e-14, linear_solver=solver))
+    np.testing.assert_allclose(ts.C_th, f["C_th"], rtol=1e-15)
+    r = ts.transient(f["q"], float(f["t_end"]), float(f["dt"]), T0=300.0,
+                     bc=BoundarySpec(xlo=Dirichlet(300.0)), store=True)
+    rise = np.abs(f["T"] - 300.0).max()
+    assert np.abs(r.fields["T"] - f["T"]).max() <= 1e-9 * rise
+    np.testing.assert_allclose(r.scalars["T_max"], f["T_max"], rtol=1e-12)
+    np.testing.assert_allclose(r.scalars["T_mean"], f["T_mean"], rtol=1e-12)

@@ -177,7 +177,7 @@ def test_parallel_plate_capacitance(gpu_lib, solver):
     assert rep["row_sum_rel"] < 1e-9 and rep["asymmetry_rel"] < 1e-9
 
 
-@pytest.mark.parametrize("solver", ["hip_pcg", "hip_mixed"])
+@pytest.mark.parametrize("solver", ["hip_pcg", "hip_mixed", "hip_fcg"])
 def test_multiconductor_capacitance_matches_oracle(gpu_lib, solver):
     _case(f"layered 6-box capacitance {solver}")
     ex = workloads.layered_boxes(nodes=(34, 30, 22), n_boxes=6, side=(2, 6), seed=4)

@@ -92,6 +92,61 @@ def test_cell_to_node_uniform():
     assert np.allclose(O.cell_to_node(g, np.full(g.shape_cells, 3.7)), 3.7)
 
 
+def _box_overlap(lo1, hi1, lo2, hi2):
+    return max(0.0, min(hi1, hi2) - max(lo1, lo2))
+
+
+def test_cell_to_node_brute_force_graded():
+    """Every node: sum over cells of prop * |cell intersect dual box|, over
+    |dual box|, the dual box spanning half the neighbouring cell widths
+    (clipped at the domain), intersected geometrically cell by cell."""
+    g = RectilinearGrid(graded_1d(0, 3.0, 5, 1.4), graded_1d(0, 2.0, 4, 0.7),
+                        graded_1d(0, 1.0, 3, 1.9))
+    rng = np.random.default_rng(11)
+    prop = rng.uniform(1.0, 9.0, g.shape_cells)
+    got = O.cell_to_node(g, prop)
+    xs, ys, zs = g.xn, g.yn, g.zn
+    mids = [np.concatenate([[a[0]], 0.5 * (a[1:] + a[:-1]), [a[-1]]]) for a in (xs, ys, zs)]
+    Nx, Ny, Nz = g.shape_cells
+    brute = np.zeros(g.shape_nodes)
+    for i in range(Nx + 1):
+        for j in range(Ny + 1):
+            for k in range(Nz + 1):
+                box = [(mids[0][i], mids[0][i + 1]), (mids[1][j], mids[1][j + 1]),
+                       (mids[2][k], mids[2][k + 1])]
+                num = vol = 0.0
+                for ci in range(Nx):
+                    ox = _box_overlap(xs[ci], xs[ci + 1], *box[0])
+                    for cj in range(Ny):
+                        oy = _box_overlap(ys[cj], ys[cj + 1], *box[1])
+                        for ck in range(Nz):
+                            v = ox * oy * _box_overlap(zs[ck], zs[ck + 1], *box[2])
+                            num += prop[ci, cj, ck] * v
+                            vol += v
+                assert vol == pytest.approx(g.node_volumes()[i, j, k], rel=1e-13)
+                assert got[i, j, k] == pytest.approx(num / vol, rel=1e-13), (i, j, k)
+                brute[i, j, k] = num / vol
+    # the check tells plausible mistakes apart: swapped low/high half widths,
+    # and an unweighted mean of the (edge-padded) 8 surrounding cells
+    p = np.pad(prop, 1, mode="edge")
+    nx, ny, nz = g.shape_nodes
+    swapped = np.zeros(g.shape_nodes)
+    den = np.zeros(g.shape_nodes)
+    halves = [(np.concatenate([0.5 * h, [0.0]]), np.concatenate([[0.0], 0.5 * h]))
+              for h in (g.hx, g.hy, g.hz)]
+    for a in (0, 1):
+        for b in (0, 1):
+            for c in (0, 1):
+                w = np.multiply.outer(np.multiply.outer(halves[0][a], halves[1][b]), halves[2][c])
+                swapped += p[a:a + nx, b:b + ny, c:c + nz] * w
+                den += w
+    swapped /= np.where(den > 0, den, 1.0)
+    mean8 = sum(p[a:a + nx, b:b + ny, c:c + nz] for a in (0, 1) for b in (0, 1)
+                for c in (0, 1)) / 8.0
+    for wrong in (swapped, mean8):
+        assert np.abs(wrong - brute).max() > 1e-3 * np.abs(brute).max()
+
+
 def test_laplacian_null_space_symmetry_and_incidence():
     g = _graded()
     G = O.grad_node_edge(g)
