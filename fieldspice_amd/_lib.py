@@ -40,6 +40,7 @@ EXPORTS = (
     "fs_pc_clear_pairs", "fs_pc_loss_grad", "fs_pc_train", "fs_sys_autotune",
     "fs_batch_create", "fs_batch_destroy", "fs_batch_add", "fs_batch_solve",
     "fs_sys_find_islands", "fs_sys_set_deflation", "fs_sys_get_deflation",
+    "fs_sys_solve_noalloc",
 )
 
 
@@ -114,6 +115,8 @@ def _declare(lib):
     lib.fs_sys_destroy.argtypes = [_vp]
     lib.fs_sys_solve.argtypes = [_vp, C.c_int, _dp, _vp, _dp, C.POINTER(SolveParams),
                                  C.POINTER(SolveStats)]
+    lib.fs_sys_solve_noalloc.argtypes = [_vp, C.c_int, _dp, _vp, _dp, C.POINTER(SolveParams),
+                                         C.POINTER(SolveStats)]
     lib.fs_capacitance.argtypes = [_vp, _vp, C.c_int, _i32p, _dp, _vp, _vp,
                                    C.POINTER(SolveParams), C.POINTER(SolveStats)]
     lib.fs_cap_prepare.argtypes = [_vp, C.c_int, _i32p, C.POINTER(_vp)]
@@ -327,9 +330,11 @@ class DeviceSystem:
         return SolveParams(float(tol), int(maxiter or 0), int(precision), float(inner_tol),
                            int(check_every))
 
-    def solve(self, b, params: SolveParams, x0=None):
+    def solve(self, b, params: SolveParams, x0=None, noalloc: bool = False):
         """Solve for one (n,) or several (k, n) right-hand sides; ``x0`` (same
-        shape as ``b``) is an initial guess."""
+        shape as ``b``) is an initial guess.  ``noalloc``: through
+        ``fs_sys_solve_noalloc`` (the workspace in the library's static
+        scratch, no device allocation)."""
         b = as_f64(b)
         single = b.ndim == 1
         b2 = b.reshape(1, -1) if single else b
@@ -344,10 +349,11 @@ class DeviceSystem:
             g = np.ascontiguousarray(g.reshape(b2.shape))
         x = np.empty_like(b2)
         st = SolveStats()
-        rc = load().fs_sys_solve(self._h, b2.shape[0], np.ascontiguousarray(b2),
-                                 None if g is None else g.ctypes.data_as(_vp), x,
-                                 C.byref(params), C.byref(st))
-        check(rc, "fs_sys_solve")
+        name = "fs_sys_solve_noalloc" if noalloc else "fs_sys_solve"
+        rc = getattr(load(), name)(self._h, b2.shape[0], np.ascontiguousarray(b2),
+                                   None if g is None else g.ctypes.data_as(_vp), x,
+                                   C.byref(params), C.byref(st))
+        check(rc, name)
         return (x[0] if single else x), st.as_dict(), rc
 
     def capacitance(self, L: DeviceOperator, term_id, nt: int, params: SolveParams,

@@ -75,12 +75,15 @@ int check_dims(int Nx, int Ny, int Nz) {
 
 // Upload a host array laid out C-order (A, B, C) into an x-fast device array
 // with leading dims (OA, OB) (entries outside (A, B, C) are zero).
+// `staging` (A*B*C elements, may be null: then allocated for the call) holds
+// the host copy before the permutation.
 template <typename T>
 int upload_permuted(fs_ctx* ctx, const T* host, int A, int B, int C, int OA, int OB, int OC,
-                    T* dev) {
-  T* tmp = nullptr;
+                    T* dev, T* staging = nullptr) {
+  T* tmp = staging;
   const size_t cnt = (size_t)A * B * C;
-  if (int rc = dalloc(&tmp, cnt)) return rc;
+  if (!tmp)
+    if (int rc = dalloc(&tmp, cnt)) return rc;
   hipError_t e = hipMemcpyAsync(tmp, host, cnt * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess)
     e = hipMemsetAsync(dev, 0, (size_t)OA * OB * OC * sizeof(T), ctx->stream);
@@ -89,7 +92,7 @@ int upload_permuted(fs_ctx* ctx, const T* host, int A, int B, int C, int OA, int
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  dfree(tmp);
+  if (!staging) dfree(tmp);
   if (e != hipSuccess) return fail(FS_EHIP, "upload failed: %s", hipGetErrorString(e));
   return FS_OK;
 }
@@ -462,6 +465,92 @@ struct DevBufs {
 };
 }  // namespace
 
+// --------------------------------------------------------------------------
+// Scratch of fs_sys_solve_noalloc: one array per GPU, reserved in device
+// memory when the library's code object is loaded on that GPU and held until
+// the process ends.  Its name is a host-side stand-in: host code uses only
+// the device address hipGetSymbolAddress returns for it on each device.
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Once per device, with that device current (fs_ctx_create): the address of
+// this device's copy of `scratch`.
+int scratch_setup(int device, unsigned char** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices)
+    return fail(FS_EINVAL, "device %d beyond the %d the scratch table covers", device,
+                kMaxDevices);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(FS_EHIP, "scratch address on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_ptr[device] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[device];
+  return FS_OK;
+}
+
+// The solve workspace of a system carved out of the scratch array for one
+// call; the system's own workspace pointers come back when it goes out of
+// scope.
+struct ScratchLoan {
+  fs_sys* s;
+  double *bs, *x, *r, *p, *q, *node_tmp;
+  float *x32, *r32, *p32, *q32, *z32;
+  int ws_nr;
+  double* x0d = nullptr;  // initial guesses (nr x vstride), if asked for
+
+  explicit ScratchLoan(fs_sys* sys)
+      : s(sys), bs(sys->bs), x(sys->x), r(sys->r), p(sys->p), q(sys->q),
+        node_tmp(sys->node_tmp), x32(sys->x32), r32(sys->r32), p32(sys->p32), q32(sys->q32),
+        z32(sys->z32), ws_nr(sys->ws_nr) {}
+  ~ScratchLoan() {
+    s->bs = bs; s->x = x; s->r = r; s->p = p; s->q = q; s->node_tmp = node_tmp;
+    s->x32 = x32; s->r32 = r32; s->p32 = p32; s->q32 = q32; s->z32 = z32;
+    s->ws_nr = ws_nr;
+  }
+
+  // Bytes the workspace of nr right-hand sides (plus guesses) takes.
+  static size_t need(const fs_sys* sys, int nr, bool with_x0) {
+    const size_t vec = ((size_t)nr * sys->vstride * sizeof(double) + 255) / 256 * 256;
+    const size_t vecf = ((size_t)nr * sys->vstride * sizeof(float) + 255) / 256 * 256;
+    return (6 + (with_x0 ? 1 : 0)) * vec + 5 * vecf;
+  }
+
+  // Point the system's workspace at `base` (a device address).
+  int lend(unsigned char* base, int nr, bool with_x0) {
+    const size_t vec = ((size_t)nr * s->vstride * sizeof(double) + 255) / 256 * 256;
+    const size_t vecf = ((size_t)nr * s->vstride * sizeof(float) + 255) / 256 * 256;
+    unsigned char* c = base;
+    auto d = [&]() { double* v = (double*)c; c += vec; return v; };
+    auto f = [&]() { float* v = (float*)c; c += vecf; return v; };
+    s->bs = d(); s->x = d(); s->r = d(); s->p = d(); s->q = d(); s->node_tmp = d();
+    s->x32 = f(); s->r32 = f(); s->p32 = f(); s->q32 = f(); s->z32 = f();
+    if (with_x0) x0d = d();
+    s->ws_nr = nr;
+    // the array keeps the previous call's contents: the flexible-CG start
+    // reads q32 into an unused partial, and z32 must start finite
+    const size_t fbytes = (size_t)nr * s->vstride * sizeof(float);
+    FS_HIP(hipMemsetAsync(s->q32, 0, fbytes, s->ctx->stream));
+    FS_HIP(hipMemsetAsync(s->z32, 0, fbytes, s->ctx->stream));
+    return FS_OK;
+  }
+};
+
+// fs_sys_solve; with `loan` every device buffer of the call comes from it
+int sys_solve(fs_sys* s, int nrhs, const double* b, const double* x0, double* x,
+              const fs_solve_params* params, fs_solve_stats* stats, ScratchLoan* loan);
+
+}  // namespace
+
 // ==========================================================================
 extern "C" {
 
@@ -487,6 +576,11 @@ int fs_ctx_create(int device, fs_ctx** out) {
     delete ctx;
     return fail(FS_EHIP, "stream/event creation failed");
   }
+  // the no-allocation solve's scratch on this device (its failure is
+  // reported by fs_sys_solve_noalloc, not here)
+  const std::string keep = fs::g_err;
+  if (scratch_setup(device, &ctx->scratch) != FS_OK) ctx->scratch = nullptr;
+  fs::g_err = keep;
   *out = ctx;
   return FS_OK;
   FS_GUARD_END
@@ -820,6 +914,39 @@ int fs_sys_destroy(fs_sys* sys) {
 int fs_sys_solve(fs_sys* s, int nrhs, const double* b, const double* x0, double* x,
                  const fs_solve_params* params, fs_solve_stats* stats) {
   FS_GUARD_BEGIN
+  return sys_solve(s, nrhs, b, x0, x, params, stats, nullptr);
+  FS_GUARD_END
+}
+
+int fs_sys_solve_noalloc(fs_sys* s, int nrhs, const double* b, const double* x0, double* x,
+                         const fs_solve_params* params, fs_solve_stats* stats) {
+  FS_GUARD_BEGIN
+  if (!s || !b || !x || !params) return fail(FS_EINVAL, "NULL argument");
+  if (nrhs < 1) return fail(FS_EINVAL, "nrhs must be >= 1");
+  if (!s->ctx->scratch)
+    return fail(FS_EHIP, "no scratch address on device %d (hipGetSymbolAddress failed when the "
+                "context was created)", s->ctx->device);
+  FS_HIP(hipSetDevice(s->ctx->device));
+  const int nr = std::min(fs::kMaxRhs, nrhs);
+  const size_t need = ScratchLoan::need(s, nr, x0 != nullptr);
+  if (need > kScratchBytes)
+    return fail(FS_EINVAL,
+                "fs_sys_solve_noalloc needs %zu bytes of scratch for %d right-hand side(s) of "
+                "%lld nodes%s; its scratch holds %zu (68 B per node and right-hand side, 76 "
+                "with initial guesses, up to 8 right-hand sides at a time)",
+                need, nr, (long long)s->d.n, x0 ? " with initial guesses" : "", kScratchBytes);
+  ScratchLoan loan(s);
+  if (int rc = loan.lend(s->ctx->scratch, nr, x0 != nullptr)) return rc;
+  return sys_solve(s, nrhs, b, x0, x, params, stats, &loan);
+  FS_GUARD_END
+}
+
+}  // extern "C"
+
+namespace {
+
+int sys_solve(fs_sys* s, int nrhs, const double* b, const double* x0, double* x,
+              const fs_solve_params* params, fs_solve_stats* stats, ScratchLoan* loan) {
   if (!s || !b || !x || !params) return fail(FS_EINVAL, "NULL argument");
   if (nrhs < 1) return fail(FS_EINVAL, "nrhs must be >= 1");
   if (!(params->tol > 0.0)) return fail(FS_EINVAL, "tol must be > 0");
@@ -838,20 +965,24 @@ int fs_sys_solve(fs_sys* s, int nrhs, const double* b, const double* x0, double*
   for (int r0 = 0; r0 < nrhs; r0 += fs::kMaxRhs) {
     const int nr = std::min(fs::kMaxRhs, nrhs - r0);
     if (int rc = ensure_ws(s, nr)) return rc;
+    // host copies go through node_tmp (written again only after the uploads)
+    // when the workspace is on loan
+    double* stage = loan ? s->node_tmp : nullptr;
     // full right-hand sides (x-fast) into q, initial guesses into x0d
     for (int r = 0; r < nr; ++r) {
       const double* hb = b + (size_t)(r0 + r) * d.n;
       if (int rc = upload_permuted(ctx, hb, d.nx, d.ny, d.nz, d.nx, d.ny, d.nz,
-                                   s->q + (size_t)r * s->vstride))
+                                   s->q + (size_t)r * s->vstride, stage))
         return rc;
     }
     DevBufs B;
     double* x0d = nullptr;
     if (x0) {
-      if (int rc = B.get(&x0d, (size_t)nr * s->vstride)) return rc;
+      if (loan) x0d = loan->x0d;
+      else if (int rc = B.get(&x0d, (size_t)nr * s->vstride)) return rc;
       for (int r = 0; r < nr; ++r)
         if (int rc = upload_permuted(ctx, x0 + (size_t)(r0 + r) * d.n, d.nx, d.ny, d.nz, d.nx,
-                                     d.ny, d.nz, x0d + (size_t)r * s->vstride))
+                                     d.ny, d.nz, x0d + (size_t)r * s->vstride, stage))
           return rc;
     }
     prof_begin(s);
@@ -896,8 +1027,11 @@ int fs_sys_solve(fs_sys* s, int nrhs, const double* b, const double* x0, double*
     fail(FS_NOT_CONVERGED, "Krylov iteration did not reach tol %.3g within %d iterations",
          params->tol, maxiter);
   return status;
-  FS_GUARD_END
 }
+
+}  // namespace
+
+extern "C" {
 
 int fs_cap_prepare(fs_sys* s, int nt, const int32_t* term_id, fs_cap** out) {
   FS_GUARD_BEGIN

@@ -74,6 +74,7 @@ struct fs_ctx {
   int n_cu = 0;
   char name[256] = {0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  unsigned char* scratch = nullptr;  // this device's fs_sys_solve_noalloc scratch, or null
 };
 
 struct fs_op {

@@ -153,6 +153,20 @@ int fs_sys_destroy(fs_sys* sys);
 int fs_sys_solve(fs_sys* sys, int nrhs, const double* b, const double* x0,
                  double* x, const fs_solve_params* params, fs_solve_stats* stats);
 
+/* fs_sys_solve that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and status codes as fs_sys_solve; the Krylov workspace,
+ * the initial guesses and the upload staging live in a static 256 MiB array
+ * per GPU, which needs 68 bytes per node and right-hand side (76 with x0),
+ * for up to 8 right-hand sides at a time (nrhs above 8 run in batches of 8):
+ * larger requests fail with FS_EINVAL.  Every call on one GPU shares that
+ * GPU's array, so calls that overlap on one GPU (several contexts or threads)
+ * race; serialise them.  The system itself (and its preconditioner or island
+ * space) must have been created beforehand. */
+int fs_sys_solve_noalloc(fs_sys* sys, int nrhs, const double* b, const double* x0,
+                         double* x, const fs_solve_params* params, fs_solve_stats* stats);
+
 /* PoissonSolver.capacitance_matrix inner loop
  * (fieldspice/solvers/poisson.py:1208-1223): sys was created from op L with
  * every terminal and shield node marked Dirichlet.  term_id per node:
