@@ -94,3 +94,30 @@ def test_zero_tcr_makes_the_coupling_one_pass_and_records_assumptions():
     assert r.meta["iterations"] <= 3
     assert "A6" not in r.meta["assumptions"] and "A1a" in r.meta["assumptions"]
     assert "self_heating" in r.meta["physics"]
+
+
+def test_floating_terminal_is_left_unconstrained():
+    """A floating terminal constrains nothing in the conduction problem (the
+    reference assembles only voltage and current terminals,
+    fieldspice/solvers/electrothermal.py:143-173): the fields equal those of
+    the run without it, and it reports the dual-volume weighted potential of
+    its nodes and no net current."""
+    _case("electro-thermal run with a floating probe terminal")
+    g, sig, kap, alpha, R_th, R0, conv, nid = _lump()
+    ta = Terminal("a", nid[0].ravel(), voltage=0.5)
+    probe = Terminal("probe", nid[10, 1:3, 1:3].ravel(), voltage=None)
+    assert probe.driven == "float"
+    r0 = _solve(ta, g, sig, kap, alpha, conv, nid)
+    tb = Terminal("b", nid[-1].ravel(), voltage=0.0)
+    o = PhysicsOptions(self_heating=True, max_coupling_iterations=3000, coupling_tolerance=1e-10)
+    s = ElectroThermalSolver(g, sig, kap, alpha, o)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        r1 = s.solve([ta, tb, probe], convection=conv, damping=0.6, T_limit=1e9)
+    assert np.array_equal(r1.fields["phi"], r0.fields["phi"])
+    assert np.array_equal(r1.fields["T"], r0.fields["T"])
+    phi = r1.fields["phi"][0]
+    w = g.node_volumes().ravel()[probe.nodes]
+    assert r1.v("probe")[0] == pytest.approx(float(w @ phi[probe.nodes]) / w.sum(), rel=1e-14)
+    assert 0.0 < r1.v("probe")[0] < 0.5
+    assert abs(r1.i("probe")[0]) <= 1e-8 * abs(r1.i("a")[0])
