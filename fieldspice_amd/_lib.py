@@ -282,12 +282,18 @@ class DeviceOperator:
         return q
 
     def edges(self) -> np.ndarray:
+        return np.concatenate([w.ravel() for w in self.edges_split()])
+
+    def edges_split(self):
+        """Edge weights per family, shaped (Nx, Ny+1, Nz+1), (Nx+1, Ny, Nz+1),
+        (Nx+1, Ny+1, Nz)."""
         Nx, Ny, Nz = self.shape_cells
         wx = np.empty(Nx * (Ny + 1) * (Nz + 1))
         wy = np.empty((Nx + 1) * Ny * (Nz + 1))
         wz = np.empty((Nx + 1) * (Ny + 1) * Nz)
         check(load().fs_op_get_edges(self._h, wx, wy, wz), "fs_op_get_edges")
-        return np.concatenate([wx, wy, wz])
+        return (wx.reshape(Nx, Ny + 1, Nz + 1), wy.reshape(Nx + 1, Ny, Nz + 1),
+                wz.reshape(Nx + 1, Ny + 1, Nz))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -675,17 +675,30 @@ def field_region_ac(grid, eps_cell, sigma_cell, terminal_nodes, omega):
 # Poisson with floating / charge-driven electrodes
 # (fieldspice/solvers/poisson.py:1039-1063, node merge :663-720)
 # --------------------------------------------------------------------------
-def poisson_floating(grid, eps_cell, fixed, vals, floating, charges, b_free=None):
-    """Each node set in ``floating`` merged into one unknown (``phi = P x``,
-    ``P^T L P x = P^T b``), its prescribed charge added to the merged row,
-    the Dirichlet nodes ``fixed`` eliminated, solved directly.  Returns the
-    node potential."""
+def poisson_floating(grid, eps_cell, fixed, vals, floating, charges, b_free=None,
+                     merge=(), return_info=False):
+    """The reference's node merge (fieldspice/solvers/poisson.py:665-719,
+    1030-1063): each node set in ``floating`` and in ``merge`` (periodic wall
+    pairs) becomes one unknown (``phi = P x``, ``P^T L P x = P^T b``), reduced
+    indices in the order of each set's smallest node (as scipy's
+    connected_components numbers them); each floating set's prescribed charge
+    is added to its merged row; the Dirichlet nodes ``fixed`` are eliminated
+    on the reduced indices (a merged unknown with several Dirichlet nodes
+    takes the last one's value); solved by ``LinearSystem`` ("direct"), which
+    pins the gauge of a singular system.  Returns the node potential (and the
+    solver's info)."""
     L = nodal_laplacian(grid, cell_to_edge(grid, eps_cell, "parallel"))
     n = grid.n_nodes
     label = np.arange(n)
-    for g in floating:
-        g = np.asarray(g, dtype=np.intp)
-        label[g] = g[0]
+    sets = [np.asarray(g, dtype=np.intp) for g in list(floating) + list(merge)]
+    changed = True
+    while changed:                       # union of overlapping sets
+        changed = False
+        for g in sets:
+            lo = label[g].min()
+            if np.any(label[g] != lo):
+                label[np.isin(label, label[g])] = lo
+                changed = True
     _, group = np.unique(label, return_inverse=True)
     nred = int(group.max()) + 1
     P = sp.csr_matrix((np.ones(n), (np.arange(n), group)), shape=(n, nred))
@@ -696,4 +709,10 @@ def poisson_floating(grid, eps_cell, fixed, vals, floating, charges, b_free=None
     b_r = P.T @ b
     A_bc, b_bc = apply_dirichlet(A_r, b_r, group[np.asarray(fixed, dtype=np.intp)],
                                  np.asarray(vals, dtype=float))
-    return P @ spla.spsolve(A_bc.tocsc(), b_bc)
+    ls = LinearSystem(A_bc, method="direct")
+    phi = P @ ls.solve(b_bc)
+    if return_info:
+        info = dict(ls.info)
+        info["group"] = group
+        return phi, info
+    return phi

@@ -74,12 +74,62 @@ def test_floating_and_charged_boxes_vs_oracle_merge(solver):
     assert r.terminals["flt"]["v"][0] == pytest.approx(ref[flt[0]], rel=1e-9)
 
 
-def test_floating_terminal_limits_are_explicit():
-    _case("poisson: floating terminal with no reference / touching a wall")
-    g = RectilinearGrid(np.linspace(0, 1e-6, 9), np.linspace(0, 1e-6, 9), np.linspace(0, 1e-6, 9))
+def test_floating_electrode_touching_a_dirichlet_wall_vs_oracle_merge():
+    """An electrode with nodes on a Dirichlet wall is eliminated with the
+    wall: it takes the wall value of its last wall node (the reference's
+    merged unknown), and a second, free electrode still floats."""
+    _case("poisson: floating electrode touching a Dirichlet wall vs oracle merge")
+    g = RectilinearGrid(np.linspace(0, 2e-6, 17), np.linspace(0, 1.5e-6, 13),
+                        np.linspace(0, 1e-6, 9))
+    rng = np.random.default_rng(3)
+    eps = rng.uniform(2.0, 6.0, g.shape_cells) * eps0
     nid = g.node_ids()
-    ps = PoissonSolver(g, np.full(g.shape_cells, eps0))
-    with pytest.raises(NotImplementedError, match="no Dirichlet node"):
-        ps.solve([Terminal("f", nid[3:5, 3:5, 3:5].ravel(), current=1e-18)])
-    with pytest.raises(NotImplementedError, match="touches a Dirichlet wall"):
-        ps.solve([Terminal("f", nid[0:2, 3:5, 3:5].ravel())], BoundarySpec(xlo=Dirichlet(0.0)))
+    bc = BoundarySpec(xlo=Dirichlet(0.25), zhi=Dirichlet(-0.1))
+    touch = nid[0:3, 4:8, 2:5].ravel()                # on the xlo wall
+    free = nid[9:13, 3:7, 3:6].ravel()
+    drv = nid[13:16, 9:12, 1:3].ravel()
+    Qf = 1.5e-18
+    ps = PoissonSolver(g, eps, SolverConfig(tol=1e-12))
+    r = ps.solve([Terminal("touch", touch, current=4e-18), Terminal("free", free, current=Qf),
+                  Terminal("drv", drv, voltage=0.6)], bc)
+    wall_idx, wall_val = bc.dirichlet_nodes(g)
+    fixed = np.concatenate([wall_idx, drv])
+    vals = np.concatenate([wall_val, np.full(drv.size, 0.6)])
+    ref = O.poisson_floating(g, eps, fixed, vals, [touch, free], [4e-18, Qf])
+    phi = r.fields["phi"][0].ravel()
+    np.testing.assert_allclose(phi, ref, rtol=0, atol=1e-9 * np.abs(ref).max())
+    assert np.ptp(phi[touch]) == 0.0 and phi[touch[0]] == pytest.approx(0.25, rel=1e-12)
+    assert r.terminals["free"]["q"][0] == pytest.approx(Qf, rel=1e-8)
+
+
+@pytest.mark.parametrize("gauge_on", ["electrode", "node"])
+def test_floating_electrodes_without_any_dirichlet_node_vs_oracle_merge(gauge_on):
+    """No voltage terminal and no Dirichlet wall: the merged system is
+    singular and the reference pins its gauge (the unknown with the first
+    largest diagonal of P^T L P): a large electrode, or with small
+    electrodes and a high-permittivity node, a free node."""
+    _case(f"poisson: floating electrodes with no Dirichlet node, gauge on {gauge_on}")
+    g = RectilinearGrid(np.linspace(0, 2e-6, 17), np.linspace(0, 1.5e-6, 13),
+                        np.linspace(0, 1e-6, 9))
+    rng = np.random.default_rng(4)
+    eps = rng.uniform(2.0, 6.0, g.shape_cells) * eps0
+    nid = g.node_ids()
+    if gauge_on == "electrode":
+        a, b = nid[2:7, 2:7, 2:6].ravel(), nid[10:13, 6:9, 3:5].ravel()
+    else:
+        a, b = nid[3, 3, 3:5].ravel(), nid[10, 8, 3:5].ravel()
+        eps[12:14, 2:4, 5:7] *= 400.0       # one node with the largest diagonal
+    Q = 2e-18
+    rho = np.zeros(g.shape_nodes)
+    rho[6:9, 9:11, 1:3] = 1e3
+    b_free = rho.ravel() * g.node_volumes().ravel()
+    ps = PoissonSolver(g, eps, SolverConfig(tol=1e-12))
+    with pytest.warns(RuntimeWarning, match="gauge"):
+        r = ps.solve([Terminal("a", a, current=Q), Terminal("b", b)], rho_node=rho)
+    with pytest.warns(RuntimeWarning):
+        ref, info = O.poisson_floating(g, eps, np.empty(0, np.intp), np.empty(0), [a, b],
+                                       [Q, 0.0], b_free=b_free, return_info=True)
+    rep = int(np.flatnonzero(info["group"] == info["pinned"])[0])
+    assert (rep in (a.min(), b.min())) == (gauge_on == "electrode")
+    phi = r.fields["phi"][0].ravel()
+    np.testing.assert_allclose(phi, ref, rtol=0, atol=1e-9 * np.abs(ref).max())
