@@ -22,7 +22,7 @@ from .. import _lib
 from ..boundaries import BoundarySpec
 from .base import Result, SolverConfig, Terminal, _Timer, check_terminals
 from .linear import GridLinearSystem
-from .transient import DirichletGroups, source_basis
+from .transient import DirichletGroups, device_time_loop, source_basis
 
 __all__ = ["EQSSolver"]
 
@@ -169,8 +169,9 @@ class EQSSolver:
             probe[t.nodes] = m
             pw[t.nodes] = self._node_weights(t.nodes)
         cur_a = self.L_eps.combine(1.0 / dt)
-        out = ls.sys.transient(
-            Kexp, phi, nsteps, group=groups.owner if groups.ngroups else None, values=values,
+        out = device_time_loop(
+            ls.sys, Kexp, phi, nsteps, group=groups.owner if groups.ngroups else None,
+            values=values,
             src=src, src_scale=scales, probe=probe if terms else None, probe_w=pw,
             nprobe=len(terms), cur_a=cur_a, cur_b=self.L_sigma, quad_a=self.L_eps,
             quad_b=self.L_sigma, store_every=max(1, store_every) if store else 0,

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import fs_oracle as O
-from fieldspice_amd.boundaries import BoundarySpec, Dirichlet
+from fieldspice_amd.boundaries import BoundarySpec, Dirichlet, wall_nodes
 from fieldspice_amd.grid import RectilinearGrid
 from fieldspice_amd.solvers.base import SolverConfig, Terminal
 from fieldspice_amd.solvers.eqs import EQSSolver
@@ -318,3 +318,62 @@ def test_joule_heating_is_exactly_conservative():
     q = joule_heating_nodes(g, se, phi)
     assert q.sum() == pytest.approx(float(phi @ (L @ phi)), rel=1e-12)
     assert (q >= -1e-30).all()
+
+
+def test_eqs_array_valued_time_dependent_wall_matches_oracle():
+    """A Dirichlet wall whose value varies over the wall and in time (one
+    device group per wall node) beside a grounded terminal, against the
+    oracle's time loop with the reference's precedence (walls win,
+    fieldspice/solvers/eqs.py:108-129)."""
+    _case("EQS with a spatially and time varying Dirichlet wall vs oracle")
+    g = RectilinearGrid(np.linspace(0, 1.2e-7, 13), np.linspace(0, 1.0e-7, 11),
+                        np.linspace(0, 0.8e-7, 9))
+    rng = np.random.default_rng(12)
+    sig = np.where(rng.random(g.shape_cells) < 0.3, 2e4, 0.0)
+    eps = rng.uniform(2.0, 6.0, g.shape_cells) * eps0
+    nid = g.node_ids()
+    wall = wall_nodes(g, "xlo")
+    y = g.node_coords()[1].ravel()[wall]
+    prof = 0.5 + y / y.max()
+    f0 = 2e9
+    bc = BoundarySpec(xlo=Dirichlet(lambda t: np.sin(2 * np.pi * f0 * t) * prof))
+    gnd = Terminal("gnd", nid[0:4, 6:10, 3:6].ravel(), voltage=0.0)   # overlaps the wall
+    dt, nsteps = 5e-12, 24
+    # tol 1e-14: the potentials then meet the direct solve to 1e-9 after 24 steps
+    r = EQSSolver(g, eps, sig, SolverConfig(tol=1e-14)).solve([gnd], t_end=nsteps * dt, dt=dt,
+                                                            bc=bc, store=True)
+
+    def fixed_fn(t):
+        w_idx, w_val = bc.dirichlet_nodes(g, t)
+        return O.unique_last(np.concatenate([gnd.nodes, w_idx]),
+                             np.concatenate([np.zeros(gnd.nodes.size), w_val]))
+    out = O.eqs_transient(g, eps, sig, fixed_fn, nsteps * dt, dt, terminal_nodes=[gnd.nodes])
+    ref = np.array(out["phis"])
+    assert np.abs(r.fields["phi"].reshape(ref.shape) - ref).max() <= 1e-9 * np.abs(ref).max()
+    i_ref = out["currents"][0]
+    assert np.abs(r.i("gnd") - i_ref).max() <= 1e-7 * np.abs(i_ref).max()
+
+
+def test_thermal_transient_with_arbitrary_source_matches_oracle():
+    """A source with a new shape every step (more shapes than one device call
+    carries), run as several device calls, against the oracle's loop."""
+    _case("thermal transient, a new source shape every step, vs oracle")
+    g = RectilinearGrid(np.linspace(0, 4e-5, 11), np.linspace(0, 3e-5, 7), np.linspace(0, 2e-5, 5))
+    kap = np.full(g.shape_cells, 1.4)
+    kap[3:7, 2:5, :2] = 148.0
+    rcp = np.full(g.shape_cells, 2200.0 * 730.0)
+    rng = np.random.default_rng(21)
+    dt, nsteps = 1e-7, 20
+    shapes = {round(k * dt, 15): rng.uniform(0.0, 1e-4, g.n_nodes) for k in range(nsteps + 1)}
+    qfn = lambda t: shapes[round(t, 15)]                      # noqa: E731
+    bc = BoundarySpec(xlo=Dirichlet(300.0))
+    ts = ThermalSolver(g, kap, rcp, SolverConfig(tol=1e-14))
+    r = ts.transient(qfn, nsteps * dt, dt, T0=300.0, bc=bc, store=True, store_every=3)
+    wall = wall_nodes(g, "xlo")
+    times, Ts = O.thermal_transient(g, kap, rcp, qfn, nsteps * dt, dt, 300.0,
+                                    fixed_fn=lambda t: (wall, np.full(wall.size, 300.0)))
+    Tref = np.array(Ts)[::3]
+    rise = np.abs(np.array(Ts) - 300.0).max()
+    assert r.fields["T"].shape == Tref.shape
+    assert np.abs(r.fields["T"] - Tref).max() <= 1e-9 * rise
+    np.testing.assert_allclose(r.scalars["T_max"], np.array(Ts).max(axis=1), rtol=1e-12)

@@ -297,3 +297,88 @@ def test_bench_refuses_more_than_one_gpu():
         assert p.returncode == 2, (wl, p.returncode, p.stderr[-500:])
         msg = json.loads(p.stdout.strip().splitlines()[-1])
         assert "one GPU" in msg["error"]
+
+
+class _FakeLoop:
+    """Stand-in for the device time loop with the same contract: x_k =
+    0.5 x_{k-1} + sum_s scale[k][s] src[s] on free nodes, Dirichlet nodes at
+    their group's value; stored every store_every steps from x_0; probe
+    'current' = sum of x_k - x_{k-1} over all nodes (row 0 copies row 1)."""
+
+    def __init__(self, group):
+        self.group = group
+        self.calls = 0
+
+    def transient(self, Aexp, x0, nsteps, group=None, values=None, src=None, src_scale=None,
+                  store_every=0, x_before=None, nprobe=0, **kw):
+        self.calls += 1
+        x = np.array(x0, dtype=float)
+        xs, prev = [x.copy()], x.copy()
+        pi = np.zeros((nsteps + 1, 1))
+        ss = np.zeros((nsteps + 1, 4))
+        ss[0] = [x.sum(), x.max(), 0, 0]
+        for k in range(1, nsteps + 1):
+            x = 0.5 * prev
+            if src is not None:
+                x = x + src_scale[k] @ np.asarray(src)
+            if values is not None:
+                fixed = group >= 0
+                x[fixed] = values[k][group[fixed]]
+            pi[k, 0] = float((x - prev).sum())
+            ss[k] = [x.sum(), x.max(), 0, 0]
+            xs.append(x.copy())
+            prev_before, prev = prev, x
+        pi[0] = pi[1]
+        stored = np.array(xs[::store_every]) if store_every > 0 else None
+        st = {k: 0 for k in ("iterations", "refreshes", "fallbacks", "pc_applies",
+                             "replacements", "stop_reason")}
+        st.update(residual=0.0, residual_eq=0.0, solve_time=0.0, converged=1)
+        return {"x": x, "x_before": prev_before, "stored": stored, "probe_i": pi,
+                "probe_v": None, "step_stats": ss, "stats": st, "rc": 0}
+
+
+@pytest.mark.parametrize("store_every", [1, 3, 4])
+def test_chunked_time_loop_equals_one_call(monkeypatch, store_every):
+    """Sources that take more shapes than one device call holds, and a values
+    array past the budget, split the run into calls; the stitched fields,
+    probes and stats equal one call's."""
+    from fieldspice_amd.solvers import transient as T
+    rng = np.random.default_rng(7)
+    n, nsteps = 12, 23
+    group = np.full(n, -1, np.int32)
+    group[[0, 5]] = [0, 1]
+    values = rng.standard_normal((nsteps + 1, 2))
+    sources = [None] + [rng.standard_normal(n) if k % 4 else None for k in range(1, nsteps + 1)]
+    x0 = rng.standard_normal(n)
+    one = _FakeLoop(group)
+    monkeypatch.setattr(T, "MAX_SOURCE_BASIS", 64)
+    ref = T.device_time_loop(one, None, x0, nsteps, group=group, values=values, sources=sources,
+                             store_every=store_every)
+    assert one.calls == 1
+    many = _FakeLoop(group)
+    monkeypatch.setattr(T, "MAX_SOURCE_BASIS", 3)
+    monkeypatch.setattr(T, "VALUES_BUDGET_BYTES", 8 * 2 * 6)
+    out = T.device_time_loop(many, None, x0, nsteps, group=group, values=values,
+                             sources=sources, store_every=store_every)
+    assert many.calls > 3
+    for key in ("x", "stored", "probe_i", "step_stats"):
+        np.testing.assert_allclose(out[key], ref[key], rtol=1e-13, atol=1e-13)
+    assert out["stored"].shape[0] == nsteps // store_every + 1
+
+
+def test_array_valued_wall_groups():
+    """A wall with a value per node becomes one group per node; later claims
+    still win (walls after terminals)."""
+    g = RectilinearGrid.uniform([(0, 1.0), (0, 1.0), (0, 1.0)], [4, 3, 2])
+    nid = g.node_ids()
+    wall = wall_nodes(g, "xlo")
+    prof = np.linspace(0.0, 1.0, wall.size)
+    bc = BoundarySpec(xlo=Dirichlet(lambda t: (1.0 + t) * prof), xhi=Dirichlet(0.5))
+    term = Terminal("t", nid[0:2, 0, 0].ravel(), voltage=2.0)
+    G = DirichletGroups(g, [term], bc, walls_last=True)
+    for t in (0.0, 0.3):
+        idx, vals = G.node_values(t)
+        w_idx, w_val = bc.dirichlet_nodes(g, t)
+        ri, rv = O.unique_last(np.concatenate([term.nodes, w_idx]),
+                               np.concatenate([np.full(term.nodes.size, 2.0), w_val]))
+        assert np.array_equal(idx, ri) and np.array_equal(vals, rv)
