@@ -40,7 +40,7 @@ EXPORTS = (
     "fs_pc_clear_pairs", "fs_pc_loss_grad", "fs_pc_train", "fs_sys_autotune",
     "fs_batch_create", "fs_batch_destroy", "fs_batch_add", "fs_batch_solve",
     "fs_sys_find_islands", "fs_sys_set_deflation", "fs_sys_get_deflation",
-    "fs_sys_solve_noalloc",
+    "fs_sys_solve_noalloc", "fs_op_create_periodic",
 )
 
 
@@ -106,6 +106,8 @@ def _declare(lib):
                                        _vp, C.POINTER(_vp)]
     lib.fs_op_create_cells.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
                                        C.c_int, C.POINTER(_vp)]
+    lib.fs_op_create_periodic.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int,
+                                          C.POINTER(_vp)]
     lib.fs_op_combine.argtypes = [C.c_double, _vp, C.c_double, _vp, C.c_double, _vp,
                                   C.POINTER(_vp)]
     lib.fs_op_destroy.argtypes = [_vp]
@@ -213,9 +215,22 @@ def as_f64(a) -> np.ndarray:
 class DeviceOperator:
     """``A = G^T diag(w) G + diag(extra)`` resident on the GPU (an ``fs_op``)."""
 
-    def __init__(self, handle, shape_cells):
+    def __init__(self, handle, shape_cells, periodic: int = 0):
         self._h = handle
         self.shape_cells = tuple(shape_cells)
+        self.periodic = int(periodic)
+
+    @classmethod
+    def from_periodic_edges(cls, shape_nodes, wx, wy, wz, periodic: int) -> "DeviceOperator":
+        """Operator of a grid whose walls are identified along the axes in
+        ``periodic`` (bit a for axis a), given on the reduced grid: along a
+        periodic axis as many edges as nodes, the last one wrapping around."""
+        nx, ny, nz = (int(v) for v in shape_nodes)
+        h = _vp()
+        check(load().fs_op_create_periodic(context(), nx, ny, nz, as_f64(wx).ravel(),
+                                           as_f64(wy).ravel(), as_f64(wz).ravel(),
+                                           int(periodic), C.byref(h)), "fs_op_create_periodic")
+        return cls(h, (nx - 1, ny - 1, nz - 1), periodic)
 
     @classmethod
     def from_cells(cls, grid, prop_cell, mode: str = "parallel") -> "DeviceOperator":
@@ -259,7 +274,7 @@ class DeviceOperator:
         check(lib.fs_op_combine(float(a), self._h, float(b), None if other is None else other._h,
                                 float(c), None if ex is None else ex.ctypes.data_as(_vp),
                                 C.byref(h)), "fs_op_combine")
-        return DeviceOperator(h, self.shape_cells)
+        return DeviceOperator(h, self.shape_cells, self.periodic)
 
     @property
     def n_nodes(self) -> int:
@@ -287,13 +302,15 @@ class DeviceOperator:
     def edges_split(self):
         """Edge weights per family, shaped (Nx, Ny+1, Nz+1), (Nx+1, Ny, Nz+1),
         (Nx+1, Ny+1, Nz)."""
-        Nx, Ny, Nz = self.shape_cells
-        wx = np.empty(Nx * (Ny + 1) * (Nz + 1))
-        wy = np.empty((Nx + 1) * Ny * (Nz + 1))
-        wz = np.empty((Nx + 1) * (Ny + 1) * Nz)
-        check(load().fs_op_get_edges(self._h, wx, wy, wz), "fs_op_get_edges")
-        return (wx.reshape(Nx, Ny + 1, Nz + 1), wy.reshape(Nx + 1, Ny, Nz + 1),
-                wz.reshape(Nx + 1, Ny + 1, Nz))
+        nodes = [c + 1 for c in self.shape_cells]
+        shapes = []
+        for a in range(3):
+            shp = list(nodes)
+            shp[a] = nodes[a] if (self.periodic >> a) & 1 else nodes[a] - 1
+            shapes.append(tuple(shp))
+        w = [np.empty(int(np.prod(shp))) for shp in shapes]
+        check(load().fs_op_get_edges(self._h, w[0], w[1], w[2]), "fs_op_get_edges")
+        return tuple(a.reshape(shp) for a, shp in zip(w, shapes))
 
     def close(self):
         if getattr(self, "_h", None):

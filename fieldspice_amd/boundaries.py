@@ -22,7 +22,7 @@ import numpy as np
 from .grid import RectilinearGrid
 
 __all__ = ["WALLS", "WALL_AXIS", "WALL_SIDE", "BC", "Dirichlet", "Neumann",
-           "Periodic", "BoundarySpec", "wall_nodes", "wall_dual_areas"]
+           "Periodic", "Symmetry", "BoundarySpec", "wall_nodes", "wall_dual_areas"]
 
 WALLS = ("xlo", "xhi", "ylo", "yhi", "zlo", "zhi")
 WALL_AXIS = {"xlo": 0, "xhi": 0, "ylo": 1, "yhi": 1, "zlo": 2, "zhi": 2}
@@ -113,6 +113,51 @@ class Neumann(BC):
     def flux_at(self, t: float = 0.0) -> np.ndarray:
         f = self.flux(float(t)) if callable(self.flux) else self.flux
         return np.asarray(f, dtype=float)
+
+
+@dataclass(frozen=True)
+class Symmetry(BC):
+    """A mirror plane (``fieldspice/boundaries.py:527-605``): ``"pmc"`` (even
+    potential, ``dphi/dn = 0``: natural, as Neumann) or ``"pec"`` (odd
+    potential, constant on the plane: Dirichlet at ``value``)."""
+
+    wall: str = "pmc"
+    value: float = 0.0
+    kind: ClassVar[str] = "symmetry"
+
+    def __post_init__(self):
+        kind = str(self.wall).lower()
+        if kind not in ("pec", "pmc"):
+            raise ValueError(f"Symmetry wall must be 'pec' or 'pmc', got {self.wall!r}")
+        object.__setattr__(self, "wall", kind)
+        try:
+            float(self.value)
+        except (TypeError, ValueError) as exc:
+            raise ValueError("Symmetry value must be a float [V]") from exc
+
+    @property
+    def is_electric(self) -> bool:
+        return self.wall == "pec"
+
+    @property
+    def is_dirichlet_like(self) -> bool:  # type: ignore[override]
+        return self.is_electric
+
+    @property
+    def is_natural(self) -> bool:  # type: ignore[override]
+        return not self.is_electric
+
+    def value_at(self, t: float = 0.0) -> np.ndarray:
+        if not self.is_electric:
+            raise ValueError("a PMC symmetry plane prescribes no potential")
+        return np.asarray(float(self.value))
+
+    def prescribed_value(self, t: float = 0.0):
+        return self.value_at(t) if self.is_electric else None
+
+    def as_scalar_bc(self) -> BC:
+        """The scalar-potential condition it stands for."""
+        return Dirichlet(float(self.value)) if self.is_electric else Neumann(0.0)
 
 
 @dataclass(frozen=True)

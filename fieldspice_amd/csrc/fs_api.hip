@@ -108,13 +108,16 @@ int download_nodes(fs_ctx* ctx, const fs::Dims& d, const double* dev, double* ho
   return FS_OK;
 }
 
-fs_op* new_op(fs_ctx* ctx, int Nx, int Ny, int Nz) {
+fs_op* new_op(fs_ctx* ctx, int Nx, int Ny, int Nz, int per = 0) {
   fs_op* op = new fs_op();
   op->ctx = ctx;
   op->Nx = Nx;
   op->Ny = Ny;
   op->Nz = Nz;
   op->d = make_dims(Nx, Ny, Nz);
+  op->d.per = per;
+  const int cells[3] = {Nx, Ny, Nz};
+  for (int a = 0; a < 3; ++a) op->ex[a] = ((per >> a) & 1) ? cells[a] + 1 : cells[a];
   return op;
 }
 
@@ -633,6 +636,39 @@ int fs_op_create_edges(fs_ctx* ctx, int Nx, int Ny, int Nz, const double* w_x,
   FS_GUARD_END
 }
 
+int fs_op_create_periodic(fs_ctx* ctx, int nx, int ny, int nz, const double* w_x,
+                          const double* w_y, const double* w_z, int periodic, fs_op** out) {
+  FS_GUARD_BEGIN
+  if (!ctx || !out || !w_x || !w_y || !w_z) return fail(FS_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (periodic < 0 || periodic > 7) return fail(FS_EINVAL, "periodic must be a mask of bits 0..2");
+  const int nodes[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; ++a)
+    if (((periodic >> a) & 1) && nodes[a] < 2)
+      return fail(FS_EINVAL, "a periodic axis needs at least 2 nodes (axis %d has %d)", a,
+                  nodes[a]);
+  if (int rc = check_dims(nx - 1, ny - 1, nz - 1)) return rc;
+  FS_HIP(hipSetDevice(ctx->device));
+  fs_op* op = new_op(ctx, nx - 1, ny - 1, nz - 1, periodic);
+  const fs::Dims& d = op->d;
+  const double* src[3] = {w_x, w_y, w_z};
+  int rc = FS_OK;
+  for (int a = 0; a < 3 && rc == FS_OK; ++a) {
+    rc = dalloc(&op->w[a], (size_t)d.n);
+    int shp[3] = {d.nx, d.ny, d.nz};
+    shp[a] = op->ex[a];
+    if (rc == FS_OK)
+      rc = upload_permuted(ctx, src[a], shp[0], shp[1], shp[2], d.nx, d.ny, d.nz, op->w[a]);
+  }
+  if (rc != FS_OK) {
+    free_op(op);
+    return rc;
+  }
+  *out = op;
+  return FS_OK;
+  FS_GUARD_END
+}
+
 int fs_op_create_cells(fs_ctx* ctx, int Nx, int Ny, int Nz, const double* hx,
                        const double* hy, const double* hz, const double* prop_cell,
                        int avg_mode, fs_op** out) {
@@ -701,11 +737,12 @@ int fs_op_combine(double a, const fs_op* A, double b, const fs_op* B, double c,
   FS_GUARD_BEGIN
   if (!A || !out) return fail(FS_EINVAL, "NULL argument");
   *out = nullptr;
-  if (B && (B->Nx != A->Nx || B->Ny != A->Ny || B->Nz != A->Nz || B->ctx != A->ctx))
-    return fail(FS_EINVAL, "operators differ in grid or context");
+  if (B && (B->Nx != A->Nx || B->Ny != A->Ny || B->Nz != A->Nz || B->ctx != A->ctx ||
+            B->d.per != A->d.per))
+    return fail(FS_EINVAL, "operators differ in grid, periodicity or context");
   fs_ctx* ctx = A->ctx;
   FS_HIP(hipSetDevice(ctx->device));
-  fs_op* op = new_op(ctx, A->Nx, A->Ny, A->Nz);
+  fs_op* op = new_op(ctx, A->Nx, A->Ny, A->Nz, A->d.per);
   const int64_t n = op->d.n;
   int rc = FS_OK;
   for (int k = 0; k < 3 && rc == FS_OK; ++k) {
@@ -778,7 +815,8 @@ int fs_op_get_edges(fs_op* op, double* w_x, double* w_y, double* w_z) {
   double* tmp = nullptr;
   if (int rc = dalloc(&tmp, (size_t)d.n)) return rc;
   double* outs[3] = {w_x, w_y, w_z};
-  const int shp[3][3] = {{op->Nx, d.ny, d.nz}, {d.nx, op->Ny, d.nz}, {d.nx, d.ny, op->Nz}};
+  const int shp[3][3] = {{op->ex[0], d.ny, d.nz}, {d.nx, op->ex[1], d.nz},
+                         {d.nx, d.ny, op->ex[2]}};
   int rc = FS_OK;
   for (int a = 0; a < 3 && rc == FS_OK; ++a) {
     fs::permute_out<double>(ctx->stream, op->w[a], tmp, shp[a][0], shp[a][1], shp[a][2], d.nx,
@@ -821,7 +859,8 @@ int fs_sys_create(fs_op* op, const uint8_t* mask, int pin, fs_sys** out, fs_sys_
                                          (int64_t)std::max(ctx->n_cu, 1) * 8);
   s->vec_blocks = std::max(s->vec_blocks, 1);
   // FP32 stencil with 4 nodes per lane when rows are a multiple of 4
-  s->use_x4 = d.nx % 4 == 0 ? 1 : 0;
+  // wrap-around edges run on the one-node-per-lane kernels only
+  s->use_x4 = d.nx % 4 == 0 && d.per == 0 ? 1 : 0;
   {
     fs::Tiling& t4 = s->tl4;
     t4.tiles_x = (d.nx / 4 + fs::kWave - 1) / fs::kWave;
@@ -834,7 +873,7 @@ int fs_sys_create(fs_op* op, const uint8_t* mask, int pin, fs_sys** out, fs_sys_
     t4.nblocks = (t4.ntiles + 7) / 8 * 8;
   }
   // FP64 stencil with 2 nodes per lane when rows are even
-  s->use_x2 = d.nx % 2 == 0 ? 1 : 0;
+  s->use_x2 = d.nx % 2 == 0 && d.per == 0 ? 1 : 0;
   {
     fs::Tiling& t2 = s->tl2;
     t2.tiles_x = (d.nx / 2 + fs::kWave - 1) / fs::kWave;
@@ -1199,6 +1238,7 @@ int fs_capacitance(fs_sys* s, const fs_op* L, int nt, const int32_t* term_id, do
 int fs_op_edge_power(fs_op* op, const double* x, double* q) {
   FS_GUARD_BEGIN
   if (!op || !x || !q) return fail(FS_EINVAL, "NULL argument");
+  if (op->d.per) return fail(FS_EINVAL, "periodic operators run fs_sys_solve and fs_capacitance only (FS_PREC_FP64 or FS_PREC_MIXED)");
   fs_ctx* ctx = op->ctx;
   FS_HIP(hipSetDevice(ctx->device));
   const fs::Dims& d = op->d;
@@ -1222,6 +1262,7 @@ int fs_transient_run(fs_sys* s, const fs_op* Aexp, const fs_transient_desc* D, d
                      const fs_solve_params* params, fs_solve_stats* stats) {
   FS_GUARD_BEGIN
   if (!s || !D || !x || !params) return fail(FS_EINVAL, "NULL argument");
+  if (s->d.per) return fail(FS_EINVAL, "periodic operators run fs_sys_solve and fs_capacitance only (FS_PREC_FP64 or FS_PREC_MIXED)");
   if (D->nsteps < 1) return fail(FS_EINVAL, "nsteps must be >= 1");
   if (D->ngroups < 0 || (D->ngroups > 0 && (!D->group || !D->values)))
     return fail(FS_EINVAL, "groups need group ids and values");
@@ -1521,6 +1562,7 @@ extern "C" {
 int fs_sys_autotune(fs_sys* s, int nr, int reps, int* kz_out) {
   FS_GUARD_BEGIN
   if (!s) return fail(FS_EINVAL, "NULL argument");
+  if (s->d.per) return fail(FS_EINVAL, "periodic operators run fs_sys_solve and fs_capacitance only (FS_PREC_FP64 or FS_PREC_MIXED)");
   if (nr < 1 || nr > fs::kMaxRhs) return fail(FS_EINVAL, "nr must be in 1..%d", fs::kMaxRhs);
   if (reps < 1) reps = 5;
   fs_ctx* ctx = s->ctx;
@@ -1673,6 +1715,7 @@ int fs_batch_add(fs_batch* b, fs_cap* cap, const fs_op* L) {
   if (!b || !cap || !L) return fail(FS_EINVAL, "NULL argument");
   fs_sys* s = cap->sys;
   if (s->ctx != b->ctx) return fail(FS_EINVAL, "system on another context");
+  if (s->d.per) return fail(FS_EINVAL, "periodic operators run fs_sys_solve and fs_capacitance only (FS_PREC_FP64 or FS_PREC_MIXED)");
   if (cap->nt > fs::kMaxRhs)
     return fail(FS_EINVAL, "a batched extraction holds at most %d terminals", fs::kMaxRhs);
   if (L->Nx != s->op->Nx || L->Ny != s->op->Ny || L->Nz != s->op->Nz)

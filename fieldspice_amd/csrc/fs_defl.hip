@@ -693,6 +693,7 @@ int fs_sys_find_islands(fs_sys* sys, double contrast, int max_islands, int32_t* 
   FS_GUARD_BEGIN
   if (!sys || !island || !n_islands) return fail(FS_EINVAL, "NULL argument");
   if (!sys->op) return fail(FS_EINVAL, "the system's operator is not available");
+  if (sys->d.per) return fail(FS_EINVAL, "periodic operators run fs_sys_solve and fs_capacitance only (FS_PREC_FP64 or FS_PREC_MIXED)");
   if (max_islands < 1 || max_islands > kMaxIslands)
     return fail(FS_EINVAL, "max_islands must lie in [1, %d], got %d", kMaxIslands, max_islands);
   if (!(contrast >= 1.0)) return fail(FS_EINVAL, "contrast must be >= 1, got %g", contrast);
@@ -750,6 +751,7 @@ int fs_sys_find_islands(fs_sys* sys, double contrast, int max_islands, int32_t* 
 int fs_sys_set_deflation(fs_sys* sys, int n_islands, const int32_t* island) {
   FS_GUARD_BEGIN
   if (!sys) return fail(FS_EINVAL, "NULL argument");
+  if (sys->d.per && n_islands > 0) return fail(FS_EINVAL, "periodic operators run fs_sys_solve and fs_capacitance only (FS_PREC_FP64 or FS_PREC_MIXED)");
   FS_HIP(hipSetDevice(sys->ctx->device));
   if (n_islands == 0 || !island) {
     FS_HIP(hipStreamSynchronize(sys->ctx->stream));

@@ -89,6 +89,38 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ pr, in
 // ----------------------------------------------------------------------------
 // raw operator row: (A x)_n = sum_e w_e (x_n - x_m) + extra_n x_n   (FP64)
 // ----------------------------------------------------------------------------
+// The six faces of node n = (i, j, k), in the order +x, -x, +y, -y, +z, -z:
+// m[f] the neighbour across face f (-1: none), e[f] the edge slot holding its
+// weight (the slot of the face's lower node).  Along an axis in d.per the
+// last node's + face and the first node's - face are the wrap-around edge,
+// whose weight sits in the last node's slot.
+struct Faces {
+  int64_t m[6];
+  int64_t e[6];
+};
+
+__device__ __forceinline__ Faces faces_of(const Dims& d, int i, int j, int k, int64_t n) {
+  Faces f;
+  const int pos[3] = {i, j, k};
+  const int len[3] = {d.nx, d.ny, d.nz};
+  const int64_t st[3] = {1, (int64_t)d.nx, d.sxy};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const bool wrap = ((d.per >> a) & 1) && len[a] > 1;
+    const int64_t span = (int64_t)(len[a] - 1) * st[a];
+    f.e[2 * a] = n;
+    f.m[2 * a] = pos[a] + 1 < len[a] ? n + st[a] : (wrap ? n - span : -1);
+    if (pos[a] > 0) {
+      f.m[2 * a + 1] = n - st[a];
+      f.e[2 * a + 1] = n - st[a];
+    } else {
+      f.m[2 * a + 1] = wrap ? n + span : -1;
+      f.e[2 * a + 1] = wrap ? n + span : n;
+    }
+  }
+  return f;
+}
+
 __device__ __forceinline__ double raw_row(const double* __restrict__ x,
                                           const double* __restrict__ wx,
                                           const double* __restrict__ wy,
@@ -96,13 +128,12 @@ __device__ __forceinline__ double raw_row(const double* __restrict__ x,
                                           const double* __restrict__ extra, const Dims& d,
                                           int i, int j, int k, int64_t n) {
   const double xc = x[n];
+  const double* w[3] = {wx, wy, wz};
+  const Faces f = faces_of(d, i, j, k, n);
   double y = 0.0;
-  if (i + 1 < d.nx) y += wx[n] * (xc - x[n + 1]);
-  if (i > 0) y += wx[n - 1] * (xc - x[n - 1]);
-  if (j + 1 < d.ny) y += wy[n] * (xc - x[n + d.nx]);
-  if (j > 0) y += wy[n - d.nx] * (xc - x[n - d.nx]);
-  if (k + 1 < d.nz) y += wz[n] * (xc - x[n + d.sxy]);
-  if (k > 0) y += wz[n - d.sxy] * (xc - x[n - d.sxy]);
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+    if (f.m[q] >= 0) y += w[q >> 1][f.e[q]] * (xc - x[f.m[q]]);
   if (extra) y += extra[n] * xc;
   return y;
 }

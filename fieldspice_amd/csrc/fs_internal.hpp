@@ -29,6 +29,8 @@ struct Dims {
   int nx, ny, nz;
   int64_t n;       // nx*ny*nz
   int64_t sxy;     // nx*ny
+  int per = 0;     // axes that wrap around (bit a for axis a): the last node
+                   // couples to the first through the last edge slot
 };
 
 // Tile decomposition of the stencil kernels.  Block b of the launch is
@@ -80,6 +82,7 @@ struct fs_ctx {
 struct fs_op {
   fs_ctx* ctx = nullptr;
   int Nx = 0, Ny = 0, Nz = 0;  // cells
+  int ex[3] = {0, 0, 0};       // edges per line along each axis (cells, or nodes when periodic)
   fs::Dims d{};
   double* w[3] = {nullptr, nullptr, nullptr};  // node-padded edge weights (x-fast)
   double* extra = nullptr;                     // node diagonal, or null

@@ -224,13 +224,13 @@ __global__ __launch_bounds__(kBlock) void k_setup_diag(const fs_op op, uint8_t* 
     // each first), as the diagonal of its G^T W G product accumulates, so
     // the diagonal, its maximum (the gauge node) and the equilibration
     // match the reference to the last bit
+    const Faces f = faces_of(d, i, j, k, n);
     double ws = 0.0;
-    if (i > 0) ws = ws + op.w[0][n - 1];
-    ws = ws + op.w[0][n];
-    if (j > 0) ws = ws + op.w[1][n - d.nx];
-    ws = ws + op.w[1][n];
-    if (k > 0) ws = ws + op.w[2][n - d.sxy];
-    ws = ws + op.w[2][n];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (f.m[2 * a + 1] >= 0) ws = ws + op.w[a][f.e[2 * a + 1]];
+      if (f.m[2 * a] >= 0) ws = ws + op.w[a][f.e[2 * a]];
+    }
     const double ex = op.extra ? op.extra[n] : 0.0;
     const double D = ws + ex;
     dsum[n] = D;
@@ -310,9 +310,10 @@ __global__ __launch_bounds__(kBlock) void k_setup_couplings(const fs_op op,
   int i, j, k;
   node_coords(d, n, i, j, k);
   const double s = dinv[n];
-  const double vx = (i + 1 < d.nx) ? s * (op.w[0][n] * dinv[n + 1]) : 0.0;
-  const double vy = (j + 1 < d.ny) ? s * (op.w[1][n] * dinv[n + d.nx]) : 0.0;
-  const double vz = (k + 1 < d.nz) ? s * (op.w[2][n] * dinv[n + d.sxy]) : 0.0;
+  const Faces f = faces_of(d, i, j, k, n);  // + faces: the edge in slot n
+  const double vx = f.m[0] >= 0 ? s * (op.w[0][n] * dinv[f.m[0]]) : 0.0;
+  const double vy = f.m[2] >= 0 ? s * (op.w[1][n] * dinv[f.m[2]]) : 0.0;
+  const double vz = f.m[4] >= 0 ? s * (op.w[2][n] * dinv[f.m[4]]) : 0.0;
   cx[n] = vx;
   cy[n] = vy;
   cz[n] = vz;
@@ -397,6 +398,10 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot(const T* __restrict__ p,
     if (i < d.nx && j < d.ny) {
       const int64_t base = (int64_t)j * d.nx + i;
       const bool has_e = i + 1 < d.nx, has_w = i > 0, has_n = j + 1 < d.ny, has_s = j > 0;
+      // wrap-around neighbours along periodic axes (offsets from the node)
+      const bool px = (d.per & 1) && d.nx > 1, py = (d.per & 2) && d.ny > 1,
+                 pz = (d.per & 4) && d.nz > 1;
+      const int64_t ox = d.nx - 1, oy = (int64_t)(d.ny - 1) * d.nx, oz = (d.nz - 1) * d.sxy;
       // One right-hand side at a time through the tile: the tile's couplings
       // come from HBM for the first live RHS and from L1/L2 for the others,
       // and each pass is the register-light single-RHS z-march.
@@ -406,19 +411,19 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot(const T* __restrict__ p,
         const T* __restrict__ pr = p + r * va.vs;
         T* __restrict__ qr = q + r * va.vs;
         int64_t nn = (int64_t)k0 * d.sxy + base;
-        T czm = k0 > 0 ? cz[nn - d.sxy] : T(0);
-        T pm = k0 > 0 ? pr[nn - d.sxy] : T(0);
+        T czm = k0 > 0 ? cz[nn - d.sxy] : (pz ? cz[nn + oz] : T(0));
+        T pm = k0 > 0 ? pr[nn - d.sxy] : (pz ? pr[nn + oz] : T(0));
         T pc = pr[nn];
         double a = 0.0;
         for (int k = k0; k < k1; ++k, nn += d.sxy) {
           const T cxc = cx[nn], cyc = cy[nn], czc = cz[nn];
-          const T cxm = has_w ? cx[nn - 1] : T(0);
-          const T cym = has_s ? cy[nn - d.nx] : T(0);
-          const T pu = k + 1 < d.nz ? pr[nn + d.sxy] : T(0);
-          const T pe = has_e ? pr[nn + 1] : T(0);
-          const T pw = has_w ? pr[nn - 1] : T(0);
-          const T pn = has_n ? pr[nn + d.nx] : T(0);
-          const T ps = has_s ? pr[nn - d.nx] : T(0);
+          const T cxm = has_w ? cx[nn - 1] : (px ? cx[nn + ox] : T(0));
+          const T cym = has_s ? cy[nn - d.nx] : (py ? cy[nn + oy] : T(0));
+          const T pu = k + 1 < d.nz ? pr[nn + d.sxy] : (pz ? pr[nn - oz] : T(0));
+          const T pe = has_e ? pr[nn + 1] : (px ? pr[nn - ox] : T(0));
+          const T pw = has_w ? pr[nn - 1] : (px ? pr[nn + ox] : T(0));
+          const T pn = has_n ? pr[nn + d.nx] : (py ? pr[nn - oy] : T(0));
+          const T ps = has_s ? pr[nn - d.nx] : (py ? pr[nn + oy] : T(0));
           const T y = pc - (cxc * pe + cxm * pw + cyc * pn + cym * ps + czc * pu + czm * pm);
           qr[nn] = y;
           a += (double)pc * (double)y;
@@ -918,12 +923,12 @@ __global__ __launch_bounds__(kBlock) void k_lift_cap(const fs_op op, const int32
     }
     int tn[6];
     double wn[6];
-    tn[0] = i + 1 < d.nx ? tid[n + 1] : -1;        wn[0] = op.w[0][n];
-    tn[1] = i > 0 ? tid[n - 1] : -1;               wn[1] = i > 0 ? op.w[0][n - 1] : 0.0;
-    tn[2] = j + 1 < d.ny ? tid[n + d.nx] : -1;     wn[2] = op.w[1][n];
-    tn[3] = j > 0 ? tid[n - d.nx] : -1;            wn[3] = j > 0 ? op.w[1][n - d.nx] : 0.0;
-    tn[4] = k + 1 < d.nz ? tid[n + d.sxy] : -1;    wn[4] = op.w[2][n];
-    tn[5] = k > 0 ? tid[n - d.sxy] : -1;           wn[5] = k > 0 ? op.w[2][n - d.sxy] : 0.0;
+    const Faces f = faces_of(d, i, j, k, n);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      tn[q] = f.m[q] >= 0 ? tid[f.m[q]] : -1;
+      wn[q] = f.m[q] >= 0 ? op.w[q >> 1][f.e[q]] : 0.0;
+    }
     const double di = dinv[n];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -1021,22 +1026,20 @@ __global__ __launch_bounds__(kBlock) void k_residual(const double* __restrict__ 
   for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < va.n; n += stride) {
     int i, j, k;
     node_coords(d, n, i, j, k);
-    const double cxc = cx[n], cyc = cy[n], czc = cz[n];
-    const double cxm = i > 0 ? cx[n - 1] : 0.0;
-    const double cym = j > 0 ? cy[n - d.nx] : 0.0;
-    const double czm = k > 0 ? cz[n - d.sxy] : 0.0;
+    const Faces f = faces_of(d, i, j, k, n);
+    const double* c[3] = {cx, cy, cz};
+    double cf[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) cf[q] = f.m[q] >= 0 ? c[q >> 1][f.e[q]] : 0.0;
     const double dh = dhalf[n];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       if (r >= va.nr) continue;
       const double* xr = x + r * va.vs;
       double s = 0.0;
-      if (i + 1 < d.nx) s += cxc * xr[n + 1];
-      if (i > 0) s += cxm * xr[n - 1];
-      if (j + 1 < d.ny) s += cyc * xr[n + d.nx];
-      if (j > 0) s += cym * xr[n - d.nx];
-      if (k + 1 < d.nz) s += czc * xr[n + d.sxy];
-      if (k > 0) s += czm * xr[n - d.sxy];
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+        if (f.m[q] >= 0) s += cf[q] * xr[f.m[q]];
       // Dirichlet and pinned rows are identity rows with x_s = b_s = 0 there
       const double rv = bs[r * va.vs + n] - (xr[n] - s);
       res[r * va.vs + n] = rv;

@@ -655,6 +655,7 @@ int fs_pc_create(fs_sys* sys, fs_net* net, int nlevels, const int32_t* groups, d
   FS_GUARD_BEGIN
   if (!sys || !out || (!groups && nlevels > 0)) return fail(FS_EINVAL, "NULL argument");
   *out = nullptr;
+  if (sys->d.per) return fail(FS_EINVAL, "periodic operators run fs_sys_solve and fs_capacitance only (FS_PREC_FP64 or FS_PREC_MIXED)");
   if (nlevels < 1 || nlevels > 12) return fail(FS_EINVAL, "nlevels must be in 1..12");
   if (!(omega > 0.0 && omega < 2.0)) return fail(FS_EINVAL, "omega must be in (0, 2)");
   if (net && net->ctx != sys->ctx) return fail(FS_EINVAL, "network and system on different contexts");

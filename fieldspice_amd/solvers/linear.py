@@ -73,6 +73,10 @@ class GridLinearSystem:
                 mask[f.ravel()] = 1
             else:
                 mask[f.astype(np.intp)] = 1
+        if getattr(op, "periodic", 0) and (self.precision == _lib.FS_PREC_FCG
+                                           or self.cfg.deflate_islands):
+            raise ValueError("periodic walls run with hip_pcg or hip_mixed (no learned "
+                             "preconditioner, no island space)")
         self.n = n
         self.op = op
         self.sys = _lib.DeviceSystem(op, mask, pin=pin)

@@ -123,6 +123,20 @@ int fs_op_create_cells(fs_ctx* ctx, int Nx, int Ny, int Nz,
                        const double* hx, const double* hy, const double* hz,
                        const double* prop_cell, int avg_mode, fs_op** out);
 
+/* Operator of a grid whose opposite walls are identified along the axes in
+ * `periodic` (bit 0 x, 1 y, 2 z): the merged node planes of fieldspice's
+ * periodic reduction (fieldspice/solvers/poisson.py:665-719,942-956:
+ * P^T L P with each wall pair one unknown), given on the reduced grid of
+ * nx*ny*nz nodes (a periodic axis keeps one of its two wall planes, so nx is
+ * the cell count along a periodic x).  Edge arrays in C order: along a
+ * periodic axis there are as many edges as nodes, the last one joining the
+ * last node to the first; along the others one fewer.  A periodic axis needs
+ * >= 2 nodes.  Such operators run fs_op_apply, fs_sys_create, fs_sys_solve
+ * and fs_capacitance (FS_PREC_FP64, FS_PREC_MIXED); the other entry points
+ * reject them. */
+int fs_op_create_periodic(fs_ctx* ctx, int nx, int ny, int nz, const double* w_x,
+                          const double* w_y, const double* w_z, int periodic, fs_op** out);
+
 /* out = a*A + b*B + diag(c*extra): the EQS/thermal step matrices
  * (fieldspice/solvers/eqs.py:279-280, thermal.py:259-263).  B and extra may
  * be NULL. */

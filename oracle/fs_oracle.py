@@ -578,14 +578,15 @@ def boltzmann_charge(doping_node, ni, T=300.0):
 
 def nonlinear_poisson(grid, eps_cell, rho, drho, fixed=None, vals=None, rho_node=None,
                       phi0=None, v_scale=None, step_limit=5.0, armijo=1e-4, newton_tol=1e-8,
-                      max_newton=60, damping=1.0, floating=(), charges=()):
+                      max_newton=60, damping=1.0, floating=(), charges=(), merge=()):
     """Damped Newton for L phi - rho(phi) V_node = rho_node V_node with
     Dirichlet rows ``fixed`` at ``vals`` (fieldspice/solvers/poisson.py:1429-1511):
     the Jacobian L - diag(drho V_node) is eliminated and solved by SuperLU each
     step, the first trial step is clamped to step_limit*v_scale, then Armijo
     backtracking.  Node sets in ``floating`` are merged into one unknown each,
-    carrying ``charges`` (the reference's reduction, :1432-1451).  Returns
-    (phi, history of |F|, damping factors)."""
+    carrying ``charges`` (the reference's reduction, :1432-1451), and so are
+    the sets in ``merge`` (periodic wall pairs).  Returns (phi, history of
+    |F|, damping factors)."""
     L = nodal_laplacian(grid, cell_to_edge(grid, eps_cell))
     vnode = np.asarray(grid.node_volumes(), dtype=float).ravel()
     n = grid.n_nodes
@@ -593,13 +594,9 @@ def nonlinear_poisson(grid, eps_cell, rho, drho, fixed=None, vals=None, rho_node
     vals = np.empty(0) if vals is None else np.asarray(vals, dtype=float)
     b_ext = np.zeros(n) if rho_node is None else np.asarray(rho_node, float).ravel() * vnode
     b_ext = b_ext.copy()
-    label = np.arange(n)
     for g, q in zip(floating, charges):
-        g = np.asarray(g, dtype=np.intp)
-        label[g] = g[0]
-        b_ext[g[0]] += float(q)
-    _, group = np.unique(label, return_inverse=True)
-    P = sp.csr_matrix((np.ones(n), (np.arange(n), group)), shape=(n, int(group.max()) + 1))
+        b_ext[np.asarray(g, dtype=np.intp)[0]] += float(q)
+    group, P = merge_map(n, list(floating) + list(merge))
     fixed_r = group[fixed]
     v_scale = KB * 300.0 / Q_ELEM if v_scale is None else v_scale
     phi = np.zeros(n) if phi0 is None else np.asarray(phi0, dtype=float).ravel().copy()
@@ -689,19 +686,7 @@ def poisson_floating(grid, eps_cell, fixed, vals, floating, charges, b_free=None
     solver's info)."""
     L = nodal_laplacian(grid, cell_to_edge(grid, eps_cell, "parallel"))
     n = grid.n_nodes
-    label = np.arange(n)
-    sets = [np.asarray(g, dtype=np.intp) for g in list(floating) + list(merge)]
-    changed = True
-    while changed:                       # union of overlapping sets
-        changed = False
-        for g in sets:
-            lo = label[g].min()
-            if np.any(label[g] != lo):
-                label[np.isin(label, label[g])] = lo
-                changed = True
-    _, group = np.unique(label, return_inverse=True)
-    nred = int(group.max()) + 1
-    P = sp.csr_matrix((np.ones(n), (np.arange(n), group)), shape=(n, nred))
+    group, P = merge_map(n, list(floating) + list(merge))
     b = np.zeros(n) if b_free is None else np.asarray(b_free, dtype=float).copy()
     for g, q in zip(floating, charges):
         b[np.asarray(g, dtype=np.intp)[0]] += float(q)
@@ -716,3 +701,73 @@ def poisson_floating(grid, eps_cell, fixed, vals, floating, charges, b_free=None
         info["group"] = group
         return phi, info
     return phi
+
+
+def merge_map(n, sets):
+    """Node merge of ``fieldspice/solvers/poisson.py:683-711``: the union of
+    the node sets as connected components, numbered in the order of each
+    component's smallest node; returns (reduced index per node, P)."""
+    label = np.arange(n)
+    sets = [np.asarray(g, dtype=np.intp) for g in sets]
+    changed = True
+    while changed:                       # union of overlapping sets
+        changed = False
+        for g in sets:
+            lo = label[g].min()
+            if np.any(label[g] != lo):
+                label[np.isin(label, label[g])] = lo
+                changed = True
+    _, group = np.unique(label, return_inverse=True)
+    nred = int(group.max()) + 1
+    return group, sp.csr_matrix((np.ones(n), (np.arange(n), group)), shape=(n, nred))
+
+
+def periodic_pairs(grid, axis):
+    """(lo, hi) node indices of the two walls normal to ``axis``, paired
+    position by position (``fieldspice/boundaries.py:226-268``)."""
+    nid = np.arange(grid.n_nodes).reshape(grid.shape_nodes)
+    return (np.take(nid, 0, axis=axis).ravel(), np.take(nid, -1, axis=axis).ravel())
+
+
+def periodic_sets(grid, axes):
+    """Merge sets identifying the wall pairs of every axis in ``axes``."""
+    out = []
+    for a in axes:
+        lo, hi = periodic_pairs(grid, a)
+        out.extend(np.array([x, y]) for x, y in zip(lo, hi))
+    return out
+
+
+def capacitance_matrix_merged(grid, eps_cell, terminal_nodes, merge, wall_fixed=None):
+    """Maxwell C with node sets merged (periodic wall pairs), as
+    ``fieldspice/solvers/poisson.py:1185-1249`` runs it: ``P^T L P`` with
+    every unknown holding a terminal or wall node fixed, excitation j setting
+    the unknowns of terminal j's nodes to 1, charges ``L phi`` summed over each
+    terminal's own nodes on the full grid.  Solved directly."""
+    L = nodal_laplacian(grid, cell_to_edge(grid, eps_cell, "parallel"))
+    n = grid.n_nodes
+    group, P = merge_map(n, merge)
+    A_r = sp.csr_matrix(P.T @ L @ P)
+    wall_idx = (np.empty(0, dtype=np.intp) if wall_fixed is None
+                else np.asarray(wall_fixed, dtype=np.intp))
+    term_nodes = np.concatenate(terminal_nodes)
+    fixed, _ = unique_last(np.concatenate([wall_idx, term_nodes]),
+                           np.zeros(wall_idx.size + term_nodes.size))
+    fixed_r = group[fixed]
+    shield = np.setdiff1d(wall_idx, term_nodes)
+    A_bc, _ = apply_dirichlet(A_r, np.zeros(A_r.shape[0]), fixed_r, np.zeros(fixed_r.size))
+    ls = LinearSystem(A_bc, method="direct")
+    nt = len(terminal_nodes)
+    C = np.zeros((nt, nt))
+    qsh = np.zeros(nt)
+    for j, tj in enumerate(terminal_nodes):
+        x0 = np.zeros(A_r.shape[0])
+        x0[group[tj]] = 1.0
+        b = -(A_r @ x0)
+        b[fixed_r] = x0[fixed_r]
+        q = L @ (P @ ls.solve(b))
+        for i, ti in enumerate(terminal_nodes):
+            C[i, j] = float(q[ti].sum())
+        if shield.size:
+            qsh[j] = float(q[shield].sum())
+    return 0.5 * (C + C.T), qsh

@@ -548,3 +548,51 @@ def test_nonlinear_poisson_floating_merge_pins():
     assert float(q[flt].sum()) == pytest.approx(Qf, rel=1e-6)
     assert np.ptp(po[flt] - guess[flt]) < 1e-12 * max(1.0, np.abs(po).max())
     assert hist[-1] < 1e-8 * hist[0]
+
+
+def test_periodic_merge_plate_and_translation_invariance():
+    """The oracle's periodic merge (the reference's wall-pair reduction):
+    (1) periodic side walls leave a parallel plate's uniform field alone,
+    C = eps A / d; (2) on a geometry periodic in x the capacitance matrix is
+    invariant under a cyclic shift of materials and electrodes along x, which
+    a pairing of the wrong nodes breaks."""
+    d, W, H = 2e-6, 1.5e-6, 1e-6
+    g = RectilinearGrid(np.linspace(0, d, 9), np.linspace(0, W, 7), np.linspace(0, H, 5))
+    nid = g.node_ids()
+    eps = np.full(g.shape_cells, 3.9 * eps0)
+    C, _ = O.capacitance_matrix_merged(g, eps, [nid[0].ravel(), nid[-1].ravel()],
+                                       O.periodic_sets(g, (1, 2)))
+    assert C[0, 0] == pytest.approx(3.9 * eps0 * W * H / d, rel=1e-12)
+    assert C[0, 1] == pytest.approx(-C[0, 0], rel=1e-12)
+    # a torus of Nx = 10 node planes along x
+    Nx = 10
+    g = RectilinearGrid(np.linspace(0, 1e-6, Nx + 1), np.linspace(0, 0.8e-6, 7),
+                        np.linspace(0, 0.6e-6, 6))
+    nid = g.node_ids()
+    rng = np.random.default_rng(5)
+    eps0_cells = rng.uniform(2.0, 6.0, g.shape_cells) * eps0
+
+    def box(x0, x1, y0, y1, z0, z1):
+        xs = [i % Nx for i in range(x0, x1 + 1)]
+        full = sorted(set(xs) | ({Nx} if 0 in xs else set()))
+        return nid[full][:, y0:y1 + 1, z0:z1 + 1].ravel()
+    merge = O.periodic_sets(g, (0,))
+    base = None
+    for shift in (0, 3, 7):
+        terms = [box(8 + shift, 11 + shift, 1, 3, 1, 2), box(3 + shift, 5 + shift, 3, 5, 2, 4)]
+        e = np.roll(eps0_cells, shift, axis=0)
+        C, _ = O.capacitance_matrix_merged(g, e, terms, merge,
+                                           wall_fixed=nid[:, :, 0].ravel())
+        if base is None:
+            base = C
+        assert np.abs(C - base).max() <= 1e-12 * np.abs(base).max(), shift
+    # a wrong pairing (the high wall identified with the plane next to the low one)
+    lo, hi = O.periodic_pairs(g, 0)
+    wrong = [np.array([a + g.shape_nodes[1] * g.shape_nodes[2], b]) for a, b in zip(lo, hi)]
+    Cs = []
+    for shift in (0, 3):
+        terms = [box(8 + shift, 11 + shift, 1, 3, 1, 2), box(3 + shift, 5 + shift, 3, 5, 2, 4)]
+        e = np.roll(eps0_cells, shift, axis=0)
+        Cs.append(O.capacitance_matrix_merged(g, e, terms, wrong,
+                                              wall_fixed=nid[:, :, 0].ravel())[0])
+    assert np.abs(Cs[0] - Cs[1]).max() > 1e-6 * np.abs(Cs[0]).max()
