@@ -7,8 +7,9 @@ from .poisson import NonlinearPoissonSolver, PoissonSolver, boltzmann_equilibriu
 from .eqs import EQSSolver
 from .thermal import ThermalSolver, joule_heating_nodes
 from .electrothermal import ElectroThermalSolver, ThermalRunaway, runaway_threshold
+from . import checks
 
 __all__ = ["ConvergenceError", "Result", "SolverConfig", "Terminal", "GridLinearSystem",
            "PoissonSolver", "NonlinearPoissonSolver", "boltzmann_equilibrium_charge",
            "EQSSolver", "ThermalSolver", "joule_heating_nodes", "ElectroThermalSolver",
-           "ThermalRunaway", "runaway_threshold"]
+           "ThermalRunaway", "runaway_threshold", "checks"]

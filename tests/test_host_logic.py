@@ -382,3 +382,23 @@ def test_array_valued_wall_groups():
         ri, rv = O.unique_last(np.concatenate([term.nodes, w_idx]),
                                np.concatenate([np.full(term.nodes.size, 2.0), w_val]))
         assert np.array_equal(idx, ri) and np.array_equal(vals, rv)
+
+
+def test_check_meshes_are_graded_and_closed():
+    """The closed-form checks' meshes: fine and uniform inside the feature,
+    growth at most the requested ratio (except a shorter closing cell), cells
+    capped, endpoints exact; the junction mesh is finest at x = 0."""
+    from fieldspice_amd.solvers import checks
+    for pad in (2.0, 3.0, 5.0, 9.0):
+        R, a = pad * 1e-6, 1e-6
+        ax = checks._graded_axis(R, a, a / 14, max(a, R / 6), 1.25)
+        d = np.diff(ax)
+        assert ax[0] == 0.0 and ax[-1] == R and np.all(d > 0)
+        assert np.allclose(d[:14], a / 14) and ax[14] == pytest.approx(a)
+        assert d.max() <= max(a, R / 6) * (1 + 1e-12)
+        assert np.all(d[1:-2] / d[:-3] <= 1.25 + 1e-12)
+    x = checks._junction_axis(3e-7, 5e-7, 1e-9, 4e-8, 1.25)
+    assert x[0] == pytest.approx(-3e-7) and x[-1] == pytest.approx(5e-7)
+    assert 0.0 in x and np.all(np.diff(x) > 0)
+    k = int(np.flatnonzero(x == 0.0)[0])
+    assert np.diff(x)[k - 1] == pytest.approx(1e-9) and np.diff(x)[k] == pytest.approx(1e-9)
