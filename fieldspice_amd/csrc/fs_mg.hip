@@ -541,15 +541,35 @@ __global__ __launch_bounds__(kBlock) void k_restrict_l0(
 // -------------------------------------------------------------------------
 // level 0 with 4 consecutive x nodes per lane (rows a multiple of 4): one
 // 16-byte load per array and plane, every right-hand side advanced in the
-// same z-march so couplings and masks are read once per node
+// same z-march so couplings and masks are read once per node.
+//
+// Both kernels issue every global load of a plane before the first use of
+// any of them and keep loads out of lane-divergent branches: lanes and rows
+// outside the grid read the grid's last node (in bounds) and store nothing,
+// and the two x-halo lanes select their neighbour from one load that every
+// lane makes.  A load under a divergent branch, merged into a default value,
+// makes the compiler wait for it at the merge, which serialised five or more
+// memory round trips per plane (256^3, 5 RHS: prolongation 418 -> 280 us,
+// restriction 185 -> 144 us).  Node offsets are 32-bit: the set-up selects
+// these kernels only when n + nx*ny < 2^31.
 // -------------------------------------------------------------------------
 __device__ __forceinline__ float4 f4(float a, float b, float c, float d) {
   return make_float4(a, b, c, d);
 }
 __device__ __forceinline__ float4 ldf4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ float2 ldf2(const float* p) { return *reinterpret_cast<const float2*>(p); }
+// component-wise select of two float4 values (the conditional operator on the
+// vector struct selects an address and keeps both operands in scratch memory)
+__device__ __forceinline__ float4 pick(bool c, float4 a, float4 b) {
+  return make_float4(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w);
+}
 
-// DIAG: a coarse level (diagonal dgv, inverse diagonal dinvv, activity
-// dinvv > 0) instead of level 0 (unit diagonal, Dirichlet mask)
+// e = z2 + omega D^-1 (r - D z2 + C z2), z2 = omega D^-1 r + ec[P(n)], on
+// active nodes; optional partials <e, r>, <e, q>.  DIAG: a coarse level
+// (diagonal dgv, inverse diagonal dinvv, activity dinvv > 0) instead of level
+// 0 (unit diagonal, Dirichlet mask).  The smoothed iterate of the rows above
+// and below comes from the neighbouring waves of the block through LDS; the
+// block's two edge rows read the one row outside it.
 template <int NR, bool DIAG>
 __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
     const float* __restrict__ r, const float* __restrict__ ec, int64_t vsc,
@@ -558,134 +578,145 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
     const float* __restrict__ dgv, const float* __restrict__ dinvv, Dims d, Tiling tl, Map m,
     int64_t vs, int nr, float omega, const float* __restrict__ q,
     double* __restrict__ partials, const CgScalars* sc, int mode, int b0, int nrt) {
-  // this launch carries right-hand sides b0 .. b0+nr-1 of nrt (vector
-  // pointers already offset to b0); partials rows b0+b and nrt+b0+b
   double acc[2 * NR];
 #pragma unroll
   for (int a = 0; a < 2 * NR; ++a) acc[a] = 0.0;
   bool live[NR];
 #pragma unroll
   for (int b = 0; b < NR; ++b) live[b] = b < nr && live_of(sc, b0 + b, mode);
-  // the smoothed iterate of each wave's row at the current plane, for the
-  // waves of the rows above and below (two planes, one barrier per plane)
   __shared__ float4 zsh[2][NR][kTileY][kWave];
   int tx, ty, tz;
   if (tile_of_block(tl, tx, ty, tz)) {  // uniform over the block
     const int lane = (int)(threadIdx.x % kWave);
-    const int ly = (int)(threadIdx.x / kWave);
+    const int ly = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
     const int i = (tx * kWave + lane) * 4;
-    const int j = ty * kTileY + ly;
+    const int j = ty * kTileY + ly;  // uniform over the wave
+    const bool valid = i < d.nx && j < d.ny;
+    // lanes and rows outside the grid read the last node's data and store nothing
+    const int ic = min(i, d.nx - 4), jc = min(j, d.ny - 1);
+    const bool has_w = ic > 0, has_e = ic + 4 < d.nx, has_n = jc + 1 < d.ny, has_s = jc > 0;
+    // the x halo: lane 0 reads its west node, the last lane its east node
+    const int xo = (lane == 0 && has_w) ? -1 : ((lane == kWave - 1 && has_e) ? 4 : 0);
+    // the one row outside the block that an edge wave reads: south for the
+    // first wave, north for the last (the other rows come through LDS)
+    const bool own_n = ly + 1 == kTileY, own_s = ly == 0;
+    const bool edge = (own_s && has_s) || (own_n && has_n);
+    const int sxy = (int)d.sxy, nx = d.nx;
+    const int oh = own_s ? -nx : nx;
+    const int os = has_s ? -nx : 0;
+    const int cnx = m.cn[0], cny = m.cn[1];
+    const int ci0 = ic >> 1;
+    const int cih = xo < 0 ? (ic - 1) >> 1 : (xo > 0 ? (ic + 4) >> 1 : ci0);
+    const int cj = jc >> 1;
+    const int cjh = own_s ? (jc - 1) >> 1 : (jc + 1) >> 1;  // used on edge waves only
+    auto z2 = [&](float4 rr, float4 dd, float2 ee) {
+      if constexpr (DIAG)
+        return f4(omega * dd.x * rr.x + ee.x, omega * dd.y * rr.y + ee.x,
+                  omega * dd.z * rr.z + ee.y, omega * dd.w * rr.w + ee.y);
+      else
+        return f4(omega * rr.x + ee.x, omega * rr.y + ee.x, omega * rr.z + ee.y,
+                  omega * rr.w + ee.y);
+    };
+    const float4 z4 = f4(0.f, 0.f, 0.f, 0.f);
+    const float4 one4 = f4(1.f, 1.f, 1.f, 1.f);
     const int k0 = tz * tl.kz;
     const int k1 = min(d.nz, k0 + tl.kz);
-    const bool valid = i < d.nx && j < d.ny;
-    const int64_t base = valid ? (int64_t)j * d.nx + i : 0;
-    const bool has_w = i > 0, has_e = i + 4 < d.nx, has_n = j + 1 < d.ny, has_s = j > 0;
-    // x and y are grouped in pairs on this path (the level-0 set-up selects
-    // it only then): the lane's nodes i..i+3 lie in coarse columns i/2 and
-    // i/2+1, one aligned 8-byte load per coarse row (i/2 and cnx are even)
-    const int cnx = m.cn[0], cny = m.cn[1];
-    const int ci0 = i >> 1, ciw = (i - 1) >> 1, cie = (i + 4) >> 1;
-    const int cj = j >> 1, cjn = has_n ? (j + 1) >> 1 : cj, cjs = has_s ? (j - 1) >> 1 : cj;
-    // z2 = omega D^-1 r + ec[P(n)] of the 4 nodes of row (jj, plane ck)
-    auto z2row = [&](const float* eb, float4 rr, float4 dd, int64_t rowbase) {
-      const float2 e = *reinterpret_cast<const float2*>(eb + rowbase + ci0);
-      if constexpr (DIAG)
-        return f4(omega * dd.x * rr.x + e.x, omega * dd.y * rr.y + e.x,
-                  omega * dd.z * rr.z + e.y, omega * dd.w * rr.w + e.y);
-      else
-        return f4(omega * rr.x + e.x, omega * rr.y + e.x, omega * rr.z + e.y, omega * rr.w + e.y);
-    };
-    const float4 one4 = f4(1.f, 1.f, 1.f, 1.f);
-    // inverse diagonal of rows (plane k, row j) and (k-1, j); 1 on level 0
-    float4 dcur = one4, dprev = one4;
-    int64_t nn = (int64_t)k0 * d.sxy + base;
-    const float4 z4 = f4(0.f, 0.f, 0.f, 0.f);
+    int nn = k0 * sxy + jc * nx + ic;
+    int ck = __builtin_amdgcn_readfirstlane(m.f2c[2][k0]);
+    // state of each live right-hand side: z2 at planes k-1 and k, r at plane k
     float4 zm[NR], zc[NR], rc[NR];
-    int ck = valid ? m.f2c[2][k0] : 0;
-    float4 czm = z4;
-    if (valid && k0 > 0) czm = ldf4(cz + nn - d.sxy);
-    if constexpr (DIAG) {
-      if (valid) {
+    float4 czm = z4, dcur = one4;
+    {
+      const int kd = k0 > 0 ? k0 - 1 : k0;
+      const int nd = nn - (k0 > 0 ? sxy : 0);
+      const int ckd = __builtin_amdgcn_readfirstlane(m.f2c[2][kd]);
+      const float4 czd = ldf4(cz + nd);
+      float4 dprev = one4;
+      if constexpr (DIAG) {
         dcur = ldf4(dinvv + nn);
-        if (k0 > 0) dprev = ldf4(dinvv + nn - d.sxy);
+        dprev = ldf4(dinvv + nd);
       }
-    }
+      float4 rd[NR];
+      float2 ed[NR], e0[NR];
 #pragma unroll
-    for (int b = 0; b < NR; ++b) {
-      zm[b] = z4;
-      zc[b] = z4;
-      rc[b] = z4;
-      if (valid && live[b]) {
+      for (int b = 0; b < NR; ++b) {
+        if (!live[b]) continue;  // uniform
         const float* rb = r + b * vs;
         const float* eb = ec + b * vsc;
-        if (k0 > 0)
-          zm[b] = z2row(eb, ldf4(rb + nn - d.sxy), dprev,
-                        ((int64_t)m.f2c[2][k0 - 1] * cny + cj) * cnx);
+        rd[b] = ldf4(rb + nd);
         rc[b] = ldf4(rb + nn);
-        zc[b] = z2row(eb, rc[b], dcur, ((int64_t)ck * cny + cj) * cnx);
+        ed[b] = ldf2(eb + (ckd * cny + cj) * cnx + ci0);
+        e0[b] = ldf2(eb + (ck * cny + cj) * cnx + ci0);
+      }
+      if (k0 > 0) czm = czd;
+#pragma unroll
+      for (int b = 0; b < NR; ++b) {
+        if (!live[b]) continue;
+        zm[b] = k0 > 0 ? z2(rd[b], dprev, ed[b]) : z4;
+        zc[b] = z2(rc[b], dcur, e0[b]);
       }
     }
-    for (int k = k0; k < k1; ++k, nn += d.sxy) {
-      // k0, k1 and live[] are uniform over the block: every thread reaches
-      // the barrier once per plane.  Rows j +- 1 inside the block come from
-      // the neighbouring waves; only the block's edge rows recompute them
-      // (measured at 256^3, 5 RHS: 373 -> ~351 us per launch)
-      const int buf = (k - k0) & 1;
-#pragma unroll
-      for (int b = 0; b < NR; ++b)
-        if (live[b]) zsh[buf][b][ly][lane] = zc[b];
-      __syncthreads();
-      const bool own_n = ly + 1 == kTileY, own_s = ly == 0;  // rows outside the block
+    for (int k = k0; k < k1; ++k, nn += sxy) {
       const bool up = k + 1 < d.nz;
-      const int cku = (valid && up) ? m.f2c[2][k + 1] : ck;
-      float4 cxc = z4, cyc = z4, czc = z4, cym = z4;
-      uchar4 mk = make_uchar4(1, 1, 1, 1);
-      float4 dup = one4, dnr = one4, dsr = one4, dgc = one4;
-      if (valid) {
-        cxc = ldf4(cx + nn);
-        cyc = ldf4(cy + nn);
-        czc = ldf4(cz + nn);
-        if (has_s) cym = ldf4(cy + nn - d.nx);
-        if constexpr (DIAG) {
-          dgc = ldf4(dgv + nn);
-          if (up) dup = ldf4(dinvv + nn + d.sxy);
-          if (has_n && own_n) dnr = ldf4(dinvv + nn + d.nx);
-          if (has_s && own_s) dsr = ldf4(dinvv + nn - d.nx);
-        } else {
-          mk = *reinterpret_cast<const uchar4*>(mask + nn);
-        }
+      const int nu = up ? nn + sxy : nn;
+      const int cku = __builtin_amdgcn_readfirstlane(up ? m.f2c[2][k + 1] : ck);
+      const int rowc = (ck * cny + cj) * cnx, rowu = (cku * cny + cj) * cnx;
+      const int rowh = (ck * cny + cjh) * cnx;
+      // ---- every load of this plane ----
+      const float4 cxc = ldf4(cx + nn), cyc = ldf4(cy + nn), czc = ldf4(cz + nn);
+      const float4 cyl = ldf4(cy + nn + os);
+      const float cxh = cx[nn + min(xo, 0)];
+      uchar4 mk = make_uchar4(0, 0, 0, 0);
+      float4 dgc = one4, dup = one4, dhr = one4;
+      float dh1 = 1.f;
+      if constexpr (DIAG) {
+        dgc = ldf4(dgv + nn);
+        dup = ldf4(dinvv + nu);
+        dh1 = dinvv[nn + xo];
+        if (edge) dhr = ldf4(dinvv + nn + oh);
+      } else {
+        mk = *reinterpret_cast<const uchar4*>(mask + nn);
       }
-      const float cxw_l = __shfl_up(cxc.w, 1, kWave);
-      float cxw = 0.f;
-      if (valid && has_w) cxw = lane > 0 ? cxw_l : cx[nn - 1];
-      const int64_t rowc = ((int64_t)ck * cny + cj) * cnx;
+      float4 ru[NR], qq[NR], rh4[NR];
+      float2 eu[NR], eh2[NR];
+      float rh[NR], eh[NR];
 #pragma unroll
       for (int b = 0; b < NR; ++b) {
         if (!live[b]) continue;  // uniform over the launch
         const float* __restrict__ rb = r + b * vs;
         const float* __restrict__ eb = ec + b * vsc;
-        float4 ru = z4, zu = z4, zn = z4, zs = z4;
-        if (valid) {
-          if (up) {
-            ru = ldf4(rb + nn + d.sxy);
-            zu = z2row(eb, ru, dup, ((int64_t)cku * cny + cj) * cnx);
-          }
-          if (has_n && !own_n) zn = zsh[buf][b][ly + 1][lane];
-          if (has_s && !own_s) zs = zsh[buf][b][ly - 1][lane];
-          if (has_n && own_n)
-            zn = z2row(eb, ldf4(rb + nn + d.nx), dnr, ((int64_t)ck * cny + cjn) * cnx);
-          if (has_s && own_s)
-            zs = z2row(eb, ldf4(rb + nn - d.nx), dsr, ((int64_t)ck * cny + cjs) * cnx);
+        ru[b] = ldf4(rb + nu);
+        eu[b] = ldf2(eb + rowu + ci0);
+        rh[b] = rb[nn + xo];
+        eh[b] = eb[rowc + cih];
+        if (partials) qq[b] = ldf4(q + b * vs + nn);
+        if (edge) {
+          rh4[b] = ldf4(rb + nn + oh);
+          eh2[b] = ldf2(eb + rowh + ci0);
         }
+      }
+      // ---- z2 of the rows above and below from the neighbouring waves ----
+      const int buf = (k - k0) & 1;
+#pragma unroll
+      for (int b = 0; b < NR; ++b)
+        if (live[b]) zsh[buf][b][ly][lane] = zc[b];
+      __syncthreads();
+      // ---- compute ----
+      const float4 cym = has_s ? cyl : z4;
+      const float cxw_s = __shfl_up(cxc.w, 1, kWave);
+      const float cxw = has_w ? (lane == 0 ? cxh : cxw_s) : 0.f;
+#pragma unroll
+      for (int b = 0; b < NR; ++b) {
+        if (!live[b]) continue;
         const float4 c = zc[b];
-        float zw = __shfl_up(c.w, 1, kWave), ze = __shfl_down(c.x, 1, kWave);
-        if (valid) {
-          if (!has_w) zw = 0.f;
-          else if (lane == 0) zw = omega * (DIAG ? dinvv[nn - 1] : 1.f) * rb[nn - 1] + eb[rowc + ciw];
-          if (!has_e) ze = 0.f;
-          else if (lane == kWave - 1)
-            ze = omega * (DIAG ? dinvv[nn + 4] : 1.f) * rb[nn + 4] + eb[rowc + cie];
-        }
+        const float4 zu = up ? z2(ru[b], dup, eu[b]) : z4;
+        float4 zn = z4, zs = z4;
+        if (has_n) zn = own_n ? z2(rh4[b], dhr, eh2[b]) : zsh[buf][b][min(ly + 1, kTileY - 1)][lane];
+        if (has_s) zs = own_s ? z2(rh4[b], dhr, eh2[b]) : zsh[buf][b][max(ly - 1, 0)][lane];
+        const float zh = omega * (DIAG ? dh1 : 1.f) * rh[b] + eh[b];
+        const float sw = __shfl_up(c.w, 1, kWave), se = __shfl_down(c.x, 1, kWave);
+        const float zw = has_w ? (lane == 0 ? zh : sw) : 0.f;
+        const float ze = has_e ? (lane == kWave - 1 ? zh : se) : 0.f;
         const float4 mm = zm[b], rr = rc[b];
         const float s0 = cxc.x * c.y + cxw * zw + cyc.x * zn.x + cym.x * zs.x + czc.x * zu.x + czm.x * mm.x;
         const float s1 = cxc.y * c.z + cxc.x * c.x + cyc.y * zn.y + cym.y * zs.y + czc.y * zu.y + czm.y * mm.y;
@@ -703,18 +734,18 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
           out.z = mk.z == 0 ? c.z + omega * (rr.z - c.z + s2) : 0.f;
           out.w = mk.w == 0 ? c.w + omega * (rr.w - c.w + s3) : 0.f;
         }
-        if (valid) {
-          *reinterpret_cast<float4*>(e + b * vs + nn) = out;
-          if (partials) {
-            const float4 qq = ldf4(q + b * vs + nn);
-            // the lane's 4 products in FP32, their sums accumulated in FP64
-            acc[b] += (double)(out.x * rr.x + out.y * rr.y + out.z * rr.z + out.w * rr.w);
-            acc[NR + b] += (double)(out.x * qq.x + out.y * qq.y + out.z * qq.z + out.w * qq.w);
-          }
+        if (partials) {
+          // the lane's 4 products in FP32, their sums accumulated in FP64
+          const float4 qv = qq[b];
+          const double a0 = (double)(out.x * rr.x + out.y * rr.y + out.z * rr.z + out.w * rr.w);
+          const double a1 = (double)(out.x * qv.x + out.y * qv.y + out.z * qv.z + out.w * qv.w);
+          acc[b] += valid ? a0 : 0.0;
+          acc[NR + b] += valid ? a1 : 0.0;
         }
+        if (valid) *reinterpret_cast<float4*>(e + b * vs + nn) = out;
         zm[b] = c;
         zc[b] = zu;
-        rc[b] = ru;
+        rc[b] = ru[b];
       }
       czm = czc;
       ck = cku;
@@ -727,14 +758,14 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
 #pragma unroll
     for (int b = 0; b < NR; ++b) {
       if (b >= nr) continue;
-      store_partial(partials + (int64_t)(b0 + b) * gridDim.x + blockIdx.x, acc[b]);
-      store_partial(partials + (int64_t)(nrt + b0 + b) * gridDim.x + blockIdx.x, acc[NR + b]);
+      partials[(int64_t)(b0 + b) * gridDim.x + blockIdx.x] = acc[b];
+      partials[(int64_t)(nrt + b0 + b) * gridDim.x + blockIdx.x] = acc[NR + b];
     }
   }
 }
 
 // rc = P^T (r - omega A r) with 4 nodes per lane: x pairs {2I, 2I+1} sum
-// inside the lane, y pairs through LDS at the end of each z-group
+// inside the lane, y pairs through LDS at the end of each z-group.
 // DIAG: a coarse level, t_n = (1 - omega) r_n + omega sum_m c_nm dinv_m r_m
 // (level 0: dinv = 1 on free nodes, and r = 0 on the others)
 template <int NR, bool DIAG>
@@ -742,114 +773,131 @@ __global__ __launch_bounds__(kBlock) void k_restrict_l0x4(
     const float* __restrict__ r, float* __restrict__ rc, int64_t vsc,
     const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
     const float* __restrict__ dinvv, Dims d, Tiling tl, const int* __restrict__ kstart, Map m,
-    int64_t vs, int nr, float omega,
-    const CgScalars* sc, int mode) {
+    int64_t vs, int nr, float omega, const CgScalars* sc, int mode) {
   __shared__ float2 lds[NR][kTileY][kWave];
   int tx, ty, tz;
   if (!tile_of_block(tl, tx, ty, tz)) return;  // uniform over the block
   bool live[NR];
 #pragma unroll
   for (int b = 0; b < NR; ++b) live[b] = b < nr && live_of(sc, b, mode);
-  const int lane = (int)(threadIdx.x % kWave), ly = (int)(threadIdx.x / kWave);
+  const int lane = (int)(threadIdx.x % kWave);
+  const int ly = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
   const int i = (tx * kWave + lane) * 4;
   const int j = ty * kTileY + ly;
-  const int k0 = kstart[tz], k1 = kstart[tz + 1];
   const bool valid = i < d.nx && j < d.ny;
-  const int64_t base = valid ? (int64_t)j * d.nx + i : 0;
-  const bool has_w = i > 0, has_e = i + 4 < d.nx, has_n = j + 1 < d.ny, has_s = j > 0;
+  const int ic = min(i, d.nx - 4), jc = min(j, d.ny - 1);
+  const bool has_w = ic > 0, has_e = ic + 4 < d.nx, has_n = jc + 1 < d.ny, has_s = jc > 0;
+  const int xo = (lane == 0 && has_w) ? -1 : ((lane == kWave - 1 && has_e) ? 4 : 0);
+  const int sxy = (int)d.sxy, nx = d.nx;
+  const int on = has_n ? nx : 0, os = has_s ? -nx : 0;
   const int cnx = m.cn[0], cny = m.cn[1];
+  const int k0 = __builtin_amdgcn_readfirstlane(kstart[tz]);
+  const int k1 = __builtin_amdgcn_readfirstlane(kstart[tz + 1]);
   const float4 z4 = f4(0.f, 0.f, 0.f, 0.f);
-  int64_t nn = (int64_t)k0 * d.sxy + base;
+  const float4 one4 = f4(1.f, 1.f, 1.f, 1.f);
+  int nn = k0 * sxy + jc * nx + ic;
   float4 rm[NR], rcur[NR];
   float2 col[NR];
-  float4 czm = z4;
-  if (valid && k0 > 0) czm = ldf4(cz + nn - d.sxy);
-  const float4 one4 = f4(1.f, 1.f, 1.f, 1.f);
-  float4 dcur = one4, dprev = one4;  // inverse diagonal of rows (k, j), (k-1, j)
-  if constexpr (DIAG) {
-    if (valid) {
-      dcur = ldf4(dinvv + nn);
-      if (k0 > 0) dprev = ldf4(dinvv + nn - d.sxy);
-    }
-  }
-#pragma unroll
-  for (int b = 0; b < NR; ++b) {
-    rm[b] = z4;
-    rcur[b] = z4;
-    col[b] = make_float2(0.f, 0.f);
-    if (valid && live[b]) {
-      if (k0 > 0) rm[b] = ldf4(r + b * vs + nn - d.sxy);
-      rcur[b] = ldf4(r + b * vs + nn);
-    }
-  }
-  for (int k = k0; k < k1; ++k, nn += d.sxy) {
-    const int ck = m.f2c[2][k];
-    const bool up = k + 1 < d.nz;
-    float4 cxc = z4, cyc = z4, czc = z4, cym = z4;
-    if (valid) {
-      cxc = ldf4(cx + nn);
-      cyc = ldf4(cy + nn);
-      czc = ldf4(cz + nn);
-      if (has_s) cym = ldf4(cy + nn - d.nx);
-    }
-    const float cxw_l = __shfl_up(cxc.w, 1, kWave);
-    float cxw = 0.f;
-    if (valid && has_w) cxw = lane > 0 ? cxw_l : cx[nn - 1];
-    float4 dup = one4, dnr = one4, dsr = one4;
-    float dw = 1.f, de = 1.f;
+  float4 czm = z4, dcur = one4, dprev = one4;
+  {
+    const int nd = nn - (k0 > 0 ? sxy : 0);
+    const float4 czd = ldf4(cz + nd);
     if constexpr (DIAG) {
-      if (valid) {
-        if (up) dup = ldf4(dinvv + nn + d.sxy);
-        if (has_n) dnr = ldf4(dinvv + nn + d.nx);
-        if (has_s) dsr = ldf4(dinvv + nn - d.nx);
-        if (has_w && lane == 0) dw = dinvv[nn - 1];
-        if (has_e && lane == kWave - 1) de = dinvv[nn + 4];
-      }
+      dcur = ldf4(dinvv + nn);
+      const float4 dd = ldf4(dinvv + nd);
+      if (k0 > 0) dprev = dd;
     }
-    auto mul = [](float4 a, float4 b) { return f4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w); };
+    float4 rd[NR];
 #pragma unroll
     for (int b = 0; b < NR; ++b) {
-      if (!live[b]) continue;  // uniform over the block
-      const float* __restrict__ rb = r + b * vs;
-      float4 ru = z4, rn = z4, rs = z4;
-      if (valid) {
-        if (up) ru = ldf4(rb + nn + d.sxy);
-        if (has_n) rn = ldf4(rb + nn + d.nx);
-        if (has_s) rs = ldf4(rb + nn - d.nx);
+      col[b] = make_float2(0.f, 0.f);
+      if (!live[b]) continue;
+      rd[b] = ldf4(r + b * vs + nd);
+      rcur[b] = ldf4(r + b * vs + nn);
+    }
+    if (k0 > 0) czm = czd;
+#pragma unroll
+    for (int b = 0; b < NR; ++b)
+      if (live[b]) rm[b] = pick(k0 > 0, rd[b], z4);
+  }
+  auto mul = [](float4 a, float4 b) { return f4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w); };
+  for (int k = k0; k < k1; ++k, nn += sxy) {
+    const int ck = __builtin_amdgcn_readfirstlane(m.f2c[2][k]);
+    const bool up = k + 1 < d.nz;
+    const int nu = up ? nn + sxy : nn;
+    // ---- every load of this plane ----
+    const float4 cxc = ldf4(cx + nn), cyc = ldf4(cy + nn), czc = ldf4(cz + nn);
+    const float4 cyl = ldf4(cy + nn + os);
+    const float cxh = cx[nn + min(xo, 0)];
+    float4 dup = one4, dnr = one4, dsr = one4;
+    float dh = 1.f;
+    if constexpr (DIAG) {
+      dup = ldf4(dinvv + nu);
+      dnr = ldf4(dinvv + nn + on);
+      dsr = ldf4(dinvv + nn + os);
+      dh = dinvv[nn + xo];
+    }
+    float4 ru[NR], rn[NR], rs[NR];
+    float rh[NR];
+#pragma unroll
+    for (int b = 0; b < NR; ++b) {
+      if (live[b]) {  // uniform over the block
+        const float* __restrict__ rb = r + b * vs;
+        ru[b] = ldf4(rb + nu);
+        rn[b] = ldf4(rb + nn + on);
+        rs[b] = ldf4(rb + nn + os);
+        rh[b] = rb[nn + xo];
       }
+    }
+    // ---- compute ----
+    const float4 cym = pick(has_s, cyl, z4);
+    const float cxw_s = __shfl_up(cxc.w, 1, kWave);
+    const float cxw = has_w ? (lane == 0 ? cxh : cxw_s) : 0.f;
+    if constexpr (DIAG) {
+      if (!up) dup = one4;
+    }
+#pragma unroll
+    for (int b = 0; b < NR; ++b) {
+      if (!live[b]) continue;
       const float4 cr = rcur[b];
       // neighbour values enter as D^-1 r (level 0: r)
-      const float4 c = DIAG ? mul(cr, dcur) : cr;
-      float rw = __shfl_up(c.w, 1, kWave), re = __shfl_down(c.x, 1, kWave);
-      if (valid) {
-        if (!has_w) rw = 0.f;
-        else if (lane == 0) rw = dw * rb[nn - 1];
-        if (!has_e) re = 0.f;
-        else if (lane == kWave - 1) re = de * rb[nn + 4];
-      }
-      const float4 ruS = DIAG ? mul(ru, dup) : ru;
+      float4 c = cr;
+      if constexpr (DIAG) c = mul(cr, dcur);
+      const float hv = (DIAG ? dh : 1.f) * rh[b];
+      const float sw = __shfl_up(c.w, 1, kWave), se = __shfl_down(c.x, 1, kWave);
+      const float rw = has_w ? (lane == 0 ? hv : sw) : 0.f;
+      const float re = has_e ? (lane == kWave - 1 ? hv : se) : 0.f;
+      const float4 ruv = pick(up, ru[b], z4);
+      float4 ruS = ruv;
+      if constexpr (DIAG) ruS = mul(ruv, dup);
+      float4 rnS = rn[b], rsS = rs[b], mm = rm[b];
       if constexpr (DIAG) {
-        rn = mul(rn, dnr);
-        rs = mul(rs, dsr);
+        rnS = mul(rnS, dnr);
+        rsS = mul(rsS, dsr);
+        mm = mul(mm, dprev);
       }
-      const float4 mm = DIAG ? mul(rm[b], dprev) : rm[b];
-      const float s0 = cxc.x * c.y + cxw * rw + cyc.x * rn.x + cym.x * rs.x + czc.x * ruS.x + czm.x * mm.x;
-      const float s1 = cxc.y * c.z + cxc.x * c.x + cyc.y * rn.y + cym.y * rs.y + czc.y * ruS.y + czm.y * mm.y;
-      const float s2 = cxc.z * c.w + cxc.y * c.y + cyc.z * rn.z + cym.z * rs.z + czc.z * ruS.z + czm.z * mm.z;
-      const float s3 = cxc.w * re + cxc.z * c.z + cyc.w * rn.w + cym.w * rs.w + czc.w * ruS.w + czm.w * mm.w;
+      rnS = pick(has_n, rnS, z4);
+      rsS = pick(has_s, rsS, z4);
+      const float s0 = cxc.x * c.y + cxw * rw + cyc.x * rnS.x + cym.x * rsS.x + czc.x * ruS.x + czm.x * mm.x;
+      const float s1 = cxc.y * c.z + cxc.x * c.x + cyc.y * rnS.y + cym.y * rsS.y + czc.y * ruS.y + czm.y * mm.y;
+      const float s2 = cxc.z * c.w + cxc.y * c.y + cyc.z * rnS.z + cym.z * rsS.z + czc.z * ruS.z + czm.z * mm.z;
+      const float s3 = cxc.w * re + cxc.z * c.z + cyc.w * rnS.w + cym.w * rsS.w + czc.w * ruS.w + czm.w * mm.w;
+      const float w1 = 1.f - omega;
+      const float cxs = (w1 * cr.x + omega * s0) + (w1 * cr.y + omega * s1);
+      const float cys = (w1 * cr.z + omega * s2) + (w1 * cr.w + omega * s3);
       if (valid) {
-        const float w1 = 1.f - omega;
-        col[b].x += (w1 * cr.x + omega * s0) + (w1 * cr.y + omega * s1);
-        col[b].y += (w1 * cr.z + omega * s2) + (w1 * cr.w + omega * s3);
+        col[b].x += cxs;
+        col[b].y += cys;
       }
       rm[b] = cr;
-      rcur[b] = ru;
+      rcur[b] = ruv;
     }
     czm = czc;
     dprev = dcur;
     dcur = dup;
-    const bool group_end = k + 1 == k1 || m.f2c[2][k + 1] != ck;  // uniform
-    if (group_end) {
+    const bool group_end =
+        k + 1 == k1 || __builtin_amdgcn_readfirstlane(m.f2c[2][k + 1 < d.nz ? k + 1 : k]) != ck;
+    if (group_end) {  // uniform over the block
 #pragma unroll
       for (int b = 0; b < NR; ++b) lds[b][ly][lane] = col[b];
       __syncthreads();

@@ -796,7 +796,8 @@ int fs_pc_create(fs_sys* sys, fs_net* net, int nlevels, const int32_t* groups, d
         return fail_pc(fail(FS_EHIP, "hierarchy upload failed"));
       a.kstart = pc->kstart;
       // 4 nodes per lane when rows are a multiple of 4 (x pairs never cross a lane)
-      a.x4 = sys->d.nx % 4 == 0 ? 1 : 0;
+      // (the 4-wide kernels address nodes with 32-bit offsets)
+      a.x4 = sys->d.nx % 4 == 0 && sys->d.n + sys->d.sxy < (int64_t)INT32_MAX ? 1 : 0;
       if (a.x4) {
         a.tl4 = sys->tl4;
         a.tlr4 = sys->tl4;
@@ -815,7 +816,7 @@ int fs_pc_create(fs_sys* sys, fs_net* net, int nlevels, const int32_t* groups, d
   pc->lx.assign(nlevels, mg::L0Args{});
   for (int l = 1; l < nlevels; ++l) {
     const mg::Level& L = pc->lv[l];
-    bool ok = L.nx % 4 == 0 && L.nx >= 16;
+    bool ok = L.nx % 4 == 0 && L.nx >= 16 && L.n + (int64_t)L.nx * L.ny < (int64_t)INT32_MAX;
     for (int q = 0; q < 2 && ok; ++q) {
       const std::vector<int32_t>& st = starts[l][q];
       for (size_t I = 0; I + 1 < st.size(); ++I)

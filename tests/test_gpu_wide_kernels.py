@@ -47,12 +47,15 @@ def test_wide_kernels_capacitance_matches_oracle(gpu_lib, label, solver):
     print(f"[gpu case] wide kernels: {label}, {solver}", flush=True)
     ex, Cr, phis = _oracle(label)
     assert ex.grid.shape_nodes[0] % 4 == 0
-    ps = PoissonSolver(ex.grid, ex.eps_cell, SolverConfig(linear_solver=solver))
+    # tol 1e-12: the potentials' error against the direct solve is bounded by
+    # the condition number times the relative residual, so a 1e-10 residual
+    # alone leaves a 1e-9 comparison of phi without margin
+    ps = PoissonSolver(ex.grid, ex.eps_cell, SolverConfig(linear_solver=solver, tol=1e-12))
     rep = {}
     C, phi = ps.capacitance_matrix(ex.terminals, report=rep, return_phi=True)
     lin = rep["linear"]
     assert (lin["stencil_width"], lin["cycle_width"]) == WIDTHS[solver], lin
-    assert lin["residual_eq"] <= 1e-10, lin
+    assert lin["residual_eq"] <= 1e-12, lin
     err = np.abs(C - Cr).max() / np.abs(Cr).max()
     assert err <= 1e-9, err
     for j in range(len(ex.terminals)):
