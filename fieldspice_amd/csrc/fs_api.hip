@@ -860,7 +860,8 @@ int fs_sys_create(fs_op* op, const uint8_t* mask, int pin, fs_sys** out, fs_sys_
   s->vec_blocks = std::max(s->vec_blocks, 1);
   // FP32 stencil with 4 nodes per lane when rows are a multiple of 4
   // wrap-around edges run on the one-node-per-lane kernels only
-  s->use_x4 = d.nx % 4 == 0 && d.per == 0 ? 1 : 0;
+  // (it addresses nodes with 32-bit offsets)
+  s->use_x4 = d.nx % 4 == 0 && d.per == 0 && d.n + d.sxy < (int64_t)INT32_MAX ? 1 : 0;
   {
     fs::Tiling& t4 = s->tl4;
     t4.tiles_x = (d.nx / 4 + fs::kWave - 1) / fs::kWave;

@@ -447,10 +447,24 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot(const T* __restrict__ p,
 // each lane owns 4 consecutive x nodes (one 16-byte load per array), a wave
 // covers 256 nodes of a row, and all right-hand sides advance together in
 // the z-march so the couplings are read once per node.  The x neighbours at
-// the ends of a lane's 4 nodes come from the adjacent lanes (shuffles), or
-// from memory at the edges of the wave.  Stores bypass the caches (q is not
-// read again by this kernel).
+// the ends of a lane's 4 nodes come from the adjacent lanes (shuffles); the
+// rows above and below come from the neighbouring waves of the block through
+// LDS, and the block's two edge rows read the one row outside it.  Every
+// global load of a plane is issued before the first use of any of them, with
+// no lane-divergent branch around a load: lanes and rows outside the grid
+// read the grid's last node and store nothing, and the two x-halo lanes
+// select their neighbour from one load every lane makes (256^3, 5 RHS: 195 ->
+// 182 us, 88% of a kernel streaming the same arrays once).  Node offsets are
+// 32-bit (the set-up selects this kernel only when n + nx*ny < 2^31).
+// Stores bypass the caches (q is not read again by this kernel).
 typedef float f4n __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+// component-wise select of two float4 values (the conditional operator on the
+// vector struct selects an address and keeps both operands in scratch memory)
+__device__ __forceinline__ float4 pick4(bool c, float4 a, float4 b) {
+  return make_float4(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w);
+}
 
 template <int NR>
 __global__ __launch_bounds__(kBlock) void k_stencil_dot4(const float* __restrict__ p,
@@ -474,74 +488,96 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot4(const float* __restrict
   double acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+  __shared__ float4 psh[2][NR][kTileY][kWave];
   int tx, ty, tz;
-  if (tile_of_block(tl, tx, ty, tz)) {
+  if (tile_of_block(tl, tx, ty, tz)) {  // uniform over the block
     const int lane = (int)(threadIdx.x % kWave);
+    const int ly = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
     const int i = (tx * kWave + lane) * 4;
-    const int j = ty * kTileY + (int)(threadIdx.x / kWave);
+    const int j = ty * kTileY + ly;  // uniform over the wave
+    const bool valid = i < d.nx && j < d.ny;
+    const int ic = min(i, d.nx - 4), jc = min(j, d.ny - 1);
+    const bool has_w = ic > 0, has_e = ic + 4 < d.nx, has_n = jc + 1 < d.ny, has_s = jc > 0;
+    // the x halo: lane 0 reads its west node, the last lane its east node
+    const int xo = (lane == 0 && has_w) ? -1 : ((lane == kWave - 1 && has_e) ? 4 : 0);
+    // the one row outside the block an edge wave reads: south for the first
+    // wave, north for the last
+    const bool own_n = ly + 1 == kTileY, own_s = ly == 0;
+    const bool edge = (own_s && has_s) || (own_n && has_n);
     const int k0 = tz * tl.kz;
     const int k1 = min(d.nz, k0 + tl.kz);
-    const bool valid = i < d.nx && j < d.ny;  // i .. i+3 inside: nx % 4 == 0
-    const int64_t base = valid ? (int64_t)j * d.nx + i : 0;
-    const bool has_w = i > 0, has_e = i + 4 < d.nx, has_n = j + 1 < d.ny, has_s = j > 0;
-    int64_t nn = (int64_t)k0 * d.sxy + base;
+    const int sxy = (int)d.sxy, nx = d.nx;
+    const int os = has_s ? -nx : 0, oh = own_s ? -nx : nx;
+    int nn = k0 * sxy + jc * nx + ic;
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     float4 pm[NR], pc[NR];
     float4 czm = z4;
-    if (valid && k0 > 0) czm = *reinterpret_cast<const float4*>(cz + nn - d.sxy);
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      pm[r] = z4;
-      pc[r] = z4;
-      if (valid && live[r]) {
-        if (k0 > 0) pm[r] = *reinterpret_cast<const float4*>(p + r * va.vs + nn - d.sxy);
-        pc[r] = *reinterpret_cast<const float4*>(p + r * va.vs + nn);
-      }
-    }
-    for (int k = k0; k < k1; ++k, nn += d.sxy) {
-      float4 cxc = z4, cyc = z4, czc = z4, cym = z4;
-      if (valid) {
-        cxc = *reinterpret_cast<const float4*>(cx + nn);
-        cyc = *reinterpret_cast<const float4*>(cy + nn);
-        czc = *reinterpret_cast<const float4*>(cz + nn);
-        if (has_s) cym = *reinterpret_cast<const float4*>(cy + nn - d.nx);
-      }
-      // coupling to the west of this lane's first node
-      const float cxw_l = __shfl_up(cxc.w, 1, kWave);
-      float cxw = 0.f;
-      if (valid && has_w) cxw = lane > 0 ? cxw_l : cx[nn - 1];
-      const bool up = k + 1 < d.nz;
+    {
+      const int nd = k0 > 0 ? nn - sxy : nn;
+      const float4 czd = ld4(cz + nd);
+      float4 pd[NR];
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
         if (!live[r]) continue;  // uniform over the launch
+        pd[r] = ld4(p + r * va.vs + nd);
+        pc[r] = ld4(p + r * va.vs + nn);
+      }
+      czm = pick4(k0 > 0, czd, z4);
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+        if (live[r]) pm[r] = pick4(k0 > 0, pd[r], z4);
+    }
+    for (int k = k0; k < k1; ++k, nn += sxy) {
+      const bool up = k + 1 < d.nz;
+      const int nu = up ? nn + sxy : nn;
+      // ---- every load of this plane ----
+      const float4 cxc = ld4(cx + nn), cyc = ld4(cy + nn), czc = ld4(cz + nn);
+      const float4 cyl = ld4(cy + nn + os);
+      const float cxh = cx[nn + min(xo, 0)];
+      float4 pu[NR], ph4[NR];
+      float ph[NR];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        if (!live[r]) continue;
         const float* __restrict__ pr = p + r * va.vs;
-        float4 pu = z4, pn = z4, ps = z4;
-        if (valid) {
-          if (up) pu = *reinterpret_cast<const float4*>(pr + nn + d.sxy);
-          if (has_n) pn = *reinterpret_cast<const float4*>(pr + nn + d.nx);
-          if (has_s) ps = *reinterpret_cast<const float4*>(pr + nn - d.nx);
-        }
+        pu[r] = ld4(pr + nu);
+        ph[r] = pr[nn + xo];
+        if (edge) ph4[r] = ld4(pr + nn + oh);
+      }
+      // ---- the rows above and below from the neighbouring waves ----
+      const int buf = (k - k0) & 1;
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+        if (live[r]) psh[buf][r][ly][lane] = pc[r];
+      __syncthreads();
+      // ---- compute ----
+      const float4 cym = pick4(has_s, cyl, z4);
+      const float cxw_s = __shfl_up(cxc.w, 1, kWave);
+      const float cxw = has_w ? (lane == 0 ? cxh : cxw_s) : 0.f;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        if (!live[r]) continue;
         const float4 c = pc[r];
-        float pw = __shfl_up(c.w, 1, kWave), pe = __shfl_down(c.x, 1, kWave);
-        if (valid) {
-          if (!has_w) pw = 0.f;
-          else if (lane == 0) pw = pr[nn - 1];
-          if (!has_e) pe = 0.f;
-          else if (lane == kWave - 1) pe = pr[nn + 4];
-        }
+        const float4 u = pick4(up, pu[r], z4);
+        const float4 ln = psh[buf][r][min(ly + 1, kTileY - 1)][lane];
+        const float4 ls = psh[buf][r][max(ly - 1, 0)][lane];
+        const float4 nb = pick4(has_n, pick4(own_n, ph4[r], ln), z4);
+        const float4 sb = pick4(has_s, pick4(own_s, ph4[r], ls), z4);
+        const float sw = __shfl_up(c.w, 1, kWave), se = __shfl_down(c.x, 1, kWave);
+        const float pw = has_w ? (lane == 0 ? ph[r] : sw) : 0.f;
+        const float pe = has_e ? (lane == kWave - 1 ? ph[r] : se) : 0.f;
         const float4 m = pm[r];
         f4n y;
-        y.x = c.x - (cxc.x * c.y + cxw * pw + cyc.x * pn.x + cym.x * ps.x + czc.x * pu.x + czm.x * m.x);
-        y.y = c.y - (cxc.y * c.z + cxc.x * c.x + cyc.y * pn.y + cym.y * ps.y + czc.y * pu.y + czm.y * m.y);
-        y.z = c.z - (cxc.z * c.w + cxc.y * c.y + cyc.z * pn.z + cym.z * ps.z + czc.z * pu.z + czm.z * m.z);
-        y.w = c.w - (cxc.w * pe + cxc.z * c.z + cyc.w * pn.w + cym.w * ps.w + czc.w * pu.w + czm.w * m.w);
-        if (valid) {
-          __builtin_nontemporal_store(y, reinterpret_cast<f4n*>(q + r * va.vs + nn));
-          // the lane's 4 products in FP32, their sum accumulated in FP64
-          acc[r] += (double)(c.x * y.x + c.y * y.y + c.z * y.z + c.w * y.w);
-        }
+        y.x = c.x - (cxc.x * c.y + cxw * pw + cyc.x * nb.x + cym.x * sb.x + czc.x * u.x + czm.x * m.x);
+        y.y = c.y - (cxc.y * c.z + cxc.x * c.x + cyc.y * nb.y + cym.y * sb.y + czc.y * u.y + czm.y * m.y);
+        y.z = c.z - (cxc.z * c.w + cxc.y * c.y + cyc.z * nb.z + cym.z * sb.z + czc.z * u.z + czm.z * m.z);
+        y.w = c.w - (cxc.w * pe + cxc.z * c.z + cyc.w * nb.w + cym.w * sb.w + czc.w * u.w + czm.w * m.w);
+        // the lane's 4 products in FP32, their sum accumulated in FP64
+        const double dp = (double)(c.x * y.x + c.y * y.y + c.z * y.z + c.w * y.w);
+        acc[r] += valid ? dp : 0.0;
+        if (valid) __builtin_nontemporal_store(y, reinterpret_cast<f4n*>(q + r * va.vs + nn));
         pm[r] = c;
-        pc[r] = pu;
+        pc[r] = u;
       }
       czm = czc;
     }
