@@ -95,18 +95,61 @@ def config_for(method):
     return SolverConfig(linear_solver=method, tol=1e-10)
 
 
-def measured_traffic(kernel, n):
-    """HBM bytes per launch of the stencil from the committed counter profile
-    (two rocprofv3 --pmc passes calibrated on a streaming kernel), when this
-    run is the profiled configuration; None otherwise."""
-    path = os.path.join(ROOT, "profiles", "stencil_traffic_256cube.md")
+TRAFFIC_PROFILE = os.path.join("profiles", "traffic_256cube.md")
+
+
+def traffic_ratio(kind, n):
+    """Measured HBM bytes over algorithmic bytes of one kernel kind, from the
+    committed counter profile (FETCH_SIZE and WRITE_SIZE passes over this
+    bench's own 256^3 learned-FCG run, calibrated on the library's streaming
+    probe, both byte counts summed over the same dispatches), when this run is
+    that configuration; None otherwise."""
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
     if n != 256 ** 3 or not os.path.exists(path):
         return None
-    key = f"TRAFFIC_BYTES_PER_LAUNCH[{kernel}]"
+    key = f"TRAFFIC_RATIO[{kind}]"
     for line in open(path):
         if line.startswith(key):
-            return int(line.split("=")[1])
+            return float(line.split("=")[1])
     return None
+
+
+def kernel_table(prof, n):
+    """Per level-0 kernel kind: launches, average launch time (HIP events at
+    the kernel's own start and end), algorithmic bytes per launch and the
+    rate they imply, and the share of the solve's device time."""
+    from fieldspice_amd._lib import PROF_NAMES
+    out = {}
+    for k, name in enumerate(PROF_NAMES):
+        t, nl, b = prof["kernel_time"][k], prof["kernel_launches"][k], prof["kernel_bytes"][k]
+        if nl <= 0 or t <= 0:
+            continue
+        avg = t / nl
+        gbs = b / t / 1e9
+        ratio = traffic_ratio(name, n)
+        out[name] = {"launches": int(nl), "time_s": t, "avg_launch_us": avg * 1e6,
+                     "algorithmic_bytes_total": b, "algorithmic_bytes_per_launch": b / nl,
+                     "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                     "traffic_bytes_per_launch": None if ratio is None else ratio * b / nl}
+    return out
+
+
+def kernel_names(nt, fp64, nx):
+    """Kernel symbol of each profiled kind for nt right-hand sides on rows of
+    nx nodes."""
+    bucket = 1 if nt <= 1 else 2 if nt <= 2 else 4 if nt <= 4 else 8
+    x4 = nx % 4 == 0
+    if fp64:
+        return {"stencil": (f"k_stencil_dot2<{nt}>" if nx % 2 == 0 else
+                            f"k_stencil_dot<double, {bucket}>"),
+                "direction": f"k_pupdate<double, {bucket}, true>",
+                "residual": f"k_update<double, {bucket}, true>"}
+    return {"stencil": f"k_stencil_dot4<{nt}>" if x4 else f"k_stencil_dot<float, {bucket}>",
+            "direction": f"k_pupdate<float, {bucket}, true>",
+            "residual": f"k_update<float, {bucket}, true>",
+            "restrict": f"k_restrict_l0x4<{nt}, false>" if x4 else "k_restrict_l0<8>",
+            "prolong": (f"k_prolong_l0x4<{min(nt, 5)}, false>" if x4 else
+                        f"k_prolong_l0<{bucket}>")}
 
 
 def run_method(ex, L, method, steps, warmup):
@@ -126,25 +169,32 @@ def run_method(ex, L, method, steps, warmup):
     params = ls.params()
     for _ in range(warmup):
         cap.solve(L, params)
-    ls.sys.set_profiling(True)
-    times, stats = [], []
+    times, results = [], []
     for _ in range(steps):
         t1 = time.perf_counter()
         C, qsh, st, rc = cap.solve(L, params)
         times.append(time.perf_counter() - t1)
-        stats.append(st)
+        results.append(st)
+    # the per-kernel table: the same steps again with every level-0 launch
+    # timed by HIP events (kept out of the timed steps: the timestamps add ~2%)
+    ls.sys.set_profiling(True)
+    stats = []
+    for _ in range(steps):
+        stats.append(cap.solve(L, params)[2])
     ls.sys.set_profiling(False)
     ms = 1e3 * float(np.mean(times))
-    st = stats[-1]
+    st = results[-1]
     prof = {k: sum(s[k] for s in stats) for k in ("stencil_time", "stencil_launches",
-                                                  "stencil_bytes")}
+                                                  "stencil_bytes", "solve_time")}
+    for k in ("kernel_time", "kernel_launches", "kernel_bytes"):
+        prof[k] = [sum(s[k][j] for s in stats) for j in range(len(stats[0][k]))]
     out = {
         "C": C, "ms_per_step": ms, "s_per_solve": ms * 1e-3 / nt, "setup_s": setup_s,
         "iterations": st["iterations"], "refreshes": st["refreshes"],
         "residual_eq": st["residual_eq"], "residual": st["residual"],
         "converged": bool(st["converged"]), "fallbacks": st["fallbacks"],
         "precond_applies": st["pc_applies"], "device_solve_s": st["solve_time"],
-        "prof": prof, "nt": nt,
+        "prof": prof, "nt": nt, "kernels": kernel_table(prof, g.n_nodes),
     }
     if ls.pc is not None:
         out["precond_levels"] = ls.pc.nlevels
@@ -458,28 +508,45 @@ def main():
             r["C_rel_dev_vs_fp64"] = float(np.abs(r["C"] - ref["C"]).max()
                                            / np.abs(ref["C"]).max())
             r["speedup_vs_hip_pcg"] = ref["s_per_solve"] / r["s_per_solve"]
-    pr = b["prof"]
-    avg_s = pr["stencil_time"] / max(pr["stencil_launches"], 1)
-    bytes_per_launch = pr["stencil_bytes"] / max(pr["stencil_launches"], 1)
-    achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
     fp64 = head == "hip_pcg"
     nt = len(ex.terminals)
-    bucket = 1 if nt <= 1 else 2 if nt <= 2 else 4 if nt <= 4 else 8
-    kernel = (f"k_stencil_dot<double,{bucket}>" if fp64 else
-              f"k_stencil_dot4<{nt}>" if g.shape_nodes[0] % 4 == 0
-              else f"k_stencil_dot<float,{bucket}>")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(kernel, g.n_nodes),
-                "traffic_source": "profiles/stencil_traffic_256cube.md (FETCH_SIZE/WRITE_SIZE, "
-                                  "calibrated on k_pupdate)",
-                "kernel": kernel,
-                "avg_launch_us": round(avg_s * 1e6, 2),
-                "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+    kt = b["kernels"]
+    # the dominant kernel of the reported method (largest summed launch time)
+    dom = max(kt, key=lambda k: kt[k]["time_s"])
+    names = kernel_names(nt, fp64, g.shape_nodes[0])
+
+    def roof(kind):
+        e = kt[kind]
+        tr = e["traffic_bytes_per_launch"]
+        return {"bound": "hbm", "achieved": round(e["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(e["frac"], 4),
+                "traffic": None if tr is None else int(tr),
+                "kernel": names.get(kind, kind), "kind": kind,
+                "avg_launch_us": round(e["avg_launch_us"], 2),
+                "algorithmic_bytes_per_launch": int(e["algorithmic_bytes_per_launch"]),
+                "share_of_device_time": round(e["time_s"] / max(b["prof"]["solve_time"], 1e-30), 4)}
+    roofline = roof(dom)
+    roofline["measured_over"] = (f"{args.steps} step(s) identical to the timed ones, every "
+                                 "level-0 launch timed by HIP events on the solver's stream")
+    roofline["traffic_source"] = (f"{TRAFFIC_PROFILE}: FETCH_SIZE/WRITE_SIZE over the same "
+                                  "run, calibrated on fs_stream_probe")
+    stencil = roof("stencil") if "stencil" in kt else None
+    if stencil is not None and not fp64 and g.shape_nodes[0] % 4 == 0:
+        # the rate at which this device streams the stencil's read/write mix
+        # (3 couplings + nt p read, nt q written) once, without neighbours
+        from fieldspice_amd import _lib
+        n4 = g.n_nodes - g.n_nodes % 4
+        sec = _lib.stream_probe(n4, 3 + nt, nt, reps=20)
+        ceil = n4 * 4.0 * (3 + 2 * nt) / sec / 1e9
+        stencil["stream_ceiling_gbs"] = round(ceil, 1)
+        stencil["stream_ceiling_frac"] = round(ceil / HBM_PEAK_GBS, 4)
+        stencil["frac_of_stream_ceiling"] = round(kt["stencil"]["achieved_gbs"] / ceil, 4)
     cpu = None
     if not args.no_cpu and ref is not None:
         log("CPU oracle baseline (bounded sample)")
         cpu = cpu_baseline(ex, max(ref["iterations"], 1))
+    for r in res.values():
+        r.pop("prof", None)
     out = {
         "metric": METRIC,
         "value": b["s_per_solve"],
@@ -495,6 +562,7 @@ def main():
         "methods": {m: {k: v for k, v in r.items() if k not in ("C", "prof")}
                     for m, r in res.items()},
         "roofline": roofline,
+        "stencil_roofline": stencil,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -29,7 +29,7 @@ FS_LOSS_CYCLE = 0
 FS_LOSS_BOTTOM = 1
 
 EXPORTS = (
-    "fs_ctx_create", "fs_ctx_destroy", "fs_last_error", "fs_ctx_device_info",
+    "fs_ctx_create", "fs_ctx_destroy", "fs_last_error", "fs_ctx_device_info", "fs_stream_probe",
     "fs_op_create_edges", "fs_op_create_cells", "fs_op_combine", "fs_op_destroy",
     "fs_op_apply", "fs_op_get_edges", "fs_sys_create", "fs_sys_destroy",
     "fs_sys_solve", "fs_capacitance", "fs_cap_prepare", "fs_cap_solve", "fs_cap_destroy",
@@ -44,6 +44,11 @@ EXPORTS = (
 )
 
 
+# fs_solve_stats.kernel_* (FS_PROF_* in include/fieldspice_hip.h)
+PROF_KINDS = 5
+PROF_NAMES = ("stencil", "direction", "residual", "restrict", "prolong")
+
+
 class SolveParams(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int), ("precision", C.c_int),
                 ("inner_tol", C.c_double), ("check_every", C.c_int)]
@@ -56,10 +61,16 @@ class SolveStats(C.Structure):
                 ("stencil_launches", C.c_int64), ("stencil_bytes", C.c_double),
                 ("fallbacks", C.c_int), ("pc_applies", C.c_int), ("stop_reason", C.c_int),
                 ("maxiter", C.c_int), ("replacements", C.c_int), ("stencil_width", C.c_int),
-                ("cycle_width", C.c_int)]
+                ("cycle_width", C.c_int), ("kernel_time", C.c_double * PROF_KINDS),
+                ("kernel_launches", C.c_int64 * PROF_KINDS),
+                ("kernel_bytes", C.c_double * PROF_KINDS)]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = list(v) if isinstance(v, C.Array) else v
+        return out
 
 
 class SysInfo(C.Structure):
@@ -102,6 +113,8 @@ def _declare(lib):
     lib.fs_ctx_create.argtypes = [C.c_int, C.POINTER(_vp)]
     lib.fs_ctx_destroy.argtypes = [_vp]
     lib.fs_ctx_device_info.argtypes = [_vp, C.c_char_p, C.c_int, C.POINTER(C.c_int)]
+    lib.fs_stream_probe.argtypes = [_vp, C.c_int64, C.c_int, C.c_int, C.c_int,
+                                    C.POINTER(C.c_double)]
     lib.fs_op_create_edges.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
                                        _vp, C.POINTER(_vp)]
     lib.fs_op_create_cells.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
@@ -206,6 +219,15 @@ def device_info() -> tuple[str, int]:
     ncu = C.c_int(0)
     check(lib.fs_ctx_device_info(context(), buf, 256, C.byref(ncu)), "fs_ctx_device_info")
     return buf.value.decode(), ncu.value
+
+
+def stream_probe(n: int, nread: int, nwrite: int, reps: int = 20) -> float:
+    """Seconds per launch of a kernel streaming nread + nwrite FP32 arrays of
+    n elements once (fs_stream_probe)."""
+    sec = C.c_double(0.0)
+    check(load().fs_stream_probe(context(), int(n), int(nread), int(nwrite), int(reps),
+                                 C.byref(sec)), "fs_stream_probe")
+    return sec.value
 
 
 def as_f64(a) -> np.ndarray:

@@ -158,31 +158,54 @@ void free_sys(fs_sys* s) {
   dfree(s->sc);
   dfree(s->prof_dev);
   if (s->sc_host) (void)hipHostFree(s->sc_host);
-  for (int k = 0; k < s->ev_cap; ++k) {
-    (void)hipEventDestroy(s->ev_a[k]);
-    (void)hipEventDestroy(s->ev_b[k]);
-  }
-  delete[] s->ev_a;
-  delete[] s->ev_b;
+  for (int c = 0; c < s->ev_cap; ++c)
+    for (int k = 0; k < FS_PROF_KINDS; ++k) {
+      (void)hipEventDestroy(s->prof_slots[c].a[k]);
+      (void)hipEventDestroy(s->prof_slots[c].b[k]);
+    }
+  delete[] s->prof_slots;
   delete s;
 }
 
 constexpr int kMaxChunk = 512;
 
 void prof_begin(fs_sys* s) {
-  s->prof_time = 0.0;
-  s->prof_launches = 0;
-  if (s->prof_dev) (void)hipMemsetAsync(s->prof_dev, 0, 2 * sizeof(unsigned long long), s->ctx->stream);
+  for (int k = 0; k < FS_PROF_KINDS; ++k) {
+    s->prof_time[k] = 0.0;
+    s->prof_launches[k] = 0;
+  }
+  if (s->prof_dev)
+    (void)hipMemsetAsync(s->prof_dev, 0, 2 * FS_PROF_KINDS * sizeof(unsigned long long),
+                         s->ctx->stream);
 }
 
+// Algorithmic bytes of each profiled kernel kind from its launches with work
+// (L) and live right-hand sides summed (R), n nodes, elt bytes per Krylov
+// value (the cycle kernels are FP32), n1 nodes on level 1:
+//   stencil   3 couplings once, p read and q written per RHS
+//   direction dir, p and x read, p and x written per RHS
+//   residual  r and q read, r written per RHS
+//   restrict  3 couplings once; r read and the level-1 residual written per RHS
+//   prolong   3 couplings and the mask once; r and q read, z written and the
+//             level-1 error read per RHS
 void prof_end(fs_sys* s, fs_solve_stats* st, bool mixed) {
   if (!st || !s->profiling || !s->prof_dev) return;
-  unsigned long long h[2] = {0, 0};
+  unsigned long long h[2 * FS_PROF_KINDS] = {};
   if (hipMemcpy(h, s->prof_dev, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) return;
-  const double elt = mixed ? 4.0 : 8.0;
-  st->stencil_time += s->prof_time;
-  st->stencil_launches += s->prof_launches;
-  st->stencil_bytes += (double)s->d.n * (3.0 * elt * (double)h[0] + 2.0 * elt * (double)h[1]);
+  const double n = (double)s->d.n, elt = mixed ? 4.0 : 8.0;
+  const double n1 = (double)fs::pc_level1_nodes(s->pc);
+  const double fixed[FS_PROF_KINDS] = {3.0 * elt * n, 0.0, 0.0, 12.0 * n, 13.0 * n};
+  const double per_rhs[FS_PROF_KINDS] = {2.0 * elt * n, 5.0 * elt * n, 3.0 * elt * n,
+                                         4.0 * n + 4.0 * n1, 12.0 * n + 4.0 * n1};
+  for (int k = 0; k < FS_PROF_KINDS; ++k) {
+    const double bytes = fixed[k] * (double)h[2 * k] + per_rhs[k] * (double)h[2 * k + 1];
+    st->kernel_time[k] += s->prof_time[k];
+    st->kernel_launches[k] += s->prof_launches[k];
+    st->kernel_bytes[k] += bytes;
+  }
+  st->stencil_time += s->prof_time[FS_PROF_STENCIL];
+  st->stencil_launches += s->prof_launches[FS_PROF_STENCIL];
+  st->stencil_bytes += fixed[FS_PROF_STENCIL] * (double)h[0] + per_rhs[FS_PROF_STENCIL] * (double)h[1];
 }
 
 int ensure_ws(fs_sys* s, int nr) {
@@ -286,14 +309,27 @@ static int run_krylov_loop(fs_sys* s, int nr, const fs_solve_params& P, int mode
     return true;
   };
   if (all([](const fs::CgScalars& c) { return c.done != 0; })) return FS_OK;
+  // profiling: the first preconditioner application of each flexible-CG
+  // cycle is timed in the slot after the chunk's
+  fs::ProfSlot* start_sl = s->profiling && s->prof_slots ? &s->prof_slots[kMaxChunk] : nullptr;
   auto start_cycle = [&]() -> int {
     fs::launch_cycle_start(s, nr, inner_tol * inner_tol);
     if (mode == 2) {
-      if (int rc = fs::launch_fcg_start(s, nr)) return rc;
+      if (start_sl) start_sl->used = 0;
+      if (int rc = fs::launch_fcg_start(s, nr, start_sl)) return rc;
       s->acc_pc_applies += 1;
     }
     FS_HIP(hipGetLastError());
     return FS_OK;
+  };
+  auto add_times = [&](fs::ProfSlot& sl) {
+    for (int k = 0; k < FS_PROF_KINDS; ++k) {
+      if (!((sl.used >> k) & 1u)) continue;
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, sl.a[k], sl.b[k]) == hipSuccess) s->prof_time[k] += 1e-3 * ms;
+      s->prof_launches[k] += 1;
+    }
+    sl.used = 0;
   };
   if (cyc) {
     if (int rc = start_cycle()) return rc;
@@ -311,21 +347,16 @@ static int run_krylov_loop(fs_sys* s, int nr, const fs_solve_params& P, int mode
   for (int guard = 0; guard < 4 * maxiter + 64; ++guard) {
     const bool prof = s->profiling && chunk <= s->ev_cap;
     for (int c = 0; c < chunk; ++c) {
-      if (int rc = fs::launch_iteration(s, nr, mode, prof ? s->ev_a[c] : nullptr,
-                                        prof ? s->ev_b[c] : nullptr))
-        return rc;
+      fs::ProfSlot* sl = prof ? &s->prof_slots[c] : nullptr;
+      if (sl) sl->used = 0;
+      if (int rc = fs::launch_iteration(s, nr, mode, sl)) return rc;
       if (mode == 2) s->acc_pc_applies += 1;
     }
     FS_HIP(hipGetLastError());
     if (int rc = fetch()) return rc;
-    if (prof) {
-      for (int c = 0; c < chunk; ++c) {
-        float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, s->ev_a[c], s->ev_b[c]) == hipSuccess)
-          s->prof_time += 1e-3 * ms;
-      }
-      s->prof_launches += chunk;
-    }
+    if (prof)
+      for (int c = 0; c < chunk; ++c) add_times(s->prof_slots[c]);
+    if (start_sl && start_sl->used) add_times(*start_sl);
     if (!cyc) {
       if (all([](const fs::CgScalars& c) { return c.done != 0; })) return FS_OK;
     } else {
@@ -605,6 +636,52 @@ int fs_ctx_device_info(fs_ctx* ctx, char* name, int name_len, int* n_cu) {
   if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", ctx->name);
   if (n_cu) *n_cu = ctx->n_cu;
   return FS_OK;
+}
+
+int fs_stream_probe(fs_ctx* ctx, int64_t n, int nread, int nwrite, int reps, double* seconds) {
+  FS_GUARD_BEGIN
+  if (!ctx || !seconds) return fail(FS_EINVAL, "NULL argument");
+  if (n <= 0 || n % 4 != 0 || reps <= 0)
+    return fail(FS_EINVAL, "n must be a positive multiple of 4 and reps positive");
+  const bool mix = (nread == 1 && nwrite == 1) || (nwrite >= 1 && nwrite <= 8 && nread == 3 + nwrite);
+  if (!mix) return fail(FS_EINVAL, "unsupported mix: %d read, %d written", nread, nwrite);
+  FS_HIP(hipSetDevice(ctx->device));
+  std::vector<float*> bufs(nread + nwrite, nullptr);
+  int rc = FS_OK;
+  for (auto& b : bufs)
+    if ((rc = dalloc(&b, (size_t)n))) break;
+  auto release = [&] {
+    for (auto& b : bufs) dfree(b);
+  };
+  if (rc) {
+    release();
+    return rc;
+  }
+  for (int k = 0; k < nread; ++k) {
+    if (hipMemsetAsync(bufs[k], 0, (size_t)n * sizeof(float), ctx->stream) != hipSuccess) {
+      release();
+      return fail(FS_EHIP, "probe buffer initialisation failed");
+    }
+  }
+  const int nblocks = std::max(1, ctx->n_cu) * 8;
+  const float* const* in = (const float* const*)bufs.data();
+  float* const* out = bufs.data() + nread;
+  fs::launch_stream_probe(ctx->stream, in, out, nread, nwrite, n, nblocks);  // warm-up
+  (void)hipEventRecord(ctx->ev0, ctx->stream);
+  for (int r = 0; r < reps; ++r)
+    fs::launch_stream_probe(ctx->stream, in, out, nread, nwrite, n, nblocks);
+  (void)hipEventRecord(ctx->ev1, ctx->stream);
+  const hipError_t e = hipEventSynchronize(ctx->ev1);
+  float ms = 0.0f;
+  if (e == hipSuccess) (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  const hipError_t le = hipGetLastError();
+  release();
+  if (e != hipSuccess || le != hipSuccess)
+    return fail(FS_EHIP, "streaming probe failed: %s",
+                hipGetErrorString(e != hipSuccess ? e : le));
+  *seconds = 1e-3 * (double)ms / reps;
+  return FS_OK;
+  FS_GUARD_END
 }
 
 int fs_op_create_edges(fs_ctx* ctx, int Nx, int Ny, int Nz, const double* w_x,
@@ -1514,16 +1591,18 @@ int fs_sys_set_profiling(fs_sys* s, int on) {
   FS_GUARD_BEGIN
   if (!s) return fail(FS_EINVAL, "NULL argument");
   FS_HIP(hipSetDevice(s->ctx->device));
-  if (on && s->ev_cap == 0) {
-    s->ev_a = new hipEvent_t[kMaxChunk];
-    s->ev_b = new hipEvent_t[kMaxChunk];
-    for (int k = 0; k < kMaxChunk; ++k) {
-      FS_HIP(hipEventCreate(&s->ev_a[k]));
-      FS_HIP(hipEventCreate(&s->ev_b[k]));
-      s->ev_cap = k + 1;
+  if (on && s->prof_slots == nullptr) {
+    // one slot per iteration of a chunk, and one for the start of a cycle
+    s->prof_slots = new fs::ProfSlot[kMaxChunk + 1];
+    for (int c = 0; c <= kMaxChunk; ++c) {
+      for (int k = 0; k < FS_PROF_KINDS; ++k) {
+        FS_HIP(hipEventCreate(&s->prof_slots[c].a[k]));
+        FS_HIP(hipEventCreate(&s->prof_slots[c].b[k]));
+      }
+      s->ev_cap = c + 1;
     }
-    if (int rc = dalloc(&s->prof_dev, 2)) return rc;
-    FS_HIP(hipMemset(s->prof_dev, 0, 2 * sizeof(unsigned long long)));
+    if (int rc = dalloc(&s->prof_dev, 2 * FS_PROF_KINDS)) return rc;
+    FS_HIP(hipMemset(s->prof_dev, 0, 2 * FS_PROF_KINDS * sizeof(unsigned long long)));
   }
   s->profiling = on ? 1 : 0;
   return FS_OK;
@@ -1825,7 +1904,7 @@ int fs_batch_solve(fs_batch* b, const fs_solve_params* params, int chunk, double
       fs_sys* s = b->inst[i].cap->sys;
       s->stream = b->streams[i % nst];
       for (int c = 0; c < chunk && crc == FS_OK; ++c)
-        crc = fs::launch_iteration(s, b->inst[i].cap->nt, mode, nullptr, nullptr);
+        crc = fs::launch_iteration(s, b->inst[i].cap->nt, mode, nullptr);
       s->stream = main;
     }
     for (int k = 0; k < nst && crc == FS_OK; ++k) {

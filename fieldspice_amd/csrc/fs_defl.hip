@@ -539,11 +539,12 @@ static void defl_scatter_t(fs_sys* s, int nr, int mode) {
 }
 
 template <int NR>
-static int iteration_defl_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {
+static int iteration_defl_t(fs_sys* s, int nr, int mode, ProfSlot* sl) {
   hipStream_t st = s->stream;
   VecArgs va{s->d.n, s->vstride, nr};
   const int vb = s->vec_blocks, sb = s->tl.nblocks;
-  unsigned long long* prof = ea ? s->prof_dev : nullptr;
+  hipEvent_t ea = prof_start(sl, FS_PROF_STENCIL), eb = prof_stop(sl, FS_PROF_STENCIL);
+  unsigned long long* prof = prof_ctr(s, sl, FS_PROF_STENCIL);
   // FP32 cycle on unscaled potentials: p = (D^-1 + 1_I E^-1 1_I^T) r + beta p
   hipLaunchKernelGGL((k_pupdate_w<NR>), dim3(vb), dim3(kBlock), 0, st, s->p32, s->x32, s->r32,
                      s->u_dinv, va, mode, s->sc);
@@ -575,11 +576,11 @@ static int iteration_defl_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent
     else FN<8>(__VA_ARGS__);                 \
   } while (0)
 
-int launch_iteration_defl(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {
-  if (nr <= 1) return iteration_defl_t<1>(s, nr, mode, ea, eb);
-  if (nr <= 2) return iteration_defl_t<2>(s, nr, mode, ea, eb);
-  if (nr <= 4) return iteration_defl_t<4>(s, nr, mode, ea, eb);
-  return iteration_defl_t<8>(s, nr, mode, ea, eb);
+int launch_iteration_defl(fs_sys* s, int nr, int mode, ProfSlot* slot) {
+  if (nr <= 1) return iteration_defl_t<1>(s, nr, mode, slot);
+  if (nr <= 2) return iteration_defl_t<2>(s, nr, mode, slot);
+  if (nr <= 4) return iteration_defl_t<4>(s, nr, mode, slot);
+  return iteration_defl_t<8>(s, nr, mode, slot);
 }
 
 void launch_defl_coef(fs_sys* s, int nr, int mode, int first) {

@@ -65,6 +65,14 @@ struct CgScalars {
   int pad0, pad1;
 };
 
+// Events of one profiled Krylov iteration: one start/stop pair per kernel
+// kind (FS_PROF_*), and the kinds launched with them in the current chunk.
+struct ProfSlot {
+  hipEvent_t a[FS_PROF_KINDS] = {};
+  hipEvent_t b[FS_PROF_KINDS] = {};
+  unsigned used = 0;
+};
+
 }  // namespace fs
 
 // --------------------------------------------------------------------------
@@ -118,14 +126,14 @@ struct fs_sys {
   fs::CgScalars* sc = nullptr;       // device scalars (kMaxRhs)
   fs::CgScalars* sc_host = nullptr;  // pinned host mirror
   int vec_blocks = 0;                // grid of the node-wise vector kernels
-  // optional timing of the stencil launches of the Krylov loop
+  // optional timing of the full-grid kernels of the Krylov loop (FS_PROF_*)
   int profiling = 0;
-  int ev_cap = 0;
-  hipEvent_t* ev_a = nullptr;
-  hipEvent_t* ev_b = nullptr;
-  double prof_time = 0.0;
-  int64_t prof_launches = 0;
-  unsigned long long* prof_dev = nullptr;  // [0] live launches, [1] live RHS summed
+  int ev_cap = 0;                        // profiled iterations per chunk
+  fs::ProfSlot* prof_slots = nullptr;    // ev_cap slots
+  double prof_time[FS_PROF_KINDS] = {};
+  int64_t prof_launches[FS_PROF_KINDS] = {};
+  // per kind k: [2k] launches that found a live RHS, [2k+1] live RHS summed
+  unsigned long long* prof_dev = nullptr;
   // flexible CG
   fs_pc* pc = nullptr;        // attached preconditioner (not owned)
   std::vector<fs_pc*> pcs;    // every preconditioner built on this system

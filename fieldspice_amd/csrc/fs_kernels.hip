@@ -702,11 +702,24 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot2(const double* __restric
 //         rho kept in aux0, sets cycle_done; the iteration cap still ends it).
 // DX: x += alpha p is deferred to the next direction update (k_pupdate DX),
 // which reads p anyway; the refresh adds the pending term of the last step.
+// Launches with work and their live right-hand sides, counted by block 0 when
+// the profiling counters are given.
+template <int NR>
+__device__ __forceinline__ void prof_count(unsigned long long* prof, const bool (&live)[NR]) {
+  if (prof && blockIdx.x == 0 && threadIdx.x == 0) {
+    unsigned long long nl = 0;
+    for (int r = 0; r < NR; ++r) nl += live[r] ? 1ull : 0ull;
+    prof[0] += 1ull;
+    prof[1] += nl;
+  }
+}
+
 template <typename T, int NR, bool DX = false>
 __global__ __launch_bounds__(kBlock) void k_update(T* __restrict__ x, T* __restrict__ r_,
                                                    const T* __restrict__ p,
                                                    const T* __restrict__ q, VecArgs va,
-                                                   int mode, CgScalars* sc, double* partials) {
+                                                   int mode, CgScalars* sc, double* partials,
+                                                   unsigned long long* prof = nullptr) {
   if (all_idle(sc, va.nr, mode)) return;
   double acc[NR];
 #pragma unroll
@@ -714,6 +727,7 @@ __global__ __launch_bounds__(kBlock) void k_update(T* __restrict__ x, T* __restr
   bool live[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) live[r] = r < va.nr && rhs_live(sc, r, mode);
+  prof_count<NR>(prof, live);
   const int64_t stride = (int64_t)gridDim.x * kBlock;
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
@@ -761,8 +775,15 @@ __global__ __launch_bounds__(kBlock) void k_flush_x(double* __restrict__ x,
 template <typename T, int NR, bool DX = false>
 __global__ __launch_bounds__(kBlock) void k_pupdate(T* __restrict__ p, const T* __restrict__ r_,
                                                     VecArgs va, int mode, const CgScalars* sc,
-                                                    T* __restrict__ x = nullptr) {
+                                                    T* __restrict__ x = nullptr,
+                                                    unsigned long long* prof = nullptr) {
   if (all_idle(sc, va.nr, mode)) return;
+  if (prof) {
+    bool live[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) live[r] = r < va.nr && rhs_live(sc, r, mode);
+    prof_count<NR>(prof, live);
+  }
   const int64_t stride = (int64_t)gridDim.x * kBlock;
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
@@ -1315,6 +1336,70 @@ __global__ __launch_bounds__(kBlock) void k_charge_final(const double* __restric
 }
 
 // --------------------------------------------------------------------------
+// Streaming probe (measurement aid): NI FP32 input streams read and NO output
+// streams written once, 16 bytes per lane and access, grid-stride, the
+// outputs stored non-temporally as the stencil stores q.  out[o] =
+// in[o % NI] * sum(in) keeps every load live.
+// --------------------------------------------------------------------------
+struct StreamArrays {
+  const float* in[16];
+  float* out[16];
+};
+
+template <int NI, int NO>
+__global__ __launch_bounds__(kBlock) void k_stream_probe(StreamArrays a, int64_t n4) {
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n4;
+       t += (int64_t)gridDim.x * kBlock) {
+    float4 v[NI];
+#pragma unroll
+    for (int k = 0; k < NI; ++k) v[k] = *reinterpret_cast<const float4*>(a.in[k] + 4 * t);
+    float4 sum = v[0];
+#pragma unroll
+    for (int k = 1; k < NI; ++k) {
+      sum.x += v[k].x;
+      sum.y += v[k].y;
+      sum.z += v[k].z;
+      sum.w += v[k].w;
+    }
+#pragma unroll
+    for (int o = 0; o < NO; ++o) {
+      const float4 w = v[o % NI];
+      f4n y;
+      y.x = w.x * sum.x;
+      y.y = w.y * sum.y;
+      y.z = w.z * sum.z;
+      y.w = w.w * sum.w;
+      __builtin_nontemporal_store(y, reinterpret_cast<f4n*>(a.out[o] + 4 * t));
+    }
+  }
+}
+
+// the stencil's mix for k live right-hand sides (3 + k read, k written), and a copy
+int launch_stream_probe(hipStream_t st, const float* const* in, float* const* out, int nread,
+                        int nwrite, int64_t n, int nblocks) {
+  StreamArrays a{};
+  for (int k = 0; k < nread; ++k) a.in[k] = in[k];
+  for (int k = 0; k < nwrite; ++k) a.out[k] = out[k];
+  const int64_t n4 = n / 4;
+#define FS_SP(I, O)                                                                     \
+  if (nread == (I) && nwrite == (O)) {                                                  \
+    hipLaunchKernelGGL((k_stream_probe<I, O>), dim3(nblocks), dim3(kBlock), 0, st, a, n4); \
+    return 0;                                                                           \
+  }
+  FS_SP(1, 1)
+  FS_SP(4, 1)
+  FS_SP(5, 2)
+  FS_SP(6, 3)
+  FS_SP(7, 4)
+  FS_SP(8, 5)
+  FS_SP(9, 6)
+  FS_SP(10, 7)
+  FS_SP(11, 8)
+#undef FS_SP
+  return -1;
+}
+
+// --------------------------------------------------------------------------
 // host launchers
 // --------------------------------------------------------------------------
 // The 4-wide FP32 stencil holds 8 registers per right-hand side and lane:
@@ -1365,56 +1450,65 @@ static void launch_dot2(hipStream_t st, fs_sys* s, const VecArgs& va, int mode,
 }
 
 template <int NR>
-static int launch_iteration_t(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {
+static int launch_iteration_t(fs_sys* s, int nr, int mode, ProfSlot* sl) {
   hipStream_t st = s->stream;
   VecArgs va{s->d.n, s->vstride, nr};
   const int vb = s->vec_blocks;
   const int sb = s->tl.nblocks;
-  unsigned long long* prof = ea ? s->prof_dev : nullptr;
   if (mode == 0) {
     // with an island space: p = (I + Z E^-1 Z^T) r + beta p, alpha = zr/pq
     const bool defl = s->defl_n > 0;
     // x += alpha p of the previous step rides on the direction update (DX);
     // the last pending step is added by launch_flush_x when the loop ends
-    hipLaunchKernelGGL((k_pupdate<double, NR, true>), dim3(vb), dim3(kBlock), 0, st, s->p, s->r,
-                       va, mode, s->sc, s->x);
+    launch_timed(k_pupdate<double, NR, true>, dim3(vb), st, prof_start(sl, FS_PROF_DIRECTION),
+                 prof_stop(sl, FS_PROF_DIRECTION), s->p, (const double*)s->r, va, mode,
+                 (const CgScalars*)s->sc, s->x, prof_ctr(s, sl, FS_PROF_DIRECTION));
     if (defl) launch_defl_scatter(s, nr, mode);
     int sbd = sb;
+    hipEvent_t ea = prof_start(sl, FS_PROF_STENCIL), eb = prof_stop(sl, FS_PROF_STENCIL);
+    unsigned long long* pst = prof_ctr(s, sl, FS_PROF_STENCIL);
     if (s->use_x2) {
       sbd = s->tl2.nblocks;
-      launch_dot2(st, s, va, mode, prof, ea, eb);
+      launch_dot2(st, s, va, mode, pst, ea, eb);
     } else {
       launch_timed(k_stencil_dot<double, NR>, dim3(sb), st, ea, eb, (const double*)s->p, s->q,
                    (const double*)s->c64[0], (const double*)s->c64[1], (const double*)s->c64[2],
-                   s->d, s->tl, va, mode, s->sc, s->partials, prof);
+                   s->d, s->tl, va, mode, s->sc, s->partials, pst);
     }
     finalize<NR>(st, s->partials, defl ? FIN_ALPHA_ZR : FIN_ALPHA, nr, sbd, mode, s->sc);
-    hipLaunchKernelGGL((k_update<double, NR, true>), dim3(vb), dim3(kBlock), 0, st, s->x, s->r,
-                       s->p, s->q, va, mode, s->sc, s->partials);
+    launch_timed(k_update<double, NR, true>, dim3(vb), st, prof_start(sl, FS_PROF_RESIDUAL),
+                 prof_stop(sl, FS_PROF_RESIDUAL), s->x, s->r, (const double*)s->p,
+                 (const double*)s->q, va, mode, s->sc, s->partials,
+                 prof_ctr(s, sl, FS_PROF_RESIDUAL));
     finalize<NR>(st, s->partials, FIN_BETA, nr, vb, mode, s->sc);
     if (defl) launch_defl_coef(s, nr, mode, 0);
     return 0;
   }
   // FP32 cycle: CG searches along r32, flexible CG along z32 = M(r32)
   const float* dir = mode == 2 ? s->z32 : s->r32;
-  hipLaunchKernelGGL((k_pupdate<float, NR, true>), dim3(vb), dim3(kBlock), 0, st, s->p32, dir, va,
-                     mode, s->sc, s->x32);
+  launch_timed(k_pupdate<float, NR, true>, dim3(vb), st, prof_start(sl, FS_PROF_DIRECTION),
+               prof_stop(sl, FS_PROF_DIRECTION), s->p32, dir, va, mode, (const CgScalars*)s->sc,
+               s->x32, prof_ctr(s, sl, FS_PROF_DIRECTION));
   int sbf = sb;
+  hipEvent_t ea = prof_start(sl, FS_PROF_STENCIL), eb = prof_stop(sl, FS_PROF_STENCIL);
+  unsigned long long* pst = prof_ctr(s, sl, FS_PROF_STENCIL);
   if (s->use_x4) {
     sbf = s->tl4.nblocks;
-    launch_dot4(st, s, s->tl4, va, mode, prof, ea, eb);
+    launch_dot4(st, s, s->tl4, va, mode, pst, ea, eb);
   } else {
     launch_timed(k_stencil_dot<float, NR>, dim3(sb), st, ea, eb, (const float*)s->p32, s->q32,
                  (const float*)s->c32[0], (const float*)s->c32[1], (const float*)s->c32[2],
-                 s->d, s->tl, va, mode, s->sc, s->partials, prof);
+                 s->d, s->tl, va, mode, s->sc, s->partials, pst);
   }
   finalize<NR>(st, s->partials, FIN_ALPHA, nr, sbf, mode, s->sc);
-  hipLaunchKernelGGL((k_update<float, NR, true>), dim3(vb), dim3(kBlock), 0, st, s->x32, s->r32, s->p32,
-                     s->q32, va, mode, s->sc, s->partials);
+  launch_timed(k_update<float, NR, true>, dim3(vb), st, prof_start(sl, FS_PROF_RESIDUAL),
+               prof_stop(sl, FS_PROF_RESIDUAL), s->x32, s->r32, (const float*)s->p32,
+               (const float*)s->q32, va, mode, s->sc, s->partials,
+               prof_ctr(s, sl, FS_PROF_RESIDUAL));
   finalize<NR>(st, s->partials, FIN_BETA, nr, vb, mode, s->sc);
   if (mode == 2) {
     int nbp = 0;
-    if (int rc = pc_apply_solver(s, nr, &nbp)) return rc;
+    if (int rc = pc_apply_solver(s, nr, &nbp, sl)) return rc;
     finalize<NR>(st, s->partials, FIN_FCG, nr, nbp, mode, s->sc);
   }
   return 0;
@@ -1455,12 +1549,12 @@ void launch_replace(fs_sys* s, int nr, unsigned restart) {
   hipLaunchKernelGGL(k_replace, dim3(1), dim3(64), 0, s->stream, nr, restart, s->sc);
 }
 
-int launch_iteration(fs_sys* s, int nr, int mode, hipEvent_t ea, hipEvent_t eb) {
-  if (s->defl_n > 0 && mode == 1) return launch_iteration_defl(s, nr, mode, ea, eb);
-  if (nr <= 1) return launch_iteration_t<1>(s, nr, mode, ea, eb);
-  if (nr <= 2) return launch_iteration_t<2>(s, nr, mode, ea, eb);
-  if (nr <= 4) return launch_iteration_t<4>(s, nr, mode, ea, eb);
-  return launch_iteration_t<8>(s, nr, mode, ea, eb);
+int launch_iteration(fs_sys* s, int nr, int mode, ProfSlot* slot) {
+  if (s->defl_n > 0 && mode == 1) return launch_iteration_defl(s, nr, mode, slot);
+  if (nr <= 1) return launch_iteration_t<1>(s, nr, mode, slot);
+  if (nr <= 2) return launch_iteration_t<2>(s, nr, mode, slot);
+  if (nr <= 4) return launch_iteration_t<4>(s, nr, mode, slot);
+  return launch_iteration_t<8>(s, nr, mode, slot);
 }
 
 template <int NR>
@@ -1553,17 +1647,17 @@ static void launch_cycle_start_t(fs_sys* s, int nr, double inner_tol2) {
                0.0, 0, inner_tol2);
 }
 template <int NR>
-static int launch_fcg_start_t(fs_sys* s, int nr) {
+static int launch_fcg_start_t(fs_sys* s, int nr, ProfSlot* slot) {
   int nbp = 0;
-  if (int rc = pc_apply_solver(s, nr, &nbp)) return rc;
+  if (int rc = pc_apply_solver(s, nr, &nbp, slot)) return rc;
   finalize<NR>(s->stream, s->partials, FIN_FCG0, nr, nbp, 2, s->sc);
   return 0;
 }
-int launch_fcg_start(fs_sys* s, int nr) {
-  if (nr <= 1) return launch_fcg_start_t<1>(s, nr);
-  if (nr <= 2) return launch_fcg_start_t<2>(s, nr);
-  if (nr <= 4) return launch_fcg_start_t<4>(s, nr);
-  return launch_fcg_start_t<8>(s, nr);
+int launch_fcg_start(fs_sys* s, int nr, ProfSlot* slot) {
+  if (nr <= 1) return launch_fcg_start_t<1>(s, nr, slot);
+  if (nr <= 2) return launch_fcg_start_t<2>(s, nr, slot);
+  if (nr <= 4) return launch_fcg_start_t<4>(s, nr, slot);
+  return launch_fcg_start_t<8>(s, nr, slot);
 }
 template <int NR>
 static void launch_stencil_only_t(fs_sys* s, int nr, const Tiling& tl, bool x4) {

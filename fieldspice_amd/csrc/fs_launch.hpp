@@ -22,6 +22,19 @@ inline void launch_timed(K kernel, dim3 grid, hipStream_t st, hipEvent_t ea, hip
     hipLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, st, args...);
 }
 
+// Profiling of one kernel kind of a Krylov iteration: the start and stop
+// events of its launch (null without a slot; the start marks the kind as
+// launched) and its device counters.
+inline hipEvent_t prof_start(ProfSlot* sl, int kind) {
+  if (!sl) return nullptr;
+  sl->used |= 1u << kind;
+  return sl->a[kind];
+}
+inline hipEvent_t prof_stop(ProfSlot* sl, int kind) { return sl ? sl->b[kind] : nullptr; }
+inline unsigned long long* prof_ctr(const fs_sys* s, ProfSlot* sl, int kind) {
+  return sl ? s->prof_dev + 2 * kind : nullptr;
+}
+
 template <typename T>
 void permute_in(hipStream_t s, const T* in, T* out, int A, int B, int C, int OA, int OB);
 template <typename T>
@@ -54,7 +67,7 @@ void launch_finalize(hipStream_t st, const double* partials, int op, int nr, int
                      CgScalars* sc, double inner_tol2 = 0.0);
 
 // mode 0 FP64 CG, 1 FP32 CG cycle, 2 FP32 flexible CG cycle (returns a status)
-int launch_iteration(fs_sys* s, int nr, int mode, hipEvent_t ev_a, hipEvent_t ev_b);
+int launch_iteration(fs_sys* s, int nr, int mode, ProfSlot* slot);
 // FP64 CG (mode 0) defers x += alpha p into the next direction update; this adds
 // the last pending step, once, when the loop ends
 void launch_flush_x(fs_sys* s, int nr);
@@ -71,14 +84,14 @@ void launch_expand(fs_sys* s, int nr, const double* b, const int32_t* tid, int j
 void launch_residual(fs_sys* s, int nr, double* res);
 void launch_cycle_start(fs_sys* s, int nr, double inner_tol2);
 // flexible CG: z32 = M(r32) and <z,r> at the start of an FP32 cycle
-int launch_fcg_start(fs_sys* s, int nr);
+int launch_fcg_start(fs_sys* s, int nr, ProfSlot* slot = nullptr);
 void launch_refresh(fs_sys* s, int nr);
 
 // island coarse space (fs_defl.hip).  Mode 0 of a system with defl_n > 0
 // adds Z E^-1 Z^T r to each search direction (launch_defl_scatter after the
 // p update, launch_defl_coef after the residual update); its FP32 cycles
 // (mode 1) run launch_iteration_defl on the unscaled difference-form operator.
-int launch_iteration_defl(fs_sys* s, int nr, int mode, hipEvent_t ev_a, hipEvent_t ev_b);
+int launch_iteration_defl(fs_sys* s, int nr, int mode, ProfSlot* slot);
 // Z^T r -> E^-1 Z^T r and zr (first: no beta, at the start of an iteration run)
 void launch_defl_coef(fs_sys* s, int nr, int mode, int first);
 void launch_defl_scatter(fs_sys* s, int nr, int mode);
@@ -114,11 +127,21 @@ void launch_step_stats(fs_sys* s, const double* x, const fs_op* Qa, const fs_op*
                        double* out);
 void launch_edge_power(hipStream_t st, const fs_op* A, const double* x, double* q, int ncu);
 
+// streaming probe: nread inputs and nwrite outputs of n floats (n % 4 == 0);
+// supported mixes 1/1 and 3+k / k for k = 1..8; -1 otherwise
+int launch_stream_probe(hipStream_t st, const float* const* in, float* const* out, int nread,
+                        int nwrite, int64_t n, int nblocks);
+
 // learned preconditioner (fs_pc.hip)
+// With a profiling slot (and the system's counters), the level-0 restriction
+// and prolongation launches are timed as FS_PROF_RESTRICT / FS_PROF_PROLONG.
 int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, double* partials,
-             int* nblocks, const CgScalars* sc, int mode);
+             int* nblocks, const CgScalars* sc, int mode, ProfSlot* slot = nullptr,
+             unsigned long long* prof_dev = nullptr);
 // z32 = M(r32) with the FCG partials ([2*nr][*nblocks])
-int pc_apply_solver(fs_sys* s, int nr, int* nblocks);
+int pc_apply_solver(fs_sys* s, int nr, int* nblocks, ProfSlot* slot = nullptr);
+// nodes of the level below level 0 (0 without a hierarchy)
+int64_t pc_level1_nodes(const fs_pc* pc);
 // the system of pc is being destroyed: pc keeps only its own memory
 void pc_orphan(fs_pc* pc);
 // solver workspace of a system for nr right-hand sides (fs_api.hip)

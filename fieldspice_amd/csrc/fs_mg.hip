@@ -19,6 +19,7 @@
 
 #include "fs_device.hpp"
 #include "fs_internal.hpp"
+#include "fs_launch.hpp"
 #include "fs_nn.hpp"
 
 namespace fs {
@@ -385,6 +386,22 @@ __device__ __forceinline__ bool live_of(const CgScalars* sc, int r, int mode) {
   return sc == nullptr || rhs_live(sc, r, mode);
 }
 
+// Profiling counters of a level-0 launch carrying right-hand sides b0 ..
+// b0+nr-1 of nrt: [0] iterations that found a live right-hand side (counted
+// by the launch that carries b0 = 0), [1] live right-hand sides summed.
+__device__ __forceinline__ void prof_level0(unsigned long long* prof, const CgScalars* sc,
+                                            int mode, int b0, int nr, int nrt) {
+  if (!prof || blockIdx.x != 0 || threadIdx.x != 0) return;
+  unsigned long long mine = 0, any = 0;
+  for (int b = 0; b < nrt; ++b) {
+    const bool l = live_of(sc, b, mode);
+    any += l ? 1ull : 0ull;
+    if (b >= b0 && b < b0 + nr) mine += l ? 1ull : 0ull;
+  }
+  if (b0 == 0 && any) prof[0] += 1ull;
+  prof[1] += mine;
+}
+
 // e = z2 + omega (r - z2 + sum_m c_nm z2_m), z2 = omega r + ec[P(n)] on free
 // nodes; optional partials <e, r>, <e, q> ([2*nr][gridDim.x]).
 template <int NR>
@@ -393,7 +410,9 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0(
     float* __restrict__ e, const float* __restrict__ cx, const float* __restrict__ cy,
     const float* __restrict__ cz, const uint8_t* __restrict__ mask, Dims d, Tiling tl, Map m,
     int64_t vs, int nr, float omega, const float* __restrict__ q,
-    double* __restrict__ partials, const CgScalars* sc, int mode) {
+    double* __restrict__ partials, const CgScalars* sc, int mode,
+    unsigned long long* prof) {
+  prof_level0(prof, sc, mode, 0, nr, nr);
   double acc[2 * NR];
 #pragma unroll
   for (int a = 0; a < 2 * NR; ++a) acc[a] = 0.0;
@@ -480,7 +499,8 @@ __global__ __launch_bounds__(kBlock) void k_restrict_l0(
     const float* __restrict__ r, float* __restrict__ rc, int64_t vsc,
     const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
     Dims d, Tiling tl, const int* __restrict__ kstart, Map m, int64_t vs, int nr, float omega,
-    const CgScalars* sc, int mode) {
+    const CgScalars* sc, int mode, unsigned long long* prof) {
+  prof_level0(prof, sc, mode, 0, nr, nr);
   __shared__ float lds[kTileY][kTileX];
   int tx, ty, tz;
   if (!tile_of_block(tl, tx, ty, tz)) return;  // uniform over the block
@@ -577,7 +597,9 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
     const float* __restrict__ cz, const uint8_t* __restrict__ mask,
     const float* __restrict__ dgv, const float* __restrict__ dinvv, Dims d, Tiling tl, Map m,
     int64_t vs, int nr, float omega, const float* __restrict__ q,
-    double* __restrict__ partials, const CgScalars* sc, int mode, int b0, int nrt) {
+    double* __restrict__ partials, const CgScalars* sc, int mode, int b0, int nrt,
+    unsigned long long* prof) {
+  prof_level0(prof, sc, mode, b0, nr, nrt);
   double acc[2 * NR];
 #pragma unroll
   for (int a = 0; a < 2 * NR; ++a) acc[a] = 0.0;
@@ -773,7 +795,8 @@ __global__ __launch_bounds__(kBlock) void k_restrict_l0x4(
     const float* __restrict__ r, float* __restrict__ rc, int64_t vsc,
     const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
     const float* __restrict__ dinvv, Dims d, Tiling tl, const int* __restrict__ kstart, Map m,
-    int64_t vs, int nr, float omega, const CgScalars* sc, int mode) {
+    int64_t vs, int nr, float omega, const CgScalars* sc, int mode, unsigned long long* prof) {
+  prof_level0(prof, sc, mode, 0, nr, nr);
   __shared__ float2 lds[NR][kTileY][kWave];
   int tx, ty, tz;
   if (!tile_of_block(tl, tx, ty, tz)) return;  // uniform over the block
@@ -934,15 +957,18 @@ void energy_grad(hipStream_t s, const Level& L, const float* y, const float* r, 
 
 int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, int64_t vsc,
                float* e, int nr, float omega, const float* q, double* partials,
-               const CgScalars* sc, int mode) {
+               const CgScalars* sc, int mode, hipEvent_t ea, hipEvent_t eb,
+               unsigned long long* prof) {
   if (a.x4) {
     // ~16 registers per RHS and lane: launched for the exact RHS count, and
-    // in two halves above 5 RHS (8 in one launch would run 1 wave per SIMD)
-#define FS_L0P4(NR_, DG, B0, NB)                                                               \
-  hipLaunchKernelGGL((k_prolong_l0x4<NR_, DG>), dim3(a.tl4.nblocks), dim3(kBlock), 0, s,        \
-                     r + (B0) * a.vs, ec + (B0) * vsc, vsc, e + (B0) * a.vs, a.c[0], a.c[1],    \
-                     a.c[2], a.mask, a.dg, a.dinv, a.d, a.tl4, a.m, a.vs, NB, omega,           \
-                     q ? q + (B0) * a.vs : nullptr, partials, sc, mode, B0, nr)
+    // in two halves above 5 RHS (8 in one launch would run 1 wave per SIMD);
+    // the start event goes with the first launch, the stop event with the last
+    hipEvent_t e0 = ea, e1 = nullptr;
+#define FS_L0P4(NR_, DG, B0, NB)                                                                 \
+  launch_timed(k_prolong_l0x4<NR_, DG>, dim3(a.tl4.nblocks), s, e0, e1, r + (B0) * a.vs,          \
+               ec + (B0) * vsc, vsc, e + (B0) * a.vs, a.c[0], a.c[1], a.c[2], a.mask, a.dg,      \
+               a.dinv, a.d, a.tl4, a.m, a.vs, NB, omega, q ? q + (B0) * a.vs : nullptr,        \
+               partials, sc, mode, B0, nr, prof)
     auto part = [&](int b0, int nb) {
       if (a.dinv) {
         switch (nb) {
@@ -964,18 +990,20 @@ int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, 
     };
 #undef FS_L0P4
     if (nr <= 5) {
+      e1 = eb;
       part(0, nr);
     } else {
       const int h = (nr + 1) / 2;
       part(0, h);
+      e0 = nullptr;
+      e1 = eb;
       part(h, nr - h);
     }
     return a.tl4.nblocks;
   }
-#define FS_L0P(NR_)                                                                        \
-  hipLaunchKernelGGL((k_prolong_l0<NR_>), dim3(a.tl.nblocks), dim3(kBlock), 0, s, r, ec, vsc, e, \
-                     a.c[0], a.c[1], a.c[2], a.mask, a.d, a.tl, a.m, a.vs, nr, omega, q,     \
-                     partials, sc, mode)
+#define FS_L0P(NR_)                                                                             \
+  launch_timed(k_prolong_l0<NR_>, dim3(a.tl.nblocks), s, ea, eb, r, ec, vsc, e, a.c[0], a.c[1], \
+               a.c[2], a.mask, a.d, a.tl, a.m, a.vs, nr, omega, q, partials, sc, mode, prof)
   if (nr <= 1) FS_L0P(1);
   else if (nr <= 2) FS_L0P(2);
   else if (nr <= 4) FS_L0P(4);
@@ -985,12 +1013,12 @@ int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, 
 }
 
 void restrict_l0(hipStream_t s, const L0Args& a, const float* r, float* rc, int64_t vsc, int nr,
-                 float omega, const CgScalars* sc, int mode) {
+                 float omega, const CgScalars* sc, int mode, hipEvent_t ea, hipEvent_t eb,
+                 unsigned long long* prof) {
   if (a.x4) {
-#define FS_L0R4(NR_, DG)                                                                       \
-  hipLaunchKernelGGL((k_restrict_l0x4<NR_, DG>), dim3(a.tlr4.nblocks), dim3(kBlock), 0, s, r, rc, \
-                     vsc, a.c[0], a.c[1], a.c[2], a.dinv, a.d, a.tlr4, a.kstart, a.m, a.vs, nr,  \
-                     omega, sc, mode)
+#define FS_L0R4(NR_, DG)                                                                        \
+  launch_timed(k_restrict_l0x4<NR_, DG>, dim3(a.tlr4.nblocks), s, ea, eb, r, rc, vsc, a.c[0],     \
+               a.c[1], a.c[2], a.dinv, a.d, a.tlr4, a.kstart, a.m, a.vs, nr, omega, sc, mode, prof)
     // exact RHS count: these kernels hold ~16 registers per RHS and lane
     if (a.dinv) {
       switch (nr) {
@@ -1018,8 +1046,8 @@ void restrict_l0(hipStream_t s, const L0Args& a, const float* r, float* rc, int6
 #undef FS_L0R4
     return;
   }
-  hipLaunchKernelGGL((k_restrict_l0<8>), dim3(a.tlr.nblocks), dim3(kBlock), 0, s, r, rc, vsc,
-                     a.c[0], a.c[1], a.c[2], a.d, a.tlr, a.kstart, a.m, a.vs, nr, omega, sc, mode);
+  launch_timed(k_restrict_l0<8>, dim3(a.tlr.nblocks), s, ea, eb, r, rc, vsc, a.c[0], a.c[1],
+               a.c[2], a.d, a.tlr, a.kstart, a.m, a.vs, nr, omega, sc, mode, prof);
 }
 
 int vec_blocks(int64_t n) {

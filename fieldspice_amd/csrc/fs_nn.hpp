@@ -131,11 +131,15 @@ struct L0Args {
   Map m;                // level 0 -> 1
 };
 // returns the number of blocks (the partials' stride)
+// optional profiling: start/stop events of the launch and the counters of
+// the kernel kind ([0] iterations with a live RHS, [1] live RHS summed)
 int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, int64_t vsc,
                float* e, int nr, float omega, const float* q, double* partials,
-               const CgScalars* sc, int mode);
+               const CgScalars* sc, int mode, hipEvent_t ea = nullptr, hipEvent_t eb = nullptr,
+               unsigned long long* prof = nullptr);
 void restrict_l0(hipStream_t s, const L0Args& a, const float* r, float* rc, int64_t vsc, int nr,
-                 float omega, const CgScalars* sc, int mode);
+                 float omega, const CgScalars* sc, int mode, hipEvent_t ea = nullptr,
+                 hipEvent_t eb = nullptr, unsigned long long* prof = nullptr);
 // bottom-level training loss: c_b = r^T D^-1 r, then g = 2 (A y - r) / (c_b nr)
 void jacobi_norm(hipStream_t s, const Level& L, const float* r, int nr, double* partials,
                  int nblocks);

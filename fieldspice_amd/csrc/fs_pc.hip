@@ -348,7 +348,15 @@ namespace fs {
 // partials of <z,r> and <z,q> ([2*nr][*nblocks]).  Right-hand sides that sc
 // marks idle (mode as the Krylov kernels) are skipped on level 0.
 int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, double* partials,
-             int* nblocks, const CgScalars* sc, int mode) {
+             int* nblocks, const CgScalars* sc, int mode, ProfSlot* slot,
+             unsigned long long* prof_dev) {
+  // level-0 timing (FS_PROF_RESTRICT / FS_PROF_PROLONG) when profiled
+  unsigned long long* pr = slot && prof_dev ? prof_dev + 2 * FS_PROF_RESTRICT : nullptr;
+  unsigned long long* pp = slot && prof_dev ? prof_dev + 2 * FS_PROF_PROLONG : nullptr;
+  // (the tiled level-0 restriction runs when level 0 is grouped in x/y pairs)
+  const hipEvent_t ra = pc->l0_pairs ? prof_start(slot, FS_PROF_RESTRICT) : nullptr;
+  const hipEvent_t rb = pc->l0_pairs ? prof_stop(slot, FS_PROF_RESTRICT) : nullptr;
+  const hipEvent_t pa = prof_start(slot, FS_PROF_PROLONG), pb = prof_stop(slot, FS_PROF_PROLONG);
   hipStream_t s = pc->sys->stream;
   const int K = pc->K;
   // in a solve, every kernel below skips its work once all right-hand sides
@@ -360,7 +368,8 @@ int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, doub
   }
   for (int l = 0; l < K; ++l) {
     if (l == 0 && pc->l0_pairs)
-      mg::restrict_l0(s, pc->l0, r0, pc->rv[1], pc->lv[1].vs, nr, pc->omega, sc, mode);
+      mg::restrict_l0(s, pc->l0, r0, pc->rv[1], pc->lv[1].vs, nr, pc->omega, sc, mode, ra, rb,
+                      pr);
     else if (l > 0 && pc->lx[l].x4)
       mg::restrict_l0(s, pc->lx[l], pc->rv[l], pc->rv[l + 1], pc->lv[l + 1].vs, nr, pc->omega,
                       sc, mode);
@@ -388,19 +397,22 @@ int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, doub
                   pc->ev[l], nr, pc->omega, nullptr, nullptr, 0, g);
   }
   const int nb = mg::prolong_l0(s, pc->l0, r0, pc->ev[1], pc->lv[1].vs, z0, nr, pc->omega, q,
-                                partials, sc, mode);
+                                partials, sc, mode, pa, pb, pp);
   if (nblocks) *nblocks = nb;
   FS_HIP(hipGetLastError());
   return FS_OK;
 }
+
+int64_t pc_level1_nodes(const fs_pc* pc) { return pc && pc->K > 0 ? pc->lv[1].n : 0; }
 
 int pc_level0_width(const fs_pc* pc) {
   return pc && pc->l0_pairs && pc->l0.x4 ? 4 : 1;
 }
 
 // Solver hook (fs_kernels.hip, FS_PREC_FCG): z32 = M(r32) with FCG partials.
-int pc_apply_solver(fs_sys* s, int nr, int* nblocks) {
-  return pc_cycle(s->pc, nr, s->r32, s->z32, s->q32, s->partials, nblocks, s->sc, 2);
+int pc_apply_solver(fs_sys* s, int nr, int* nblocks, ProfSlot* slot) {
+  return pc_cycle(s->pc, nr, s->r32, s->z32, s->q32, s->partials, nblocks, s->sc, 2, slot,
+                  s->prof_dev);
 }
 
 void pc_orphan(fs_pc* pc) {

@@ -91,7 +91,24 @@ typedef struct {
                             FP64 1 or 2 (even rows), FP32 1 or 4 (rows a multiple of 4) */
     int cycle_width;     /* FS_PREC_FCG: nodes per lane of the level-0 cycle kernels (1 or 4);
                             0 without a preconditioner */
+    /* with fs_sys_set_profiling(sys, 1): every full-grid kernel of a Krylov
+     * iteration, indexed by FS_PROF_*: seconds summed over its launches (HIP
+     * events at the kernel's own start and end), launches, and algorithmic
+     * bytes (each array once per launch: couplings and masks once, the
+     * vectors of every live right-hand side).  Kinds a method does not run
+     * stay 0.  Index FS_PROF_STENCIL repeats stencil_time/launches/bytes. */
+    double kernel_time[5];
+    int64_t kernel_launches[5];
+    double kernel_bytes[5];
 } fs_solve_stats;
+
+/* fs_solve_stats.kernel_* index */
+#define FS_PROF_STENCIL   0  /* q = A_s p with the p.q partials */
+#define FS_PROF_DIRECTION 1  /* p = z + beta p, carrying the deferred x += alpha p */
+#define FS_PROF_RESIDUAL  2  /* r -= alpha q with the r.r partials */
+#define FS_PROF_RESTRICT  3  /* FS_PREC_FCG: level-0 pre-smoothing and restriction */
+#define FS_PROF_PROLONG   4  /* FS_PREC_FCG: level-0 prolongation, post-smoothing, <z,r>, <z,q> */
+#define FS_PROF_KINDS     5
 
 typedef struct {
     int64_t n_nodes;
@@ -108,6 +125,16 @@ int fs_ctx_destroy(fs_ctx* ctx);
 const char* fs_last_error(void);
 /* device name and compute-unit count, for reports */
 int fs_ctx_device_info(fs_ctx* ctx, char* name, int name_len, int* n_cu);
+
+/* Measurement aid, no counterpart in the reference: seconds per launch of a
+ * kernel that streams nread + nwrite FP32 arrays of n elements once (16-byte
+ * accesses, non-temporal stores, grid-stride), averaged over reps launches
+ * after one warm-up.  It prices a kernel's read/write mix at the rate this
+ * device streams it (nread = 3 + k, nwrite = k is the FP32 stencil's mix for
+ * k right-hand sides, k = 1..8; 1/1 is a copy) and calibrates hardware
+ * traffic counters against exactly n*4*(nread + nwrite) bytes.  Allocates and
+ * frees its own buffers; n a positive multiple of 4. */
+int fs_stream_probe(fs_ctx* ctx, int64_t n, int nread, int nwrite, int reps, double* seconds);
 
 /* ---- operators: A = G^T diag(w) G + diag(extra) ------------------------- */
 /* Replaces edge_mass + nodal_laplacian (fieldspice/operators.py:302-315,
