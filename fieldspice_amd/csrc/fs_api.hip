@@ -326,7 +326,10 @@ static int run_krylov_loop(fs_sys* s, int nr, const fs_solve_params& P, int mode
     for (int k = 0; k < FS_PROF_KINDS; ++k) {
       if (!((sl.used >> k) & 1u)) continue;
       float ms = 0.0f;
-      if (hipEventElapsedTime(&ms, sl.a[k], sl.b[k]) == hipSuccess) s->prof_time[k] += 1e-3 * ms;
+      if (hipEventElapsedTime(&ms, sl.a[k], sl.b[k]) == hipSuccess)
+        s->prof_time[k] += 1e-3 * ms;
+      else
+        (void)hipGetLastError();  // an unrecorded pair is skipped, not left as a sticky error
       s->prof_launches[k] += 1;
     }
     sl.used = 0;

@@ -13,10 +13,12 @@ namespace fs {
 // A launch whose start/stop events, when given, are timestamped at the
 // kernel's own start and end (hipExtLaunchKernelGGL), so that the profiled
 // duration excludes the dispatch gap an event pair around the launch holds.
+// Either event may be null (a kernel split into several launches carries the
+// start event on its first launch and the stop event on its last).
 template <typename K, typename... A>
 inline void launch_timed(K kernel, dim3 grid, hipStream_t st, hipEvent_t ea, hipEvent_t eb,
                          A... args) {
-  if (ea)
+  if (ea || eb)
     hipExtLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, st, ea, eb, 0, args...);
   else
     hipLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, st, args...);
