@@ -134,21 +134,24 @@ def kernel_table(prof, n):
     return out
 
 
-def kernel_names(nt, fp64, nx):
+def kernel_names(nt, fp64, nx, width):
     """Kernel symbol of each profiled kind for nt right-hand sides on rows of
-    nx nodes."""
+    nx nodes, the Krylov stencil running `width` nodes per lane."""
     bucket = 1 if nt <= 1 else 2 if nt <= 2 else 4 if nt <= 4 else 8
     x4 = nx % 4 == 0
     if fp64:
-        return {"stencil": (f"k_stencil_dot2<{nt}>" if nx % 2 == 0 else
+        return {"stencil": (f"k_stencil_dot2<{nt}>" if width == 2 else
                             f"k_stencil_dot<double, {bucket}>"),
                 "direction": f"k_pupdate<double, {bucket}, true>",
                 "residual": f"k_update<double, {bucket}, true>"}
-    return {"stencil": f"k_stencil_dot4<{nt}>" if x4 else f"k_stencil_dot<float, {bucket}>",
+    half = (nt + 1) // 2
+    return {"stencil": (f"k_stencil_dotw<{nt}, {width}>" if width > 1 else
+                        f"k_stencil_dot<float, {bucket}>"),
             "direction": f"k_pupdate<float, {bucket}, true>",
             "residual": f"k_update<float, {bucket}, true>",
             "restrict": f"k_restrict_l0x4<{nt}, false>" if x4 else "k_restrict_l0<8>",
-            "prolong": (f"k_prolong_l0x4<{min(nt, 5)}, false>" if x4 else
+            "prolong": ((f"k_prolong_l0x4<{nt}, false>" if nt <= 5 else
+                         f"k_prolong_l0x4<{half}, false> + <{nt - half}, false>") if x4 else
                         f"k_prolong_l0<{bucket}>")}
 
 
@@ -194,6 +197,7 @@ def run_method(ex, L, method, steps, warmup):
         "residual_eq": st["residual_eq"], "residual": st["residual"],
         "converged": bool(st["converged"]), "fallbacks": st["fallbacks"],
         "precond_applies": st["pc_applies"], "device_solve_s": st["solve_time"],
+        "stencil_width": st["stencil_width"], "cycle_width": st["cycle_width"],
         "prof": prof, "nt": nt, "kernels": kernel_table(prof, g.n_nodes),
     }
     if ls.pc is not None:
@@ -513,7 +517,7 @@ def main():
     kt = b["kernels"]
     # the dominant kernel of the reported method (largest summed launch time)
     dom = max(kt, key=lambda k: kt[k]["time_s"])
-    names = kernel_names(nt, fp64, g.shape_nodes[0])
+    names = kernel_names(nt, fp64, g.shape_nodes[0], b["stencil_width"])
 
     def roof(kind):
         e = kt[kind]
@@ -531,7 +535,7 @@ def main():
     roofline["traffic_source"] = (f"{TRAFFIC_PROFILE}: FETCH_SIZE/WRITE_SIZE over the same "
                                   "run, calibrated on fs_stream_probe")
     stencil = roof("stencil") if "stencil" in kt else None
-    if stencil is not None and not fp64 and g.shape_nodes[0] % 4 == 0:
+    if stencil is not None and not fp64 and b["stencil_width"] > 1:
         # the rate at which this device streams the stencil's read/write mix
         # (3 couplings + nt p read, nt q written) once, without neighbours
         from fieldspice_amd import _lib

@@ -427,7 +427,7 @@ int run_krylov(fs_sys* s, int nr, const fs_solve_params& P) {
 // which kernels a solve in `precision` runs on this system (fs_solve_stats)
 void kernel_widths(const fs_sys* s, int precision, fs_solve_stats* st) {
   if (!st) return;
-  st->stencil_width = precision == FS_PREC_FP64 ? (s->use_x2 ? 2 : 1) : (s->use_x4 ? 4 : 1);
+  st->stencil_width = precision == FS_PREC_FP64 ? (s->use_x2 ? 2 : 1) : (s->xw ? s->xw : 1);
   st->cycle_width = precision == FS_PREC_FCG && s->pc ? fs::pc_level0_width(s->pc) : 0;
 }
 
@@ -953,6 +953,23 @@ int fs_sys_create(fs_op* op, const uint8_t* mask, int pin, fs_sys** out, fs_sys_
     t4.ntiles = (int)(t4xy * t4.tiles_z);
     t4.nblocks = (t4.ntiles + 7) / 8 * 8;
   }
+  // FP32 Krylov stencil: W nodes per lane, the widest of 4, 3, 2 that divides
+  // the rows (fs_sys_autotune may pick another width or the one-node kernel)
+  {
+    const bool wide_ok = d.per == 0 && d.n + d.sxy < (int64_t)INT32_MAX;
+    s->xw = !wide_ok ? 0 : d.nx % 4 == 0 ? 4 : d.nx % 3 == 0 ? 3 : d.nx % 2 == 0 ? 2 : 0;
+    if (s->xw) {
+      fs::Tiling& tw = s->tlw;
+      tw.tiles_x = (d.nx / s->xw + fs::kWave - 1) / fs::kWave;
+      tw.tiles_y = (d.ny + fs::kTileY - 1) / fs::kTileY;
+      const int64_t twxy = (int64_t)tw.tiles_x * tw.tiles_y;
+      tw.kz = (int)std::min<int64_t>(16, std::max<int64_t>(4, (d.nz * twxy + 1023) / 1024));
+      tw.kz = std::max(1, std::min(tw.kz, d.nz));
+      tw.tiles_z = (d.nz + tw.kz - 1) / tw.kz;
+      tw.ntiles = (int)(twxy * tw.tiles_z);
+      tw.nblocks = (tw.ntiles + 7) / 8 * 8;
+    }
+  }
   // FP64 stencil with 2 nodes per lane when rows are even
   s->use_x2 = d.nx % 2 == 0 && d.per == 0 ? 1 : 0;
   {
@@ -968,7 +985,7 @@ int fs_sys_create(fs_op* op, const uint8_t* mask, int pin, fs_sys** out, fs_sys_
   }
   const int64_t maxb = std::max<int64_t>(
       std::max<int64_t>(std::max<int64_t>(tl.nblocks, s->tl4.nblocks), s->tl2.nblocks),
-      s->vec_blocks);
+      std::max<int64_t>(s->vec_blocks, s->xw ? s->tlw.nblocks : 0));
   s->partials_cap = maxb * fs::kMaxRhs * 2 + 64 * (fs::kMaxRhs + 1) * fs::kMaxRhs;
   int rc = FS_OK;
   double* dsum = nullptr;
@@ -1620,9 +1637,11 @@ int fs_sys_set_profiling(fs_sys* s, int on) {
 
 namespace {
 
-fs::Tiling tiling_for(const fs::Dims& d, int kz, bool x4 = false) {
+// tiles of the FP32 stencil with w nodes per lane (1: the 64-wide tiles of
+// the one-node-per-thread kernel)
+fs::Tiling tiling_for(const fs::Dims& d, int kz, int w = 1) {
   fs::Tiling tl;
-  tl.tiles_x = x4 ? (d.nx / 4 + fs::kWave - 1) / fs::kWave : (d.nx + fs::kTileX - 1) / fs::kTileX;
+  tl.tiles_x = w > 1 ? (d.nx / w + fs::kWave - 1) / fs::kWave : (d.nx + fs::kTileX - 1) / fs::kTileX;
   tl.tiles_y = (d.ny + fs::kTileY - 1) / fs::kTileY;
   tl.kz = std::max(1, std::min(kz, d.nz));
   tl.tiles_z = (d.nz + tl.kz - 1) / tl.kz;
@@ -1632,7 +1651,9 @@ fs::Tiling tiling_for(const fs::Dims& d, int kz, bool x4 = false) {
 }
 
 struct TuneResult {
-  int kz = 0, kz4 = 0, use_x4 = 0;
+  int kz = 0;    // z-march depth of the one-node-per-thread kernel
+  int kzw = 0;   // ... and of the wide kernel at width xw
+  int xw = 0;    // fastest width (0: the one-node kernel)
 };
 
 std::mutex g_tune_mu;
@@ -1668,15 +1689,20 @@ int fs_sys_autotune(fs_sys* s, int nr, int reps, int* kz_out) {
     s->partials_cap = need;
     return FS_OK;
   };
-  const bool x4_ok = d.nx % 4 == 0;
+  const bool wide_ok = d.per == 0 && d.n + d.sxy < (int64_t)INT32_MAX;
   if (best.kz == 0) {
     if (int rc = ensure_ws(s, nr)) return rc;
-    // candidates: z-march depths from one plane to 128, for both kernels
+    // candidates: z-march depths from one plane to 128, for the one-node
+    // kernel and every width that divides the rows
     std::vector<int> cand;
     for (int kz = 1; kz <= 128; kz *= 2)
       if (kz <= d.nz) cand.push_back(kz);
+    std::vector<int> widths{1};
+    for (int w : {2, 3, 4})
+      if (wide_ok && d.nx % w == 0) widths.push_back(w);
     int64_t need = 0;
-    for (int kz : cand) need = std::max<int64_t>(need, tiling_for(d, kz).nblocks);
+    for (int w : widths)
+      for (int kz : cand) need = std::max<int64_t>(need, tiling_for(d, kz, w).nblocks);
     if (int rc = grow_partials(need)) return rc;
     // every right-hand side live for the timed launches; the next solve
     // re-initialises the scalars
@@ -1684,37 +1710,45 @@ int fs_sys_autotune(fs_sys* s, int nr, int reps, int* kz_out) {
     FS_HIP(hipMemcpy(s->sc, s->sc_host, sizeof(fs::CgScalars) * fs::kMaxRhs,
                      hipMemcpyHostToDevice));
     FS_HIP(hipMemsetAsync(s->p32, 0, sizeof(float) * nr * s->vstride, s->stream));
-    float t_best[2] = {1e30f, 1e30f};
-    for (int x4 = 0; x4 < (x4_ok ? 2 : 1); ++x4) {
+    float t_one = 1e30f, t_wide = 1e30f;
+    for (int w : widths) {
       for (int kz : cand) {
-        const fs::Tiling tl = tiling_for(d, kz, x4 != 0);
-        fs::launch_stencil_only(s, nr, tl, x4 != 0);  // warm
+        const fs::Tiling tl = tiling_for(d, kz, w);
+        fs::launch_stencil_only(s, nr, tl, w);  // warm
         FS_HIP(hipEventRecord(ctx->ev0, s->stream));
-        for (int k = 0; k < reps; ++k) fs::launch_stencil_only(s, nr, tl, x4 != 0);
+        for (int k = 0; k < reps; ++k) fs::launch_stencil_only(s, nr, tl, w);
         FS_HIP(hipEventRecord(ctx->ev1, s->stream));
         FS_HIP(hipEventSynchronize(ctx->ev1));
         float ms = 0.f;
         FS_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        if (ms < t_best[x4]) {
-          t_best[x4] = ms;
-          (x4 ? best.kz4 : best.kz) = kz;
+        if (w == 1 && ms < t_one) {
+          t_one = ms;
+          best.kz = kz;
+        } else if (w > 1 && ms < t_wide) {
+          t_wide = ms;
+          best.kzw = kz;
+          best.xw = w;
         }
       }
     }
-    best.use_x4 = x4_ok && t_best[1] < t_best[0] ? 1 : 0;
+    if (!(t_wide < t_one)) best.xw = 0;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[key] = best;
   }
   const fs::Tiling tl = tiling_for(d, best.kz);
   if (int rc = grow_partials(tl.nblocks)) return rc;
   s->tl = tl;
-  if (x4_ok && best.kz4 > 0) {
-    const fs::Tiling t4 = tiling_for(d, best.kz4, true);
-    if (int rc = grow_partials(t4.nblocks)) return rc;
-    s->tl4 = t4;
+  s->xw = best.xw;
+  if (best.xw) {
+    const fs::Tiling tw = tiling_for(d, best.kzw, best.xw);
+    if (int rc = grow_partials(tw.nblocks)) return rc;
+    s->tlw = tw;
   }
-  s->use_x4 = best.use_x4;
-  const int best_kz = best.use_x4 ? best.kz4 : best.kz;
+  // the 4-wide layout of the preconditioner and the deflated stencil follows
+  // the tuned depth when the 4-wide stencil won
+  s->use_x4 = best.xw == 4 ? 1 : 0;
+  if (best.xw == 4) s->tl4 = s->tlw;
+  const int best_kz = best.xw ? best.kzw : best.kz;
   if (kz_out) *kz_out = best_kz;
   return FS_OK;
   FS_GUARD_END

@@ -104,8 +104,12 @@ struct fs_sys {
   const fs_op* op = nullptr;  // not owned; must outlive the system
   fs::Dims d{};
   fs::Tiling tl{};
-  fs::Tiling tl4{};            // 4-nodes-per-lane FP32 stencil (x tiles of 256 nodes)
-  int use_x4 = 0;              // rows are a multiple of 4 and that kernel is selected
+  fs::Tiling tl4{};            // 4-nodes-per-lane layout (x tiles of 256 nodes): the
+                               // preconditioner's level-0 kernels and the deflated stencil
+  int use_x4 = 0;              // rows are a multiple of 4 and that layout is selected
+  int xw = 0;                  // nodes per lane of the FP32 Krylov stencil (2, 3, 4; 0: one
+                               // node per thread on the 64-wide tiles)
+  fs::Tiling tlw{};            // its tiles (x tiles of 64*xw nodes)
   fs::Tiling tl2{};            // 2-nodes-per-lane FP64 stencil (x tiles of 128 nodes)
   int use_x2 = 0;              // rows are a multiple of 2
   int64_t vstride = 0;

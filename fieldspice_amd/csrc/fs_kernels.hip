@@ -443,31 +443,72 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot(const T* __restrict__ p,
   }
 }
 
-// FP32 variant of k_stencil_dot for rows whose length is a multiple of 4:
-// each lane owns 4 consecutive x nodes (one 16-byte load per array), a wave
-// covers 256 nodes of a row, and all right-hand sides advance together in
-// the z-march so the couplings are read once per node.  The x neighbours at
-// the ends of a lane's 4 nodes come from the adjacent lanes (shuffles); the
+// FP32 variant of k_stencil_dot with W consecutive x nodes per lane (W = 2,
+// 3 or 4; rows a multiple of W): one 8-, 12- or 16-byte load per array, a
+// wave covers 64*W nodes of a row, and all right-hand sides advance together
+// in the z-march so the couplings are read once per node.  The x neighbours
+// at the ends of a lane's W nodes come from the adjacent lanes (shuffles); the
 // rows above and below come from the neighbouring waves of the block through
 // LDS, and the block's two edge rows read the one row outside it.  Every
 // global load of a plane is issued before the first use of any of them, with
 // no lane-divergent branch around a load: lanes and rows outside the grid
 // read the grid's last node and store nothing, and the two x-halo lanes
-// select their neighbour from one load every lane makes (256^3, 5 RHS: 195 ->
-// 182 us, 88% of a kernel streaming the same arrays once).  Node offsets are
-// 32-bit (the set-up selects this kernel only when n + nx*ny < 2^31).
-// Stores bypass the caches (q is not read again by this kernel).
+// select their neighbour from one load every lane makes (256^3, 5 RHS, W = 4:
+// 195 -> 176 us, 79% of a kernel streaming the same arrays once).  Node
+// offsets are 32-bit (the set-up selects this kernel only when
+// n + nx*ny < 2^31).  Stores bypass the caches (q is not read again here).
 typedef float f4n __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
-// component-wise select of two float4 values (the conditional operator on the
-// vector struct selects an address and keeps both operands in scratch memory)
-__device__ __forceinline__ float4 pick4(bool c, float4 a, float4 b) {
-  return make_float4(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w);
+// W consecutive floats of one lane
+template <int W>
+struct Lw {
+  float v[W];
+};
+template <int W>
+__device__ __forceinline__ Lw<W> ldw(const float* p) {
+  Lw<W> r;
+  if constexpr (W == 4) {
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    r.v[0] = t.x; r.v[1] = t.y; r.v[2] = t.z; r.v[3] = t.w;
+  } else if constexpr (W == 3) {
+    const float3 t = *reinterpret_cast<const float3*>(p);
+    r.v[0] = t.x; r.v[1] = t.y; r.v[2] = t.z;
+  } else {
+    const float2 t = *reinterpret_cast<const float2*>(p);
+    r.v[0] = t.x; r.v[1] = t.y;
+  }
+  return r;
+}
+template <int W>
+__device__ __forceinline__ void st_nt(float* p, const Lw<W>& y) {
+  if constexpr (W == 4) {
+    f4n t;
+    t.x = y.v[0]; t.y = y.v[1]; t.z = y.v[2]; t.w = y.v[3];
+    __builtin_nontemporal_store(t, reinterpret_cast<f4n*>(p));
+  } else {
+#pragma unroll
+    for (int w = 0; w < W; ++w) __builtin_nontemporal_store(y.v[w], p + w);
+  }
+}
+// component-wise select (the conditional operator on a vector struct selects
+// an address and keeps both operands in scratch memory)
+template <int W>
+__device__ __forceinline__ Lw<W> pickw(bool c, const Lw<W>& a, const Lw<W>& b) {
+  Lw<W> r;
+#pragma unroll
+  for (int w = 0; w < W; ++w) r.v[w] = c ? a.v[w] : b.v[w];
+  return r;
+}
+template <int W>
+__device__ __forceinline__ Lw<W> zerow() {
+  Lw<W> r;
+#pragma unroll
+  for (int w = 0; w < W; ++w) r.v[w] = 0.f;
+  return r;
 }
 
-template <int NR>
-__global__ __launch_bounds__(kBlock) void k_stencil_dot4(const float* __restrict__ p,
+template <int NR, int W>
+__global__ __launch_bounds__(kBlock) void k_stencil_dotw(const float* __restrict__ p,
                                                          float* __restrict__ q,
                                                          const float* __restrict__ cx,
                                                          const float* __restrict__ cy,
@@ -488,18 +529,18 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot4(const float* __restrict
   double acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) acc[r] = 0.0;
-  __shared__ float4 psh[2][NR][kTileY][kWave];
+  __shared__ Lw<W> psh[2][NR][kTileY][kWave];
   int tx, ty, tz;
   if (tile_of_block(tl, tx, ty, tz)) {  // uniform over the block
     const int lane = (int)(threadIdx.x % kWave);
     const int ly = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
-    const int i = (tx * kWave + lane) * 4;
+    const int i = (tx * kWave + lane) * W;
     const int j = ty * kTileY + ly;  // uniform over the wave
     const bool valid = i < d.nx && j < d.ny;
-    const int ic = min(i, d.nx - 4), jc = min(j, d.ny - 1);
-    const bool has_w = ic > 0, has_e = ic + 4 < d.nx, has_n = jc + 1 < d.ny, has_s = jc > 0;
+    const int ic = min(i, d.nx - W), jc = min(j, d.ny - 1);
+    const bool has_w = ic > 0, has_e = ic + W < d.nx, has_n = jc + 1 < d.ny, has_s = jc > 0;
     // the x halo: lane 0 reads its west node, the last lane its east node
-    const int xo = (lane == 0 && has_w) ? -1 : ((lane == kWave - 1 && has_e) ? 4 : 0);
+    const int xo = (lane == 0 && has_w) ? -1 : ((lane == kWave - 1 && has_e) ? W : 0);
     // the one row outside the block an edge wave reads: south for the first
     // wave, north for the last
     const bool own_n = ly + 1 == kTileY, own_s = ly == 0;
@@ -509,40 +550,40 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot4(const float* __restrict
     const int sxy = (int)d.sxy, nx = d.nx;
     const int os = has_s ? -nx : 0, oh = own_s ? -nx : nx;
     int nn = k0 * sxy + jc * nx + ic;
-    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 pm[NR], pc[NR];
-    float4 czm = z4;
+    const Lw<W> zw = zerow<W>();
+    Lw<W> pm[NR], pc[NR];
+    Lw<W> czm = zw;
     {
       const int nd = k0 > 0 ? nn - sxy : nn;
-      const float4 czd = ld4(cz + nd);
-      float4 pd[NR];
+      const Lw<W> czd = ldw<W>(cz + nd);
+      Lw<W> pd[NR];
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
         if (!live[r]) continue;  // uniform over the launch
-        pd[r] = ld4(p + r * va.vs + nd);
-        pc[r] = ld4(p + r * va.vs + nn);
+        pd[r] = ldw<W>(p + r * va.vs + nd);
+        pc[r] = ldw<W>(p + r * va.vs + nn);
       }
-      czm = pick4(k0 > 0, czd, z4);
+      czm = pickw<W>(k0 > 0, czd, zw);
 #pragma unroll
       for (int r = 0; r < NR; ++r)
-        if (live[r]) pm[r] = pick4(k0 > 0, pd[r], z4);
+        if (live[r]) pm[r] = pickw<W>(k0 > 0, pd[r], zw);
     }
     for (int k = k0; k < k1; ++k, nn += sxy) {
       const bool up = k + 1 < d.nz;
       const int nu = up ? nn + sxy : nn;
       // ---- every load of this plane ----
-      const float4 cxc = ld4(cx + nn), cyc = ld4(cy + nn), czc = ld4(cz + nn);
-      const float4 cyl = ld4(cy + nn + os);
+      const Lw<W> cxc = ldw<W>(cx + nn), cyc = ldw<W>(cy + nn), czc = ldw<W>(cz + nn);
+      const Lw<W> cyl = ldw<W>(cy + nn + os);
       const float cxh = cx[nn + min(xo, 0)];
-      float4 pu[NR], ph4[NR];
+      Lw<W> pu[NR], ph4[NR];
       float ph[NR];
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
         if (!live[r]) continue;
         const float* __restrict__ pr = p + r * va.vs;
-        pu[r] = ld4(pr + nu);
+        pu[r] = ldw<W>(pr + nu);
         ph[r] = pr[nn + xo];
-        if (edge) ph4[r] = ld4(pr + nn + oh);
+        if (edge) ph4[r] = ldw<W>(pr + nn + oh);
       }
       // ---- the rows above and below from the neighbouring waves ----
       const int buf = (k - k0) & 1;
@@ -551,31 +592,38 @@ __global__ __launch_bounds__(kBlock) void k_stencil_dot4(const float* __restrict
         if (live[r]) psh[buf][r][ly][lane] = pc[r];
       __syncthreads();
       // ---- compute ----
-      const float4 cym = pick4(has_s, cyl, z4);
-      const float cxw_s = __shfl_up(cxc.w, 1, kWave);
+      const Lw<W> cym = pickw<W>(has_s, cyl, zw);
+      const float cxw_s = __shfl_up(cxc.v[W - 1], 1, kWave);
       const float cxw = has_w ? (lane == 0 ? cxh : cxw_s) : 0.f;
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
         if (!live[r]) continue;
-        const float4 c = pc[r];
-        const float4 u = pick4(up, pu[r], z4);
-        const float4 ln = psh[buf][r][min(ly + 1, kTileY - 1)][lane];
-        const float4 ls = psh[buf][r][max(ly - 1, 0)][lane];
-        const float4 nb = pick4(has_n, pick4(own_n, ph4[r], ln), z4);
-        const float4 sb = pick4(has_s, pick4(own_s, ph4[r], ls), z4);
-        const float sw = __shfl_up(c.w, 1, kWave), se = __shfl_down(c.x, 1, kWave);
+        const Lw<W> c = pc[r];
+        const Lw<W> u = pickw<W>(up, pu[r], zw);
+        const Lw<W> ln = psh[buf][r][min(ly + 1, kTileY - 1)][lane];
+        const Lw<W> ls = psh[buf][r][max(ly - 1, 0)][lane];
+        const Lw<W> nb = pickw<W>(has_n, pickw<W>(own_n, ph4[r], ln), zw);
+        const Lw<W> sb = pickw<W>(has_s, pickw<W>(own_s, ph4[r], ls), zw);
+        const float sw = __shfl_up(c.v[W - 1], 1, kWave), se = __shfl_down(c.v[0], 1, kWave);
         const float pw = has_w ? (lane == 0 ? ph[r] : sw) : 0.f;
         const float pe = has_e ? (lane == kWave - 1 ? ph[r] : se) : 0.f;
-        const float4 m = pm[r];
-        f4n y;
-        y.x = c.x - (cxc.x * c.y + cxw * pw + cyc.x * nb.x + cym.x * sb.x + czc.x * u.x + czm.x * m.x);
-        y.y = c.y - (cxc.y * c.z + cxc.x * c.x + cyc.y * nb.y + cym.y * sb.y + czc.y * u.y + czm.y * m.y);
-        y.z = c.z - (cxc.z * c.w + cxc.y * c.y + cyc.z * nb.z + cym.z * sb.z + czc.z * u.z + czm.z * m.z);
-        y.w = c.w - (cxc.w * pe + cxc.z * c.z + cyc.w * nb.w + cym.w * sb.w + czc.w * u.w + czm.w * m.w);
-        // the lane's 4 products in FP32, their sum accumulated in FP64
-        const double dp = (double)(c.x * y.x + c.y * y.y + c.z * y.z + c.w * y.w);
+        const Lw<W> m = pm[r];
+        Lw<W> y;
+        double dp = 0.0;
+        float dps = 0.f;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          const float east = w + 1 < W ? c.v[w + 1] : pe;   // x neighbour above
+          const float cwest = w > 0 ? cxc.v[w - 1] : cxw;   // coupling to the one below
+          const float west = w > 0 ? c.v[w - 1] : pw;
+          y.v[w] = c.v[w] - (cxc.v[w] * east + cwest * west + cyc.v[w] * nb.v[w] +
+                             cym.v[w] * sb.v[w] + czc.v[w] * u.v[w] + czm.v[w] * m.v[w]);
+          // the lane's W products in FP32, their sum accumulated in FP64
+          dps = w == 0 ? c.v[0] * y.v[0] : dps + c.v[w] * y.v[w];
+        }
+        dp = (double)dps;
         acc[r] += valid ? dp : 0.0;
-        if (valid) __builtin_nontemporal_store(y, reinterpret_cast<f4n*>(q + r * va.vs + nn));
+        if (valid) st_nt<W>(q + r * va.vs + nn, y);
         pm[r] = c;
         pc[r] = u;
       }
@@ -1402,28 +1450,33 @@ int launch_stream_probe(hipStream_t st, const float* const* in, float* const* ou
 // --------------------------------------------------------------------------
 // host launchers
 // --------------------------------------------------------------------------
-// The 4-wide FP32 stencil holds 8 registers per right-hand side and lane:
-// it is instantiated for the exact RHS count (occupancy 3 waves/SIMD at 8
-// RHS, more below), where the other kernels use the 1/2/4/8 buckets.
+// The wide FP32 stencil holds ~2W+6 registers per right-hand side and lane:
+// it is instantiated for the exact RHS count, where the other kernels use the
+// 1/2/4/8 buckets, and for each width W in {2, 3, 4}.
 template <int NR>
-static void dot4_t(hipStream_t st, fs_sys* s, const Tiling& tl, const VecArgs& va, int mode,
-                   unsigned long long* prof, hipEvent_t ea, hipEvent_t eb) {
-  launch_timed(k_stencil_dot4<NR>, dim3(tl.nblocks), st, ea, eb, (const float*)s->p32, s->q32,
-               (const float*)s->c32[0], (const float*)s->c32[1], (const float*)s->c32[2], s->d,
-               tl, va, mode, s->sc, s->partials, prof);
+static void dotw_t(hipStream_t st, fs_sys* s, const Tiling& tl, int w, const VecArgs& va,
+                   int mode, unsigned long long* prof, hipEvent_t ea, hipEvent_t eb) {
+#define FS_DW(W_)                                                                               \
+  launch_timed(k_stencil_dotw<NR, W_>, dim3(tl.nblocks), st, ea, eb, (const float*)s->p32,       \
+               s->q32, (const float*)s->c32[0], (const float*)s->c32[1],                          \
+               (const float*)s->c32[2], s->d, tl, va, mode, s->sc, s->partials, prof)
+  if (w == 2) FS_DW(2);
+  else if (w == 3) FS_DW(3);
+  else FS_DW(4);
+#undef FS_DW
 }
-static void launch_dot4(hipStream_t st, fs_sys* s, const Tiling& tl, const VecArgs& va, int mode,
-                        unsigned long long* prof, hipEvent_t ea = nullptr,
+static void launch_dotw(hipStream_t st, fs_sys* s, const Tiling& tl, int w, const VecArgs& va,
+                        int mode, unsigned long long* prof, hipEvent_t ea = nullptr,
                         hipEvent_t eb = nullptr) {
   switch (va.nr) {
-    case 1: dot4_t<1>(st, s, tl, va, mode, prof, ea, eb); break;
-    case 2: dot4_t<2>(st, s, tl, va, mode, prof, ea, eb); break;
-    case 3: dot4_t<3>(st, s, tl, va, mode, prof, ea, eb); break;
-    case 4: dot4_t<4>(st, s, tl, va, mode, prof, ea, eb); break;
-    case 5: dot4_t<5>(st, s, tl, va, mode, prof, ea, eb); break;
-    case 6: dot4_t<6>(st, s, tl, va, mode, prof, ea, eb); break;
-    case 7: dot4_t<7>(st, s, tl, va, mode, prof, ea, eb); break;
-    default: dot4_t<8>(st, s, tl, va, mode, prof, ea, eb); break;
+    case 1: dotw_t<1>(st, s, tl, w, va, mode, prof, ea, eb); break;
+    case 2: dotw_t<2>(st, s, tl, w, va, mode, prof, ea, eb); break;
+    case 3: dotw_t<3>(st, s, tl, w, va, mode, prof, ea, eb); break;
+    case 4: dotw_t<4>(st, s, tl, w, va, mode, prof, ea, eb); break;
+    case 5: dotw_t<5>(st, s, tl, w, va, mode, prof, ea, eb); break;
+    case 6: dotw_t<6>(st, s, tl, w, va, mode, prof, ea, eb); break;
+    case 7: dotw_t<7>(st, s, tl, w, va, mode, prof, ea, eb); break;
+    default: dotw_t<8>(st, s, tl, w, va, mode, prof, ea, eb); break;
   }
 }
 
@@ -1492,9 +1545,9 @@ static int launch_iteration_t(fs_sys* s, int nr, int mode, ProfSlot* sl) {
   int sbf = sb;
   hipEvent_t ea = prof_start(sl, FS_PROF_STENCIL), eb = prof_stop(sl, FS_PROF_STENCIL);
   unsigned long long* pst = prof_ctr(s, sl, FS_PROF_STENCIL);
-  if (s->use_x4) {
-    sbf = s->tl4.nblocks;
-    launch_dot4(st, s, s->tl4, va, mode, pst, ea, eb);
+  if (s->xw) {
+    sbf = s->tlw.nblocks;
+    launch_dotw(st, s, s->tlw, s->xw, va, mode, pst, ea, eb);
   } else {
     launch_timed(k_stencil_dot<float, NR>, dim3(sb), st, ea, eb, (const float*)s->p32, s->q32,
                  (const float*)s->c32[0], (const float*)s->c32[1], (const float*)s->c32[2],
@@ -1660,17 +1713,17 @@ int launch_fcg_start(fs_sys* s, int nr, ProfSlot* slot) {
   return launch_fcg_start_t<8>(s, nr, slot);
 }
 template <int NR>
-static void launch_stencil_only_t(fs_sys* s, int nr, const Tiling& tl, bool x4) {
+static void launch_stencil_only_t(fs_sys* s, int nr, const Tiling& tl, int w) {
   VecArgs va{s->d.n, s->vstride, nr};
-  if (x4)
-    launch_dot4(s->stream, s, tl, va, 1, nullptr);
+  if (w > 1)
+    launch_dotw(s->stream, s, tl, w, va, 1, nullptr);
   else
     hipLaunchKernelGGL((k_stencil_dot<float, NR>), dim3(tl.nblocks), dim3(kBlock), 0, s->stream,
                        s->p32, s->q32, s->c32[0], s->c32[1], s->c32[2], s->d, tl, va, 1, s->sc,
                        s->partials, nullptr);
 }
-void launch_stencil_only(fs_sys* s, int nr, const Tiling& tl, bool x4) {
-  FS_DISPATCH_NR(nr, launch_stencil_only_t, s, nr, tl, x4);
+void launch_stencil_only(fs_sys* s, int nr, const Tiling& tl, int w) {
+  FS_DISPATCH_NR(nr, launch_stencil_only_t, s, nr, tl, w);
 }
 
 void launch_cycle_start(fs_sys* s, int nr, double inner_tol2) {

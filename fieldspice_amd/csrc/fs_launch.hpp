@@ -104,7 +104,8 @@ void launch_refresh_finish(fs_sys* s, int nr);
 void free_defl(fs_sys* s);
 // one FP32 multi-RHS stencil launch (p32 -> q32) on tiling tl (autotuning);
 // x4: the 4-nodes-per-lane kernel (tl in units of 256-node row segments)
-void launch_stencil_only(fs_sys* s, int nr, const Tiling& tl, bool x4);
+// w: nodes per lane (1: the one-node-per-thread kernel)
+void launch_stencil_only(fs_sys* s, int nr, const Tiling& tl, int w);
 void launch_charge(fs_sys* s, int nr, const fs_op* L, const double* phi, const int64_t* nodes,
                    const int64_t* offs, int nbins, int nbx, double* out);
 

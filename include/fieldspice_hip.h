@@ -88,7 +88,9 @@ typedef struct {
     int replacements;    /* FS_PREC_FP64: restarts from the recomputed FP64 residual after
                             the recursive residual met tol but the true one did not */
     int stencil_width;   /* nodes per lane of the Krylov stencil kernel that ran:
-                            FP64 1 or 2 (even rows), FP32 1 or 4 (rows a multiple of 4) */
+                            FP64 1 or 2 (even rows); FP32 1, or 2, 3 or 4 (a width that
+                            divides the rows: the widest by default, the fastest after
+                            fs_sys_autotune) */
     int cycle_width;     /* FS_PREC_FCG: nodes per lane of the level-0 cycle kernels (1 or 4);
                             0 without a preconditioner */
     /* with fs_sys_set_profiling(sys, 1): every full-grid kernel of a Krylov
@@ -345,9 +347,10 @@ int fs_pc_train(fs_pc* pc, int kind, int steps, int batch, double lr, uint64_t s
  * CPU (fieldspice/solvers/poisson.py:1100-1249 per extraction). */
 typedef struct fs_batch fs_batch;
 /* Time the FP32 multi-RHS stencil of this system for every z-march depth
- * (1, 2, 4, ... <= nz planes per tile) and keep the fastest (cached per grid
- * shape and nr).  Run before fs_pc_create: the preconditioner copies the
- * system's tiling. */
+ * (1, 2, 4, ... <= nz planes per tile), with one node per thread and with
+ * every width of 2, 3 or 4 nodes per lane that divides the rows, and keep
+ * the fastest (cached per grid shape and nr).  Run before fs_pc_create: the
+ * preconditioner copies the system's 4-wide tiling. */
 int fs_sys_autotune(fs_sys* sys, int nr, int reps, int* kz);
 /* Many capacitance extractions (each <= 8 terminals) solved together: the
  * iterations of every system, `chunk` at a time, run as one captured hipGraph
