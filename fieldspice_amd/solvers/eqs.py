@@ -132,8 +132,6 @@ class EQSSolver:
             raise ValueError("t_end must exceed t_start")
         if not 0.0 <= theta <= 1.0:
             raise ValueError("theta must lie in [0, 1]")
-        if monitors is not None:
-            raise NotImplementedError("field monitors are not part of the GPU time loop")
         timer = _Timer()
         g = self.grid
         terms = check_terminals(terminals, g)
@@ -174,14 +172,21 @@ class EQSSolver:
             values=values,
             src=src, src_scale=scales, probe=probe if terms else None, probe_w=pw,
             nprobe=len(terms), cur_a=cur_a, cur_b=self.L_sigma, quad_a=self.L_eps,
-            quad_b=self.L_sigma, store_every=max(1, store_every) if store else 0,
+            quad_b=self.L_sigma,
+            # monitors see every step: the loop then keeps them all, as the
+            # reference's list of potentials does
+            store_every=1 if monitors is not None else (max(1, store_every) if store else 0),
             initial_guess=initial_guess, x_before=phi_prev, params=ls.params())
         ls._verdict(out["stats"], out["rc"])
         res = Result(grid=g, t=times)
+        se = max(1, store_every)
+        if monitors is not None:
+            self._run_monitors(monitors, out["stored"], times, dt)
+            res.meta["monitors"] = monitors.finalize()
         if store:
-            res.fields["phi"] = out["stored"]
-            se = max(1, store_every)
-            res.scalars["t_stored"] = times[::se][:out["stored"].shape[0]]
+            stored = out["stored"][::se] if monitors is not None else out["stored"]
+            res.fields["phi"] = stored
+            res.scalars["t_stored"] = times[::se][:stored.shape[0]]
         for m, t in enumerate(terms):
             res.terminals[t.name] = {"v": out["probe_v"][:, m].copy(),
                                      "i": out["probe_i"][:, m].copy()}
@@ -194,6 +199,29 @@ class EQSSolver:
         return timer.finish(res, self.name, self.assumptions, g, mode="transient", theta=theta,
                             dt=dt, n_factorisations=1,
                             speedup_vs_courant=dt / g.courant_dt(eps_r_min=1.0))
+
+    # -- monitors -----------------------------------------------------------------
+    def _edge_properties(self):
+        """Permittivity and conductivity averaged onto edges [F/m, S/m]: the
+        device operators' edge masses divided by each edge's A_dual / L."""
+        ratio = np.concatenate([(a / l).ravel() for a, l in
+                                zip(self.grid.edge_dual_areas(), self.grid.edge_lengths())])
+        return self.L_eps.edges() / ratio, self.L_sigma.edges() / ratio
+
+    def _run_monitors(self, mset, phis, times, dt):
+        """``mset.record(state, t)`` for the initial field and every step, in
+        order, with the state keys of ``fieldspice/solvers/eqs.py:377-386``
+        (``ops`` is None: a monitor builds the gradient from the grid); a
+        monitor's exception is reported and never stops the solve."""
+        eps_edge, sigma_edge = self._edge_properties()
+        for phi, t in zip(phis, times):
+            state = {"phi": phi, "grid": self.grid, "ops": None, "eps_edge": eps_edge,
+                     "sigma_edge": sigma_edge, "step": 0, "dt": dt}
+            try:
+                mset.record(state, float(t))
+            except Exception as exc:  # a monitor must never kill a solve
+                if self.cfg.verbose >= 1:
+                    print(f"[{self.name}] monitor error at t={t:.4g}: {exc}", flush=True)
 
     # -- derived quantities ------------------------------------------------------
     def terminal_current(self, phi_new, phi_old, dt, nodes, theta: float = 1.0) -> float:

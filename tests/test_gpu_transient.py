@@ -377,3 +377,56 @@ def test_thermal_transient_with_arbitrary_source_matches_oracle():
     assert r.fields["T"].shape == Tref.shape
     assert np.abs(r.fields["T"] - Tref).max() <= 1e-9 * rise
     np.testing.assert_allclose(r.scalars["T_max"], np.array(Ts).max(axis=1), rtol=1e-12)
+
+
+class _Recorder:
+    """A monitor set in the reference's protocol (record(state, t),
+    finalize()) that keeps what it saw."""
+
+    def __init__(self, node, fail_at=None):
+        self.node, self.fail_at = node, fail_at
+        self.t, self.v, self.keys, self.edge = [], [], None, None
+
+    def record(self, state, t):
+        self.t.append(t)
+        self.v.append(float(state["phi"][self.node]))
+        self.keys = sorted(state)
+        self.edge = (state["eps_edge"], state["sigma_edge"])
+        if self.fail_at is not None and len(self.t) == self.fail_at:
+            raise RuntimeError("a failing monitor")
+
+    def finalize(self):
+        return {"t": np.array(self.t), "v": np.array(self.v)}
+
+
+def test_eqs_monitors_see_every_step():
+    """fieldspice/solvers/eqs.py:287-319,377-386: the monitor set records the
+    initial field and every step with the solver's state, its finalize()
+    lands in meta["monitors"], a raising monitor never stops the solve, and
+    eps_edge / sigma_edge are the edge averages of the reference's
+    cell_to_edge (the oracle's restatement)."""
+    _case("EQS monitors")
+    g, eps, sig, R1, C1, C2, n = _rc_stack(30)
+    tau = R1 * (C1 + C2)
+    nid = g.node_ids()
+    node = int(nid[n // 2, 1, 1])
+    s = EQSSolver(g, eps, sig, SolverConfig(verbose=0))
+    hi = Terminal("hi", nid[0].ravel(), voltage=1.0)
+    lo = Terminal("lo", nid[-1].ravel(), voltage=0.0)
+    dt = tau / 10
+    plain = s.solve([hi, lo], t_end=2 * tau, dt=dt, store=True, store_every=1)
+    mon = _Recorder(node, fail_at=5)
+    r = s.solve([hi, lo], t_end=2 * tau, dt=dt, store=True, store_every=4, monitors=mon)
+    nsteps = len(plain.t) - 1
+    assert len(mon.t) == nsteps + 1  # initial field and every step, the failing call included
+    np.testing.assert_allclose(mon.t, plain.t, rtol=0, atol=1e-30)
+    np.testing.assert_allclose(mon.v, plain.fields["phi"][:, node], rtol=1e-12, atol=1e-15)
+    assert r.meta["monitors"]["v"].shape == (nsteps + 1,)
+    assert mon.keys == sorted(["phi", "grid", "ops", "eps_edge", "sigma_edge", "step", "dt"])
+    # storing every 4th step is unaffected by the monitors seeing all of them
+    np.testing.assert_allclose(r.fields["phi"], plain.fields["phi"][::4], rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(r.scalars["t_stored"], plain.t[::4])
+    e_ref = O.cell_to_edge(g, eps)
+    s_ref = O.cell_to_edge(g, sig)
+    np.testing.assert_allclose(mon.edge[0], e_ref, rtol=1e-13, atol=0)
+    np.testing.assert_allclose(mon.edge[1], s_ref, rtol=1e-13, atol=1e-300)
