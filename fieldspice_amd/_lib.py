@@ -704,11 +704,12 @@ class DevicePreconditioner:
 
 
 class DeviceBatch:
-    """Many capacitance extractions solved together (an ``fs_batch``): one
-    captured hipGraph of every system's iterations, replayed until all
-    converge."""
+    """Many capacitance extractions solved together (an ``fs_batch``): the
+    systems in ``nstreams`` groups, each group's iterations and device-side
+    control one captured hipGraph on its own stream, replayed by its own host
+    thread until the device counts the group finished."""
 
-    def __init__(self, nstreams: int = 16):
+    def __init__(self, nstreams: int = 32):
         h = _vp()
         check(load().fs_batch_create(context(), int(nstreams), C.byref(h)), "fs_batch_create")
         self._h = h

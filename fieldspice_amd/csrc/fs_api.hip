@@ -17,6 +17,7 @@
 #include <new>
 #include <tuple>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fieldspice_hip.h"
@@ -1757,13 +1758,15 @@ int fs_sys_autotune(fs_sys* s, int nr, int reps, int* kz_out) {
 }  // extern "C"
 
 // --------------------------------------------------------------------------
-// batched extraction: many prepared systems, one captured graph per solve
+// batched extraction: many prepared systems in groups, one captured graph
+// per group, each replayed by its own host thread
 // --------------------------------------------------------------------------
 struct fs_batch {
   fs_ctx* ctx = nullptr;
-  std::vector<hipStream_t> streams;
-  hipEvent_t fork = nullptr;
-  std::vector<hipEvent_t> joins;
+  std::vector<hipStream_t> streams;  // one per group
+  hipEvent_t fork = nullptr;         // the systems' initial state is on the main stream
+  std::vector<hipEvent_t> joins;     // a group's last replay
+  std::vector<hipEvent_t> polled;    // [2 per group]: replays in flight
   struct Inst {
     fs_cap* cap;
     const fs_op* L;
@@ -1771,6 +1774,11 @@ struct fs_batch {
   std::vector<Inst> inst;
   fs::CgScalars* sc_all = nullptr;       // [ninst][kMaxRhs] device
   fs::CgScalars* sc_all_host = nullptr;  // pinned mirror
+  fs::BatchCtl* ctl = nullptr;           // [ninst] device control
+  fs::BatchCtl* ctl_host = nullptr;      // pinned mirror
+  int* ndone = nullptr;                  // device [group]: finished systems
+  unsigned long long* clock0 = nullptr;  // device wall clock at the start of a solve
+  int* ndone_host = nullptr;             // pinned [2 per group]: two replays in flight
   int sc_cap = 0;
 };
 
@@ -1781,8 +1789,14 @@ void free_batch(fs_batch* b) {
   for (auto st : b->streams) (void)hipStreamDestroy(st);
   if (b->fork) (void)hipEventDestroy(b->fork);
   for (auto e : b->joins) (void)hipEventDestroy(e);
+  for (auto e : b->polled) (void)hipEventDestroy(e);
   dfree(b->sc_all);
+  dfree(b->ctl);
+  dfree(b->ndone);
+  dfree(b->clock0);
   if (b->sc_all_host) (void)hipHostFree(b->sc_all_host);
+  if (b->ctl_host) (void)hipHostFree(b->ctl_host);
+  if (b->ndone_host) (void)hipHostFree(b->ndone_host);
   delete b;
 }
 
@@ -1800,17 +1814,19 @@ int fs_batch_create(fs_ctx* ctx, int nstreams, fs_batch** out) {
   b->ctx = ctx;
   for (int k = 0; k < nstreams; ++k) {
     hipStream_t st = nullptr;
-    hipEvent_t ev = nullptr;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
       free_batch(b);
       return fail(FS_EHIP, "stream creation failed");
     }
     b->streams.push_back(st);
-    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
-      free_batch(b);
-      return fail(FS_EHIP, "event creation failed");
+    for (int e = 0; e < 3; ++e) {
+      hipEvent_t ev = nullptr;
+      if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        free_batch(b);
+        return fail(FS_EHIP, "event creation failed");
+      }
+      (e == 0 ? b->joins : b->polled).push_back(ev);
     }
-    b->joins.push_back(ev);
   }
   if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
     free_batch(b);
@@ -1867,16 +1883,33 @@ int fs_batch_solve(fs_batch* b, const fs_solve_params* params, int chunk, double
       return fail(FS_EINVAL, "FS_PREC_FCG needs a preconditioner on every system of the batch");
   if (b->sc_cap < ni) {
     dfree(b->sc_all);
+    dfree(b->ctl);
     if (b->sc_all_host) (void)hipHostFree(b->sc_all_host);
+    if (b->ctl_host) (void)hipHostFree(b->ctl_host);
     b->sc_all_host = nullptr;
+    b->ctl_host = nullptr;
     b->sc_cap = 0;
     if (int rc = dalloc(&b->sc_all, (size_t)ni * fs::kMaxRhs)) return rc;
+    if (int rc = dalloc(&b->ctl, (size_t)ni)) return rc;
     if (hipHostMalloc((void**)&b->sc_all_host, sizeof(fs::CgScalars) * ni * fs::kMaxRhs) !=
-        hipSuccess)
+            hipSuccess ||
+        hipHostMalloc((void**)&b->ctl_host, sizeof(fs::BatchCtl) * ni) != hipSuccess)
       return fail(FS_ENOMEM, "pinned allocation failed");
     b->sc_cap = ni;
   }
-  FS_HIP(hipMemset(b->sc_all, 0, sizeof(fs::CgScalars) * ni * fs::kMaxRhs));
+  const int ng = std::min((int)b->streams.size(), ni);
+  if (!b->ndone) {
+    const int nst = (int)b->streams.size();
+    if (int rc = dalloc(&b->ndone, (size_t)nst)) return rc;
+    if (int rc = dalloc(&b->clock0, 1)) return rc;
+    if (hipHostMalloc((void**)&b->ndone_host, 2 * sizeof(int) * nst) != hipSuccess)
+      return fail(FS_ENOMEM, "pinned allocation failed");
+  }
+  int clock_khz = 0;
+  FS_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, ctx->device));
+  if (clock_khz <= 0) return fail(FS_EHIP, "device wall clock rate unavailable");
+  FS_HIP(hipMemsetAsync(b->sc_all, 0, sizeof(fs::CgScalars) * ni * fs::kMaxRhs, main));
+  FS_HIP(hipMemsetAsync(b->ctl, 0, sizeof(fs::BatchCtl) * ni, main));
   // workspaces first (an allocation failure returns before any system is
   // changed), then lend every system the batch's scalars and the main stream
   std::vector<fs::CgScalars*> own_sc(ni);
@@ -1900,144 +1933,226 @@ int fs_batch_solve(fs_batch* b, const fs_solve_params* params, int chunk, double
       fs_sys* s = b->inst[i].cap->sys;
       s->sc = own_sc[i];
       s->stream = ctx->stream;
+      s->gate = nullptr;
+      s->pc_count = nullptr;
     }
   };
-  enum { ITER, REFRESH, FINISHING, DONE };
-  std::vector<int> state(ni, ITER);
   std::vector<std::vector<double>> hout(ni);
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  std::vector<double> fallback_time(ni, -1.0);
+  // groups of about equal work (nodes x right-hand sides, longest first to
+  // the least loaded group); each group's systems take consecutive control
+  // slots so that one kernel counts the group's finished systems
+  struct Group {
+    std::vector<int> sys;
+    double work = 0.0;
+    int slot0 = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipError_t err = hipSuccess;
+    bool finished = false;
+  };
+  std::vector<Group> grp(ng);
+  {
+    std::vector<int> order(ni);
+    for (int i = 0; i < ni; ++i) order[i] = i;
+    auto work = [&](int i) {
+      return (double)b->inst[i].cap->sys->d.n * (double)b->inst[i].cap->nt;
+    };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return work(x) > work(y); });
+    for (int i : order) {
+      Group* best = &grp[0];
+      for (auto& g : grp)
+        if (g.work < best->work) best = &g;
+      best->sys.push_back(i);
+      best->work += work(i);
+    }
+  }
+  std::vector<int> slot(ni);
+  for (int g = 0, k = 0; g < ng; ++g) {
+    grp[g].slot0 = k;
+    for (int i : grp[g].sys) slot[i] = k++;
+  }
   int rc = FS_OK;
   auto t0 = std::chrono::steady_clock::now();
   auto start_cycle = [&](fs_sys* s, int nt) -> int {
     fs::launch_cycle_start(s, nt, inner_tol * inner_tol);
-    if (mode == 2) {
+    if (mode == 2)
       if (int r = fs::launch_fcg_start(s, nt)) return r;
-      s->acc_pc_applies += 1;
-    }
     return FS_OK;
+  };
+  auto destroy_graphs = [&]() {
+    for (int g = 0; g < ng; ++g)
+      if (grp[g].exec) (void)hipStreamSynchronize(b->streams[g]);  // replays still queued
+    for (auto& g : grp) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      if (g.graph) (void)hipGraphDestroy(g.graph);
+      g.exec = nullptr;
+      g.graph = nullptr;
+    }
   };
   auto body = [&]() -> int {
     // initial state of every system (lift, zero guess, first cycle)
+    fs::launch_batch_clock(main, b->clock0);
     for (int i = 0; i < ni; ++i) {
       fs_cap* cap = b->inst[i].cap;
       fs_sys* s = cap->sys;
       fs::launch_lift_cap(s, cap->nt, b->inst[i].L, cap->tid, 0);
       fs::launch_init(s, cap->nt, params->tol, maxit[i]);
-      if (cyc)
+      if (cyc) {
         if (int r = start_cycle(s, cap->nt)) return r;
+        s->acc_pc_applies += mode == 2 ? 1 : 0;
+      }
     }
     FS_HIP(hipGetLastError());
-    // the graph: `chunk` iterations of every system, systems spread over the
-    // batch's streams (parallel branches), idle systems exit at once
-    const int nst = (int)b->streams.size();
-    FS_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
-    int crc = FS_OK;
-    if (hipEventRecord(b->fork, main) != hipSuccess) crc = fail(FS_EHIP, "capture fork failed");
-    for (int k = 0; k < nst && crc == FS_OK; ++k)
-      if (hipStreamWaitEvent(b->streams[k], b->fork, 0) != hipSuccess)
-        crc = fail(FS_EHIP, "capture fork failed");
-    for (int i = 0; i < ni && crc == FS_OK; ++i) {
-      fs_sys* s = b->inst[i].cap->sys;
-      s->stream = b->streams[i % nst];
-      for (int c = 0; c < chunk && crc == FS_OK; ++c)
-        crc = fs::launch_iteration(s, b->inst[i].cap->nt, mode, nullptr);
-      s->stream = main;
+    FS_HIP(hipEventRecord(b->fork, main));
+    // One graph per group, on the group's stream: per system, `chunk`
+    // iterations, then the system's device control: when every running
+    // right-hand side has ended its FP32 cycle, the FP64 refresh and, if one
+    // is still running, the next cycle's start.  The gated kernels exit at
+    // once on the replays that do not need them, and finished systems skip
+    // everything, so the host only replays each graph until the device
+    // counts the group's systems finished.
+    for (int g = 0; g < ng; ++g) {
+      Group& G = grp[g];
+      const hipStream_t st = b->streams[g];
+      FS_HIP(hipStreamWaitEvent(st, b->fork, 0));
+      FS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int crc = FS_OK;
+      for (int i : G.sys) {
+        if (crc != FS_OK) break;
+        fs_sys* s = b->inst[i].cap->sys;
+        const int nt = b->inst[i].cap->nt;
+        fs::BatchCtl* c = b->ctl + slot[i];
+        s->stream = st;
+        s->pc_count = &c->pc_applies;
+        for (int k = 0; k < chunk && crc == FS_OK; ++k)
+          crc = fs::launch_iteration(s, nt, mode, nullptr);
+        if (crc == FS_OK && cyc) {
+          fs::launch_batch_refresh_gate(s, nt, c);
+          s->gate = &c->refresh_skip;
+          fs::launch_refresh(s, nt);
+        }
+        if (crc == FS_OK) fs::launch_batch_start_gate(s, nt, cyc ? 1 : 0, c);
+        if (crc == FS_OK && cyc) {
+          s->gate = &c->start_skip;
+          crc = start_cycle(s, nt);
+        }
+        s->gate = nullptr;
+        s->pc_count = nullptr;
+        s->stream = main;
+      }
+      if (crc == FS_OK)
+        fs::launch_batch_count(st, b->ctl + G.slot0, (int)G.sys.size(), b->ndone + g);
+      const hipError_t ee = hipStreamEndCapture(st, &G.graph);
+      if (crc != FS_OK) return crc;
+      if (ee != hipSuccess)
+        return fail(FS_EHIP, "graph capture failed: %s", hipGetErrorString(ee));
+      FS_HIP(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
     }
-    for (int k = 0; k < nst && crc == FS_OK; ++k) {
-      if (hipEventRecord(b->joins[k], b->streams[k]) != hipSuccess ||
-          hipStreamWaitEvent(main, b->joins[k], 0) != hipSuccess)
-        crc = fail(FS_EHIP, "capture join failed");
-    }
-    const hipError_t ee = hipStreamEndCapture(main, &graph);
-    if (crc != FS_OK) return crc;
-    if (ee != hipSuccess) return fail(FS_EHIP, "graph capture failed: %s", hipGetErrorString(ee));
-    FS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     int max_rounds = 0;
     for (int i = 0; i < ni; ++i) max_rounds = std::max(max_rounds, maxit[i]);
     max_rounds = 4 * (max_rounds / chunk + 2) + 64;
-    for (int round = 0; round < max_rounds; ++round) {
-      bool any_iter = false, all_done = true;
-      for (int i = 0; i < ni; ++i) {
-        any_iter = any_iter || state[i] == ITER;
-        all_done = all_done && state[i] == DONE;
+    // each group's graph is replayed by a host thread of its own (graph
+    // launches cost host time per node, so one thread would feed the device
+    // too slowly), two replays in flight: the thread reads the finished
+    // count of replay k while replay k+1 runs; the replay queued behind the
+    // last one finds the group finished and exits kernel by kernel
+    auto drive = [&](int g) {
+      Group& G = grp[g];
+      const hipStream_t st = b->streams[g];
+      int* cnt = b->ndone_host + 2 * g;
+      hipEvent_t* ev = b->polled.data() + 2 * g;
+      auto chk = [&](hipError_t e) {
+        if (e != hipSuccess && G.err == hipSuccess) G.err = e;
+        return e == hipSuccess;
+      };
+      if (!chk(hipSetDevice(ctx->device))) return;
+      auto replay = [&](int k) {
+        return chk(hipGraphLaunch(G.exec, st)) &&
+               chk(hipMemcpyAsync(cnt + (k & 1), b->ndone + g, sizeof(int),
+                                  hipMemcpyDeviceToHost, st)) &&
+               chk(hipEventRecord(ev[k & 1], st));
+      };
+      const int want = (int)G.sys.size();
+      if (!replay(0)) return;
+      for (int k = 1; k <= max_rounds && !G.finished; ++k) {
+        if (k < max_rounds && !replay(k)) return;
+        if (!chk(hipEventSynchronize(ev[(k - 1) & 1]))) return;
+        G.finished = cnt[(k - 1) & 1] >= want;
       }
-      if (all_done) return FS_OK;
-      if (any_iter) FS_HIP(hipGraphLaunch(exec, main));
-      FS_HIP(hipMemcpyAsync(b->sc_all_host, b->sc_all, sizeof(fs::CgScalars) * ni * fs::kMaxRhs,
+      (void)chk(hipEventRecord(b->joins[g], st));
+    };
+    std::vector<std::thread> workers;
+    workers.reserve(ng);
+    for (int g = 0; g < ng; ++g) workers.emplace_back(drive, g);
+    for (auto& w : workers) w.join();
+    for (int g = 0; g < ng; ++g) {
+      if (grp[g].err != hipSuccess)
+        return fail(FS_EHIP, "batch replay failed: %s", hipGetErrorString(grp[g].err));
+      if (!grp[g].finished) return fail(FS_EINVAL, "batch driver exceeded its round guard");
+      FS_HIP(hipStreamWaitEvent(main, b->joins[g], 0));
+    }
+    FS_HIP(hipMemcpyAsync(b->sc_all_host, b->sc_all, sizeof(fs::CgScalars) * ni * fs::kMaxRhs,
+                          hipMemcpyDeviceToHost, main));
+    FS_HIP(hipMemcpyAsync(b->ctl_host, b->ctl, sizeof(fs::BatchCtl) * ni,
+                          hipMemcpyDeviceToHost, main));
+    FS_HIP(hipStreamSynchronize(main));
+    // per system: the FP64 redo of a flexible-CG solve that failed, the last
+    // FP64 step and true-residual check, then the charges
+    for (int i = 0; i < ni; ++i) {
+      fs_cap* cap = b->inst[i].cap;
+      fs_sys* s = cap->sys;
+      const int nt = cap->nt;
+      const fs::CgScalars* c = b->sc_all_host + (size_t)i * fs::kMaxRhs;
+      s->acc_pc_applies += b->ctl_host[slot[i]].pc_applies;
+      bool ok = true;
+      for (int r = 0; r < nt; ++r) ok = ok && c[r].done == 1;
+      if (mode == 2 && !ok) {
+        // plain FP64 CG redoes this system (blocking)
+        fs_solve_params Q = *params;
+        Q.precision = FS_PREC_FP64;
+        Q.maxiter = 0;
+        s->acc_fallbacks += 1;
+        fs::launch_init(s, nt, params->tol, default_maxiter(s, &Q));
+        if (int r = run_krylov_mode(s, nt, Q, 0)) return r;
+        FS_HIP(hipStreamSynchronize(main));
+        fallback_time[i] =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      } else if (mode == 0) {
+        // the pending x += alpha p, and the true-residual check (blocking)
+        if (int r = fp64_finish(s, nt, *params)) return r;
+      }
+      fs::launch_residual(s, nt, s->r);
+      fs::launch_expand(s, nt, nullptr, cap->tid, 0, s->q);
+      fs::launch_charge(s, nt, b->inst[i].L, s->q, cap->nodes, cap->offs, cap->nbins,
+                        cap->nbx, cap->out);
+      hout[i].assign((size_t)cap->nbins * fs::kMaxRhs, 0.0);
+      FS_HIP(hipMemcpyAsync(hout[i].data(), cap->out, sizeof(double) * cap->nbins * nt,
                             hipMemcpyDeviceToHost, main));
-      FS_HIP(hipStreamSynchronize(main));
-      const double now =
-          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      for (int i = 0; i < ni; ++i) {
-        fs_cap* cap = b->inst[i].cap;
-        fs_sys* s = cap->sys;
-        const int nt = cap->nt;
-        const fs::CgScalars* c = b->sc_all_host + (size_t)i * fs::kMaxRhs;
-        if (state[i] == DONE) continue;
-        if (state[i] == FINISHING) {
-          if (inst_time) inst_time[i] = now;
-          state[i] = DONE;
-          continue;
-        }
-        if (any_iter && state[i] == ITER) s->acc_pc_applies += mode == 2 ? chunk : 0;
-        bool done = true, waiting = true, ok = true;
-        for (int r = 0; r < nt; ++r) {
-          done = done && c[r].done != 0;
-          waiting = waiting && (c[r].done != 0 || c[r].cycle_done != 0);
-          ok = ok && c[r].done == 1;
-        }
-        if (state[i] == ITER && cyc && !done && waiting) {
-          fs::launch_refresh(s, nt);
-          state[i] = REFRESH;
-          continue;
-        }
-        if (state[i] == REFRESH && !done) {
-          if (int r = start_cycle(s, nt)) return r;
-          state[i] = ITER;
-          continue;
-        }
-        if (!done) continue;
-        if (mode == 2 && !ok) {
-          // plain FP64 CG redoes this system (blocking)
-          fs_solve_params Q = *params;
-          Q.precision = FS_PREC_FP64;
-          Q.maxiter = 0;
-          s->acc_fallbacks += 1;
-          fs::launch_init(s, nt, params->tol, default_maxiter(s, &Q));
-          if (int r = run_krylov_mode(s, nt, Q, 0)) return r;
-          FS_HIP(hipMemcpyAsync(b->sc_all_host + (size_t)i * fs::kMaxRhs, s->sc,
-                                sizeof(fs::CgScalars) * nt, hipMemcpyDeviceToHost, main));
-        } else if (mode == 0) {
-          // the pending x += alpha p, and the true-residual check (blocking)
-          if (int r = fp64_finish(s, nt, *params)) return r;
-        }
-        fs::launch_residual(s, nt, s->r);
-        fs::launch_expand(s, nt, nullptr, cap->tid, 0, s->q);
-        fs::launch_charge(s, nt, b->inst[i].L, s->q, cap->nodes, cap->offs, cap->nbins,
-                          cap->nbx, cap->out);
-        hout[i].assign((size_t)cap->nbins * fs::kMaxRhs, 0.0);
-        FS_HIP(hipMemcpyAsync(hout[i].data(), cap->out, sizeof(double) * cap->nbins * nt,
-                              hipMemcpyDeviceToHost, main));
-        state[i] = FINISHING;
-      }
       FS_HIP(hipGetLastError());
     }
-    return fail(FS_EINVAL, "batch driver exceeded its round guard");
+    return FS_OK;
   };
   rc = body();
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
+  destroy_graphs();
+  unsigned long long clock0 = 0;
   if (rc == FS_OK) {
     // final scalars (residuals of the finishing pass)
     if (hipMemcpy(b->sc_all_host, b->sc_all, sizeof(fs::CgScalars) * ni * fs::kMaxRhs,
-                  hipMemcpyDeviceToHost) != hipSuccess)
+                  hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&clock0, b->clock0, sizeof(clock0), hipMemcpyDeviceToHost) != hipSuccess)
       rc = fail(FS_EHIP, "scalar download failed");
   }
   restore();
   if (rc != FS_OK) return rc;
   const double total =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (inst_time)
+    for (int i = 0; i < ni; ++i)
+      inst_time[i] = fallback_time[i] >= 0.0
+                         ? fallback_time[i]
+                         : (double)(b->ctl_host[slot[i]].stamp - clock0) / (1e3 * clock_khz);
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->converged = 1;

@@ -65,6 +65,17 @@ struct CgScalars {
   int pad0, pad1;
 };
 
+// Device control of one system of a batched solve, written by the control
+// kernels that follow each chunk of the system's iterations in the batch's
+// captured graph.
+struct BatchCtl {
+  int refresh_skip;          // 0: this replay refreshes the system's FP64 residual
+  int start_skip;            // 0: this replay starts a new FP32 cycle
+  int pc_applies;            // preconditioner applications that ran
+  int finished;              // every right-hand side done
+  unsigned long long stamp;  // device wall clock when first seen finished
+};
+
 // Events of one profiled Krylov iteration: one start/stop pair per kernel
 // kind (FS_PROF_*), and the kinds launched with them in the current chunk.
 struct ProfSlot {
@@ -144,6 +155,11 @@ struct fs_sys {
   int acc_fallbacks = 0;      // counters of the current solve call
   int acc_pc_applies = 0;
   int acc_replacements = 0;   // FP64 residual replacements of the current solve call
+  // device control of a batched solve, set while its graph is captured: the
+  // refresh and cycle-start kernels then exit at once while *gate is nonzero,
+  // and pc_count (when set) counts the preconditioner applications that ran
+  const int* gate = nullptr;
+  int* pc_count = nullptr;
   // island coarse space (fs_sys_set_deflation): Z = D^1/2 1_I per island I
   // in the equilibrated space, E = Z^T A_s Z
   int defl_n = 0;                  // islands (0: off)

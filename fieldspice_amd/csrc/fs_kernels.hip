@@ -875,7 +875,9 @@ struct FinArgs {
 // Launched with one block per right-hand side: block r owns sc[r].
 __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ partials,
                                                      FinArgs fa, CgScalars* sc,
-                                                     unsigned long long* prof) {
+                                                     unsigned long long* prof,
+                                                     const int* __restrict__ gate) {
+  if (gate && *gate) return;  // a batch's device control skipped this step
   const int r = blockIdx.x;
   if (fa.op == FIN_ALPHA || fa.op == FIN_ALPHA_ZR || fa.op == FIN_BETA || fa.op == FIN_FCG) {
     if (!rhs_live(sc, r, fa.mode)) return;  // uniform over the block
@@ -954,15 +956,16 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
 template <int NR>
 static void finalize(hipStream_t st, const double* partials, int op, int nr, int nb, int mode,
                      CgScalars* sc, unsigned long long* prof = nullptr, double tol = 0.0,
-                     int maxiter = 0, double inner_tol2 = 0.0) {
+                     int maxiter = 0, double inner_tol2 = 0.0, const int* gate = nullptr) {
   FinArgs fa{op, nr, nb, mode, tol, maxiter, inner_tol2};
-  hipLaunchKernelGGL(k_finalize, dim3(nr), dim3(kBlock), 0, st, partials, fa, sc, prof);
+  hipLaunchKernelGGL(k_finalize, dim3(nr), dim3(kBlock), 0, st, partials, fa, sc, prof, gate);
 }
 
 void launch_finalize(hipStream_t st, const double* partials, int op, int nr, int nb, int mode,
                      CgScalars* sc, double inner_tol2) {
   FinArgs fa{op, nr, nb, mode, 0.0, 0, inner_tol2};
-  hipLaunchKernelGGL(k_finalize, dim3(nr), dim3(kBlock), 0, st, partials, fa, sc, nullptr);
+  hipLaunchKernelGGL(k_finalize, dim3(nr), dim3(kBlock), 0, st, partials, fa, sc, nullptr,
+                     (const int*)nullptr);
 }
 
 // --------------------------------------------------------------------------
@@ -1123,7 +1126,9 @@ __global__ __launch_bounds__(kBlock) void k_residual(const double* __restrict__ 
                                                      const double* __restrict__ cy,
                                                      const double* __restrict__ cz,
                                                      const double* __restrict__ dhalf, Dims d,
-                                                     VecArgs va, double* partials) {
+                                                     VecArgs va, double* partials,
+                                                     const int* __restrict__ gate) {
+  if (gate && *gate) return;  // uniform over the launch
   double acc[2 * NR];
 #pragma unroll
   for (int r = 0; r < 2 * NR; ++r) acc[r] = 0.0;
@@ -1174,7 +1179,9 @@ __global__ __launch_bounds__(kBlock) void k_residual2(const double* __restrict__
                                                       const double* __restrict__ cy,
                                                       const double* __restrict__ cz,
                                                       const double* __restrict__ dhalf, Dims d,
-                                                      Tiling tl, VecArgs va, double* partials) {
+                                                      Tiling tl, VecArgs va, double* partials,
+                                                      const int* __restrict__ gate) {
+  if (gate && *gate) return;  // uniform over the launch
   double acc[2 * NR];
 #pragma unroll
   for (int a = 0; a < 2 * NR; ++a) acc[a] = 0.0;
@@ -1275,7 +1282,9 @@ __global__ __launch_bounds__(kBlock) void k_cycle_start(const double* __restrict
                                                         float* __restrict__ r32,
                                                         float* __restrict__ d32,
                                                         float* __restrict__ p32, VecArgs va,
-                                                        CgScalars* sc, double* partials) {
+                                                        CgScalars* sc, double* partials,
+                                                        const int* __restrict__ gate) {
+  if (gate && *gate) return;  // uniform over the launch
   double acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) acc[r] = 0.0;
@@ -1309,7 +1318,9 @@ template <int NR>
 __global__ __launch_bounds__(kBlock) void k_cycle_accumulate(double* __restrict__ x64,
                                                              const float* __restrict__ d32,
                                                              const float* __restrict__ p32,
-                                                             VecArgs va, const CgScalars* sc) {
+                                                             VecArgs va, const CgScalars* sc,
+                                                             const int* __restrict__ gate) {
+  if (gate && *gate) return;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
@@ -1323,8 +1334,9 @@ __global__ __launch_bounds__(kBlock) void k_cycle_accumulate(double* __restrict_
 
 // After k_residual wrote r64 and aux3 = ||r64||^2: new rho, refresh count,
 // convergence test (done=1), iteration cap (done=2).
-__global__ void k_refresh_finish(int nr, CgScalars* sc) {
+__global__ void k_refresh_finish(int nr, CgScalars* sc, const int* __restrict__ gate) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (gate && *gate) return;
   for (int r = 0; r < nr; ++r) {
     CgScalars& c = sc[r];
     if (c.done != 0) continue;
@@ -1334,6 +1346,60 @@ __global__ void k_refresh_finish(int nr, CgScalars* sc) {
     else if (c.iter >= c.maxiter) c.done = 2;
     else if (!isfinite(c.rho)) c.done = 3;
   }
+}
+
+// --------------------------------------------------------------------------
+// device control of a batched solve (one thread each)
+// --------------------------------------------------------------------------
+// After a chunk: refresh when some right-hand side is still running and every
+// running one has ended its FP32 cycle.
+__global__ void k_batch_refresh_gate(const CgScalars* sc, int nr, BatchCtl* c) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  bool live = false, waiting = true;
+  for (int r = 0; r < nr; ++r) {
+    live = live || sc[r].done == 0;
+    waiting = waiting && (sc[r].done != 0 || sc[r].cycle_done != 0);
+  }
+  c->refresh_skip = live && waiting ? 0 : 1;
+}
+
+// After the (gated) refresh: start a new cycle when the refresh ran and left
+// a right-hand side running; stamp the first replay that finds every
+// right-hand side done.
+__global__ void k_batch_start_gate(const CgScalars* sc, int nr, int cycles, BatchCtl* c) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  bool live = false;
+  for (int r = 0; r < nr; ++r) live = live || sc[r].done == 0;
+  c->start_skip = cycles && c->refresh_skip == 0 && live ? 0 : 1;
+  if (!live && !c->finished) {
+    c->finished = 1;
+    c->stamp = (unsigned long long)wall_clock64();
+  }
+}
+
+// finished systems of the batch -> *out
+__global__ void k_batch_count(const BatchCtl* c, int ni, int* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int n = 0;
+  for (int i = 0; i < ni; ++i) n += c[i].finished;
+  *out = n;
+}
+
+__global__ void k_batch_clock(unsigned long long* t) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *t = (unsigned long long)wall_clock64();
+}
+
+void launch_batch_refresh_gate(fs_sys* s, int nr, BatchCtl* c) {
+  hipLaunchKernelGGL(k_batch_refresh_gate, dim3(1), dim3(64), 0, s->stream, s->sc, nr, c);
+}
+void launch_batch_start_gate(fs_sys* s, int nr, int cycles, BatchCtl* c) {
+  hipLaunchKernelGGL(k_batch_start_gate, dim3(1), dim3(64), 0, s->stream, s->sc, nr, cycles, c);
+}
+void launch_batch_count(hipStream_t st, const BatchCtl* c, int ni, int* out) {
+  hipLaunchKernelGGL(k_batch_count, dim3(1), dim3(64), 0, st, c, ni, out);
+}
+void launch_batch_clock(hipStream_t st, unsigned long long* t) {
+  hipLaunchKernelGGL(k_batch_clock, dim3(1), dim3(64), 0, st, t);
 }
 
 // --------------------------------------------------------------------------
@@ -1661,7 +1727,7 @@ static void residual2_t(fs_sys* s, const VecArgs& va, double* res) {
   hipLaunchKernelGGL((k_residual2<NR>), dim3(s->tl2.nblocks), dim3(kBlock), 0, s->stream,
                      (const double*)s->bs, (const double*)s->x, res, (const double*)s->c64[0],
                      (const double*)s->c64[1], (const double*)s->c64[2],
-                     (const double*)s->dhalf, s->d, s->tl2, va, s->partials);
+                     (const double*)s->dhalf, s->d, s->tl2, va, s->partials, s->gate);
 }
 
 template <int NR>
@@ -1679,13 +1745,15 @@ static void launch_residual_t(fs_sys* s, int nr, double* res) {
       case 7: residual2_t<7>(s, va, res); break;
       default: residual2_t<8>(s, va, res); break;
     }
-    finalize<NR>(s->stream, s->partials, FIN_RESID, nr, s->tl2.nblocks, 0, s->sc);
+    finalize<NR>(s->stream, s->partials, FIN_RESID, nr, s->tl2.nblocks, 0, s->sc, nullptr, 0.0,
+                 0, 0.0, s->gate);
     return;
   }
   hipLaunchKernelGGL((k_residual<NR>), dim3(s->vec_blocks), dim3(kBlock), 0, s->stream,
                      s->bs, s->x, res, s->c64[0], s->c64[1], s->c64[2], s->dhalf, s->d, va,
-                     s->partials);
-  finalize<NR>(s->stream, s->partials, FIN_RESID, nr, s->vec_blocks, 0, s->sc);
+                     s->partials, s->gate);
+  finalize<NR>(s->stream, s->partials, FIN_RESID, nr, s->vec_blocks, 0, s->sc, nullptr, 0.0, 0,
+               0.0, s->gate);
 }
 void launch_residual(fs_sys* s, int nr, double* res) {
   FS_DISPATCH_NR(nr, launch_residual_t, s, nr, res);
@@ -1695,15 +1763,16 @@ template <int NR>
 static void launch_cycle_start_t(fs_sys* s, int nr, double inner_tol2) {
   VecArgs va{s->d.n, s->vstride, nr};
   hipLaunchKernelGGL((k_cycle_start<NR>), dim3(s->vec_blocks), dim3(kBlock), 0,
-                     s->stream, s->r, s->r32, s->x32, s->p32, va, s->sc, s->partials);
+                     s->stream, s->r, s->r32, s->x32, s->p32, va, s->sc, s->partials, s->gate);
   finalize<NR>(s->stream, s->partials, FIN_CYCLE, nr, s->vec_blocks, 1, s->sc, nullptr,
-               0.0, 0, inner_tol2);
+               0.0, 0, inner_tol2, s->gate);
 }
 template <int NR>
 static int launch_fcg_start_t(fs_sys* s, int nr, ProfSlot* slot) {
   int nbp = 0;
   if (int rc = pc_apply_solver(s, nr, &nbp, slot)) return rc;
-  finalize<NR>(s->stream, s->partials, FIN_FCG0, nr, nbp, 2, s->sc);
+  finalize<NR>(s->stream, s->partials, FIN_FCG0, nr, nbp, 2, s->sc, nullptr, 0.0, 0, 0.0,
+               s->gate);
   return 0;
 }
 int launch_fcg_start(fs_sys* s, int nr, ProfSlot* slot) {
@@ -1738,12 +1807,12 @@ template <int NR>
 static void launch_refresh_t(fs_sys* s, int nr) {
   VecArgs va{s->d.n, s->vstride, nr};
   hipLaunchKernelGGL((k_cycle_accumulate<NR>), dim3(s->vec_blocks), dim3(kBlock), 0,
-                     s->stream, s->x, s->x32, s->p32, va, s->sc);
+                     s->stream, s->x, s->x32, s->p32, va, s->sc, s->gate);
   launch_residual_t<NR>(s, nr, s->r);
-  hipLaunchKernelGGL(k_refresh_finish, dim3(1), dim3(64), 0, s->stream, nr, s->sc);
+  hipLaunchKernelGGL(k_refresh_finish, dim3(1), dim3(64), 0, s->stream, nr, s->sc, s->gate);
 }
 void launch_refresh_finish(fs_sys* s, int nr) {
-  hipLaunchKernelGGL(k_refresh_finish, dim3(1), dim3(64), 0, s->stream, nr, s->sc);
+  hipLaunchKernelGGL(k_refresh_finish, dim3(1), dim3(64), 0, s->stream, nr, s->sc, s->gate);
 }
 void launch_refresh(fs_sys* s, int nr) {
   if (s->defl_n > 0) {

@@ -109,6 +109,14 @@ void launch_stencil_only(fs_sys* s, int nr, const Tiling& tl, int w);
 void launch_charge(fs_sys* s, int nr, const fs_op* L, const double* phi, const int64_t* nodes,
                    const int64_t* offs, int nbins, int nbx, double* out);
 
+// device control of a batched solve: the gates that follow a chunk of a
+// system's iterations (refresh, then cycle start and the finish stamp), the
+// count of finished systems, and a device wall-clock stamp
+void launch_batch_refresh_gate(fs_sys* s, int nr, BatchCtl* c);
+void launch_batch_start_gate(fs_sys* s, int nr, int cycles, BatchCtl* c);
+void launch_batch_count(hipStream_t st, const BatchCtl* c, int ni, int* out);
+void launch_batch_clock(hipStream_t st, unsigned long long* t);
+
 // time loops (fs_transient.hip)
 void launch_explicit(fs_sys* s, const fs_op* A, const double* x, const double* src,
                      const double* scale, int nsrc, int64_t vs, double* rhs);

@@ -411,7 +411,8 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0(
     const float* __restrict__ cz, const uint8_t* __restrict__ mask, Dims d, Tiling tl, Map m,
     int64_t vs, int nr, float omega, const float* __restrict__ q,
     double* __restrict__ partials, const CgScalars* sc, int mode,
-    unsigned long long* prof) {
+    unsigned long long* prof, const int* __restrict__ gate) {
+  if (gate && *gate) return;  // a batch's device control skipped this cycle
   prof_level0(prof, sc, mode, 0, nr, nr);
   double acc[2 * NR];
 #pragma unroll
@@ -499,7 +500,8 @@ __global__ __launch_bounds__(kBlock) void k_restrict_l0(
     const float* __restrict__ r, float* __restrict__ rc, int64_t vsc,
     const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
     Dims d, Tiling tl, const int* __restrict__ kstart, Map m, int64_t vs, int nr, float omega,
-    const CgScalars* sc, int mode, unsigned long long* prof) {
+    const CgScalars* sc, int mode, unsigned long long* prof, const int* __restrict__ gate) {
+  if (gate && *gate) return;  // a batch's device control skipped this cycle
   prof_level0(prof, sc, mode, 0, nr, nr);
   __shared__ float lds[kTileY][kTileX];
   int tx, ty, tz;
@@ -598,7 +600,8 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
     const float* __restrict__ dgv, const float* __restrict__ dinvv, Dims d, Tiling tl, Map m,
     int64_t vs, int nr, float omega, const float* __restrict__ q,
     double* __restrict__ partials, const CgScalars* sc, int mode, int b0, int nrt,
-    unsigned long long* prof) {
+    unsigned long long* prof, const int* __restrict__ gate) {
+  if (gate && *gate) return;  // a batch's device control skipped this cycle
   prof_level0(prof, sc, mode, b0, nr, nrt);
   double acc[2 * NR];
 #pragma unroll
@@ -795,7 +798,9 @@ __global__ __launch_bounds__(kBlock) void k_restrict_l0x4(
     const float* __restrict__ r, float* __restrict__ rc, int64_t vsc,
     const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
     const float* __restrict__ dinvv, Dims d, Tiling tl, const int* __restrict__ kstart, Map m,
-    int64_t vs, int nr, float omega, const CgScalars* sc, int mode, unsigned long long* prof) {
+    int64_t vs, int nr, float omega, const CgScalars* sc, int mode, unsigned long long* prof,
+    const int* __restrict__ gate) {
+  if (gate && *gate) return;  // a batch's device control skipped this cycle
   prof_level0(prof, sc, mode, 0, nr, nr);
   __shared__ float2 lds[NR][kTileY][kWave];
   int tx, ty, tz;
@@ -958,7 +963,7 @@ void energy_grad(hipStream_t s, const Level& L, const float* y, const float* r, 
 int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, int64_t vsc,
                float* e, int nr, float omega, const float* q, double* partials,
                const CgScalars* sc, int mode, hipEvent_t ea, hipEvent_t eb,
-               unsigned long long* prof) {
+               unsigned long long* prof, const int* gate) {
   if (a.x4) {
     // ~16 registers per RHS and lane: launched for the exact RHS count, and
     // in two halves above 5 RHS (8 in one launch would run 1 wave per SIMD);
@@ -968,7 +973,7 @@ int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, 
   launch_timed(k_prolong_l0x4<NR_, DG>, dim3(a.tl4.nblocks), s, e0, e1, r + (B0) * a.vs,          \
                ec + (B0) * vsc, vsc, e + (B0) * a.vs, a.c[0], a.c[1], a.c[2], a.mask, a.dg,      \
                a.dinv, a.d, a.tl4, a.m, a.vs, NB, omega, q ? q + (B0) * a.vs : nullptr,        \
-               partials, sc, mode, B0, nr, prof)
+               partials, sc, mode, B0, nr, prof, gate)
     auto part = [&](int b0, int nb) {
       if (a.dinv) {
         switch (nb) {
@@ -1003,7 +1008,7 @@ int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, 
   }
 #define FS_L0P(NR_)                                                                             \
   launch_timed(k_prolong_l0<NR_>, dim3(a.tl.nblocks), s, ea, eb, r, ec, vsc, e, a.c[0], a.c[1], \
-               a.c[2], a.mask, a.d, a.tl, a.m, a.vs, nr, omega, q, partials, sc, mode, prof)
+               a.c[2], a.mask, a.d, a.tl, a.m, a.vs, nr, omega, q, partials, sc, mode, prof, gate)
   if (nr <= 1) FS_L0P(1);
   else if (nr <= 2) FS_L0P(2);
   else if (nr <= 4) FS_L0P(4);
@@ -1014,11 +1019,12 @@ int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, 
 
 void restrict_l0(hipStream_t s, const L0Args& a, const float* r, float* rc, int64_t vsc, int nr,
                  float omega, const CgScalars* sc, int mode, hipEvent_t ea, hipEvent_t eb,
-                 unsigned long long* prof) {
+                 unsigned long long* prof, const int* gate) {
   if (a.x4) {
 #define FS_L0R4(NR_, DG)                                                                        \
   launch_timed(k_restrict_l0x4<NR_, DG>, dim3(a.tlr4.nblocks), s, ea, eb, r, rc, vsc, a.c[0],     \
-               a.c[1], a.c[2], a.dinv, a.d, a.tlr4, a.kstart, a.m, a.vs, nr, omega, sc, mode, prof)
+               a.c[1], a.c[2], a.dinv, a.d, a.tlr4, a.kstart, a.m, a.vs, nr, omega, sc, mode, prof, \
+               gate)
     // exact RHS count: these kernels hold ~16 registers per RHS and lane
     if (a.dinv) {
       switch (nr) {
@@ -1047,7 +1053,7 @@ void restrict_l0(hipStream_t s, const L0Args& a, const float* r, float* rc, int6
     return;
   }
   launch_timed(k_restrict_l0<8>, dim3(a.tlr.nblocks), s, ea, eb, r, rc, vsc, a.c[0], a.c[1],
-               a.c[2], a.d, a.tlr, a.kstart, a.m, a.vs, nr, omega, sc, mode, prof);
+               a.c[2], a.d, a.tlr, a.kstart, a.m, a.vs, nr, omega, sc, mode, prof, gate);
 }
 
 int vec_blocks(int64_t n) {
@@ -1099,12 +1105,19 @@ void bottom_jacobi(hipStream_t s, const Level& L, const float* r, float* e, int 
                      omega, gate);
 }
 
-__global__ void k_gate(const CgScalars* sc, int nr, int mode, int* out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *out = all_idle(sc, nr, mode) ? 1 : 0;
+// out = 1 (skip the cycle) when the external gate is set or every
+// right-hand side is idle; count, when given, counts the cycles that run
+__global__ void k_gate(const CgScalars* sc, int nr, int mode, const int* ext, int* out,
+                       int* count) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int skip = (ext && *ext) || all_idle(sc, nr, mode) ? 1 : 0;
+  *out = skip;
+  if (count && !skip) *count += 1;
 }
 
-void gate(hipStream_t s, const CgScalars* sc, int nr, int mode, int* out) {
-  hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, s, sc, nr, mode, out);
+void gate(hipStream_t s, const CgScalars* sc, int nr, int mode, int* out, const int* ext,
+          int* count) {
+  hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, s, sc, nr, mode, ext, out, count);
 }
 
 void loss_grad(hipStream_t s, const Level& L, const float* z, const float* r,

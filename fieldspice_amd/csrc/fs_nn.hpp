@@ -105,7 +105,9 @@ void prolong(hipStream_t s, const Level& f, const Map& m, const float* r, const 
 void bottom_jacobi(hipStream_t s, const Level& L, const float* r, float* e, int nr,
                    float omega, const int* gate = nullptr);
 // *out = 1 when every right-hand side of sc[0..nr) is idle (Krylov mode)
-void gate(hipStream_t s, const CgScalars* sc, int nr, int mode, int* out);
+// ext: an external skip flag (a batch's device control), count: cycles run
+void gate(hipStream_t s, const CgScalars* sc, int nr, int mode, int* out,
+          const int* ext = nullptr, int* count = nullptr);
 // training loss on level 0: w = A z; g = 2 (w - r) * gscale[b]; partials
 // [2*nr][nblocks] of z.w and z.r
 void loss_grad(hipStream_t s, const Level& L, const float* z, const float* r,
@@ -132,14 +134,16 @@ struct L0Args {
 };
 // returns the number of blocks (the partials' stride)
 // optional profiling: start/stop events of the launch and the counters of
-// the kernel kind ([0] iterations with a live RHS, [1] live RHS summed)
+// the kernel kind ([0] iterations with a live RHS, [1] live RHS summed);
+// gate (optional): a device flag, nonzero = skip the launch
 int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, int64_t vsc,
                float* e, int nr, float omega, const float* q, double* partials,
                const CgScalars* sc, int mode, hipEvent_t ea = nullptr, hipEvent_t eb = nullptr,
-               unsigned long long* prof = nullptr);
+               unsigned long long* prof = nullptr, const int* gate = nullptr);
 void restrict_l0(hipStream_t s, const L0Args& a, const float* r, float* rc, int64_t vsc, int nr,
                  float omega, const CgScalars* sc, int mode, hipEvent_t ea = nullptr,
-                 hipEvent_t eb = nullptr, unsigned long long* prof = nullptr);
+                 hipEvent_t eb = nullptr, unsigned long long* prof = nullptr,
+                 const int* gate = nullptr);
 // bottom-level training loss: c_b = r^T D^-1 r, then g = 2 (A y - r) / (c_b nr)
 void jacobi_norm(hipStream_t s, const Level& L, const float* r, int nr, double* partials,
                  int nblocks);

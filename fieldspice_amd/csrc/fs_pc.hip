@@ -360,19 +360,24 @@ int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, doub
   hipStream_t s = pc->sys->stream;
   const int K = pc->K;
   // in a solve, every kernel below skips its work once all right-hand sides
-  // are idle (graph replays keep launching finished systems)
+  // are idle (graph replays keep launching finished systems), or, in a
+  // batch's captured cycle start, while the batch's device control says so
+  // (the level-0 kernels, which otherwise skip idle right-hand sides one by
+  // one, then take the gate too)
   const int* g = nullptr;
+  const int* ext = sc ? pc->sys->gate : nullptr;
   if (sc) {
-    mg::gate(s, sc, nr, mode, pc->gate);
+    mg::gate(s, sc, nr, mode, pc->gate, ext, pc->sys->pc_count);
     g = pc->gate;
   }
+  const int* g0 = ext ? g : nullptr;
   for (int l = 0; l < K; ++l) {
     if (l == 0 && pc->l0_pairs)
       mg::restrict_l0(s, pc->l0, r0, pc->rv[1], pc->lv[1].vs, nr, pc->omega, sc, mode, ra, rb,
-                      pr);
+                      pr, g0);
     else if (l > 0 && pc->lx[l].x4)
       mg::restrict_l0(s, pc->lx[l], pc->rv[l], pc->rv[l + 1], pc->lv[l + 1].vs, nr, pc->omega,
-                      sc, mode);
+                      sc, mode, nullptr, nullptr, nullptr, g0);
     else
       mg::restrict_(s, pc->lv[l], pc->maps[l], l == 0 ? r0 : pc->rv[l], pc->rv[l + 1],
                     pc->lv[l + 1].vs, nr, pc->omega, g);
@@ -391,13 +396,13 @@ int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, doub
   for (int l = K - 1; l >= 1; --l) {
     if (pc->lx[l].x4)
       mg::prolong_l0(s, pc->lx[l], pc->rv[l], pc->ev[l + 1], pc->lv[l + 1].vs, pc->ev[l], nr,
-                     pc->omega, nullptr, nullptr, sc, mode);
+                     pc->omega, nullptr, nullptr, sc, mode, nullptr, nullptr, nullptr, g0);
     else
       mg::prolong(s, pc->lv[l], pc->maps[l], pc->rv[l], pc->ev[l + 1], pc->lv[l + 1].vs,
                   pc->ev[l], nr, pc->omega, nullptr, nullptr, 0, g);
   }
   const int nb = mg::prolong_l0(s, pc->l0, r0, pc->ev[1], pc->lv[1].vs, z0, nr, pc->omega, q,
-                                partials, sc, mode, pa, pb, pp);
+                                partials, sc, mode, pa, pb, pp, g0);
   if (nblocks) *nblocks = nb;
   FS_HIP(hipGetLastError());
   return FS_OK;

@@ -352,17 +352,24 @@ typedef struct fs_batch fs_batch;
  * the fastest (cached per grid shape and nr).  Run before fs_pc_create: the
  * preconditioner copies the system's 4-wide tiling. */
 int fs_sys_autotune(fs_sys* sys, int nr, int reps, int* kz);
-/* Many capacitance extractions (each <= 8 terminals) solved together: the
- * iterations of every system, `chunk` at a time, run as one captured hipGraph
- * whose parallel branches spread the systems over `nstreams` streams; systems
- * that finish skip their kernels on later replays; FP64 refreshes, cycle
- * starts and the final charge integration run per system between replays. */
+/* Many capacitance extractions (each <= 8 terminals) solved together.  The
+ * systems are split into min(nstreams, systems) groups of about equal work,
+ * each captured as one hipGraph on a stream of its own.  Per system a graph
+ * holds `chunk` iterations and then the system's device-side control: when
+ * every running right-hand side has ended its FP32 cycle, the FP64 refresh
+ * and the next cycle's start run; on other replays those kernels, and every
+ * kernel of a finished system, exit at once.  One host thread per group
+ * replays its graph, two replays in flight, until a device counter reports
+ * the group's systems finished; then the charges are integrated (and any
+ * flexible-CG system that failed is redone in FP64). */
 int fs_batch_create(fs_ctx* ctx, int nstreams, fs_batch** out);
 int fs_batch_destroy(fs_batch* batch);
 int fs_batch_add(fs_batch* batch, fs_cap* cap, const fs_op* L);
 /* C: the systems' nt_i x nt_i matrices concatenated (Maxwell form, not
- * symmetrised); inst_time[i]: seconds from the start of the call until system
- * i was seen finished; inst_iters[i]: its Krylov iterations (NULL to skip). */
+ * symmetrised); inst_time[i]: seconds from the start of the iterations until
+ * the device found system i finished (device wall clock; a system redone in
+ * FP64: until that redo ended, host clock); inst_iters[i]: its Krylov
+ * iterations (NULL to skip). */
 int fs_batch_solve(fs_batch* batch, const fs_solve_params* params, int chunk, double* C,
                    double* inst_time, int* inst_iters, fs_solve_stats* stats);
 

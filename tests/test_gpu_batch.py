@@ -1,5 +1,6 @@
-"""Batched extraction of uneven systems in one captured graph, and the stencil
-tile autotuner: every batched capacitance matrix against the oracle's direct
+"""Batched extraction of uneven systems in captured graphs under device
+control, and the stencil tile autotuner: every batched capacitance matrix
+against the oracle's direct
 solve (fieldspice/solvers/poisson.py:1185-1249 restated) in all three
 precisions, and the full-size 64-system workload through the properties
 every Maxwell capacitance matrix has."""
@@ -51,9 +52,12 @@ def test_batch_matches_oracle_in_every_precision():
     inst = _instances(with_pc=True)
     ref = [O.capacitance_matrix(ex.grid, ex.eps_cell, [t.nodes for t in ex.terminals],
                                 method="direct")[0] for ex, *_ in inst]
-    for prec, chunk in ((_lib.FS_PREC_FCG, 4), (_lib.FS_PREC_FP64, 16), (_lib.FS_PREC_MIXED, 8)):
-        print(f"[gpu case] batch precision {prec}", flush=True)
-        batch = _lib.DeviceBatch(nstreams=3)
+    # groups (one stream and host thread each): fewer than the systems, one,
+    # and more than the systems (then one system per group)
+    for prec, chunk, ns in ((_lib.FS_PREC_FCG, 4, 3), (_lib.FS_PREC_FP64, 16, 1),
+                            (_lib.FS_PREC_MIXED, 8, 8)):
+        print(f"[gpu case] batch precision {prec}, {ns} groups", flush=True)
+        batch = _lib.DeviceBatch(nstreams=ns)
         for ex, L, sysd, pc, cap in inst:
             batch.add(cap, L)
         Cs, t, its, st, rc = batch.solve(_lib.DeviceSystem.params(TOL, precision=prec), chunk)
@@ -61,6 +65,14 @@ def test_batch_matches_oracle_in_every_precision():
         # the largest FP64-recomputed residual of every system's every RHS
         assert st["residual_eq"] <= TOL, (prec, st)
         assert np.all(t > 0) and np.all(its > 0)
+        # finish times come from the device clock, inside the call
+        assert np.all(t <= st["solve_time"]), (t, st["solve_time"])
+        if prec != _lib.FS_PREC_FP64:
+            # refreshes and cycle starts ran under device control
+            assert st["refreshes"] >= 1, st
+        if prec == _lib.FS_PREC_FCG:
+            # one cycle per iteration with a running RHS, plus one per cycle start
+            assert st["pc_applies"] >= its.sum() + len(inst), (st["pc_applies"], its)
         for k, (C, R) in enumerate(zip(Cs, ref)):
             err = np.abs(_sym(C) - R).max() / np.abs(R).max()
             assert err <= 1e-9, (prec, k, err)
