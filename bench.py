@@ -77,6 +77,8 @@ def _parse():
                          "reference's)")
     ap.add_argument("--instances", type=int, default=64, help="c5: number of systems")
     ap.add_argument("--chunk", type=int, default=4, help="c5: iterations per graph replay")
+    ap.add_argument("--no-looped", action="store_true",
+                    help="c5: time only the batched solve (no looped FP64 PCG leg)")
     ap.add_argument("--streams", type=int, default=32,
                     help="c5: groups of systems, each on its own stream and host thread")
     ap.add_argument("--n", type=int, default=256, help="c4: nodes per side")
@@ -283,18 +285,20 @@ def run_c5(args):
     pf = _lib.DeviceSystem.params(1e-10, precision=_lib.FS_PREC_FCG)
     # plain FP64 PCG looped over the systems (the reference's algorithm, one
     # extraction after the other)
-    for w in range(args.warmup):
-        inst[0]["cap"].solve(inst[0]["L"], p64)
     loop_t, loop_C, loop_it = [], [], []
-    t1 = time.perf_counter()
-    for i in inst:
-        ta = time.perf_counter()
-        C, _, st, _ = i["cap"].solve(i["L"], p64)
-        loop_t.append(time.perf_counter() - ta)
-        loop_C.append(C)
-        loop_it.append(st["iterations"])
-    loop_total = time.perf_counter() - t1
-    log(f"looped FP64 PCG: {loop_total:.2f} s")
+    loop_total = None
+    if not args.no_looped:
+        for w in range(args.warmup):
+            inst[0]["cap"].solve(inst[0]["L"], p64)
+        t1 = time.perf_counter()
+        for i in inst:
+            ta = time.perf_counter()
+            C, _, st, _ = i["cap"].solve(i["L"], p64)
+            loop_t.append(time.perf_counter() - ta)
+            loop_C.append(C)
+            loop_it.append(st["iterations"])
+        loop_total = time.perf_counter() - t1
+        log(f"looped FP64 PCG: {loop_total:.2f} s")
     # every system at once: learned FCG in captured graphs, one per group
     res = []
     for step in range(args.warmup + args.steps):
@@ -309,8 +313,15 @@ def run_c5(args):
             res.append((wall, Cs, fin, its, st))
         log(f"batched learned FCG: {wall:.2f} s (step {step})")
     wall, Cs, fin, its, st = min(res, key=lambda r: r[0])
-    dev = max(float(np.abs(C - R).max() / np.abs(R).max()) for C, R in zip(Cs, loop_C))
-    loop_fin = np.cumsum(loop_t)
+    looped = None
+    if loop_total is not None:
+        loop_fin = np.cumsum(loop_t)
+        looped = {"wall_s": loop_total, "solves_per_s": nrhs / loop_total,
+                  "p50_instance_s": float(np.percentile(loop_t, 50)),
+                  "p99_instance_s": float(np.percentile(loop_t, 99)),
+                  "p50_finish_s": float(np.percentile(loop_fin, 50)),
+                  "p99_finish_s": float(np.percentile(loop_fin, 99)),
+                  "iterations_max": int(max(loop_it))}
     out = {
         "metric": "solves/s, batched uneven extraction: learned FCG in captured graphs vs looped PCG",
         "value": nrhs / wall, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -326,14 +337,10 @@ def run_c5(args):
                         "p99_finish_s": float(np.percentile(fin, 99)),
                         "iterations_max": int(its.max()), "converged": bool(st["converged"]),
                         "fallbacks": st["fallbacks"], "residual_eq_max": st["residual_eq"]},
-        "looped_fp64_pcg": {"wall_s": loop_total, "solves_per_s": nrhs / loop_total,
-                            "p50_instance_s": float(np.percentile(loop_t, 50)),
-                            "p99_instance_s": float(np.percentile(loop_t, 99)),
-                            "p50_finish_s": float(np.percentile(loop_fin, 50)),
-                            "p99_finish_s": float(np.percentile(loop_fin, 99)),
-                            "iterations_max": int(max(loop_it))},
-        "speedup_vs_looped_pcg": loop_total / wall,
-        "C_rel_dev_vs_fp64": dev,
+        "looped_fp64_pcg": looped,
+        "speedup_vs_looped_pcg": None if looped is None else loop_total / wall,
+        "C_rel_dev_vs_fp64": None if looped is None else max(
+            float(np.abs(C - R).max() / np.abs(R).max()) for C, R in zip(Cs, loop_C)),
         "autotuned_kz": sorted(set(int(i["kz"]) for i in inst)),
         "setup_s": setup,
     }
