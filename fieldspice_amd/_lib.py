@@ -371,7 +371,7 @@ class DeviceSystem:
     def params(tol, maxiter=None, precision=FS_PREC_FP64, inner_tol=0.0,
                check_every=0) -> SolveParams:
         """``inner_tol`` 0: the library's default per method (1e-4 mixed CG,
-        1e-5 flexible CG)."""
+        1e-3 flexible CG)."""
         return SolveParams(float(tol), int(maxiter or 0), int(precision), float(inner_tol),
                            int(check_every))
 
@@ -635,7 +635,7 @@ class DevicePreconditioner:
     """Aggregation hierarchy of one prepared system with the network (or one
     Jacobi step) at its bottom (an ``fs_pc``)."""
 
-    def __init__(self, system: DeviceSystem, net: DeviceNet | None, groups, omega: float = 0.8):
+    def __init__(self, system: DeviceSystem, net: DeviceNet | None, groups, omega: float = 0.95):
         self.system = system
         self.net = net
         flat = np.ascontiguousarray(np.concatenate([np.asarray(g, np.int32).ravel()

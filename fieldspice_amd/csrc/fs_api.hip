@@ -1892,18 +1892,27 @@ int fs_batch_solve(fs_batch* b, const fs_solve_params* params, int chunk, double
     if (int rc = dalloc(&b->sc_all, (size_t)ni * fs::kMaxRhs)) return rc;
     if (int rc = dalloc(&b->ctl, (size_t)ni)) return rc;
     if (hipHostMalloc((void**)&b->sc_all_host, sizeof(fs::CgScalars) * ni * fs::kMaxRhs) !=
-            hipSuccess ||
-        hipHostMalloc((void**)&b->ctl_host, sizeof(fs::BatchCtl) * ni) != hipSuccess)
+        hipSuccess) {
+      b->sc_all_host = nullptr;
       return fail(FS_ENOMEM, "pinned allocation failed");
+    }
+    if (hipHostMalloc((void**)&b->ctl_host, sizeof(fs::BatchCtl) * ni) != hipSuccess) {
+      b->ctl_host = nullptr;
+      return fail(FS_ENOMEM, "pinned allocation failed");
+    }
     b->sc_cap = ni;
   }
   const int ng = std::min((int)b->streams.size(), ni);
-  if (!b->ndone) {
+  if (!b->ndone_host) {  // allocated last: set only when all three are
     const int nst = (int)b->streams.size();
+    dfree(b->ndone);
+    dfree(b->clock0);
     if (int rc = dalloc(&b->ndone, (size_t)nst)) return rc;
     if (int rc = dalloc(&b->clock0, 1)) return rc;
-    if (hipHostMalloc((void**)&b->ndone_host, 2 * sizeof(int) * nst) != hipSuccess)
+    if (hipHostMalloc((void**)&b->ndone_host, 2 * sizeof(int) * nst) != hipSuccess) {
+      b->ndone_host = nullptr;
       return fail(FS_ENOMEM, "pinned allocation failed");
+    }
   }
   int clock_khz = 0;
   FS_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, ctx->device));
@@ -2085,8 +2094,14 @@ int fs_batch_solve(fs_batch* b, const fs_solve_params* params, int chunk, double
     };
     std::vector<std::thread> workers;
     workers.reserve(ng);
-    for (int g = 0; g < ng; ++g) workers.emplace_back(drive, g);
+    bool spawned = true;
+    try {
+      for (int g = 0; g < ng; ++g) workers.emplace_back(drive, g);
+    } catch (const std::exception&) {
+      spawned = false;  // the threads already running finish their groups first
+    }
     for (auto& w : workers) w.join();
+    if (!spawned) return fail(FS_EHIP, "could not start the batch's replay threads");
     for (int g = 0; g < ng; ++g) {
       if (grp[g].err != hipSuccess)
         return fail(FS_EHIP, "batch replay failed: %s", hipGetErrorString(grp[g].err));

@@ -36,7 +36,12 @@ from . import _lib
 __all__ = ["axis_groups", "hierarchy_groups", "LearnedPreconditioner", "default_weights_path",
            "load_net", "save_net", "uniform_widths", "DEFAULT_OMEGA", "DEFAULT_MAX_BOTTOM"]
 
-DEFAULT_OMEGA = 0.8
+# Jacobi weight of the cycle's smoothing.  Below 1 the smoother damps every
+# mode of the equilibrated operator (whose spectrum lies in (0, 2)), so the
+# cycle stays symmetric positive definite.  0.95 against 0.8 (the weight the
+# shipped network was trained with): 119 vs 128 flexible-CG iterations at
+# 256^3, 84 vs 86 on the graded 192x192x96 case, 89 vs 97 at 160^3.
+DEFAULT_OMEGA = 0.95
 DEFAULT_MAX_BOTTOM = 8_000
 WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
 
