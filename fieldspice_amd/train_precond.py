@@ -133,6 +133,8 @@ def main(argv=None):
     ap.add_argument("--val-n", type=int, nargs="*", default=[128])
     ap.add_argument("--out", default=default_weights_path())
     ap.add_argument("--max-pairs", type=int, default=2000)
+    ap.add_argument("--max-bottom", type=int, default=DEFAULT_MAX_BOTTOM,
+                    help="largest bottom grid (nodes) of the training hierarchies")
     a = ap.parse_args(argv)
 
     name, ncu = _lib.device_info()
@@ -150,7 +152,7 @@ def main(argv=None):
         case = Case(multi_conductor((a.n,) * 3, seed=a.seed * 100000 + gi))
         E, st = solver_pairs(case)
         E = E[:max(0, a.max_pairs - npairs)]
-        groups = hierarchy_groups(case.widths)
+        groups = hierarchy_groups(case.widths, a.max_bottom)
         pc = _lib.DevicePreconditioner(case.sys, net, groups, DEFAULT_OMEGA)
         pc.add_pairs(E)
         npairs += E.shape[0]
@@ -189,7 +191,7 @@ def main(argv=None):
     log(f"saved {a.out} ({net.num_params} parameters) after {train_time:.1f}s of training")
     results = {}
     for n in a.val_n:
-        results[str(n)] = evaluate(net, a.val_seeds, n)
+        results[str(n)] = evaluate(net, a.val_seeds, n, max_bottom=a.max_bottom)
     summary = os.path.splitext(a.out)[0] + "_validation.json"
     with open(summary, "w") as f:
         json.dump({"train": {"pairs": npairs, "geometries": len(hist) // max(a.rounds, 1),
