@@ -26,6 +26,14 @@ namespace fs {
 namespace mg {
 
 __device__ __forceinline__ void node_ijk(const Level& L, int64_t n, int& i, int& j, int& k) {
+  if (L.n <= 0xffffffffLL) {  // 32-bit divisions (64-bit ones are long emulated sequences)
+    const uint32_t u = (uint32_t)n, nx = (uint32_t)L.nx, ny = (uint32_t)L.ny;
+    const uint32_t row = u / nx;
+    i = (int)(u - row * nx);
+    k = (int)(row / ny);
+    j = (int)(row - (uint32_t)k * ny);
+    return;
+  }
   i = (int)(n % L.nx);
   j = (int)((n / L.nx) % L.ny);
   k = (int)(n / ((int64_t)L.nx * L.ny));
@@ -116,62 +124,57 @@ __global__ __launch_bounds__(kBlock) void k_bottom_features(Level L, float* __re
 // -------------------------------------------------------------------------
 // cycle kernels
 // -------------------------------------------------------------------------
+// rc_I = sum over the fine nodes n of aggregate I of t_n,
 // t_n = (r - A omega D^-1 r)_n = (1 - omega) r_n + omega sum_m c_nm dinv_m r_m
 // (on inactive nodes r = 0 and every coupling is 0, so t = 0 there)
-template <bool UNIT>
-__device__ __forceinline__ float smooth_res(const Level& L, const float* __restrict__ r,
-                                            int i, int j, int k, int64_t n, float omega) {
-  const int64_t sxy = (int64_t)L.nx * L.ny;
-  float s = 0.f;
-  if (i + 1 < L.nx) s += L.c[0][n] * dinv_of<UNIT>(L, n + 1) * r[n + 1];
-  if (i > 0) s += L.c[0][n - 1] * dinv_of<UNIT>(L, n - 1) * r[n - 1];
-  if (j + 1 < L.ny) s += L.c[1][n] * dinv_of<UNIT>(L, n + L.nx) * r[n + L.nx];
-  if (j > 0) s += L.c[1][n - L.nx] * dinv_of<UNIT>(L, n - L.nx) * r[n - L.nx];
-  if (k + 1 < L.nz) s += L.c[2][n] * dinv_of<UNIT>(L, n + sxy) * r[n + sxy];
-  if (k > 0) s += L.c[2][n - sxy] * dinv_of<UNIT>(L, n - sxy) * r[n - sxy];
-  return (1.f - omega) * r[n] + omega * s;
-}
-
 template <bool UNIT>
 __global__ __launch_bounds__(kBlock) void k_restrict(Level f, Map m, const float* __restrict__ r,
                                                      float* __restrict__ rc, int64_t vsc, int nr,
                                                      float omega,
     const int* __restrict__ gate) {
   if (gate && *gate) return;  // every right-hand side of the batch is idle
+  // one thread per (coarse node, right-hand side), the right-hand side
+  // slowest: a small level has few coarse nodes, and each sums a group of
+  // fine nodes in sequence
   const int64_t nc = (int64_t)m.cn[0] * m.cn[1] * m.cn[2];
-  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < nc;
-       t += (int64_t)gridDim.x * kBlock) {
-    const int I = (int)(t % m.cn[0]), J = (int)((t / m.cn[0]) % m.cn[1]),
-              K = (int)(t / ((int64_t)m.cn[0] * m.cn[1]));
+  const Level cl{m.cn[0], m.cn[1], m.cn[2], nc, 0, {}, nullptr, nullptr, nullptr};
+  const int64_t sxy = (int64_t)f.nx * f.ny;
+  const int64_t nt = nc * nr;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int b = nt <= 0xffffffffLL ? (int)((uint32_t)t / (uint32_t)nc) : (int)(t / nc);
+    const int64_t c = t - (int64_t)b * nc;
+    int I, J, K;
+    node_ijk(cl, c, I, J, K);
     const int i0 = m.start[0][I], i1 = m.start[0][I + 1];
     const int j0 = m.start[1][J], j1 = m.start[1][J + 1];
     const int k0 = m.start[2][K], k1 = m.start[2][K + 1];
-    for (int b = 0; b < nr; ++b) {
-      const float* rb = r + b * f.vs;
-      float s = 0.f;
-      for (int k = k0; k < k1; ++k)
-        for (int j = j0; j < j1; ++j)
-          for (int i = i0; i < i1; ++i) {
-            const int64_t n = ((int64_t)k * f.ny + j) * f.nx + i;
-            s += smooth_res<UNIT>(f, rb, i, j, k, n, omega);
-          }
-      rc[b * vsc + t] = s;
-    }
+    const float* __restrict__ rb = r + b * f.vs;
+    float s = 0.f;
+    for (int k = k0; k < k1; ++k)
+      for (int j = j0; j < j1; ++j)
+        for (int i = i0; i < i1; ++i) {
+          // t_n = (1 - omega) r_n + omega sum_m c_nm dinv_m r_m
+          const int64_t n = ((int64_t)k * f.ny + j) * f.nx + i;
+          float a = 0.f;
+          if (i + 1 < f.nx) a += f.c[0][n] * dinv_of<UNIT>(f, n + 1) * rb[n + 1];
+          if (i > 0) a += f.c[0][n - 1] * dinv_of<UNIT>(f, n - 1) * rb[n - 1];
+          if (j + 1 < f.ny) a += f.c[1][n] * dinv_of<UNIT>(f, n + f.nx) * rb[n + f.nx];
+          if (j > 0) a += f.c[1][n - f.nx] * dinv_of<UNIT>(f, n - f.nx) * rb[n - f.nx];
+          if (k + 1 < f.nz) a += f.c[2][n] * dinv_of<UNIT>(f, n + sxy) * rb[n + sxy];
+          if (k > 0) a += f.c[2][n - sxy] * dinv_of<UNIT>(f, n - sxy) * rb[n - sxy];
+          s += (1.f - omega) * rb[n] + omega * a;
+        }
+    rc[b * vsc + c] = s;
   }
 }
 
-// z2_n = omega dinv_n r_n + (active ? ec[P(n)] : 0)
-template <bool UNIT>
-__device__ __forceinline__ float z2_of(const Level& L, const Map& m, const float* __restrict__ r,
-                                       const float* __restrict__ ec, int i, int j, int k,
-                                       int64_t n, float omega) {
-  const float di = dinv_of<UNIT>(L, n);
-  if (!(di > 0.f)) return 0.f;
-  const int64_t c = ((int64_t)m.f2c[2][k] * m.cn[1] + m.f2c[1][j]) * m.cn[0] + m.f2c[0][i];
-  return omega * di * r[n] + ec[c];
-}
-
-template <bool UNIT>
+// e_n = z2_n + omega dinv_n (r - A z2)_n with z2_n = omega dinv_n r_n +
+// (active ? ec[P(n)] : 0); optional FCG partials <e, r>, <e, q>
+// SPLIT (a small level without partials): one thread per (node, right-hand
+// side), the right-hand side slowest, for enough threads to hide the
+// gathers' latency; otherwise one thread per node for every right-hand side.
+template <bool UNIT, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_prolong(Level f, Map m, const float* __restrict__ r,
                                                     const float* __restrict__ ec, int64_t vsc,
                                                     float* __restrict__ e, int nr, float omega,
@@ -183,37 +186,66 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Level f, Map m, const float*
 #pragma unroll
   for (int a = 0; a < 2 * kMaxRhs; ++a) acc[a] = 0.0;
   const int64_t sxy = (int64_t)f.nx * f.ny;
-  for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < f.n;
-       n += (int64_t)gridDim.x * kBlock) {
+  const int64_t csxy = (int64_t)m.cn[0] * m.cn[1];
+  const int64_t nt = SPLIT ? f.n * nr : f.n;
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < nt;
+       t += (int64_t)gridDim.x * kBlock) {
+    int b0 = 0;
+    int64_t n = t;
+    if constexpr (SPLIT) {
+      b0 = nt <= 0xffffffffLL ? (int)((uint32_t)t / (uint32_t)f.n) : (int)(t / f.n);
+      n = t - (int64_t)b0 * f.n;
+    }
     int i, j, k;
     node_ijk(f, n, i, j, k);
     const float di = dinv_of<UNIT>(f, n);
     const float dd = diag_of<UNIT>(f, n);
+    // the node's 6 neighbours (+x, -x, +y, -y, +z, -z): index, coarse index,
+    // inverse diagonal and coupling, read once for every right-hand side
+    const bool has[6] = {i + 1 < f.nx, i > 0, j + 1 < f.ny, j > 0, k + 1 < f.nz, k > 0};
+    const int64_t nb[6] = {n + 1, n - 1, n + f.nx, n - f.nx, n + sxy, n - sxy};
+    const int ci = m.f2c[0][i], cj = m.f2c[1][j], ck = m.f2c[2][k];
+    const int64_t cc = ((int64_t)ck * m.cn[1] + cj) * m.cn[0] + ci;
+    int64_t cb[6];
+    float dn[6], cw[6];
+    cb[0] = cc + ((has[0] ? m.f2c[0][i + 1] : ci) - ci);
+    cb[1] = cc + ((has[1] ? m.f2c[0][i - 1] : ci) - ci);
+    cb[2] = cc + (int64_t)((has[2] ? m.f2c[1][j + 1] : cj) - cj) * m.cn[0];
+    cb[3] = cc + (int64_t)((has[3] ? m.f2c[1][j - 1] : cj) - cj) * m.cn[0];
+    cb[4] = cc + (int64_t)((has[4] ? m.f2c[2][k + 1] : ck) - ck) * csxy;
+    cb[5] = cc + (int64_t)((has[5] ? m.f2c[2][k - 1] : ck) - ck) * csxy;
+    cw[0] = has[0] ? f.c[0][n] : 0.f;
+    cw[1] = has[1] ? f.c[0][n - 1] : 0.f;
+    cw[2] = has[2] ? f.c[1][n] : 0.f;
+    cw[3] = has[3] ? f.c[1][n - f.nx] : 0.f;
+    cw[4] = has[4] ? f.c[2][n] : 0.f;
+    cw[5] = has[5] ? f.c[2][n - sxy] : 0.f;
 #pragma unroll
-    for (int b = 0; b < kMaxRhs; ++b) {
-      if (b >= nr) break;
+    for (int q = 0; q < 6; ++q) dn[q] = has[q] ? dinv_of<UNIT>(f, nb[q]) : 0.f;
+#pragma unroll
+    for (int bb = 0; bb < (SPLIT ? 1 : kMaxRhs); ++bb) {
+      const int b = SPLIT ? b0 : bb;
+      if (!SPLIT && b >= nr) break;
       const float* rb = r + b * f.vs;
       const float* eb = ec + b * vsc;
       float out = 0.f;
       if (di > 0.f) {
-        const float zc = z2_of<UNIT>(f, m, rb, eb, i, j, k, n, omega);
+        // z2_m = omega dinv_m r_m + ec[P(m)] on active nodes, 0 elsewhere
+        const float zc = omega * di * rb[n] + eb[cc];
         float s = 0.f;
-        if (i + 1 < f.nx) s += f.c[0][n] * z2_of<UNIT>(f, m, rb, eb, i + 1, j, k, n + 1, omega);
-        if (i > 0) s += f.c[0][n - 1] * z2_of<UNIT>(f, m, rb, eb, i - 1, j, k, n - 1, omega);
-        if (j + 1 < f.ny) s += f.c[1][n] * z2_of<UNIT>(f, m, rb, eb, i, j + 1, k, n + f.nx, omega);
-        if (j > 0) s += f.c[1][n - f.nx] * z2_of<UNIT>(f, m, rb, eb, i, j - 1, k, n - f.nx, omega);
-        if (k + 1 < f.nz) s += f.c[2][n] * z2_of<UNIT>(f, m, rb, eb, i, j, k + 1, n + sxy, omega);
-        if (k > 0) s += f.c[2][n - sxy] * z2_of<UNIT>(f, m, rb, eb, i, j, k - 1, n - sxy, omega);
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+          if (has[q]) s += cw[q] * (dn[q] > 0.f ? omega * dn[q] * rb[nb[q]] + eb[cb[q]] : 0.f);
         out = zc + omega * di * (rb[n] - dd * zc + s);
       }
       e[b * f.vs + n] = out;
-      if (partials) {
-        acc[b] += (double)out * (double)rb[n];
-        acc[kMaxRhs + b] += (double)out * (double)q[b * f.vs + n];
+      if (!SPLIT && partials) {
+        acc[bb] += (double)out * (double)rb[n];
+        acc[kMaxRhs + bb] += (double)out * (double)q[b * f.vs + n];
       }
     }
   }
-  if (!partials) return;  // uniform over the launch
+  if (SPLIT || !partials) return;  // uniform over the launch
   block_sum<2 * kMaxRhs>(acc);
   if (threadIdx.x == 0) {
     for (int b = 0; b < nr; ++b) {
@@ -1078,25 +1110,38 @@ void bottom_features(hipStream_t s, const Level& L, float* feat4, float* dh) {
 
 void restrict_(hipStream_t s, const Level& f, const Map& m, const float* r, float* rc,
                int64_t vsc, int nr, float omega, const int* gate) {
-  const int64_t nc = (int64_t)m.cn[0] * m.cn[1] * m.cn[2];
+  const int64_t nt = (int64_t)m.cn[0] * m.cn[1] * m.cn[2] * nr;
+  const int nb = (int)std::min<int64_t>(std::max<int64_t>((nt + kBlock - 1) / kBlock, 1), 16384);
   if (f.dg == nullptr)
-    hipLaunchKernelGGL(k_restrict<true>, dim3(vec_blocks(nc)), dim3(kBlock), 0, s, f, m, r, rc,
-                       vsc, nr, omega, gate);
+    hipLaunchKernelGGL(k_restrict<true>, dim3(nb), dim3(kBlock), 0, s, f, m, r, rc, vsc, nr,
+                       omega, gate);
   else
-    hipLaunchKernelGGL(k_restrict<false>, dim3(vec_blocks(nc)), dim3(kBlock), 0, s, f, m, r, rc,
-                       vsc, nr, omega, gate);
+    hipLaunchKernelGGL(k_restrict<false>, dim3(nb), dim3(kBlock), 0, s, f, m, r, rc, vsc, nr,
+                       omega, gate);
 }
 
 void prolong(hipStream_t s, const Level& f, const Map& m, const float* r, const float* ec,
              int64_t vsc, float* e, int nr, float omega, const float* q, double* partials,
              int nblocks, const int* gate) {
+  // levels below 2^20 nodes without partials: a thread per (node, RHS)
+  if (!partials && f.n < (int64_t)1 << 20) {
+    const int64_t nt = f.n * nr;
+    const int nbs = (int)std::min<int64_t>(std::max<int64_t>((nt + kBlock - 1) / kBlock, 1), 16384);
+    if (f.dg == nullptr)
+      hipLaunchKernelGGL((k_prolong<true, true>), dim3(nbs), dim3(kBlock), 0, s, f, m, r, ec, vsc,
+                         e, nr, omega, q, partials, gate);
+    else
+      hipLaunchKernelGGL((k_prolong<false, true>), dim3(nbs), dim3(kBlock), 0, s, f, m, r, ec,
+                         vsc, e, nr, omega, q, partials, gate);
+    return;
+  }
   const int nb = nblocks > 0 ? nblocks : vec_blocks(f.n);
   if (f.dg == nullptr)
-    hipLaunchKernelGGL(k_prolong<true>, dim3(nb), dim3(kBlock), 0, s, f, m, r, ec, vsc, e, nr,
-                       omega, q, partials, gate);
+    hipLaunchKernelGGL((k_prolong<true, false>), dim3(nb), dim3(kBlock), 0, s, f, m, r, ec, vsc, e,
+                       nr, omega, q, partials, gate);
   else
-    hipLaunchKernelGGL(k_prolong<false>, dim3(nb), dim3(kBlock), 0, s, f, m, r, ec, vsc, e, nr,
-                       omega, q, partials, gate);
+    hipLaunchKernelGGL((k_prolong<false, false>), dim3(nb), dim3(kBlock), 0, s, f, m, r, ec, vsc,
+                       e, nr, omega, q, partials, gate);
 }
 
 void bottom_jacobi(hipStream_t s, const Level& L, const float* r, float* e, int nr,
