@@ -1026,7 +1026,10 @@ int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, 
       }
     };
 #undef FS_L0P4
-    if (nr <= 5) {
+    // a coarse level's kernel (with its diagonal) holds more registers: 5
+    // right-hand sides there need 256 VGPRs and some AGPRs (1 wave per SIMD),
+    // 4 fit in 230 (2 waves)
+    if (nr <= (a.dinv ? 4 : 5)) {
       e1 = eb;
       part(0, nr);
     } else {
@@ -1059,6 +1062,18 @@ void restrict_l0(hipStream_t s, const L0Args& a, const float* r, float* rc, int6
                gate)
     // exact RHS count: these kernels hold ~16 registers per RHS and lane
     if (a.dinv) {
+      if (nr == 8) {
+        // a coarse level's 8 right-hand sides need 256 VGPRs and some AGPRs
+        // (1 wave per SIMD): two launches of 4 (2 waves); no profiling here
+        launch_timed(k_restrict_l0x4<4, true>, dim3(a.tlr4.nblocks), s, nullptr, nullptr, r, rc,
+                     vsc, a.c[0], a.c[1], a.c[2], a.dinv, a.d, a.tlr4, a.kstart, a.m, a.vs, 4,
+                     omega, sc, mode, (unsigned long long*)nullptr, gate);
+        launch_timed(k_restrict_l0x4<4, true>, dim3(a.tlr4.nblocks), s, nullptr, nullptr,
+                     r + 4 * a.vs, rc + 4 * vsc, vsc, a.c[0], a.c[1], a.c[2], a.dinv, a.d, a.tlr4,
+                     a.kstart, a.m, a.vs, 4, omega, sc ? sc + 4 : sc, mode,
+                     (unsigned long long*)nullptr, gate);
+        return;
+      }
       switch (nr) {
         case 1: FS_L0R4(1, true); break;
         case 2: FS_L0R4(2, true); break;
@@ -1066,8 +1081,7 @@ void restrict_l0(hipStream_t s, const L0Args& a, const float* r, float* rc, int6
         case 4: FS_L0R4(4, true); break;
         case 5: FS_L0R4(5, true); break;
         case 6: FS_L0R4(6, true); break;
-        case 7: FS_L0R4(7, true); break;
-        default: FS_L0R4(8, true); break;
+        default: FS_L0R4(7, true); break;
       }
     } else {
       switch (nr) {
