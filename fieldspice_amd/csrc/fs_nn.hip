@@ -37,6 +37,14 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 __device__ __forceinline__ void voxel_ijk(const Grid3& g, int64_t v, int& i, int& j, int& k) {
+  if (g.n <= 0xffffffffLL) {  // 32-bit divisions (64-bit ones are long emulated sequences)
+    const uint32_t u = (uint32_t)v, nx = (uint32_t)g.nx, ny = (uint32_t)g.ny;
+    const uint32_t row = u / nx;
+    i = (int)(u - row * nx);
+    k = (int)(row / ny);
+    j = (int)(row - (uint32_t)k * ny);
+    return;
+  }
   i = (int)(v % g.nx);
   j = (int)((v / g.nx) % g.ny);
   k = (int)(v / ((int64_t)g.nx * g.ny));
@@ -154,14 +162,17 @@ __global__ __launch_bounds__(64) void k_conv_wgrad(const uint16_t* __restrict__ 
   floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
   for (int64_t vb = vb0; vb < vb1; vb += 32) {
     bf16x8 a, bv;
+    // the lane's 8 consecutive voxels: coordinates of the first by division,
+    // the others by stepping x with carries into y, z and the batch
+    const int64_t v0 = vb + 8 * kq;
+    int64_t bt = v0 / g.n;
+    int i, j, k;
+    voxel_ijk(g, v0 - bt * g.n, i, j, k);
 #pragma unroll
     for (int jj = 0; jj < 8; ++jj) {
-      const int64_t v = vb + 8 * kq + jj;
+      const int64_t v = v0 + jj;
       uint16_t xa = 0, yb = 0;
       if (v < vb1) {
-        const int64_t bt = v / g.n, vl = v % g.n;
-        int i, j, k;
-        voxel_ijk(g, vl, i, j, k);
         const int si = i + di, sj = j + dj, sk = k + dk;
         if (si >= 0 && si < g.nx && sj >= 0 && sj < g.ny && sk >= 0 && sk < g.nz)
           xa = X[(bt * g.n + ((int64_t)sk * g.ny + sj) * g.nx + si) * CIN + ci];
@@ -169,6 +180,16 @@ __global__ __launch_bounds__(64) void k_conv_wgrad(const uint16_t* __restrict__ 
       }
       a[jj] = __builtin_bit_cast(__bf16, xa);
       bv[jj] = __builtin_bit_cast(__bf16, yb);
+      if (++i == g.nx) {
+        i = 0;
+        if (++j == g.ny) {
+          j = 0;
+          if (++k == g.nz) {
+            k = 0;
+            ++bt;
+          }
+        }
+      }
     }
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc, 0, 0, 0);
   }
