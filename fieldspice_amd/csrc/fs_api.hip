@@ -2092,15 +2092,19 @@ int fs_batch_solve(fs_batch* b, const fs_solve_params* params, int chunk, double
       }
       (void)chk(hipEventRecord(b->joins[g], st));
     };
-    std::vector<std::thread> workers;
-    workers.reserve(ng);
     bool spawned = true;
-    try {
-      for (int g = 0; g < ng; ++g) workers.emplace_back(drive, g);
-    } catch (const std::exception&) {
-      spawned = false;  // the threads already running finish their groups first
+    if (ng == 1) {
+      drive(0);  // one group: the calling thread replays it
+    } else {
+      std::vector<std::thread> workers;
+      workers.reserve(ng);
+      try {
+        for (int g = 0; g < ng; ++g) workers.emplace_back(drive, g);
+      } catch (const std::exception&) {
+        spawned = false;  // the threads already running finish their groups first
+      }
+      for (auto& w : workers) w.join();
     }
-    for (auto& w : workers) w.join();
     if (!spawned) return fail(FS_EHIP, "could not start the batch's replay threads");
     for (int g = 0; g < ng; ++g) {
       if (grp[g].err != hipSuccess)
