@@ -53,6 +53,22 @@ __device__ __forceinline__ void voxel_ijk(const Grid3& g, int64_t v, int& i, int
 #define FS_GRID_STRIDE(t, cnt) \
   for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < (cnt); t += (int64_t)gridDim.x * 256)
 
+// element t = (b n + v) C + c of a channels-last tensor of cnt elements ->
+// (b, v, c), with 32-bit divisions when cnt fits
+__device__ __forceinline__ void split_bvc(int64_t t, int C, int64_t n, int64_t cnt, int64_t& b,
+                                          int64_t& v, int& c) {
+  if (cnt <= 0xffffffffLL) {
+    const uint32_t u = (uint32_t)t, q = u / (uint32_t)C, bb = q / (uint32_t)n;
+    c = (int)(u - q * (uint32_t)C);
+    v = (int64_t)(q - bb * (uint32_t)n);
+    b = (int64_t)bb;
+    return;
+  }
+  c = (int)(t % C);
+  v = (t / C) % n;
+  b = t / ((int64_t)C * n);
+}
+
 // ---------------------------------------------------------------------------
 // forward convolution: Y = mask(act(conv(X, W) + b + skip))
 // block = 4 waves over 16 voxels along x each; for short rows (nx <= 32) the
@@ -237,9 +253,11 @@ __global__ __launch_bounds__(256) void k_pool(const uint16_t* __restrict__ Xf, G
                                               uint16_t* __restrict__ Xc, Grid3 gc, int C,
     const int* __restrict__ gate) {
   if (gate && *gate) return;  // every right-hand side of the batch is idle
-  FS_GRID_STRIDE(t, gc.nb * gc.n * C) {
-    const int c = (int)(t % C);
-    const int64_t v = (t / C) % gc.n, b = t / (C * gc.n);
+  const int64_t cnt = gc.nb * gc.n * C;
+  FS_GRID_STRIDE(t, cnt) {
+    int c;
+    int64_t v, b;
+    split_bvc(t, C, gc.n, cnt, b, v, c);
     int I, J, K;
     voxel_ijk(gc, v, I, J, K);
     float s = 0.f;
@@ -259,9 +277,11 @@ __global__ __launch_bounds__(256) void k_pool_bwd(const uint16_t* __restrict__ d
                                                   uint16_t* __restrict__ dXf, Grid3 gf, int C,
                                                   const uint16_t* __restrict__ add,
                                                   const uint16_t* __restrict__ mask) {
-  FS_GRID_STRIDE(t, gf.nb * gf.n * C) {
-    const int c = (int)(t % C);
-    const int64_t v = (t / C) % gf.n, b = t / (C * gf.n);
+  const int64_t cnt = gf.nb * gf.n * C;
+  FS_GRID_STRIDE(t, cnt) {
+    int c;
+    int64_t v, b;
+    split_bvc(t, C, gf.n, cnt, b, v, c);
     int i, j, k;
     voxel_ijk(gf, v, i, j, k);
     const int I = i / 2, J = j / 2, K = k / 2;
@@ -277,9 +297,11 @@ __global__ __launch_bounds__(256) void k_up(const uint16_t* __restrict__ Xc, Gri
                                             uint16_t* __restrict__ Xf, Grid3 gf, int C,
     const int* __restrict__ gate) {
   if (gate && *gate) return;  // every right-hand side of the batch is idle
-  FS_GRID_STRIDE(t, gf.nb * gf.n * C) {
-    const int c = (int)(t % C);
-    const int64_t v = (t / C) % gf.n, b = t / (C * gf.n);
+  const int64_t cnt = gf.nb * gf.n * C;
+  FS_GRID_STRIDE(t, cnt) {
+    int c;
+    int64_t v, b;
+    split_bvc(t, C, gf.n, cnt, b, v, c);
     int i, j, k;
     voxel_ijk(gf, v, i, j, k);
     Xf[t] = Xc[(b * gc.n + ((int64_t)(k / 2) * gc.ny + j / 2) * gc.nx + i / 2) * C + c];
@@ -290,9 +312,11 @@ __global__ __launch_bounds__(256) void k_up(const uint16_t* __restrict__ Xc, Gri
 __global__ __launch_bounds__(256) void k_up_bwd(const uint16_t* __restrict__ dXf, Grid3 gf,
                                                 uint16_t* __restrict__ dXc, Grid3 gc, int C,
                                                 const uint16_t* __restrict__ mask) {
-  FS_GRID_STRIDE(t, gc.nb * gc.n * C) {
-    const int c = (int)(t % C);
-    const int64_t v = (t / C) % gc.n, b = t / (C * gc.n);
+  const int64_t cnt = gc.nb * gc.n * C;
+  FS_GRID_STRIDE(t, cnt) {
+    int c;
+    int64_t v, b;
+    split_bvc(t, C, gc.n, cnt, b, v, c);
     int I, J, K;
     voxel_ijk(gc, v, I, J, K);
     float s = 0.f;
@@ -338,9 +362,11 @@ __global__ __launch_bounds__(256) void k_features(const float* __restrict__ r, i
                                                   uint16_t* __restrict__ X,
     const int* __restrict__ gate) {
   if (gate && *gate) return;  // every right-hand side of the batch is idle
-  FS_GRID_STRIDE(t, g.nb * g.n * 16) {
-    const int c = (int)(t % 16);
-    const int64_t v = (t / 16) % g.n, b = t / (16 * g.n);
+  const int64_t cnt = g.nb * g.n * 16;
+  FS_GRID_STRIDE(t, cnt) {
+    int c;
+    int64_t v, b;
+    split_bvc(t, 16, g.n, cnt, b, v, c);
     float val = 0.f;
     if (c == 0) val = r[b * vs + v] * dh[v] * sc[2 * b + 1];
     else if (c <= 4) val = feat4[v * 4 + (c - 1)];
@@ -354,8 +380,11 @@ __global__ __launch_bounds__(256) void k_take0(const uint16_t* __restrict__ Y,
                                                float* __restrict__ e, int64_t vs,
     const int* __restrict__ gate) {
   if (gate && *gate) return;  // every right-hand side of the batch is idle
-  FS_GRID_STRIDE(t, g.nb * g.n) {
-    const int64_t v = t % g.n, b = t / g.n;
+  const int64_t cnt = g.nb * g.n;
+  FS_GRID_STRIDE(t, cnt) {
+    int c;
+    int64_t v, b;
+    split_bvc(t, 1, g.n, cnt, b, v, c);
     e[b * vs + v] = sc[2 * b] * dh[v] * bf2f(Y[t * 16]);
   }
 }
@@ -364,9 +393,11 @@ __global__ __launch_bounds__(256) void k_put0(const float* __restrict__ ge, int6
                                               const float* __restrict__ dh,
                                               const float* __restrict__ sc, Grid3 g,
                                               uint16_t* __restrict__ dY) {
-  FS_GRID_STRIDE(t, g.nb * g.n * 16) {
-    const int c = (int)(t % 16);
-    const int64_t v = (t / 16) % g.n, b = t / (16 * g.n);
+  const int64_t cnt = g.nb * g.n * 16;
+  FS_GRID_STRIDE(t, cnt) {
+    int c;
+    int64_t v, b;
+    split_bvc(t, 16, g.n, cnt, b, v, c);
     dY[t] = c == 0 ? f2bf(sc[2 * b] * dh[v] * ge[b * vs + v]) : (uint16_t)0;
   }
 }
