@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <chrono>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <new>
@@ -1800,6 +1801,131 @@ void free_batch(fs_batch* b) {
   delete b;
 }
 
+
+// A group of a batch's systems: one captured graph on one stream, replayed
+// by one host thread.
+struct BatchGroup {
+  std::vector<int> sys;
+  double work = 0.0;
+  int slot0 = 0;  // the group's systems take control slots slot0, slot0 + 1, ...
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipError_t err = hipSuccess;
+  bool finished = false;
+};
+
+// ng groups of about equal work (nodes x right-hand sides, longest first to
+// the least loaded group); slot[i]: system i's control slot, consecutive
+// within a group so that one kernel counts the group's finished systems
+std::vector<BatchGroup> batch_groups(const fs_batch* b, int ng, std::vector<int>& slot) {
+  const int ni = (int)b->inst.size();
+  std::vector<BatchGroup> grp(ng);
+  std::vector<int> order(ni);
+  for (int i = 0; i < ni; ++i) order[i] = i;
+  auto work = [&](int i) {
+    return (double)b->inst[i].cap->sys->d.n * (double)b->inst[i].cap->nt;
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+  for (int i : order) {
+    BatchGroup* best = &grp[0];
+    for (auto& g : grp)
+      if (g.work < best->work) best = &g;
+    best->sys.push_back(i);
+    best->work += work(i);
+  }
+  slot.assign(ni, 0);
+  for (int g = 0, k = 0; g < ng; ++g) {
+    grp[g].slot0 = k;
+    for (int i : grp[g].sys) slot[i] = k++;
+  }
+  return grp;
+}
+
+// the start of an FP32 cycle (and, for flexible CG, its first
+// preconditioner application)
+int batch_start_cycle(fs_sys* s, int nt, int mode, double inner_tol2) {
+  fs::launch_cycle_start(s, nt, inner_tol2);
+  if (mode == 2)
+    if (int r = fs::launch_fcg_start(s, nt)) return r;
+  return FS_OK;
+}
+
+// One group's graph on stream st: per system, `chunk` iterations, then the
+// system's device control: when every running right-hand side has ended its
+// FP32 cycle, the FP64 refresh and, if one is still running, the next
+// cycle's start.  The gated kernels exit at once on the replays that do not
+// need them, and finished systems skip everything, so the host only replays
+// the graph until the device counts the group's systems finished.
+int capture_group(fs_batch* b, BatchGroup& G, int g, const std::vector<int>& slot, int chunk,
+                  int mode, double inner_tol2) {
+  const hipStream_t st = b->streams[g];
+  const hipStream_t main = b->ctx->stream;
+  const bool cyc = mode != 0;
+  FS_HIP(hipStreamWaitEvent(st, b->fork, 0));
+  FS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int crc = FS_OK;
+  for (int i : G.sys) {
+    if (crc != FS_OK) break;
+    fs_sys* s = b->inst[i].cap->sys;
+    const int nt = b->inst[i].cap->nt;
+    fs::BatchCtl* c = b->ctl + slot[i];
+    s->stream = st;
+    s->pc_count = &c->pc_applies;
+    for (int k = 0; k < chunk && crc == FS_OK; ++k)
+      crc = fs::launch_iteration(s, nt, mode, nullptr);
+    if (crc == FS_OK && cyc) {
+      fs::launch_batch_refresh_gate(s, nt, c);
+      s->gate = &c->refresh_skip;
+      fs::launch_refresh(s, nt);
+    }
+    if (crc == FS_OK) fs::launch_batch_start_gate(s, nt, cyc ? 1 : 0, c);
+    if (crc == FS_OK && cyc) {
+      s->gate = &c->start_skip;
+      crc = batch_start_cycle(s, nt, mode, inner_tol2);
+    }
+    s->gate = nullptr;
+    s->pc_count = nullptr;
+    s->stream = main;
+  }
+  if (crc == FS_OK)
+    fs::launch_batch_count(st, b->ctl + G.slot0, (int)G.sys.size(), b->ndone + g);
+  const hipError_t ee = hipStreamEndCapture(st, &G.graph);
+  if (crc != FS_OK) return crc;
+  if (ee != hipSuccess) return fail(FS_EHIP, "graph capture failed: %s", hipGetErrorString(ee));
+  FS_HIP(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+  return FS_OK;
+}
+
+// Replays group g's graph until the device counts its systems finished (at
+// most max_rounds replays), two replays in flight: the finished count of
+// replay k is read while replay k+1 runs; the replay queued behind the last
+// one finds the group finished and exits kernel by kernel.  Runs on a host
+// thread of its own (graph launches cost host time per node, so one thread
+// would feed the device too slowly); errors go to G.err.
+void replay_group(fs_batch* b, BatchGroup& G, int g, int max_rounds) {
+  const hipStream_t st = b->streams[g];
+  int* cnt = b->ndone_host + 2 * g;
+  hipEvent_t* ev = b->polled.data() + 2 * g;
+  auto chk = [&](hipError_t e) {
+    if (e != hipSuccess && G.err == hipSuccess) G.err = e;
+    return e == hipSuccess;
+  };
+  if (!chk(hipSetDevice(b->ctx->device))) return;
+  auto replay = [&](int k) {
+    return chk(hipGraphLaunch(G.exec, st)) &&
+           chk(hipMemcpyAsync(cnt + (k & 1), b->ndone + g, sizeof(int), hipMemcpyDeviceToHost,
+                              st)) &&
+           chk(hipEventRecord(ev[k & 1], st));
+  };
+  const int want = (int)G.sys.size();
+  if (!replay(0)) return;
+  for (int k = 1; k <= max_rounds && !G.finished; ++k) {
+    if (k < max_rounds && !replay(k)) return;
+    if (!chk(hipEventSynchronize(ev[(k - 1) & 1]))) return;
+    G.finished = cnt[(k - 1) & 1] >= want;
+  }
+  (void)chk(hipEventRecord(b->joins[g], st));
+}
 }  // namespace
 
 extern "C" {
@@ -1948,48 +2074,10 @@ int fs_batch_solve(fs_batch* b, const fs_solve_params* params, int chunk, double
   };
   std::vector<std::vector<double>> hout(ni);
   std::vector<double> fallback_time(ni, -1.0);
-  // groups of about equal work (nodes x right-hand sides, longest first to
-  // the least loaded group); each group's systems take consecutive control
-  // slots so that one kernel counts the group's finished systems
-  struct Group {
-    std::vector<int> sys;
-    double work = 0.0;
-    int slot0 = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipError_t err = hipSuccess;
-    bool finished = false;
-  };
-  std::vector<Group> grp(ng);
-  {
-    std::vector<int> order(ni);
-    for (int i = 0; i < ni; ++i) order[i] = i;
-    auto work = [&](int i) {
-      return (double)b->inst[i].cap->sys->d.n * (double)b->inst[i].cap->nt;
-    };
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return work(x) > work(y); });
-    for (int i : order) {
-      Group* best = &grp[0];
-      for (auto& g : grp)
-        if (g.work < best->work) best = &g;
-      best->sys.push_back(i);
-      best->work += work(i);
-    }
-  }
-  std::vector<int> slot(ni);
-  for (int g = 0, k = 0; g < ng; ++g) {
-    grp[g].slot0 = k;
-    for (int i : grp[g].sys) slot[i] = k++;
-  }
+  std::vector<int> slot;
+  std::vector<BatchGroup> grp = batch_groups(b, ng, slot);
   int rc = FS_OK;
   auto t0 = std::chrono::steady_clock::now();
-  auto start_cycle = [&](fs_sys* s, int nt) -> int {
-    fs::launch_cycle_start(s, nt, inner_tol * inner_tol);
-    if (mode == 2)
-      if (int r = fs::launch_fcg_start(s, nt)) return r;
-    return FS_OK;
-  };
   auto destroy_graphs = [&]() {
     for (int g = 0; g < ng; ++g)
       if (grp[g].exec) (void)hipStreamSynchronize(b->streams[g]);  // replays still queued
@@ -2009,97 +2097,26 @@ int fs_batch_solve(fs_batch* b, const fs_solve_params* params, int chunk, double
       fs::launch_lift_cap(s, cap->nt, b->inst[i].L, cap->tid, 0);
       fs::launch_init(s, cap->nt, params->tol, maxit[i]);
       if (cyc) {
-        if (int r = start_cycle(s, cap->nt)) return r;
+        if (int r = batch_start_cycle(s, cap->nt, mode, inner_tol * inner_tol)) return r;
         s->acc_pc_applies += mode == 2 ? 1 : 0;
       }
     }
     FS_HIP(hipGetLastError());
     FS_HIP(hipEventRecord(b->fork, main));
-    // One graph per group, on the group's stream: per system, `chunk`
-    // iterations, then the system's device control: when every running
-    // right-hand side has ended its FP32 cycle, the FP64 refresh and, if one
-    // is still running, the next cycle's start.  The gated kernels exit at
-    // once on the replays that do not need them, and finished systems skip
-    // everything, so the host only replays each graph until the device
-    // counts the group's systems finished.
-    for (int g = 0; g < ng; ++g) {
-      Group& G = grp[g];
-      const hipStream_t st = b->streams[g];
-      FS_HIP(hipStreamWaitEvent(st, b->fork, 0));
-      FS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      int crc = FS_OK;
-      for (int i : G.sys) {
-        if (crc != FS_OK) break;
-        fs_sys* s = b->inst[i].cap->sys;
-        const int nt = b->inst[i].cap->nt;
-        fs::BatchCtl* c = b->ctl + slot[i];
-        s->stream = st;
-        s->pc_count = &c->pc_applies;
-        for (int k = 0; k < chunk && crc == FS_OK; ++k)
-          crc = fs::launch_iteration(s, nt, mode, nullptr);
-        if (crc == FS_OK && cyc) {
-          fs::launch_batch_refresh_gate(s, nt, c);
-          s->gate = &c->refresh_skip;
-          fs::launch_refresh(s, nt);
-        }
-        if (crc == FS_OK) fs::launch_batch_start_gate(s, nt, cyc ? 1 : 0, c);
-        if (crc == FS_OK && cyc) {
-          s->gate = &c->start_skip;
-          crc = start_cycle(s, nt);
-        }
-        s->gate = nullptr;
-        s->pc_count = nullptr;
-        s->stream = main;
-      }
-      if (crc == FS_OK)
-        fs::launch_batch_count(st, b->ctl + G.slot0, (int)G.sys.size(), b->ndone + g);
-      const hipError_t ee = hipStreamEndCapture(st, &G.graph);
-      if (crc != FS_OK) return crc;
-      if (ee != hipSuccess)
-        return fail(FS_EHIP, "graph capture failed: %s", hipGetErrorString(ee));
-      FS_HIP(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
-    }
+    for (int g = 0; g < ng; ++g)
+      if (int r = capture_group(b, grp[g], g, slot, chunk, mode, inner_tol * inner_tol)) return r;
     int max_rounds = 0;
     for (int i = 0; i < ni; ++i) max_rounds = std::max(max_rounds, maxit[i]);
     max_rounds = 4 * (max_rounds / chunk + 2) + 64;
-    // each group's graph is replayed by a host thread of its own (graph
-    // launches cost host time per node, so one thread would feed the device
-    // too slowly), two replays in flight: the thread reads the finished
-    // count of replay k while replay k+1 runs; the replay queued behind the
-    // last one finds the group finished and exits kernel by kernel
-    auto drive = [&](int g) {
-      Group& G = grp[g];
-      const hipStream_t st = b->streams[g];
-      int* cnt = b->ndone_host + 2 * g;
-      hipEvent_t* ev = b->polled.data() + 2 * g;
-      auto chk = [&](hipError_t e) {
-        if (e != hipSuccess && G.err == hipSuccess) G.err = e;
-        return e == hipSuccess;
-      };
-      if (!chk(hipSetDevice(ctx->device))) return;
-      auto replay = [&](int k) {
-        return chk(hipGraphLaunch(G.exec, st)) &&
-               chk(hipMemcpyAsync(cnt + (k & 1), b->ndone + g, sizeof(int),
-                                  hipMemcpyDeviceToHost, st)) &&
-               chk(hipEventRecord(ev[k & 1], st));
-      };
-      const int want = (int)G.sys.size();
-      if (!replay(0)) return;
-      for (int k = 1; k <= max_rounds && !G.finished; ++k) {
-        if (k < max_rounds && !replay(k)) return;
-        if (!chk(hipEventSynchronize(ev[(k - 1) & 1]))) return;
-        G.finished = cnt[(k - 1) & 1] >= want;
-      }
-      (void)chk(hipEventRecord(b->joins[g], st));
-    };
     bool spawned = true;
     if (ng == 1) {
-      drive(0);  // one group: the calling thread replays it
+      replay_group(b, grp[0], 0, max_rounds);  // one group: the calling thread
     } else {
       std::vector<std::thread> workers;
       workers.reserve(ng);
       try {
-        for (int g = 0; g < ng; ++g) workers.emplace_back(drive, g);
+        for (int g = 0; g < ng; ++g)
+          workers.emplace_back(replay_group, b, std::ref(grp[g]), g, max_rounds);
       } catch (const std::exception&) {
         spawned = false;  // the threads already running finish their groups first
       }
