@@ -190,15 +190,19 @@ void prof_begin(fs_sys* s) {
 //   restrict  3 couplings once; r read and the level-1 residual written per RHS
 //   prolong   3 couplings and the mask once; r and q read, z written and the
 //             level-1 error read per RHS
-void prof_end(fs_sys* s, fs_solve_stats* st, bool mixed) {
+// (flexible CG over a 4-wide cycle: z is FP16, 2 bytes written and read)
+void prof_end(fs_sys* s, fs_solve_stats* st, int precision) {
   if (!st || !s->profiling || !s->prof_dev) return;
   unsigned long long h[2 * FS_PROF_KINDS] = {};
   if (hipMemcpy(h, s->prof_dev, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) return;
+  const bool mixed = precision != FS_PREC_FP64;
+  const double zb = precision == FS_PREC_FCG && fs::pc_z_half(s->pc) ? 2.0 : 4.0;
   const double n = (double)s->d.n, elt = mixed ? 4.0 : 8.0;
   const double n1 = (double)fs::pc_level1_nodes(s->pc);
   const double fixed[FS_PROF_KINDS] = {3.0 * elt * n, 0.0, 0.0, 12.0 * n, 13.0 * n};
-  const double per_rhs[FS_PROF_KINDS] = {2.0 * elt * n, 5.0 * elt * n, 3.0 * elt * n,
-                                         4.0 * n + 4.0 * n1, 12.0 * n + 4.0 * n1};
+  const double per_rhs[FS_PROF_KINDS] = {2.0 * elt * n, (mixed ? 16.0 + zb : 40.0) * n,
+                                         3.0 * elt * n, 4.0 * n + 4.0 * n1,
+                                         (8.0 + zb) * n + 4.0 * n1};
   for (int k = 0; k < FS_PROF_KINDS; ++k) {
     const double bytes = fixed[k] * (double)h[2 * k] + per_rhs[k] * (double)h[2 * k + 1];
     st->kernel_time[k] += s->prof_time[k];
@@ -1150,7 +1154,7 @@ int sys_solve(fs_sys* s, int nrhs, const double* b, const double* x0, double* x,
     ms_total += ms;
     int st = FS_OK;
     collect_stats(s, nr, stats, st);
-    prof_end(s, stats, params->precision != FS_PREC_FP64);
+    prof_end(s, stats, params->precision);
     if (st != FS_OK) {
       status = st;
       if (stats) stats->converged = 0;
@@ -1298,7 +1302,7 @@ int fs_cap_solve(fs_cap* cap, const fs_op* L, double* C, double* q_shield, doubl
     ms_total += ms;
     int st = FS_OK;
     collect_stats(s, nr, stats, st);
-    prof_end(s, stats, mixed);
+    prof_end(s, stats, params->precision);
     if (st != FS_OK) {
       status = st;
       if (stats) stats->converged = 0;

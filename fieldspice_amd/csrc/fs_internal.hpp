@@ -65,6 +65,14 @@ struct CgScalars {
   int pad0, pad1;
 };
 
+// Flexible CG over a 4-wide level-0 cycle keeps its preconditioned residual
+// z in FP16 as z / zscale: the FP32 cycle's current residual norm |r32|_2
+// (aux0 = r32.r32), which bounds |z_n| / zscale by |M^-1|_2.
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float zscale(const CgScalars& c) {
+  return c.aux0 > 0.0 ? (float)sqrt(c.aux0) : 1.f;
+}
+
 // Device control of one system of a batched solve, written by the control
 // kernels that follow each chunk of the system's iterations in the batch's
 // captured graph.

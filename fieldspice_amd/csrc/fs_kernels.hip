@@ -854,6 +854,36 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(T* __restrict__ p, const T* 
   }
 }
 
+// flexible CG with an FP16 preconditioned residual: p = zscale * zh + beta p,
+// after the deferred x += alpha p
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_pupdate_h(float* __restrict__ p,
+                                                      const _Float16* __restrict__ zh,
+                                                      VecArgs va, int mode, const CgScalars* sc,
+                                                      float* __restrict__ x,
+                                                      unsigned long long* prof = nullptr) {
+  if (all_idle(sc, va.nr, mode)) return;
+  if (prof) {
+    bool live[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) live[r] = r < va.nr && rhs_live(sc, r, mode);
+    prof_count<NR>(prof, live);
+  }
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (!(r < va.nr && rhs_live(sc, r, mode))) continue;
+    const float b = (float)sc[r].beta, a = (float)sc[r].alpha, hs = zscale(sc[r]);
+    float* pr = p + r * va.vs;
+    float* xr = x + r * va.vs;
+    const _Float16* zr = zh + r * va.vs;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < va.n; i += stride) {
+      const float pv = pr[i];
+      xr[i] += a * pv;
+      pr[i] = (float)zr[i] * hs + b * pv;
+    }
+  }
+}
 
 // --------------------------------------------------------------------------
 // Finalisation of the block partials: one single-block kernel launched right
@@ -1605,9 +1635,15 @@ static int launch_iteration_t(fs_sys* s, int nr, int mode, ProfSlot* sl) {
   }
   // FP32 cycle: CG searches along r32, flexible CG along z32 = M(r32)
   const float* dir = mode == 2 ? s->z32 : s->r32;
-  launch_timed(k_pupdate<float, NR, true>, dim3(vb), st, prof_start(sl, FS_PROF_DIRECTION),
-               prof_stop(sl, FS_PROF_DIRECTION), s->p32, dir, va, mode, (const CgScalars*)s->sc,
-               s->x32, prof_ctr(s, sl, FS_PROF_DIRECTION));
+  if (mode == 2 && pc_z_half(s->pc))
+    launch_timed(k_pupdate_h<NR>, dim3(vb), st, prof_start(sl, FS_PROF_DIRECTION),
+                 prof_stop(sl, FS_PROF_DIRECTION), s->p32,
+                 reinterpret_cast<const _Float16*>(s->z32), va, mode, (const CgScalars*)s->sc,
+                 s->x32, prof_ctr(s, sl, FS_PROF_DIRECTION));
+  else
+    launch_timed(k_pupdate<float, NR, true>, dim3(vb), st, prof_start(sl, FS_PROF_DIRECTION),
+                 prof_stop(sl, FS_PROF_DIRECTION), s->p32, dir, va, mode,
+                 (const CgScalars*)s->sc, s->x32, prof_ctr(s, sl, FS_PROF_DIRECTION));
   int sbf = sb;
   hipEvent_t ea = prof_start(sl, FS_PROF_STENCIL), eb = prof_stop(sl, FS_PROF_STENCIL);
   unsigned long long* pst = prof_ctr(s, sl, FS_PROF_STENCIL);

@@ -78,6 +78,8 @@ void launch_flush_x(fs_sys* s, int nr);
 void launch_replace(fs_sys* s, int nr, unsigned restart);
 // nodes per lane of the level-0 cycle kernels of a preconditioner (1 or 4)
 int pc_level0_width(const fs_pc* pc);
+// 1: flexible CG stores this preconditioner's output in FP16 (4-wide level 0)
+int pc_z_half(const fs_pc* pc);
 void launch_init(fs_sys* s, int nr, double tol, int maxiter);
 void launch_lift_general(fs_sys* s, int nr, const double* b);
 void launch_lift_cap(fs_sys* s, int nr, const fs_op* L, const int32_t* tid, int j0);
@@ -148,8 +150,9 @@ int launch_stream_probe(hipStream_t st, const float* const* in, float* const* ou
 // and prolongation launches are timed as FS_PROF_RESTRICT / FS_PROF_PROLONG.
 int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, double* partials,
              int* nblocks, const CgScalars* sc, int mode, ProfSlot* slot = nullptr,
-             unsigned long long* prof_dev = nullptr);
-// z32 = M(r32) with the FCG partials ([2*nr][*nblocks])
+             unsigned long long* prof_dev = nullptr, int zhalf = 0);
+// z32 = M(r32) with the FCG partials ([2*nr][*nblocks]); z32 holds FP16
+// values (z / zscale) when pc_z_half(s->pc)
 int pc_apply_solver(fs_sys* s, int nr, int* nblocks, ProfSlot* slot = nullptr);
 // nodes of the level below level 0 (0 without a hierarchy)
 int64_t pc_level1_nodes(const fs_pc* pc);

@@ -624,7 +624,11 @@ __device__ __forceinline__ float4 pick(bool c, float4 a, float4 b) {
 // 0 (unit diagonal, Dirichlet mask).  The smoothed iterate of the rows above
 // and below comes from the neighbouring waves of the block through LDS; the
 // block's two edge rows read the one row outside it.
-template <int NR, bool DIAG>
+// HZ (level 0 inside flexible CG): e is stored as FP16 e / |r|_2 per
+// right-hand side, |r|_2 = sqrt(sc.aux0), the FP32 cycle's current residual
+// norm (saturated at the FP16 range), and the partials use the stored value;
+// the direction update reads it back with the same scale.
+template <int NR, bool DIAG, bool HZ = false>
 __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
     const float* __restrict__ r, const float* __restrict__ ec, int64_t vsc,
     float* __restrict__ e, const float* __restrict__ cx, const float* __restrict__ cy,
@@ -791,6 +795,16 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
           out.z = mk.z == 0 ? c.z + omega * (rr.z - c.z + s2) : 0.f;
           out.w = mk.w == 0 ? c.w + omega * (rr.w - c.w + s3) : 0.f;
         }
+        if constexpr (HZ) {  // the value the direction update will read
+          const float hs = zscale(sc[b0 + b]);
+          const float zi = 1.f / hs;
+          const h4 hv = {(_Float16)fminf(fmaxf(out.x * zi, -65504.f), 65504.f),
+                         (_Float16)fminf(fmaxf(out.y * zi, -65504.f), 65504.f),
+                         (_Float16)fminf(fmaxf(out.z * zi, -65504.f), 65504.f),
+                         (_Float16)fminf(fmaxf(out.w * zi, -65504.f), 65504.f)};
+          if (valid) *reinterpret_cast<h4*>(reinterpret_cast<_Float16*>(e) + b * vs + nn) = hv;
+          out = f4((float)hv[0] * hs, (float)hv[1] * hs, (float)hv[2] * hs, (float)hv[3] * hs);
+        }
         if (partials) {
           // the lane's 4 products in FP32, their sums accumulated in FP64
           const float4 qv = qq[b];
@@ -799,7 +813,8 @@ __global__ __launch_bounds__(kBlock) void k_prolong_l0x4(
           acc[b] += valid ? a0 : 0.0;
           acc[NR + b] += valid ? a1 : 0.0;
         }
-        if (valid) *reinterpret_cast<float4*>(e + b * vs + nn) = out;
+        if constexpr (!HZ)
+          if (valid) *reinterpret_cast<float4*>(e + b * vs + nn) = out;
         zm[b] = c;
         zc[b] = zu;
         rc[b] = ru[b];
@@ -995,7 +1010,8 @@ void energy_grad(hipStream_t s, const Level& L, const float* y, const float* r, 
 int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, int64_t vsc,
                float* e, int nr, float omega, const float* q, double* partials,
                const CgScalars* sc, int mode, hipEvent_t ea, hipEvent_t eb,
-               unsigned long long* prof, const int* gate) {
+               unsigned long long* prof, const int* gate, int zhalf) {
+  if (zhalf && (!a.x4 || a.dinv || !sc)) return -1;  // FP16 output: level 0, 4-wide, in a solve
   if (a.x4) {
     // ~16 registers per RHS and lane: launched for the exact RHS count, and
     // in two halves above 5 RHS (8 in one launch would run 1 wave per SIMD);
@@ -1006,8 +1022,22 @@ int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, 
                ec + (B0) * vsc, vsc, e + (B0) * a.vs, a.c[0], a.c[1], a.c[2], a.mask, a.dg,      \
                a.dinv, a.d, a.tl4, a.m, a.vs, NB, omega, q ? q + (B0) * a.vs : nullptr,        \
                partials, sc, mode, B0, nr, prof, gate)
+#define FS_L0P4H(NR_, B0, NB)                                                                    \
+  launch_timed(k_prolong_l0x4<NR_, false, true>, dim3(a.tl4.nblocks), s, e0, e1,                 \
+               r + (B0) * a.vs, ec + (B0) * vsc, vsc,                                          \
+               reinterpret_cast<float*>(reinterpret_cast<_Float16*>(e) + (B0) * a.vs), a.c[0],  \
+               a.c[1], a.c[2], a.mask, a.dg, a.dinv, a.d, a.tl4, a.m, a.vs, NB, omega,          \
+               q ? q + (B0) * a.vs : nullptr, partials, sc, mode, B0, nr, prof, gate)
     auto part = [&](int b0, int nb) {
-      if (a.dinv) {
+      if (zhalf) {
+        switch (nb) {
+          case 1: FS_L0P4H(1, b0, nb); break;
+          case 2: FS_L0P4H(2, b0, nb); break;
+          case 3: FS_L0P4H(3, b0, nb); break;
+          case 4: FS_L0P4H(4, b0, nb); break;
+          default: FS_L0P4H(5, b0, nb); break;
+        }
+      } else if (a.dinv) {
         switch (nb) {
           case 1: FS_L0P4(1, true, b0, nb); break;
           case 2: FS_L0P4(2, true, b0, nb); break;
@@ -1026,6 +1056,7 @@ int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, 
       }
     };
 #undef FS_L0P4
+#undef FS_L0P4H
     // a coarse level's kernel (with its diagonal) holds more registers: 5
     // right-hand sides there need 256 VGPRs and some AGPRs (1 wave per SIMD),
     // 4 fit in 230 (2 waves)

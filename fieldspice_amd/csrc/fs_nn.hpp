@@ -136,10 +136,12 @@ struct L0Args {
 // optional profiling: start/stop events of the launch and the counters of
 // the kernel kind ([0] iterations with a live RHS, [1] live RHS summed);
 // gate (optional): a device flag, nonzero = skip the launch
+// zhalf: store e in FP16 scaled by zscale(sc[b]) (level 0 of a 4-wide cycle in
+// a flexible-CG solve; returns -1 otherwise)
 int prolong_l0(hipStream_t s, const L0Args& a, const float* r, const float* ec, int64_t vsc,
                float* e, int nr, float omega, const float* q, double* partials,
                const CgScalars* sc, int mode, hipEvent_t ea = nullptr, hipEvent_t eb = nullptr,
-               unsigned long long* prof = nullptr, const int* gate = nullptr);
+               unsigned long long* prof = nullptr, const int* gate = nullptr, int zhalf = 0);
 void restrict_l0(hipStream_t s, const L0Args& a, const float* r, float* rc, int64_t vsc, int nr,
                  float omega, const CgScalars* sc, int mode, hipEvent_t ea = nullptr,
                  hipEvent_t eb = nullptr, unsigned long long* prof = nullptr,

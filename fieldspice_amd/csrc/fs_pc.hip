@@ -349,7 +349,7 @@ namespace fs {
 // marks idle (mode as the Krylov kernels) are skipped on level 0.
 int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, double* partials,
              int* nblocks, const CgScalars* sc, int mode, ProfSlot* slot,
-             unsigned long long* prof_dev) {
+             unsigned long long* prof_dev, int zhalf) {
   // level-0 timing (FS_PROF_RESTRICT / FS_PROF_PROLONG) when profiled
   unsigned long long* pr = slot && prof_dev ? prof_dev + 2 * FS_PROF_RESTRICT : nullptr;
   unsigned long long* pp = slot && prof_dev ? prof_dev + 2 * FS_PROF_PROLONG : nullptr;
@@ -402,13 +402,16 @@ int pc_cycle(fs_pc* pc, int nr, const float* r0, float* z0, const float* q, doub
                   pc->ev[l], nr, pc->omega, nullptr, nullptr, 0, g);
   }
   const int nb = mg::prolong_l0(s, pc->l0, r0, pc->ev[1], pc->lv[1].vs, z0, nr, pc->omega, q,
-                                partials, sc, mode, pa, pb, pp, g0);
+                                partials, sc, mode, pa, pb, pp, g0, zhalf);
+  if (nb < 0) return FS_EINVAL;
   if (nblocks) *nblocks = nb;
   FS_HIP(hipGetLastError());
   return FS_OK;
 }
 
 int64_t pc_level1_nodes(const fs_pc* pc) { return pc && pc->K > 0 ? pc->lv[1].n : 0; }
+
+int pc_z_half(const fs_pc* pc) { return pc && pc->K > 0 && pc->l0.x4 ? 1 : 0; }
 
 int pc_level0_width(const fs_pc* pc) {
   return pc && pc->l0_pairs && pc->l0.x4 ? 4 : 1;
@@ -417,7 +420,7 @@ int pc_level0_width(const fs_pc* pc) {
 // Solver hook (fs_kernels.hip, FS_PREC_FCG): z32 = M(r32) with FCG partials.
 int pc_apply_solver(fs_sys* s, int nr, int* nblocks, ProfSlot* slot) {
   return pc_cycle(s->pc, nr, s->r32, s->z32, s->q32, s->partials, nblocks, s->sc, 2, slot,
-                  s->prof_dev);
+                  s->prof_dev, pc_z_half(s->pc));
 }
 
 void pc_orphan(fs_pc* pc) {

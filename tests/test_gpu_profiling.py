@@ -61,9 +61,11 @@ def test_profiled_fcg_times_every_level0_kernel(gpu_lib, nt):
     b = st1["kernel_bytes"][names.index("stencil")]
     assert b <= it * n * (12 + 8 * nt) * (1 + 1e-12)
     assert b >= it * n * (12 + 8 * 1) * (1 - 1e-12)
-    # direction update: 20 n bytes per live RHS and launch
+    # direction update per live RHS and launch: p and x read and written, z
+    # read (FP16 over a 4-wide cycle: 18 n bytes, else 20 n)
+    assert st1["cycle_width"] == 4  # 64-node rows, x and y in pairs
     bd = st1["kernel_bytes"][names.index("direction")]
-    assert bd <= it * n * 20 * nt * (1 + 1e-12) and bd % (20 * n) == 0
+    assert bd <= it * n * 18 * nt * (1 + 1e-12) and bd % (18 * n) == 0
 
 
 def test_stream_probe_times_a_mix(gpu_lib):
