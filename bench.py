@@ -140,7 +140,7 @@ def kernel_table(prof, n):
     return out
 
 
-def kernel_names(nt, fp64, nx, width):
+def kernel_names(nt, fp64, nx, width, fcg=False):
     """Kernel symbol of each profiled kind for nt right-hand sides on rows of
     nx nodes, the Krylov stencil running `width` nodes per lane."""
     bucket = 1 if nt <= 1 else 2 if nt <= 2 else 4 if nt <= 4 else 8
@@ -153,11 +153,14 @@ def kernel_names(nt, fp64, nx, width):
     half = (nt + 1) // 2
     return {"stencil": (f"k_stencil_dotw<{nt}, {width}>" if width > 1 else
                         f"k_stencil_dot<float, {bucket}>"),
-            "direction": f"k_pupdate<float, {bucket}, true>",
+            # flexible CG over the 4-wide cycle keeps z in FP16
+            "direction": (f"k_pupdate_h<{bucket}>" if fcg and x4 else
+                          f"k_pupdate<float, {bucket}, true>"),
             "residual": f"k_update<float, {bucket}, true>",
             "restrict": f"k_restrict_l0x4<{nt}, false>" if x4 else "k_restrict_l0<8>",
-            "prolong": ((f"k_prolong_l0x4<{nt}, false>" if nt <= 5 else
-                         f"k_prolong_l0x4<{half}, false> + <{nt - half}, false>") if x4 else
+            "prolong": ((f"k_prolong_l0x4<{nt}, false, true>" if nt <= 5 else
+                         f"k_prolong_l0x4<{half}, false, true> + <{nt - half}, false, true>")
+                        if x4 else
                         f"k_prolong_l0<{bucket}>")}
 
 
@@ -528,7 +531,7 @@ def main():
     kt = b["kernels"]
     # the dominant kernel of the reported method (largest summed launch time)
     dom = max(kt, key=lambda k: kt[k]["time_s"])
-    names = kernel_names(nt, fp64, g.shape_nodes[0], b["stencil_width"])
+    names = kernel_names(nt, fp64, g.shape_nodes[0], b["stencil_width"], head.startswith("hip_fcg"))
 
     def roof(kind):
         e = kt[kind]
